@@ -1,0 +1,303 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module; the product package never does.  ``Oracle`` drives
+oracle/liboracle.so (the C restatement in camp_oracle.c) with the same call
+sequence CAMP's Fortran layer uses on the C ABI (camp_solver_data.F90:
+612-778): new -> add reactions -> initialize -> update_data -> run.
+``RefLib`` drives oracle/_ref/libcampref.so, the reference's own C sources.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libcampref.so")
+REF_ROOT = "/root/reference"
+
+_i = C.c_int
+_d = C.c_double
+_p = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+
+def build(ref=True, quiet=True):
+    """Compile liboracle.so, and oracle/_ref when the reference tree exists."""
+    targets = ["oracle"]
+    if ref and os.path.isdir(os.path.join(REF_ROOT, "src")):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+def _ip_(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _dp_(a):
+    return a.ctypes.data_as(_dp)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build(ref=False)
+        L = C.CDLL(LIB_PATH)
+        L.oracle_new.restype = _p
+        L.oracle_new.argtypes = [_i, _i, _ip, _i, _i, _i, _i]
+        L.oracle_add_rxn.argtypes = [_p, _i, _i, _i, _i, _ip, _dp]
+        L.oracle_initialize.argtypes = [_p, _dp, _d, _i, _i]
+        L.oracle_free.argtypes = [_p]
+        L.oracle_update_data.argtypes = [_p, _i, _ip, _i, _i, _d]
+        L.oracle_update_data.restype = _i
+        for name in ("oracle_n_dep", "oracle_nnz", "oracle_nnz_rxn", "oracle_n_mapped",
+                     "oracle_rxn_int_size", "oracle_n_env", "oracle_lu_nnz"):
+            getattr(L, name).argtypes = [_p]
+            getattr(L, name).restype = _i
+        L.oracle_get_pattern.argtypes = [_p, _ip, _ip, _ip, _ip, _ip]
+        L.oracle_get_jac_map.argtypes = [_p, _ip, _ip, _ip]
+        L.oracle_get_rxn_int_data.argtypes = [_p, _ip, _i]
+        L.oracle_get_rxn_env.argtypes = [_p, _dp]
+        L.oracle_update_env_state.argtypes = [_p, _dp]
+        L.oracle_eval_f.argtypes = [_p, _i, _dp, _dp, _d, _i]
+        L.oracle_eval_f.restype = _i
+        L.oracle_eval_jac.argtypes = [_p, _i, _dp, _dp, _d]
+        L.oracle_eval_jac.restype = _i
+        L.oracle_reset_jac_history.argtypes = [_p]
+        L.oracle_solver_run.argtypes = [_p, _dp, _dp, _d, _d, _i]
+        L.oracle_solver_run.restype = _i
+        L.oracle_solver_run_cells.argtypes = [_p, _dp, _dp, _d, _d, _i, _i]
+        L.oracle_solver_run_cells.restype = _i
+        L.oracle_get_stats.argtypes = [_p, C.POINTER(C.c_long), _dp]
+        L.oracle_set_guess_helper.argtypes = [_p, _i]
+        L.oracle_lu_solve_sparse.argtypes = [_p, _dp, _d, _dp, _dp]
+        L.oracle_lu_solve_sparse.restype = _i
+        L.oracle_lu_solve_dense.argtypes = [_p, _dp, _d, _dp, _dp]
+        L.oracle_lu_solve_dense.restype = _i
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """One oracle model: mechanism packed into the reference layout."""
+
+    def __init__(self, mech, n_cells, abs_tol=None, rel_tol=None,
+                 max_steps=10000, max_conv_fails=1000):
+        from camp_amd.mechanisms import pack
+        L = lib()
+        self.mech = mech
+        self.n_cells = n_cells
+        self.ns = mech.n_state
+        packed = pack(mech)
+        n_int = sum(len(p[1]) for p in packed)
+        n_float = sum(len(p[2]) for p in packed)
+        n_env = sum(p[3] for p in packed)
+        vt = np.asarray(mech.var_type, dtype=np.int32)
+        self.h = L.oracle_new(self.ns, n_cells, _ip_(vt), len(packed), n_int, n_float, n_env)
+        for t, ints, floats, ne in packed:
+            ia = np.asarray(ints, dtype=np.int32)
+            fa = np.asarray(floats, dtype=np.float64)
+            L.oracle_add_rxn(self.h, t, len(ints), len(floats), ne, _ip_(ia), _dp_(fa))
+        at = np.asarray(mech.abs_tol if abs_tol is None else abs_tol, dtype=np.float64)
+        rt = mech.rel_tol if rel_tol is None else rel_tol
+        L.oracle_initialize(self.h, _dp_(at), rt, max_steps, max_conv_fails)
+        self.nv = L.oracle_n_dep(self.h)
+        self.nnz = L.oracle_nnz(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.oracle_free(self.h)
+            self.h = None
+
+    def set_rates(self, rates, cells=None):
+        """rates: {rxn_id: array over cells}; uses rxn_update_data per cell."""
+        L = lib()
+        payload = self.mech
+        types = {r.rxn_id: r.type for r in payload.reactions if r.rxn_id > 0}
+        from camp_amd.mechanisms import TYPE_CODES
+        cells = range(self.n_cells) if cells is None else cells
+        rid = C.c_int(0)
+        for uid, vals in rates.items():
+            code = TYPE_CODES[types[uid]]
+            vals = np.broadcast_to(np.asarray(vals, dtype=np.float64), (self.n_cells,))
+            for c in cells:
+                rid.value = 0
+                L.oracle_update_data(self.h, c, C.byref(rid), code, uid, float(vals[c]))
+
+    def pattern(self):
+        L = lib()
+        nnz_r = L.oracle_nnz_rxn(self.h)
+        rcp = np.zeros(self.ns + 1, np.int32)
+        rrv = np.zeros(max(nnz_r, 1), np.int32)
+        cp = np.zeros(self.nv + 1, np.int32)
+        rv = np.zeros(max(self.nnz, 1), np.int32)
+        di = np.zeros(self.ns, np.int32)
+        L.oracle_get_pattern(self.h, _ip_(rcp), _ip_(rrv), _ip_(cp), _ip_(rv), _ip_(di))
+        return dict(rxn_colptr=rcp, rxn_rowval=rrv[:nnz_r], colptr=cp, rowval=rv[:self.nnz],
+                    deriv_ids=di)
+
+    def jac_map(self):
+        L = lib()
+        n = L.oracle_n_mapped(self.h)
+        a, b, c = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+        L.oracle_get_jac_map(self.h, _ip_(a), _ip_(b), _ip_(c))
+        return a[:n], b[:n], c[:n]
+
+    def int_data(self):
+        L = lib()
+        n = L.oracle_rxn_int_size(self.h)
+        a = np.zeros(n, np.int32)
+        L.oracle_get_rxn_int_data(self.h, _ip_(a), n)
+        return a
+
+    def update_env(self, env):
+        env = np.ascontiguousarray(env, dtype=np.float64)
+        lib().oracle_update_env_state(self.h, _dp_(env))
+
+    def rxn_env(self):
+        L = lib()
+        a = np.zeros(self.n_cells * L.oracle_n_env(self.h))
+        L.oracle_get_rxn_env(self.h, _dp_(a))
+        return a.reshape(self.n_cells, -1)
+
+    def eval_f(self, cell, state_cell, time_step=1.0, use_deriv_est=False):
+        s = np.ascontiguousarray(state_cell, dtype=np.float64)
+        out = np.zeros(self.nv)
+        rc = lib().oracle_eval_f(self.h, cell, _dp_(s), _dp_(out), time_step, int(use_deriv_est))
+        return rc, out
+
+    def eval_jac(self, cell, state_cell, time_step=1.0):
+        s = np.ascontiguousarray(state_cell, dtype=np.float64)
+        out = np.zeros(self.nnz)
+        rc = lib().oracle_eval_jac(self.h, cell, _dp_(s), _dp_(out), time_step)
+        return rc, out
+
+    def reset_jac_history(self):
+        lib().oracle_reset_jac_history(self.h)
+
+    def run(self, state, env, t0, t1, mode=1):
+        """solver_run; state (n_cells, ns) is updated in place."""
+        assert state.flags.c_contiguous and state.dtype == np.float64
+        env = np.ascontiguousarray(env, dtype=np.float64)
+        return lib().oracle_solver_run(self.h, _dp_(state), _dp_(env), t0, t1, mode)
+
+    def run_cells(self, state, env, t0, t1, c0, c1):
+        env = np.ascontiguousarray(env, dtype=np.float64)
+        return lib().oracle_solver_run_cells(self.h, _dp_(state), _dp_(env), t0, t1, c0, c1)
+
+    def stats(self):
+        c = (C.c_long * 8)()
+        h = np.zeros(2)
+        lib().oracle_get_stats(self.h, c, _dp_(h))
+        keys = ("num_steps", "RHS_evals", "LS_setups", "error_test_fails", "NLS_iters",
+                "NLS_convergence_fails", "DLS_Jac_evals", "solver_flag")
+        d = dict(zip(keys, list(c)))
+        d["last_time_step__s"], d["next_time_step__s"] = float(h[0]), float(h[1])
+        return d
+
+    def set_guess_helper(self, on):
+        lib().oracle_set_guess_helper(self.h, int(on))
+
+    def lu_solve(self, jac, gamma, b, dense=False):
+        jac = np.ascontiguousarray(jac, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.nv)
+        fn = lib().oracle_lu_solve_dense if dense else lib().oracle_lu_solve_sparse
+        rc = fn(self.h, _dp_(jac), gamma, _dp_(b), _dp_(x))
+        return rc, x
+
+    def lu_nnz(self):
+        return lib().oracle_lu_nnz(self.h)
+
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_LIB_PATH)
+
+
+def reflib():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_LIB_PATH)
+        L.ref_new.restype = _p
+        L.ref_new.argtypes = [_i, _i, _ip, _i, _i, _i, _i]
+        L.ref_add_rxn.argtypes = [_p, _i, _i, _i, _i, _ip, _dp]
+        L.ref_build_pattern.argtypes = [_p, _ip, _ip, _i]
+        L.ref_build_pattern.restype = _i
+        L.ref_update_ids.argtypes = [_p, _ip]
+        L.ref_get_int_data.argtypes = [_p, _ip, _i]
+        L.ref_update_env.argtypes = [_p, _i, _dp]
+        L.ref_update_data.argtypes = [_p, _i, _ip, _i, _i, _d]
+        L.ref_get_env_data.argtypes = [_p, _dp, _i]
+        L.ref_time_derivative.argtypes = [_i, _i, _ip, _dp, _dp, _dp]
+        _ref = L
+    return _ref
+
+
+class RefModel:
+    """The reference's own rxn_* / Jacobian functions on a packed mechanism."""
+
+    def __init__(self, mech, n_cells):
+        from camp_amd.mechanisms import pack
+        L = reflib()
+        self.mech = mech
+        self.n_cells = n_cells
+        packed = pack(mech)
+        self.n_int = sum(len(p[1]) for p in packed) + len(packed)
+        n_float = sum(len(p[2]) for p in packed)
+        self.n_env = sum(p[3] for p in packed)
+        vt = np.asarray(mech.var_type, dtype=np.int32)
+        self._keep = []
+        self.h = L.ref_new(mech.n_state, n_cells, _ip_(vt), len(packed),
+                           self.n_int - len(packed), n_float, self.n_env)
+        for t, ints, floats, ne in packed:
+            ia = np.asarray(ints, dtype=np.int32)
+            fa = np.asarray(floats if floats else [0.0], dtype=np.float64)
+            L.ref_add_rxn(self.h, t, len(ints), len(floats), ne, _ip_(ia), _dp_(fa))
+
+    def pattern(self):
+        ns = self.mech.n_state
+        cp = np.zeros(ns + 1, np.int32)
+        rv = np.zeros(ns * ns, np.int32)
+        nnz = reflib().ref_build_pattern(self.h, _ip_(cp), _ip_(rv), ns * ns)
+        return cp, rv[:nnz]
+
+    def update_ids(self, deriv_ids):
+        di = np.ascontiguousarray(deriv_ids, dtype=np.int32)
+        reflib().ref_update_ids(self.h, _ip_(di))
+        out = np.zeros(self.n_int, np.int32)
+        reflib().ref_get_int_data(self.h, _ip_(out), self.n_int)
+        return out
+
+    def update_env(self, cell, T, P):
+        e = np.array([T, P], dtype=np.float64)
+        reflib().ref_update_env(self.h, cell, _dp_(e))
+
+    def update_data(self, cell, rtype, uid, rate):
+        rid = C.c_int(0)
+        reflib().ref_update_data(self.h, cell, C.byref(rid), rtype, uid, rate)
+
+    def env_data(self):
+        out = np.zeros(self.n_cells * self.n_env)
+        reflib().ref_get_env_data(self.h, _dp_(out), out.size)
+        return out.reshape(self.n_cells, self.n_env)
+
+
+def ref_time_derivative(n_spec, spec, contrib, est=None):
+    spec = np.ascontiguousarray(spec, dtype=np.int32)
+    contrib = np.ascontiguousarray(contrib, dtype=np.float64)
+    out = np.zeros(n_spec)
+    e = None if est is None else np.ascontiguousarray(est, dtype=np.float64)
+    reflib().ref_time_derivative(n_spec, len(spec), _ip_(spec), _dp_(contrib),
+                                 None if e is None else _dp_(e), _dp_(out))
+    return out
