@@ -1,0 +1,71 @@
+"""ctypes binding of libcamp_mi355x.so (the HIP solver's C ABI).
+
+The library is built in-tree by camp_amd/build.py (hipcc, gfx950).  There is
+no fallback: if the library is missing or fails to load, every entry point
+raises, so nothing can silently run on the CPU instead of the GPU.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcamp_mi355x.so")
+
+_i, _d, _p = C.c_int, C.c_double, C.c_void_p
+_ip, _dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every symbol declared in include/camp_solver.h
+SIGNATURES = {
+    "solver_new": (_p, [_i, _i, _ip, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i]),
+    "solver_initialize": (None, [_p, _dp, _d, _i, _i]),
+    "solver_run": (_i, [_p, _dp, _dp, _d, _d]),
+    "solver_get_statistics": (None, [_p, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp,
+                                     _ip, _ip, _ip, _dp, _dp, _dp]),
+    "solver_reset_timers": (None, [_p]),
+    "solver_free": (None, [_p]),
+    "rxn_add_condensed_data": (None, [_i, _i, _i, _i, _ip, _dp, _p]),
+    "rxn_update_data": (None, [_i, _ip, _i, _p, _p]),
+    "rxn_print_data": (None, [_p]),
+    "rxn_free_update_data": (None, [_p]),
+    "rxn_photolysis_create_rate_update_data": (_p, []),
+    "rxn_photolysis_set_rate_update_data": (None, [_p, _i, _d]),
+    "rxn_emission_create_rate_update_data": (_p, []),
+    "rxn_emission_set_rate_update_data": (None, [_p, _i, _d]),
+    "rxn_first_order_loss_create_rate_update_data": (_p, []),
+    "rxn_first_order_loss_set_rate_update_data": (None, [_p, _i, _d]),
+    "aero_phase_add_condensed_data": (None, [_i, _i, _ip, _dp, _p]),
+    "aero_rep_add_condensed_data": (None, [_i, _i, _i, _i, _ip, _dp, _p]),
+    "sub_model_add_condensed_data": (None, [_i, _i, _i, _i, _ip, _dp, _p]),
+    "camp_solver_set_mode": (_i, [_p, _i]),
+    "camp_solver_set_cells_per_block": (_i, [_p, _i]),
+    "camp_solver_set_guess_helper": (None, [_p, _i]),
+    "camp_solver_run_device": (_i, [_p, _p, _p, _d, _d]),
+    "camp_solver_set_base_rates": (_i, [_p, _i, _i, _dp]),
+    "camp_solver_dims": (_i, [_p, _ip, _ip, _ip, _ip]),
+    "camp_solver_get_pattern": (None, [_p, _ip, _ip, _ip, _ip, _ip, _ip]),
+    "camp_eval_deriv": (_i, [_p, _dp, _dp, _d, _i, _dp]),
+    "camp_eval_jac": (_i, [_p, _dp, _dp, _d, _dp]),
+    "camp_eval_lu_solve": (_i, [_p, _dp, _d, _dp]),
+    "camp_solver_get_cell_stats": (None, [_p, _dp]),
+    "camp_solver_last_kernel_ms": (_d, [_p]),
+    "camp_solver_get_rxn_env": (None, [_p, _dp]),
+    "camp_solver_time_fj": (_i, [_p, _p, _p, _i, _dp, _dp]),
+    "camp_solver_work_model": (None, [_p, _dp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libcamp_mi355x.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcamp_mi355x.so is not built: run __graft_entry__.build() "
+                               "or python camp_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,677 @@
+// Persistent per-cell BDF kernel.
+//
+// Each wavefront pops cells from an atomic queue and integrates each cell
+// from t0 to t1 on its own: variable-order (1-5) variable-step BDF in
+// Nordsieck form with modified Newton and WRMS error control -- the CVODE
+// algorithm that CAMP's solver_run drives (camp_solver.c:544-693; CVODE:
+// Hindmarsh et al., ACM TOMS 31(3), 2005) -- but with order, step size,
+// Newton iteration, Jacobian reuse and error test decided per cell instead
+// of once for the whole block of cells.  Everything a step touches lives
+// in LDS (Nordsieck history, work vectors, rate constants, LU factors);
+// only the cell's saved Jacobian history (J_solver, J_state, J_deriv,
+// camp_solver.c:1111-1114) is in HBM, because CAMP keeps it across calls.
+#include "camp_device.hpp"
+#include "camp_kernels.hpp"
+
+namespace camp {
+
+namespace {
+
+constexpr int QMAX = 5;
+constexpr double ETAMX1 = 10000.0, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2;
+constexpr double ETAMIN = 0.1, ETACF = 0.25, ADDON = 0.000001;
+constexpr double BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0, THRESH = 1.5;
+constexpr int SMALL_NST = 10, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10;
+constexpr double CRDOWN = 0.3, DGMAX = 0.3, RDIV = 2.0, NLSCOEF = 0.1;
+constexpr int MSBP = 20, MAXCOR = 3, MSBJ = 50;
+constexpr double DGMAX_JBAD = 0.2;
+constexpr int GUESS_MAX_ITER = 5;
+
+enum { CV_SUCCESS = 0, CV_TOO_MUCH_WORK = -1, CV_TOO_MUCH_ACC = -2, CV_ERR_FAILURE = -3,
+       CV_CONV_FAILURE = -4, CV_LSETUP_FAIL = -6, CV_LSOLVE_FAIL = -7, CV_RHSFUNC_FAIL = -8,
+       CV_FIRST_RHSFUNC_ERR = -9, CV_REPTD_RHSFUNC_ERR = -10, CV_UNREC_RHSFUNC_ERR = -11,
+       CV_ILL_INPUT = -22 };
+enum { FIRST_CALL = 1, PREV_CONV_FAIL, PREV_ERR_FAIL };
+enum { DO_ERROR_TEST = 2, PREDICT_AGAIN, CONV_FAIL, TRY_AGAIN, RHSFUNC_RECVR };
+enum { CV_NO_FAILURES = 0, CV_FAIL_BAD_J = 1, CV_FAIL_OTHER = 2 };
+
+__device__ __forceinline__ double rpowerr(double b, double e) { return b <= 0.0 ? 0.0 : pow(b, e); }
+__device__ __forceinline__ double rpoweri(double b, int e) {
+  double p = 1.0;
+  for (int i = 1; i <= (e < 0 ? -e : e); ++i) p *= b;
+  return e < 0 ? 1.0 / p : p;
+}
+
+struct CellCv {
+  DevProgram P;
+  int lane, nv;
+  double *s_state, *s_k, *zn[6], *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2,
+      *w, *lu;
+  double *Js, *Jy, *Jf;
+  double tn, h, hscale, hprime, next_h, eta, etamax, etaq, etaqm1, etaqp1, hu, init_h, rtol;
+  double l[6], tq[6], tau[8];
+  double rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
+  int q, qprime, L, qwait, jcur, use_acinit, guess_on, maxcf, max_steps;
+  int nst, nfe, nsetups, nni, ncfn, netf, nje, nstlp, nstlj;
+
+  __device__ double wrms(const double* x) {
+    double s = 0.0;
+    for (int i = lane; i < nv; i += kWave) {
+      double p = x[i] * ewt[i];
+      s += p * p;
+    }
+    s = wave_sum(s);
+    return sqrt(s / nv);
+  }
+
+  __device__ void ewt_set(const double* ycur) {
+    for (int i = lane; i < nv; i += kWave)
+      ewt[i] = 1.0 / (rtol * fabs(ycur[i]) + P.abs_tol_dep[i]);
+    wsync();
+  }
+
+  // CAMP f(): camp_solver.c:843-958 for one cell (use_deriv_est = 1)
+  __device__ int f(const double* yv, double* out) {
+    nfe++;
+    double ts = next_h > 0.0 ? next_h : init_h;
+    if (cell_set_state(P, lane, yv, s_state)) return 1;
+    cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, true, ts, out);
+    wsync();
+    return 0;
+  }
+
+  // CAMP Jac(): pure f into fy, J into the cell's J_solver, save history
+  __device__ int jac(const double* yv, double* fy) {
+    double ts = next_h > 0.0 ? next_h : init_h;
+    if (cell_set_state(P, lane, yv, s_state)) return 1;
+    cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy);
+    cell_jac(P, lane, s_state, s_k, ts, Js);
+    for (int i = lane; i < nv; i += kWave) {
+      Jy[i] = yv[i];
+      Jf[i] = fy[i];
+    }
+    __threadfence_block();
+    wsync();
+    return 0;
+  }
+
+  __device__ void rescale() {
+    double factor = eta;
+    for (int j = 1; j <= q; ++j) {
+      for (int i = lane; i < nv; i += kWave) zn[j][i] *= factor;
+      factor *= eta;
+    }
+    h = hscale * eta;
+    next_h = h;
+    hscale = h;
+    wsync();
+  }
+
+  __device__ void predict() {
+    tn += h;
+    for (int i = lane; i < nv; i += kWave)
+      for (int k = 1; k <= q; ++k)
+        for (int j = q; j >= k; --j) zn[j - 1][i] += zn[j][i];
+    wsync();
+  }
+
+  __device__ void restore(double saved_t) {
+    tn = saved_t;
+    for (int i = lane; i < nv; i += kWave)
+      for (int k = 1; k <= q; ++k)
+        for (int j = q; j >= k; --j) zn[j - 1][i] -= zn[j][i];
+    wsync();
+  }
+
+  __device__ void set_tq(double hsum, double alpha0, double alpha0_hat, double xi_inv,
+                         double xistar_inv) {
+    double A1 = 1.0 - alpha0_hat + alpha0;
+    double A2 = 1.0 + q * A1;
+    tq[2] = fabs(A1 / (alpha0 * A2));
+    tq[5] = fabs(A2 * xistar_inv / (l[q] * xi_inv));
+    if (qwait == 1) {
+      if (q > 1) {
+        double C = xistar_inv / l[q];
+        double A3 = alpha0 + 1.0 / q;
+        double A4 = alpha0_hat + xi_inv;
+        double Cpinv = (1.0 - A4 + A3) / A3;
+        tq[1] = fabs(C * Cpinv);
+      } else {
+        tq[1] = 1.0;
+      }
+      hsum += tau[q];
+      xi_inv = h / hsum;
+      double A5 = alpha0 - (1.0 / (q + 1));
+      double A6 = alpha0_hat - xi_inv;
+      double Cppinv = (1.0 - A6 + A5) / A2;
+      tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+    }
+    tq[4] = NLSCOEF / tq[2];
+  }
+
+  __device__ void set() {
+    double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
+    l[0] = l[1] = xi_inv = xistar_inv = 1.0;
+    for (int z = 2; z <= q; ++z) l[z] = 0.0;
+    alpha0 = alpha0_hat = -1.0;
+    hsum = h;
+    if (q > 1) {
+      for (int j = 2; j < q; ++j) {
+        hsum += tau[j - 1];
+        xi_inv = h / hsum;
+        alpha0 -= 1.0 / j;
+        for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+      }
+      alpha0 -= 1.0 / q;
+      xistar_inv = -l[1] - alpha0;
+      hsum += tau[q - 1];
+      xi_inv = h / hsum;
+      alpha0_hat = -l[1] - xi_inv;
+      for (int i = q; i >= 1; --i) l[i] += l[i - 1] * xistar_inv;
+    }
+    set_tq(hsum, alpha0, alpha0_hat, xi_inv, xistar_inv);
+    rl1 = 1.0 / l[1];
+    gamma = h * rl1;
+    if (nst == 0) gammap = gamma;
+    gamrat = (nst > 0) ? gamma / gammap : 1.0;
+  }
+
+  __device__ void increase_bdf() {
+    double alpha0, alpha1, prod, xi, xiold, hsum, A1;
+    for (int j = 0; j <= QMAX; ++j) l[j] = 0.0;
+    l[2] = alpha1 = prod = xiold = 1.0;
+    alpha0 = -1.0;
+    hsum = hscale;
+    if (q > 1) {
+      for (int j = 1; j < q; ++j) {
+        hsum += tau[j + 1];
+        xi = hsum / hscale;
+        prod *= xi;
+        alpha0 -= 1.0 / (j + 1);
+        alpha1 += 1.0 / xi;
+        for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xiold + l[i - 1];
+        xiold = xi;
+      }
+    }
+    A1 = (-alpha0 - alpha1) / prod;
+    for (int i = lane; i < nv; i += kWave) {
+      double v = A1 * zn[QMAX][i];
+      zn[L][i] = v;
+      for (int j = 2; j <= q; ++j) zn[j][i] += l[j] * v;
+    }
+    wsync();
+  }
+
+  __device__ void decrease_bdf() {
+    double hsum, xi;
+    for (int j = 0; j <= QMAX; ++j) l[j] = 0.0;
+    l[2] = 1.0;
+    hsum = 0.0;
+    for (int j = 1; j <= q - 2; ++j) {
+      hsum += tau[j];
+      xi = hsum / hscale;
+      for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xi + l[i - 1];
+    }
+    for (int i = lane; i < nv; i += kWave)
+      for (int j = 2; j < q; ++j) zn[j][i] += (-l[j]) * zn[q][i];
+    wsync();
+  }
+
+  __device__ void adjust_params() {
+    if (qprime != q) {
+      int dq = qprime - q;
+      if (dq == 1) increase_bdf();
+      else if (dq == -1) decrease_bdf();
+      q = qprime;
+      L = q + 1;
+      qwait = L;
+    }
+    rescale();
+  }
+
+  // guess_helper, camp_solver.c:1321-1408 (called with h_n = h > 0)
+  __device__ int guess_helper(double t_n, double h_n, const double* y_n, const double* y_n1,
+                              const double* hf, double* tmp1, double* corr) {
+    double mn = 1.0e308;
+    for (int i = lane; i < nv; i += kWave) mn = fmin(mn, y_n[i]);
+    mn = wave_min(mn);
+    if (mn > -1.0e-30) return 0;
+    double c = 1.0 / h_n;
+    for (int i = lane; i < nv; i += kWave) {
+      tmp1[i] = y_n1[i];
+      corr[i] = c * hf[i];
+    }
+    wsync();
+    double t_0 = t_n - h_n;
+    double t_j = 0.0;
+    for (int iter = 0; iter < GUESS_MAX_ITER && t_0 + t_j < t_n; iter++) {
+      double h_j = t_n - (t_0 + t_j);
+      double best = h_j;
+      int found = 0;
+      for (int i = lane; i < nv; i += kWave) {
+        double t_star = -tmp1[i] / corr[i];
+        if ((t_star > 0.0 || (t_star == 0.0 && corr[i] < 0.0)) && t_star < best) {
+          best = t_star;
+          found = 1;
+        }
+      }
+      best = wave_min(best);
+      found = wave_or(found);
+      h_j = best;
+      if (found) h_j *= 0.95 + 0.1 * iter / (double)GUESS_MAX_ITER;
+      h_j = t_n < t_0 + t_j + h_j ? t_n - (t_0 + t_j) : h_j;
+      for (int i = lane; i < nv; i += kWave) tmp1[i] += h_j * corr[i];
+      wsync();
+      t_j += h_j;
+      if (f(tmp1, corr)) return -1;
+    }
+    for (int i = lane; i < nv; i += kWave) corr[i] = (tmp1[i] - y_n[i]) * 0.999;
+    wsync();
+    return 1;
+  }
+
+  __device__ int lsetup(int convfail, const double* ypred, double* fpred) {
+    double dgamma = fabs(gamma / gammap - 1.0);
+    int jbad = (nst == 0) || (nst > nstlj + MSBJ) ||
+               ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_JBAD)) ||
+               (convfail == CV_FAIL_OTHER);
+    if (!jbad) {
+      jcur = 0;
+    } else {
+      nje++;
+      nstlj = nst;
+      jcur = 1;
+      if (jac(ypred, fpred)) return 1;
+    }
+    return cell_lu_factor(P, lane, Js, gamma, lu);
+  }
+
+  __device__ void lsolve(double* b) {
+    cell_lu_solve(P, lane, lu, b, w);
+    if (gamrat != 1.0) {
+      double s = 2.0 / (1.0 + gamrat);
+      for (int i = lane; i < nv; i += kWave) b[i] *= s;
+      wsync();
+    }
+  }
+
+  __device__ int newton_iteration() {
+    int m = 0;
+    double del = 0.0, delp = 0.0;
+    for (;;) {
+      for (int i = lane; i < nv; i += kWave) {
+        double t = rl1 * zn[1][i] + acor[i];
+        tempv[i] = gamma * ftemp[i] - t;
+      }
+      wsync();
+      lsolve(tempv);
+      nni++;
+      del = wrms(tempv);
+      for (int i = lane; i < nv; i += kWave) {
+        double a = acor[i] + tempv[i];
+        acor[i] = a;
+        y[i] = zn[0][i] + a;
+      }
+      wsync();
+      if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
+      double dcon = del * fmin(1.0, crate) / tq[4];
+      if (dcon <= 1.0) {
+        // Non-negativity safeguard: CAMP's f() rejects y < -SMALL
+        // (camp_solver.c:816-822); a converged iterate below that would make
+        // the next step's first f() fail at every step size, so it is
+        // treated here as the recoverable f() failure it implies.
+        double mn = 1.0e308;
+        for (int i = lane; i < nv; i += kWave) mn = fmin(mn, y[i]);
+        if (wave_min(mn) < -1.0e-30) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        acnrm = (m == 0) ? del : wrms(acor);
+        jcur = 0;
+        return CV_SUCCESS;
+      }
+      m++;
+      if ((m == MAXCOR) || ((m >= 2) && (del > RDIV * delp))) return jcur ? CONV_FAIL : TRY_AGAIN;
+      delp = del;
+      if (f(y, ftemp)) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+    }
+  }
+
+  __device__ int nls(int nflag) {
+    int convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? CV_NO_FAILURES : CV_FAIL_OTHER;
+    int callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (nst == 0) ||
+                    (nst >= nstlp + MSBP) || (fabs(gamrat - 1.0) > DGMAX);
+    use_acinit = 0;
+    if (guess_on) {
+      for (int i = lane; i < nv; i += kWave) acinit[i] = zn[0][i] - last[i];
+      wsync();
+      int gh = guess_helper(tn, h, zn[0], last, acinit, gh1, gh2);
+      if (gh == 1) {
+        for (int i = lane; i < nv; i += kWave) acinit[i] = gh2[i];
+        wsync();
+        use_acinit = 1;
+      }
+    }
+    for (;;) {
+      for (int i = lane; i < nv; i += kWave) y[i] = use_acinit ? zn[0][i] + acinit[i] : zn[0][i];
+      wsync();
+      if (f(y, ftemp)) return RHSFUNC_RECVR;
+      if (callSetup) {
+        int ier = lsetup(convfail, y, ftemp);
+        nsetups++;
+        callSetup = 0;
+        gamrat = crate = 1.0;
+        gammap = gamma;
+        nstlp = nst;
+        if (ier > 0) return CONV_FAIL;
+      }
+      for (int i = lane; i < nv; i += kWave) acor[i] = use_acinit ? acinit[i] : 0.0;
+      wsync();
+      int ier = newton_iteration();
+      if (ier != TRY_AGAIN) return ier;
+      callSetup = 1;
+      convfail = CV_FAIL_BAD_J;
+    }
+  }
+
+  __device__ int handle_nflag(int* nflagPtr, double saved_t, int* ncfPtr) {
+    int nflag = *nflagPtr;
+    if (nflag == CV_SUCCESS) return DO_ERROR_TEST;
+    ncfn++;
+    restore(saved_t);
+    if (nflag == CV_LSETUP_FAIL || nflag == CV_LSOLVE_FAIL || nflag == CV_RHSFUNC_FAIL) return nflag;
+    (*ncfPtr)++;
+    etamax = 1.0;
+    if (*ncfPtr == maxcf) {
+      if (nflag == CONV_FAIL) return CV_CONV_FAILURE;
+      if (nflag == RHSFUNC_RECVR) return CV_REPTD_RHSFUNC_ERR;
+    }
+    eta = ETACF;
+    *nflagPtr = PREV_CONV_FAIL;
+    rescale();
+    return PREDICT_AGAIN;
+  }
+
+  __device__ int do_error_test(int* nflagPtr, double saved_t, int* nefPtr, double* dsmPtr) {
+    double dsm = acnrm * tq[2];
+    *dsmPtr = dsm;
+    if (dsm <= 1.0) return CV_SUCCESS;
+    (*nefPtr)++;
+    netf++;
+    *nflagPtr = PREV_ERR_FAIL;
+    restore(saved_t);
+    if (*nefPtr == maxcf) return CV_ERR_FAILURE;
+    etamax = 1.0;
+    if (*nefPtr <= MXNEF1) {
+      eta = 1.0 / (rpowerr(BIAS2 * dsm, 1.0 / L) + ADDON);
+      eta = fmax(ETAMIN, fmax(eta, 0.0));
+      if (*nefPtr >= SMALL_NEF) eta = fmin(eta, ETAMXF);
+      rescale();
+      return TRY_AGAIN;
+    }
+    if (q > 1) {
+      eta = ETAMIN;
+      decrease_bdf();
+      L = q;
+      q--;
+      qwait = L;
+      rescale();
+      return TRY_AGAIN;
+    }
+    eta = ETAMIN;
+    h *= eta;
+    next_h = h;
+    hscale = h;
+    qwait = LONG_WAIT;
+    if (f(zn[0], tempv)) return CV_UNREC_RHSFUNC_ERR;
+    for (int i = lane; i < nv; i += kWave) zn[1][i] = h * tempv[i];
+    wsync();
+    return TRY_AGAIN;
+  }
+
+  __device__ void complete_step() {
+    nst++;
+    hu = h;
+    for (int i = q; i >= 2; --i) tau[i] = tau[i - 1];
+    if ((q == 1) && (nst > 1)) tau[2] = tau[1];
+    tau[1] = h;
+    qwait--;
+    bool save = (qwait == 1) && (q != QMAX);
+    for (int i = lane; i < nv; i += kWave) {
+      double a = acor[i];
+      for (int j = 0; j <= q; ++j) zn[j][i] += l[j] * a;
+      if (save) zn[QMAX][i] = a;
+      last[i] = zn[0][i];
+    }
+    if (save) saved_tq5 = tq[5];
+    wsync();
+  }
+
+  __device__ void set_eta() {
+    if (eta < THRESH) {
+      eta = 1.0;
+      hprime = h;
+    } else {
+      eta = fmin(eta, etamax);
+      hprime = h * eta;
+    }
+  }
+
+  __device__ void prepare_next_step(double dsm) {
+    if (etamax == 1.0) {
+      qwait = qwait > 2 ? qwait : 2;
+      qprime = q;
+      hprime = h;
+      eta = 1.0;
+      return;
+    }
+    etaq = 1.0 / (rpowerr(BIAS2 * dsm, 1.0 / L) + ADDON);
+    if (qwait != 0) {
+      eta = etaq;
+      qprime = q;
+      set_eta();
+      return;
+    }
+    qwait = 2;
+    etaqm1 = 0.0;
+    if (q > 1) {
+      double ddn = wrms(zn[q]) * tq[1];
+      etaqm1 = 1.0 / (rpowerr(BIAS1 * ddn, 1.0 / q) + ADDON);
+    }
+    etaqp1 = 0.0;
+    if (q != QMAX && saved_tq5 != 0.0) {
+      double cquot = (tq[5] / saved_tq5) * rpoweri(h / tau[2], L);
+      for (int i = lane; i < nv; i += kWave) tempv[i] = -cquot * zn[QMAX][i] + acor[i];
+      wsync();
+      double dup = wrms(tempv) * tq[3];
+      etaqp1 = 1.0 / (rpowerr(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+    }
+    double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+    if (etam < THRESH) {
+      eta = 1.0;
+      qprime = q;
+    } else if (etam == etaq) {
+      eta = etaq;
+      qprime = q;
+    } else if (etam == etaqm1) {
+      eta = etaqm1;
+      qprime = q - 1;
+    } else {
+      eta = etaqp1;
+      qprime = q + 1;
+      for (int i = lane; i < nv; i += kWave) zn[QMAX][i] = acor[i];
+      wsync();
+    }
+    set_eta();
+  }
+
+  __device__ int step() {
+    double saved_t = tn, dsm = 0.0;
+    int ncf = 0, nef = 0, nflag = FIRST_CALL;
+    if ((nst > 0) && (hprime != h)) adjust_params();
+    for (;;) {
+      predict();
+      set();
+      nflag = nls(nflag);
+      int kflag = handle_nflag(&nflag, saved_t, &ncf);
+      if (kflag == PREDICT_AGAIN) continue;
+      if (kflag != DO_ERROR_TEST) return kflag;
+      int eflag = do_error_test(&nflag, saved_t, &nef, &dsm);
+      if (eflag == TRY_AGAIN) continue;
+      if (eflag != CV_SUCCESS) return eflag;
+      break;
+    }
+    complete_step();
+    prepare_next_step(dsm);
+    etamax = (nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
+    for (int i = lane; i < nv; i += kWave) acor[i] *= tq[2];
+    wsync();
+    return CV_SUCCESS;
+  }
+
+  // CVodeReInit + CVodeSetInitStep + CVode(tout, CV_NORMAL); zn[0] holds y0
+  // on entry, y holds y(tout) on successful exit.
+  __device__ int solve(double t0, double tout) {
+    for (int i = lane; i < nv; i += kWave) last[i] = zn[0][i];
+    tn = t0;
+    q = 1;
+    L = 2;
+    qwait = L;
+    qprime = 1;
+    etamax = ETAMX1;
+    hu = 0.0;
+    nst = nfe = ncfn = netf = nni = nsetups = nstlp = nje = nstlj = 0;
+    saved_tq5 = 0.0;
+    crate = 1.0;
+    gamrat = 1.0;
+    gammap = 1.0;
+    jcur = 0;
+    eta = 1.0;
+    for (int i = 0; i < 8; ++i) tau[i] = 0.0;
+    for (int i = 0; i < 6; ++i) tq[i] = 0.0;
+    init_h = tout - t0;
+    next_h = 0.0;
+    wsync();
+    ewt_set(zn[0]);
+    if (f(zn[0], zn[1])) return CV_FIRST_RHSFUNC_ERR;
+    h = init_h;
+    for (int i = lane; i < nv; i += kWave) zn[1][i] *= h;
+    wsync();
+    hscale = h;
+    hprime = h;
+    int nstloc = 0;
+    for (;;) {
+      next_h = h;
+      if (nst > 0) ewt_set(zn[0]);
+      if (nstloc >= max_steps) return CV_TOO_MUCH_WORK;
+      double nrm = wrms(zn[0]);
+      if (2.220446049250313e-16 * nrm > 1.0) return CV_TOO_MUCH_ACC;
+      int kflag = step();
+      if (kflag != CV_SUCCESS) return kflag;
+      nstloc++;
+      if ((tn - tout) * h >= 0.0) {
+        double s = (tout - tn) / h;
+        for (int i = lane; i < nv; i += kWave) {
+          double v = zn[q][i];
+          for (int j = q - 1; j >= 0; --j) v = zn[j][i] + s * v;
+          y[i] = v;
+        }
+        wsync();
+        next_h = hprime;
+        return CV_SUCCESS;
+      }
+    }
+  }
+};
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_cell_bdf(CellBdfArgs a) {
+  extern __shared__ double smem[];
+  const DevProgram& P = a.P;
+  int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  int nv = P.nv;
+  size_t per_cell = cell_bdf_lds_doubles(P.ns, P.nv, P.n_env, P.lu_nnz);
+  double* base = smem + (size_t)wv * per_cell;
+  CellCv cv;
+  cv.P = P;
+  cv.lane = lane;
+  cv.nv = nv;
+  cv.s_state = base;
+  cv.s_k = cv.s_state + P.ns;
+  double* v = cv.s_k + P.n_env;
+  for (int j = 0; j < 6; ++j) cv.zn[j] = v + (size_t)j * nv;
+  v += 6 * (size_t)nv;
+  cv.ewt = v; v += nv;
+  cv.acor = v; v += nv;
+  cv.y = v; v += nv;
+  cv.tempv = v; v += nv;
+  cv.ftemp = v; v += nv;
+  cv.last = v; v += nv;
+  cv.acinit = v; v += nv;
+  cv.gh1 = v; v += nv;
+  cv.gh2 = v; v += nv;
+  cv.w = v; v += nv;
+  cv.lu = v;
+  cv.rtol = a.rtol;
+  cv.guess_on = a.guess_on;
+  cv.maxcf = a.max_conv_fails;
+  cv.max_steps = a.max_steps;
+
+  for (;;) {
+    int cell = 0;
+    if (lane == 0) cell = atomicAdd(a.counter, 1);
+    cell = __shfl(cell, 0, kWave);
+    if (cell >= a.nc) break;
+    double* state_c = a.state + (size_t)cell * P.ns;
+    double* renv_c = a.rxn_env + (size_t)cell * P.n_env;
+    cv.Js = a.Js + (size_t)cell * P.nnz;
+    cv.Jy = a.Jy + (size_t)cell * nv;
+    cv.Jf = a.Jf + (size_t)cell * nv;
+    // stage state (constants clamped in place) and rate data
+    for (int i = lane; i < P.ns; i += kWave) {
+      double s = state_c[i];
+      cv.s_state[i] = s;
+      if (a.var_type[i] != 1) {
+        s = s > 1.0e-60 ? s : 1.0e-60;
+        cv.s_state[i] = s;
+        state_c[i] = s;
+      }
+    }
+    for (int i = lane; i < P.n_env; i += kWave) cv.s_k[i] = renv_c[i];
+    wsync();
+    // rate constants for the cell's environment (rxn_update_env_state)
+    double T = a.env[2 * cell], Pa = a.env[2 * cell + 1];
+    for (int r = lane; r < P.n_rxn; r += kWave) rate_constant(P, r, T, Pa, cv.s_k);
+    wsync();
+    for (int i = lane; i < P.n_env; i += kWave) renv_c[i] = cv.s_k[i];
+    for (int i = lane; i < nv; i += kWave) {
+      double s = cv.s_state[P.dep_state[i]];
+      cv.zn[0][i] = s > 1.0e-60 ? s : 1.0e-60;
+    }
+    wsync();
+    int flag = cv.solve(a.t0, a.t1);
+    if (flag >= 0) {
+      for (int i = lane; i < nv; i += kWave) {
+        double yv = cv.y[i];
+        state_c[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
+      }
+    }
+    if (lane == 0) {
+      double* st = a.stats + (size_t)cell * kCellStats;
+      st[0] = cv.nst; st[1] = cv.nfe; st[2] = cv.nsetups; st[3] = cv.netf;
+      st[4] = cv.nni; st[5] = cv.ncfn; st[6] = cv.nje; st[7] = flag;
+      st[8] = cv.hu; st[9] = cv.next_h;
+    }
+    wsync();
+  }
+}
+
+void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL(k_cell_bdf, dim3(grid), dim3(block), lds, s, a);
+}
+
+int cell_bdf_blocks_per_cu(int block, size_t lds) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cell_bdf, block, lds) != hipSuccess) return 0;
+  return n;
+}
+
+}  // namespace camp

@@ -1,0 +1,283 @@
+// Device-side building blocks shared by the plain-path kernels and the
+// persistent per-cell BDF kernel.
+//
+// Execution model: one wavefront (64 lanes) owns one grid cell at a time.
+// The cell's state, rate constants, BDF vectors and LU factors live in LDS;
+// lanes split the work of each sparse phase (rate terms and species in
+// f(y), Jacobian elements in J(y), updates of one pivot in the LU, the
+// column of one pivot in the triangular solves).  Every decision that
+// controls the cell's integration is made on wave-uniform values (lane 0's
+// reduction result is broadcast), so a wavefront never diverges between
+// cells and never needs a workgroup barrier.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace camp {
+
+constexpr int kWave = 64;
+
+struct DevProgram {
+  int ns, nv, nnz, n_env, n_terms, lu_nnz, n_rxn;
+  const int *dep_state, *t_kidx, *t_react, *t_nreact, *t_flags;
+  const int *f_ptr, *f_term, *f_guard;
+  const double* f_coef;
+  const int *j_ptr, *j_term, *j_skip, *j_guard, *j_rind;
+  const double* j_coef;
+  const int *colptr, *rowval, *r_ptr, *r_elem, *r_col;
+  const int *perm, *m_to_lu, *diag_pos, *e_diag;
+  const int *l_ptr, *l_pos, *l_row, *u_ptr, *u_pos, *u_row, *up_ptr;
+  const uint16_t *up_dst, *up_l, *up_u;
+  // rate-constant program (rxn_update_env_state)
+  const int *rx_type, *rx_fidx, *rx_eidx, *rx_nreact;
+  const double* rx_float;
+  const double* abs_tol_dep;  // [nv]
+};
+
+// ---------------------------------------------------------------------
+// Double-double arithmetic: the accumulators of f and J use it in place of
+// CAMP's 80-bit long double production/loss sums (time_derivative.h:21-28,
+// Jacobian.h:31-40).  Error-free transforms: Knuth TwoSum, FMA TwoProd.
+// ---------------------------------------------------------------------
+struct dd {
+  double hi, lo;
+};
+
+__device__ __forceinline__ dd two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+__device__ __forceinline__ dd fast_two_sum(double a, double b) {
+  double s = a + b;
+  double e = b - (s - a);
+  return {s, e};
+}
+__device__ __forceinline__ dd dd_mul_d(dd a, double b) {
+  double p = a.hi * b;
+  double e = fma(a.hi, b, -p);
+  e = fma(a.lo, b, e);
+  return fast_two_sum(p, e);
+}
+__device__ __forceinline__ dd dd_add(dd a, dd b) {
+  dd s = two_sum(a.hi, b.hi);
+  double lo = s.lo + (a.lo + b.lo);
+  return fast_two_sum(s.hi, lo);
+}
+
+// ---------------------------------------------------------------------
+// Wavefront helpers
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return __shfl(v, 0, kWave);
+}
+__device__ __forceinline__ double wave_min(double v) {
+  for (int off = kWave / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
+  return __shfl(v, 0, kWave);
+}
+__device__ __forceinline__ int wave_or(int v) {
+  for (int off = kWave / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
+  return __shfl(v, 0, kWave);
+}
+
+// ---------------------------------------------------------------------
+// Rate constants of one cell: rxn_update_env_state (rxn_solver.c:224-313)
+// with rxn_arrhenius.c:103-118, rxn_troe.c:106-130, rxn_photolysis.c:
+// 137-147, rxn_emission.c:109-119, rxn_first_order_loss.c:119-129.
+// env_cell points at the cell's rxn_env block (k slots written in place).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void rate_constant(const DevProgram& P, int r, double T, double Pa,
+                                              double* env_cell) {
+  const double* f = P.rx_float + P.rx_fidx[r];
+  double* e = env_cell + P.rx_eidx[r];
+  int t = P.rx_type[r];
+  int nr = P.rx_nreact[r];
+  if (t == 1) {  // ARRHENIUS
+    e[0] = f[0] * exp(f[2] / T) * (f[1] == 0.0 ? 1.0 : pow(T / f[3], f[1])) *
+           (f[4] == 0.0 ? 1.0 : (1.0 + f[4] * Pa)) * pow(f[5] * Pa / T, (double)(nr - 1));
+  } else if (t == 2) {  // TROE
+    double conv = f[9] * Pa / T;
+    double k0 = f[0] * (f[2] == 0.0 ? 1.0 : exp(f[2] / T)) *
+                (f[1] == 0.0 ? 1.0 : pow(T / 300.0, f[1])) * conv;
+    double kinf = k0 / (f[3] * (f[5] == 0.0 ? 1.0 : exp(f[5] / T)) *
+                        (f[4] == 0.0 ? 1.0 : pow(T / 300.0, f[4])));
+    double lg = log10(kinf) / f[7];
+    e[0] = (k0 / (1.0 + kinf)) * pow(f[6], (1.0 / (1.0 + pow(lg, 2.0)))) *
+           pow(conv, (double)(nr - 1)) * f[8];
+  } else {  // PHOTOLYSIS, EMISSION, FIRST_ORDER_LOSS: SCALING * BASE_RATE
+    e[0] = f[0] * e[1];
+  }
+}
+
+// ---------------------------------------------------------------------
+// f(y) of one cell: per-type calc_deriv_contrib + time_derivative_output
+// (camp_solver.c:901-954, time_derivative.c:46-97).  s_state holds the
+// cell state with the solver variables already substituted, s_k the
+// cell's rxn_env block.  Lane i handles species i, i+64, ...: it gathers
+// its signed contributions into a double-double D (= production - loss)
+// and their magnitudes into S (= production + loss).
+// est != nullptr selects the Jacobian-extrapolated blend; est is then
+// computed from the cell's J history (J_deriv + J_solver (y - J_state)).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ dd term_rate(const DevProgram& P, int t, const double* s_state,
+                                        const double* s_k) {
+  double k = s_k[P.t_kidx[t]];
+  int nr = P.t_nreact[t];
+  if (P.t_flags[t] & 2) return {k * s_state[P.t_react[3 * t]], 0.0};
+  dd rate = {k, 0.0};
+  for (int j = 0; j < nr; ++j) rate = dd_mul_d(rate, s_state[P.t_react[3 * t + j]]);
+  return rate;
+}
+
+__device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const double* s_state,
+                                           const double* s_k, const double* y,
+                                           const double* Js, const double* Jy,
+                                           const double* Jf, bool use_est, double h,
+                                           double* out) {
+  for (int i = lane; i < P.nv; i += kWave) {
+    dd D = {0.0, 0.0};
+    double S = 0.0;
+    for (int q = P.f_ptr[i]; q < P.f_ptr[i + 1]; ++q) {
+      int t = P.f_term[q];
+      double c = P.f_coef[q];
+      int g = P.f_guard[q];
+      dd rate = term_rate(P, t, s_state, s_k);
+      if ((P.t_flags[t] & 1) && rate.hi == 0.0) continue;
+      if (g >= 0 && !(-rate.hi * c * h <= s_state[g])) continue;
+      dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
+      D = dd_add(D, contrib);
+      S += fabs(contrib.hi);
+    }
+    double v = 0.0;
+    if (S != 0.0) {
+      if (use_est) {
+        double acc = 0.0;
+        for (int q = P.r_ptr[i]; q < P.r_ptr[i + 1]; ++q) {
+          int j = P.r_col[q];
+          acc += Js[P.r_elem[q]] * (y[j] - Jy[j]);
+        }
+        double est = Jf[i] + acc;
+        double inv = 1.0 / S;
+        double sf = inv / (inv + 1.0e-14 / fabs(D.hi));
+        v = sf * D.hi + (1.0 - sf) * est;
+      } else {
+        v = D.hi;
+      }
+    }
+    out[i] = v;
+  }
+}
+
+// camp_solver_update_model_state (camp_solver.c:805-833) for one cell:
+// returns 1 (wave-uniform) if any y < -SMALL, else copies y into s_state.
+__device__ __forceinline__ int cell_set_state(const DevProgram& P, int lane, const double* y,
+                                              double* s_state) {
+  int bad = 0;
+  for (int i = lane; i < P.nv; i += kWave) bad |= (y[i] < -1.0e-30);
+  bad = wave_or(bad);
+  if (bad) return 1;
+  for (int i = lane; i < P.nv; i += kWave) {
+    double v = y[i];
+    s_state[P.dep_state[i]] = v > -1.0e-30 ? v : 1.0e-60;
+  }
+  wsync();
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// J(y) of one cell into the solver Jacobian (CSC element order): per-type
+// calc_jac_contrib + jacobian_output + jac_map (camp_solver.c:1052-1106,
+// Jacobian.c:223-241).  Lane e gathers the contributions of element e.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const double* s_state,
+                                         const double* s_k, double h, double* J) {
+  for (int e = lane; e < P.nnz; e += kWave) {
+    dd D = {0.0, 0.0};
+    for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q) {
+      int t = P.j_term[q];
+      int skip = P.j_skip[q];
+      double c = P.j_coef[q];
+      double v = s_k[P.t_kidx[t]];
+      int nr = P.t_nreact[t];
+      for (int j = 0; j < nr; ++j)
+        if (j != skip) v *= s_state[P.t_react[3 * t + j]];
+      if (c == -1.0) {
+        D = dd_add(D, dd{-v, 0.0});
+      } else {
+        int g = P.j_guard[q];
+        if (g >= 0 && !(-v * s_state[P.j_rind[q]] * c * h <= s_state[g])) continue;
+        D = dd_add(D, dd{c * v, 0.0});
+      }
+    }
+    J[e] = D.hi;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Sparse LU of M = I - gamma*J (SUNMatScaleAddI) with the static pivot
+// order of the program; right-looking, one pivot at a time.  Returns 1 on
+// a zero or non-finite pivot (a recoverable linear-setup failure).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, const double* J,
+                                              double gamma, double* lu) {
+  for (int p = lane; p < P.lu_nnz; p += kWave) lu[p] = 0.0;
+  wsync();
+  for (int e = lane; e < P.nnz; e += kWave) {
+    double v = -gamma * J[e];
+    if (P.e_diag[e]) v = v + 1.0;
+    lu[P.m_to_lu[e]] = v;
+  }
+  wsync();
+  for (int k = 0; k < P.nv; ++k) {
+    double piv = lu[P.diag_pos[k]];
+    if (piv == 0.0 || !isfinite(piv)) return 1;
+    for (int q = P.l_ptr[k] + lane; q < P.l_ptr[k + 1]; q += kWave) {
+      int p = P.l_pos[q];
+      lu[p] = lu[p] / piv;
+    }
+    wsync();
+    int q1 = P.up_ptr[k + 1];
+    for (int q = P.up_ptr[k] + lane; q < q1; q += kWave) {
+      int d = P.up_dst[q];
+      lu[d] = lu[d] - lu[P.up_l[q]] * lu[P.up_u[q]];
+    }
+    wsync();
+  }
+  return 0;
+}
+
+// Solve (LU) x = b in place; b in dep order, w a scratch vector (nv).
+__device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, const double* lu,
+                                              double* b, double* w) {
+  for (int k = lane; k < P.nv; k += kWave) w[k] = b[P.perm[k]];
+  wsync();
+  for (int k = 0; k < P.nv; ++k) {
+    double xk = w[k];
+    for (int q = P.l_ptr[k] + lane; q < P.l_ptr[k + 1]; q += kWave) {
+      int i = P.l_row[q];
+      w[i] = w[i] - lu[P.l_pos[q]] * xk;
+    }
+    wsync();
+  }
+  for (int k = P.nv - 1; k >= 0; --k) {
+    double xk = w[k] / lu[P.diag_pos[k]];
+    wsync();
+    if (lane == 0) w[k] = xk;
+    for (int q = P.u_ptr[k] + lane; q < P.u_ptr[k + 1]; q += kWave) {
+      int i = P.u_row[q];
+      w[i] = w[i] - lu[P.u_pos[q]] * xk;
+    }
+    wsync();
+  }
+  for (int k = lane; k < P.nv; k += kWave) b[P.perm[k]] = w[k];
+  wsync();
+}
+
+}  // namespace camp

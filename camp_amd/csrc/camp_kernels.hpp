@@ -1,0 +1,70 @@
+// Launch interface between the host solver (camp_solver.cpp) and the HIP
+// kernels (camp_kernels.hip, camp_cell_bdf.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "camp_device.hpp"
+
+namespace camp {
+
+constexpr int kCellStats = 10;  // nst nfe nsetups netf nni ncfn nje flag hu next_h
+
+// LDS doubles one wavefront needs for a cell in the persistent kernel:
+// state, rate data, Nordsieck history (6 vectors), 10 work vectors, LU.
+__host__ __device__ inline size_t cell_bdf_lds_doubles(int ns, int nv, int n_env, int lu_nnz) {
+  return (size_t)ns + n_env + 16 * (size_t)nv + lu_nnz;
+}
+
+struct CellBdfArgs {
+  DevProgram P;
+  int nc;
+  double* state;        // [nc][ns] in/out
+  const double* env;    // [nc][2]
+  double* rxn_env;      // [nc][n_env] in/out (rate constants written back)
+  double *Js, *Jy, *Jf; // per-cell J history
+  const int* var_type;  // [ns]
+  double rtol, t0, t1;
+  int max_steps, max_conv_fails, guess_on;
+  int* counter;         // work queue head (zeroed before launch)
+  double* stats;        // [nc][kCellStats]
+};
+
+void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);
+int cell_bdf_blocks_per_cu(int block, size_t lds);
+
+__global__ void k_rates(DevProgram P, int nc, const double* env, double* rxn_env);
+__global__ void k_deriv(DevProgram P, int nc, const double* y, const double* state,
+                        const double* rxn_env, const double* Js, const double* Jy,
+                        const double* Jf, double* fout, double h, int use_est, int* fail);
+__global__ void k_jac(DevProgram P, int nc, const double* y, const double* state,
+                      const double* rxn_env, double* J, double h);
+__global__ void k_lu_factor(DevProgram P, int nc, const double* J, double gamma, double* LU,
+                            int* fail);
+__global__ void k_lu_solve(DevProgram P, int nc, const double* LU, double* b, double s);
+__global__ void k_vec_lin(long n, double a, const double* x, double b, const double* y,
+                          double* z);
+__global__ void k_vec_scale(long n, double a, double* x);
+__global__ void k_vec_pascal(long n, int q, double* zn, int sign);
+__global__ void k_vec_rescale(long n, int q, double eta, double* zn);
+struct Coef6 {
+  double v[6];
+};
+__global__ void k_vec_complete(long n, int q, Coef6 l6, const double* acor, double* zn, int save,
+                               double* last);
+__global__ void k_vec_resid(long n, double rl1, const double* zn1, const double* acor,
+                            double gamma, const double* ftemp, double* tempv);
+__global__ void k_vec_newton_update(long n, const double* b, double* acor, const double* zn0,
+                                    double* y);
+__global__ void k_vec_ewt(long n, int nv, double rtol, const double* atol, const double* y,
+                          double* ewt);
+__global__ void k_vec_interp(long n, int q, double s, const double* zn, double* y);
+__global__ void k_vec_wsq(long n, const double* x, const double* w, double* out);
+__global__ void k_vec_min(long n, const double* x, double* out);
+__global__ void k_vec_tstar(long n, const double* tmp1, const double* corr, double h_j,
+                            double* out);
+__global__ void k_clamp_constants(DevProgram P, int nc, const int* var_type, double* state);
+__global__ void k_state_gather(DevProgram P, int nc, const double* state, double* y);
+__global__ void k_state_scatter(DevProgram P, int nc, const double* y, double* state);
+
+}  // namespace camp

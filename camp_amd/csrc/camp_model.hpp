@@ -1,0 +1,103 @@
+// Host-side model of the CAMP chemistry hot path and the device "program"
+// compiled from it.
+//
+// The model holds what CAMP's ModelData holds for the gas-phase path
+// (src/camp_common.h:67-179): state-variable types, the packed reaction
+// parameters appended by rxn_add_condensed_data (rxn_solver.c:658-693) and
+// the per-cell environment-dependent reaction data.  At solver_initialize
+// it derives the bit-exact CAMP sparsity pattern and ids (camp_solver.c:
+// 1417-1712) and compiles them into flat tables the HIP kernels read:
+//   * rate terms (one per reaction) and per-species gather lists for f(y),
+//   * per-Jacobian-element gather lists for J(y),
+//   * a static-pivot sparse LU program (ordering, fill, per-pivot lists).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace camp {
+
+// src/camp_common.h:34-45, src/time_derivative.h:18
+constexpr int kVariable = 1;
+constexpr double kSmall = 1.0e-30;
+constexpr double kTiny = 1.0e-60;
+constexpr double kMaxPrecisionLoss = 1.0e-14;
+
+// src/rxn_solver.c:19-35
+constexpr int kRxnArrhenius = 1;
+constexpr int kRxnTroe = 2;
+constexpr int kRxnPhotolysis = 5;
+constexpr int kRxnFirstOrderLoss = 12;
+constexpr int kRxnEmission = 13;
+
+// Flags of a rate term.
+constexpr int kTermMassAction = 1;   // skip contributions when the rate is 0
+constexpr int kTermRoundRate = 2;    // rate product rounded to double (loss)
+
+// Sparse matrix with per-column sorted rows (CSC), as built by Jacobian.c.
+struct Csc {
+  int n = 0;
+  std::vector<int> colptr, rowval;
+  int nnz() const { return (int)rowval.size(); }
+  int find(int dep, int ind) const;
+};
+
+// Flat tables consumed by the kernels.  Index conventions: "state" indices
+// address a cell's full state vector (n_state entries), "dep" indices its
+// solver variables, "element" indices the per-cell solver Jacobian in CSC
+// order, "pos" indices the per-cell LU value array.
+struct Program {
+  int ns = 0, nv = 0, nnz = 0, n_env = 0, n_terms = 0, lu_nnz = 0;
+  int max_f_len = 0, max_j_len = 0;
+  std::vector<int> dep_state;                       // [nv]
+  std::vector<int> t_kidx, t_react, t_nreact, t_flags;  // terms; t_react[3*t+j]
+  std::vector<int> f_ptr, f_term, f_guard;          // per dep species
+  std::vector<double> f_coef;
+  std::vector<int> j_ptr, j_term, j_skip, j_guard, j_rind;  // per element
+  std::vector<double> j_coef;
+  std::vector<int> colptr, rowval;                  // solver pattern (dep ids)
+  std::vector<int> r_ptr, r_elem, r_col;            // row view of the pattern
+  // LU (positions in the permuted LU value array)
+  std::vector<int> perm, iperm;                     // perm[k] = dep index of pivot k
+  std::vector<int> m_to_lu, diag_pos, e_diag;       // [nnz], [nv], [nnz]
+  std::vector<int> l_ptr, l_pos, l_row;             // per pivot k: L(:,k) below k
+  std::vector<int> u_ptr, u_pos, u_row;             // per pivot k: U(:,k) above k
+  std::vector<int> up_ptr;                          // per pivot k: update range
+  std::vector<uint16_t> up_dst, up_l, up_u;         // LU[dst] -= LU[l] * LU[u]
+  long n_updates() const { return (long)up_dst.size(); }
+};
+
+struct Model {
+  // solver_new arguments (camp_solver.c:83-362)
+  int ns = 0, nc = 0, nv = 0;
+  std::vector<int> var_type, deriv_ids, dep_state;
+  int n_rxn = 0, n_added = 0, n_env = 0;
+  std::vector<int> rxn_int, rxn_int_idx, rxn_float_idx, rxn_env_idx;
+  std::vector<double> rxn_float;
+  std::vector<double> rxn_env;  // [nc][n_env], calloc-zeroed as in CAMP
+  // solver_initialize arguments
+  std::vector<double> abs_tol;  // [ns]
+  double rel_tol = 1.0e-8;
+  int max_steps = 10000, max_conv_fails = 1000;
+  // pattern (camp_solver.c:1417-1712)
+  Csc rjac, pjac, sjac;
+  std::vector<int> map_solver, map_rxn, map_param;
+  Program prog;
+  bool initialized = false;
+
+  void add_rxn(int type, int n_int, int n_float, int n_env_param,
+               const int* int_param, const double* float_param);
+  // Builds pattern, ids, the kernel program and the LU ordering.
+  // Returns an empty string on success, else an error message.
+  std::string initialize();
+  bool update_data(int cell, int* rxn_id, int type, int uid, double rate);
+  int* rint(int r) { return &rxn_int[rxn_int_idx[r]]; }
+  double* rflt(int r) { return &rxn_float[rxn_float_idx[r]]; }
+};
+
+// Markowitz ordering of a square pattern (dep-indexed CSC).
+std::vector<int> markowitz_order(int n, const std::vector<int>& colptr,
+                                 const std::vector<int>& rowval);
+
+}  // namespace camp

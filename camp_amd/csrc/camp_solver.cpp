@@ -1,0 +1,1202 @@
+// C ABI of the MI355X-native CAMP solver and its host-side drivers.
+//
+//  * solver_new / rxn_add_condensed_data / solver_initialize build the host
+//    Model (camp_model.cpp) and upload its kernel program once;
+//  * solver_run moves the host state to HBM and runs either
+//      - the per-cell path (default): the persistent k_cell_bdf kernel,
+//        every cell with its own BDF control, or
+//      - the block path (the reference's semantics, camp_solver.c:544-693):
+//        one BDF over all cells driven from the host, one kernel launch per
+//        f(y), J(y), LU factorisation and triangular solve;
+//  * camp_solver_run_device runs on arrays already resident in HBM.
+//
+// Error convention follows the reference: setup errors print a message and
+// exit(EXIT_FAILURE) (camp_solver.c:93-96); solver_run returns
+// CAMP_SOLVER_SUCCESS (0) or CAMP_SOLVER_FAIL (1).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/camp_solver.h"
+#include "camp_kernels.hpp"
+#include "camp_model.hpp"
+
+namespace camp {
+
+[[noreturn]] static void die(const char* what) {
+  std::fprintf(stderr, "\n\nERROR %s\n\n", what);
+  std::fflush(stderr);
+  std::exit(EXIT_FAILURE);
+}
+
+#define HIP_OK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      std::exit(EXIT_FAILURE);                                                            \
+    }                                                                                     \
+  } while (0)
+
+template <class T>
+static T* dev_upload(const std::vector<T>& v, std::vector<void*>& owned) {
+  T* p = nullptr;
+  size_t n = std::max<size_t>(v.size(), 1);
+  HIP_OK(hipMalloc(&p, n * sizeof(T)));
+  if (!v.empty()) HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  owned.push_back(p);
+  return p;
+}
+
+template <class T>
+static T* dev_alloc(size_t n, std::vector<void*>& owned) {
+  T* p = nullptr;
+  HIP_OK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  HIP_OK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  owned.push_back(p);
+  return p;
+}
+
+struct BlockStats {
+  long nst = 0, nfe = 0, nsetups = 0, netf = 0, nni = 0, ncfn = 0, nje = 0;
+  int flag = 0;
+  double hu = 0.0, next_h = 0.0;
+};
+
+struct Solver {
+  Model m;
+  int mode = CAMP_MODE_PER_CELL;
+  int cells_per_block = 1;
+  int guess_on = 1;
+  bool renv_dirty = true;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_kernel_ms = 0.0;
+  std::vector<void*> owned;  // device allocations
+  DevProgram P{};
+  // per-cell persistent buffers
+  double *d_state = nullptr, *d_env = nullptr, *d_renv = nullptr;
+  double *d_Js = nullptr, *d_Jy = nullptr, *d_Jf = nullptr, *d_stats = nullptr;
+  int *d_counter = nullptr, *d_fail = nullptr, *d_var_type = nullptr;
+  // block-path buffers
+  double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_y = nullptr,
+         *d_tempv = nullptr, *d_ftemp = nullptr, *d_last = nullptr, *d_acinit = nullptr,
+         *d_gh1 = nullptr, *d_gh2 = nullptr, *d_LU = nullptr, *d_red = nullptr,
+         *d_jtmp = nullptr;
+  std::vector<double> h_red;
+  BlockStats bstats;
+  bool stats_from_cells = false;
+
+  ~Solver() {
+    for (void* p : owned) hipFree(p);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  void setup_device();
+  // Device buffers are created on first use, so that solver_initialize (and
+  // the pattern getters) work on a host without a GPU.
+  void ensure_device() {
+    if (!stream) setup_device();
+  }
+  void upload_renv();
+  int run(double* d_state_in, double* d_env_in, double t0, double t1);
+  int run_cells(double* st, double* env, double t0, double t1);
+  int run_block(double* st, double* env, double t0, double t1);
+};
+
+static inline int nblk(long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+void Solver::setup_device() {
+  const Program& G = m.prog;
+  HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&ev0));
+  HIP_OK(hipEventCreate(&ev1));
+  P.ns = G.ns;
+  P.nv = G.nv;
+  P.nnz = G.nnz;
+  P.n_env = G.n_env;
+  P.n_terms = G.n_terms;
+  P.lu_nnz = G.lu_nnz;
+  P.n_rxn = m.n_rxn;
+  P.dep_state = dev_upload(G.dep_state, owned);
+  P.t_kidx = dev_upload(G.t_kidx, owned);
+  P.t_react = dev_upload(G.t_react, owned);
+  P.t_nreact = dev_upload(G.t_nreact, owned);
+  P.t_flags = dev_upload(G.t_flags, owned);
+  P.f_ptr = dev_upload(G.f_ptr, owned);
+  P.f_term = dev_upload(G.f_term, owned);
+  P.f_guard = dev_upload(G.f_guard, owned);
+  P.f_coef = dev_upload(G.f_coef, owned);
+  P.j_ptr = dev_upload(G.j_ptr, owned);
+  P.j_term = dev_upload(G.j_term, owned);
+  P.j_skip = dev_upload(G.j_skip, owned);
+  P.j_guard = dev_upload(G.j_guard, owned);
+  P.j_rind = dev_upload(G.j_rind, owned);
+  P.j_coef = dev_upload(G.j_coef, owned);
+  P.colptr = dev_upload(G.colptr, owned);
+  P.rowval = dev_upload(G.rowval, owned);
+  P.r_ptr = dev_upload(G.r_ptr, owned);
+  P.r_elem = dev_upload(G.r_elem, owned);
+  P.r_col = dev_upload(G.r_col, owned);
+  P.perm = dev_upload(G.perm, owned);
+  P.m_to_lu = dev_upload(G.m_to_lu, owned);
+  P.diag_pos = dev_upload(G.diag_pos, owned);
+  P.e_diag = dev_upload(G.e_diag, owned);
+  P.l_ptr = dev_upload(G.l_ptr, owned);
+  P.l_pos = dev_upload(G.l_pos, owned);
+  P.l_row = dev_upload(G.l_row, owned);
+  P.u_ptr = dev_upload(G.u_ptr, owned);
+  P.u_pos = dev_upload(G.u_pos, owned);
+  P.u_row = dev_upload(G.u_row, owned);
+  P.up_ptr = dev_upload(G.up_ptr, owned);
+  P.up_dst = dev_upload(G.up_dst, owned);
+  P.up_l = dev_upload(G.up_l, owned);
+  P.up_u = dev_upload(G.up_u, owned);
+  std::vector<int> rx_type(m.n_rxn), rx_fidx(m.n_rxn), rx_eidx(m.n_rxn), rx_nreact(m.n_rxn);
+  for (int r = 0; r < m.n_rxn; ++r) {
+    int* id = m.rint(r);
+    rx_type[r] = id[0];
+    rx_fidx[r] = m.rxn_float_idx[r];
+    rx_eidx[r] = m.rxn_env_idx[r];
+    rx_nreact[r] = (id[0] == kRxnArrhenius || id[0] == kRxnTroe) ? id[1] : 0;
+  }
+  P.rx_type = dev_upload(rx_type, owned);
+  P.rx_fidx = dev_upload(rx_fidx, owned);
+  P.rx_eidx = dev_upload(rx_eidx, owned);
+  P.rx_nreact = dev_upload(rx_nreact, owned);
+  P.rx_float = dev_upload(m.rxn_float, owned);
+  std::vector<double> atol_dep(G.nv);
+  for (int d = 0; d < G.nv; ++d) atol_dep[d] = m.abs_tol[G.dep_state[d]];
+  P.abs_tol_dep = dev_upload(atol_dep, owned);
+
+  size_t nc = m.nc;
+  d_state = dev_alloc<double>(nc * G.ns, owned);
+  d_env = dev_alloc<double>(nc * 2, owned);
+  d_renv = dev_alloc<double>(nc * G.n_env, owned);
+  d_Js = dev_alloc<double>(nc * G.nnz, owned);
+  d_Jy = dev_alloc<double>(nc * G.nv, owned);
+  d_Jf = dev_alloc<double>(nc * G.nv, owned);
+  d_stats = dev_alloc<double>(nc * kCellStats, owned);
+  d_counter = dev_alloc<int>(1, owned);
+  d_fail = dev_alloc<int>(1, owned);
+  d_var_type = dev_upload(m.var_type, owned);
+}
+
+void Solver::upload_renv() {
+  if (!renv_dirty) return;
+  HIP_OK(hipMemcpyAsync(d_renv, m.rxn_env.data(), m.rxn_env.size() * sizeof(double),
+                        hipMemcpyHostToDevice, stream));
+  renv_dirty = false;
+}
+
+int Solver::run(double* st, double* env, double t0, double t1) {
+  upload_renv();
+  if (m.n_rxn == 0) return CAMP_SOLVER_SUCCESS;  // no_solve, camp_solver.c:240
+  return mode == CAMP_MODE_BLOCK ? run_block(st, env, t0, t1) : run_cells(st, env, t0, t1);
+}
+
+int Solver::run_cells(double* st, double* env, double t0, double t1) {
+  CellBdfArgs a;
+  a.P = P;
+  a.nc = m.nc;
+  a.state = st;
+  a.env = env;
+  a.rxn_env = d_renv;
+  a.Js = d_Js;
+  a.Jy = d_Jy;
+  a.Jf = d_Jf;
+  a.var_type = d_var_type;
+  a.rtol = m.rel_tol;
+  a.t0 = t0;
+  a.t1 = t1;
+  a.max_steps = m.max_steps;
+  a.max_conv_fails = m.max_conv_fails;
+  a.guess_on = guess_on;
+  a.counter = d_counter;
+  a.stats = d_stats;
+  int cpb = cells_per_block;
+  int block = kWave * cpb;
+  size_t lds = (size_t)cpb * cell_bdf_lds_doubles(P.ns, P.nv, P.n_env, P.lu_nnz) * sizeof(double);
+  int dev = 0, ncu = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int per_cu = cell_bdf_blocks_per_cu(block, lds);
+  if (per_cu <= 0) die("per-cell kernel does not fit on a compute unit (LDS)");
+  int grid = std::min<long>((long)per_cu * ncu, (m.nc + cpb - 1) / cpb);
+  HIP_OK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
+  HIP_OK(hipEventRecord(ev0, stream));
+  launch_cell_bdf(a, grid, block, lds, stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(ev1, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
+  last_kernel_ms = ms;
+  std::vector<double> stats((size_t)m.nc * kCellStats);
+  HIP_OK(hipMemcpy(stats.data(), d_stats, stats.size() * sizeof(double), hipMemcpyDeviceToHost));
+  BlockStats s;
+  int fail = 0;
+  for (int c = 0; c < m.nc; ++c) {
+    const double* x = &stats[(size_t)c * kCellStats];
+    s.nst += (long)x[0]; s.nfe += (long)x[1]; s.nsetups += (long)x[2]; s.netf += (long)x[3];
+    s.nni += (long)x[4]; s.ncfn += (long)x[5]; s.nje += (long)x[6];
+    if (x[7] < 0) {
+      fail = 1;
+      s.flag = (int)x[7];
+    }
+    s.hu = x[8];
+    s.next_h = x[9];
+  }
+  bstats = s;
+  stats_from_cells = true;
+  return fail ? CAMP_SOLVER_FAIL : CAMP_SOLVER_SUCCESS;
+}
+
+// -------------------------------------------------------------------------
+// Block path: host-driven BDF over all cells (CVODE algorithm, same
+// constants as the per-cell kernel), one launch per vector operation,
+// f(y), J(y), LU and solve.  Norms are reduced on the device and finished
+// on the host.
+// -------------------------------------------------------------------------
+namespace {
+constexpr int QMAX = 5;
+constexpr double ETAMX1 = 10000.0, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2;
+constexpr double ETAMIN = 0.1, ETACF = 0.25, ADDON = 0.000001;
+constexpr double BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0, THRESH = 1.5;
+constexpr int SMALL_NST = 10, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10;
+constexpr double CRDOWN = 0.3, DGMAX = 0.3, RDIV = 2.0, NLSCOEF = 0.1;
+constexpr int MSBP = 20, MAXCOR = 3, MSBJ = 50;
+constexpr double DGMAX_JBAD = 0.2;
+constexpr int GUESS_MAX_ITER = 5;
+constexpr int kRedBlocks = 512;
+enum { CV_SUCCESS = 0, CV_TOO_MUCH_WORK = -1, CV_TOO_MUCH_ACC = -2, CV_ERR_FAILURE = -3,
+       CV_CONV_FAILURE = -4, CV_FIRST_RHSFUNC_ERR = -9, CV_REPTD_RHSFUNC_ERR = -10,
+       CV_UNREC_RHSFUNC_ERR = -11 };
+enum { FIRST_CALL = 1, PREV_CONV_FAIL, PREV_ERR_FAIL };
+enum { DO_ERROR_TEST = 2, PREDICT_AGAIN, CONV_FAIL, TRY_AGAIN, RHSFUNC_RECVR };
+enum { CV_NO_FAILURES = 0, CV_FAIL_BAD_J = 1, CV_FAIL_OTHER = 2 };
+
+double rpowerr(double b, double e) { return b <= 0.0 ? 0.0 : std::pow(b, e); }
+double rpoweri(double b, int e) {
+  double p = 1.0;
+  for (int i = 1; i <= std::abs(e); ++i) p *= b;
+  return e < 0 ? 1.0 / p : p;
+}
+
+struct BlockCv {
+  Solver& S;
+  long n;
+  int nc;
+  double* st;
+  hipStream_t s;
+  double *zn[6], *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2;
+  double tn = 0, h = 0, hscale = 0, hprime = 0, next_h = 0, eta = 1, etamax = ETAMX1, etaq = 0,
+         etaqm1 = 0, etaqp1 = 0, hu = 0, init_h = 0;
+  double l[6] = {0}, tq[6] = {0}, tau[8] = {0};
+  double rl1 = 0, gamma = 0, gammap = 1, gamrat = 1, crate = 1, acnrm = 0, saved_tq5 = 0;
+  int q = 1, qprime = 1, L = 2, qwait = 2, jcur = 0, use_acinit = 0;
+  long nst = 0, nfe = 0, nsetups = 0, nni = 0, ncfn = 0, netf = 0, nje = 0, nstlp = 0,
+       nstlj = 0;
+
+  BlockCv(Solver& S_, double* st_) : S(S_), st(st_) {
+    nc = S.m.nc;
+    n = (long)S.P.nv * nc;
+    s = S.stream;
+    for (int j = 0; j < 6; ++j) zn[j] = S.d_zn + j * n;
+    ewt = S.d_ewt; acor = S.d_acor; y = S.d_y; tempv = S.d_tempv; ftemp = S.d_ftemp;
+    last = S.d_last; acinit = S.d_acinit; gh1 = S.d_gh1; gh2 = S.d_gh2;
+  }
+
+  void copy(double* dst, const double* src) {
+    HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  void lin(double a, const double* x, double b, const double* yv, double* z) {
+    hipLaunchKernelGGL(k_vec_lin, dim3(nblk(n)), dim3(256), 0, s, n, a, x, b, yv, z);
+  }
+  double wrms(const double* x) {
+    hipLaunchKernelGGL(k_vec_wsq, dim3(kRedBlocks), dim3(256), 0, s, n, x, ewt, S.d_red);
+    HIP_OK(hipMemcpyAsync(S.h_red.data(), S.d_red, kRedBlocks * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    double acc = 0.0;
+    for (int i = 0; i < kRedBlocks; ++i) acc += S.h_red[i];
+    return std::sqrt(acc / (double)n);
+  }
+  double vmin(const double* x) {
+    hipLaunchKernelGGL(k_vec_min, dim3(kRedBlocks), dim3(256), 0, s, n, x, S.d_red);
+    HIP_OK(hipMemcpyAsync(S.h_red.data(), S.d_red, kRedBlocks * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    double v = S.h_red[0];
+    for (int i = 1; i < kRedBlocks; ++i) v = std::min(v, S.h_red[i]);
+    return v;
+  }
+  void ewt_set(const double* ycur) {
+    hipLaunchKernelGGL(k_vec_ewt, dim3(nblk(n)), dim3(256), 0, s, n, S.P.nv, S.m.rel_tol,
+                       S.P.abs_tol_dep, ycur, ewt);
+  }
+  int check_fail() {
+    int f = 0;
+    HIP_OK(hipMemcpyAsync(&f, S.d_fail, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return f;
+  }
+  int deriv(const double* yv, double* out, int use_est) {
+    double ts = next_h > 0.0 ? next_h : init_h;
+    HIP_OK(hipMemsetAsync(S.d_fail, 0, sizeof(int), s));
+    size_t lds = 4 * (size_t)(S.P.ns + S.P.n_env) * sizeof(double);
+    hipLaunchKernelGGL(k_deriv, dim3(std::min(nblk(nc, 4), 4096)), dim3(256), lds, s, S.P, nc,
+                       yv, st, S.d_renv, S.d_Js, S.d_Jy, S.d_Jf, out, ts, use_est, S.d_fail);
+    return check_fail();
+  }
+  int f(const double* yv, double* out) {
+    nfe++;
+    return deriv(yv, out, 1);
+  }
+  int jac(const double* yv, double* fy) {
+    if (deriv(yv, fy, 0)) return 1;
+    double ts = next_h > 0.0 ? next_h : init_h;
+    size_t lds = 4 * (size_t)(S.P.ns + S.P.n_env) * sizeof(double);
+    hipLaunchKernelGGL(k_jac, dim3(std::min(nblk(nc, 4), 4096)), dim3(256), lds, s, S.P, nc, yv,
+                       st, S.d_renv, S.d_Js, ts);
+    copy(S.d_Jy, yv);
+    copy(S.d_Jf, fy);
+    return 0;
+  }
+  void rescale() {
+    hipLaunchKernelGGL(k_vec_rescale, dim3(nblk(n)), dim3(256), 0, s, n, q, eta, S.d_zn);
+    h = hscale * eta;
+    next_h = h;
+    hscale = h;
+  }
+  void predict() {
+    tn += h;
+    hipLaunchKernelGGL(k_vec_pascal, dim3(nblk(n)), dim3(256), 0, s, n, q, S.d_zn, 1);
+  }
+  void restore(double saved_t) {
+    tn = saved_t;
+    hipLaunchKernelGGL(k_vec_pascal, dim3(nblk(n)), dim3(256), 0, s, n, q, S.d_zn, -1);
+  }
+  void set_tq(double hsum, double alpha0, double alpha0_hat, double xi_inv, double xistar_inv) {
+    double A1 = 1.0 - alpha0_hat + alpha0;
+    double A2 = 1.0 + q * A1;
+    tq[2] = std::fabs(A1 / (alpha0 * A2));
+    tq[5] = std::fabs(A2 * xistar_inv / (l[q] * xi_inv));
+    if (qwait == 1) {
+      if (q > 1) {
+        double C = xistar_inv / l[q];
+        double A3 = alpha0 + 1.0 / q;
+        double A4 = alpha0_hat + xi_inv;
+        double Cpinv = (1.0 - A4 + A3) / A3;
+        tq[1] = std::fabs(C * Cpinv);
+      } else {
+        tq[1] = 1.0;
+      }
+      hsum += tau[q];
+      xi_inv = h / hsum;
+      double A5 = alpha0 - (1.0 / (q + 1));
+      double A6 = alpha0_hat - xi_inv;
+      double Cppinv = (1.0 - A6 + A5) / A2;
+      tq[3] = std::fabs(Cppinv / (xi_inv * (q + 2) * A5));
+    }
+    tq[4] = NLSCOEF / tq[2];
+  }
+  void set() {
+    double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
+    l[0] = l[1] = xi_inv = xistar_inv = 1.0;
+    for (int z = 2; z <= q; ++z) l[z] = 0.0;
+    alpha0 = alpha0_hat = -1.0;
+    hsum = h;
+    if (q > 1) {
+      for (int j = 2; j < q; ++j) {
+        hsum += tau[j - 1];
+        xi_inv = h / hsum;
+        alpha0 -= 1.0 / j;
+        for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+      }
+      alpha0 -= 1.0 / q;
+      xistar_inv = -l[1] - alpha0;
+      hsum += tau[q - 1];
+      xi_inv = h / hsum;
+      alpha0_hat = -l[1] - xi_inv;
+      for (int i = q; i >= 1; --i) l[i] += l[i - 1] * xistar_inv;
+    }
+    set_tq(hsum, alpha0, alpha0_hat, xi_inv, xistar_inv);
+    rl1 = 1.0 / l[1];
+    gamma = h * rl1;
+    if (nst == 0) gammap = gamma;
+    gamrat = (nst > 0) ? gamma / gammap : 1.0;
+  }
+  void increase_bdf() {
+    double alpha0, alpha1, prod, xi, xiold, hsum, A1;
+    for (int j = 0; j <= QMAX; ++j) l[j] = 0.0;
+    l[2] = alpha1 = prod = xiold = 1.0;
+    alpha0 = -1.0;
+    hsum = hscale;
+    if (q > 1) {
+      for (int j = 1; j < q; ++j) {
+        hsum += tau[j + 1];
+        xi = hsum / hscale;
+        prod *= xi;
+        alpha0 -= 1.0 / (j + 1);
+        alpha1 += 1.0 / xi;
+        for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xiold + l[i - 1];
+        xiold = xi;
+      }
+    }
+    A1 = (-alpha0 - alpha1) / prod;
+    lin(A1, zn[QMAX], 0.0, zn[QMAX], zn[L]);
+    for (int j = 2; j <= q; ++j) lin(l[j], zn[L], 1.0, zn[j], zn[j]);
+  }
+  void decrease_bdf() {
+    double hsum, xi;
+    for (int j = 0; j <= QMAX; ++j) l[j] = 0.0;
+    l[2] = 1.0;
+    hsum = 0.0;
+    for (int j = 1; j <= q - 2; ++j) {
+      hsum += tau[j];
+      xi = hsum / hscale;
+      for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xi + l[i - 1];
+    }
+    for (int j = 2; j < q; ++j) lin(-l[j], zn[q], 1.0, zn[j], zn[j]);
+  }
+  void adjust_params() {
+    if (qprime != q) {
+      int dq = qprime - q;
+      if (dq == 1) increase_bdf();
+      else if (dq == -1) decrease_bdf();
+      q = qprime;
+      L = q + 1;
+      qwait = L;
+    }
+    rescale();
+  }
+  // guess_helper (camp_solver.c:1321-1408) over the whole block vector
+  int guess_helper(double t_n, double h_n, const double* y_n, const double* y_n1, const double* hf,
+                   double* tmp1, double* corr) {
+    if (vmin(y_n) > -1.0e-30) return 0;
+    copy(tmp1, y_n1);
+    lin(1.0 / h_n, hf, 0.0, hf, corr);
+    double t_0 = t_n - h_n, t_j = 0.0;
+    for (int iter = 0; iter < GUESS_MAX_ITER && t_0 + t_j < t_n; iter++) {
+      double h_j = t_n - (t_0 + t_j);
+      hipLaunchKernelGGL(k_vec_tstar, dim3(kRedBlocks), dim3(256), 0, s, n, tmp1, corr, h_j,
+                         S.d_red);
+      std::vector<double> red(2 * kRedBlocks);
+      HIP_OK(hipMemcpyAsync(red.data(), S.d_red, red.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      bool found = false;
+      double best = h_j;
+      for (int b = 0; b < kRedBlocks; ++b)
+        if (red[2 * b + 1] >= 0.0) {
+          found = true;
+          best = std::min(best, red[2 * b]);
+        }
+      h_j = best;
+      if (found) h_j *= 0.95 + 0.1 * iter / (double)GUESS_MAX_ITER;
+      h_j = t_n < t_0 + t_j + h_j ? t_n - (t_0 + t_j) : h_j;
+      lin(1.0, tmp1, h_j, corr, tmp1);
+      t_j += h_j;
+      if (f(tmp1, corr)) return -1;
+    }
+    lin(1.0, tmp1, -1.0, y_n, corr);
+    hipLaunchKernelGGL(k_vec_scale, dim3(nblk(n)), dim3(256), 0, s, n, 0.999, corr);
+    return 1;
+  }
+  int lsetup(int convfail, const double* ypred, double* fpred) {
+    double dgamma = std::fabs(gamma / gammap - 1.0);
+    int jbad = (nst == 0) || (nst > nstlj + MSBJ) ||
+               ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_JBAD)) || (convfail == CV_FAIL_OTHER);
+    if (!jbad) {
+      jcur = 0;
+    } else {
+      nje++;
+      nstlj = nst;
+      jcur = 1;
+      if (jac(ypred, fpred)) return 1;
+    }
+    HIP_OK(hipMemsetAsync(S.d_fail, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_lu_factor, dim3(std::min(nc, 16384)), dim3(64),
+                       (size_t)S.P.lu_nnz * sizeof(double), s, S.P, nc, S.d_Js, gamma, S.d_LU,
+                       S.d_fail);
+    return check_fail();
+  }
+  void lsolve(double* b) {
+    double sc = gamrat != 1.0 ? 2.0 / (1.0 + gamrat) : 1.0;
+    hipLaunchKernelGGL(k_lu_solve, dim3(std::min(nc, 16384)), dim3(64),
+                       ((size_t)S.P.lu_nnz + 2 * (size_t)S.P.nv) * sizeof(double), s, S.P, nc,
+                       S.d_LU, b, sc);
+  }
+  int newton_iteration() {
+    int m = 0;
+    double del = 0.0, delp = 0.0;
+    for (;;) {
+      hipLaunchKernelGGL(k_vec_resid, dim3(nblk(n)), dim3(256), 0, s, n, rl1, zn[1], acor, gamma,
+                         ftemp, tempv);
+      lsolve(tempv);
+      nni++;
+      del = wrms(tempv);
+      hipLaunchKernelGGL(k_vec_newton_update, dim3(nblk(n)), dim3(256), 0, s, n, tempv, acor,
+                         zn[0], y);
+      if (m > 0) crate = std::fmax(CRDOWN * crate, del / delp);
+      double dcon = del * std::fmin(1.0, crate) / tq[4];
+      if (dcon <= 1.0) {
+        // non-negativity safeguard (same rule as the per-cell kernel)
+        if (vmin(y) < -1.0e-30) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        acnrm = (m == 0) ? del : wrms(acor);
+        jcur = 0;
+        return CV_SUCCESS;
+      }
+      m++;
+      if ((m == MAXCOR) || ((m >= 2) && (del > RDIV * delp))) return jcur ? CONV_FAIL : TRY_AGAIN;
+      delp = del;
+      if (f(y, ftemp)) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+    }
+  }
+  int nls(int nflag) {
+    int convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? CV_NO_FAILURES : CV_FAIL_OTHER;
+    int callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (nst == 0) ||
+                    (nst >= nstlp + MSBP) || (std::fabs(gamrat - 1.0) > DGMAX);
+    use_acinit = 0;
+    if (S.guess_on) {
+      lin(1.0, zn[0], -1.0, last, acinit);
+      if (guess_helper(tn, h, zn[0], last, acinit, gh1, gh2) == 1) {
+        copy(acinit, gh2);
+        use_acinit = 1;
+      }
+    }
+    for (;;) {
+      if (use_acinit) lin(1.0, zn[0], 1.0, acinit, y);
+      else copy(y, zn[0]);
+      if (f(y, ftemp)) return RHSFUNC_RECVR;
+      if (callSetup) {
+        int ier = lsetup(convfail, y, ftemp);
+        nsetups++;
+        callSetup = 0;
+        gamrat = crate = 1.0;
+        gammap = gamma;
+        nstlp = nst;
+        if (ier > 0) return CONV_FAIL;
+      }
+      if (use_acinit) copy(acor, acinit);
+      else HIP_OK(hipMemsetAsync(acor, 0, n * sizeof(double), s));
+      int ier = newton_iteration();
+      if (ier != TRY_AGAIN) return ier;
+      callSetup = 1;
+      convfail = CV_FAIL_BAD_J;
+    }
+  }
+  int handle_nflag(int* nflagPtr, double saved_t, int* ncfPtr) {
+    int nflag = *nflagPtr;
+    if (nflag == CV_SUCCESS) return DO_ERROR_TEST;
+    ncfn++;
+    restore(saved_t);
+    (*ncfPtr)++;
+    etamax = 1.0;
+    if (*ncfPtr == S.m.max_conv_fails) {
+      if (nflag == CONV_FAIL) return CV_CONV_FAILURE;
+      if (nflag == RHSFUNC_RECVR) return CV_REPTD_RHSFUNC_ERR;
+    }
+    eta = ETACF;
+    *nflagPtr = PREV_CONV_FAIL;
+    rescale();
+    return PREDICT_AGAIN;
+  }
+  int do_error_test(int* nflagPtr, double saved_t, int* nefPtr, double* dsmPtr) {
+    double dsm = acnrm * tq[2];
+    *dsmPtr = dsm;
+    if (dsm <= 1.0) return CV_SUCCESS;
+    (*nefPtr)++;
+    netf++;
+    *nflagPtr = PREV_ERR_FAIL;
+    restore(saved_t);
+    if (*nefPtr == S.m.max_conv_fails) return CV_ERR_FAILURE;
+    etamax = 1.0;
+    if (*nefPtr <= MXNEF1) {
+      eta = 1.0 / (rpowerr(BIAS2 * dsm, 1.0 / L) + ADDON);
+      eta = std::fmax(ETAMIN, std::fmax(eta, 0.0));
+      if (*nefPtr >= SMALL_NEF) eta = std::fmin(eta, ETAMXF);
+      rescale();
+      return TRY_AGAIN;
+    }
+    if (q > 1) {
+      eta = ETAMIN;
+      decrease_bdf();
+      L = q;
+      q--;
+      qwait = L;
+      rescale();
+      return TRY_AGAIN;
+    }
+    eta = ETAMIN;
+    h *= eta;
+    next_h = h;
+    hscale = h;
+    qwait = LONG_WAIT;
+    if (f(zn[0], tempv)) return CV_UNREC_RHSFUNC_ERR;
+    lin(h, tempv, 0.0, tempv, zn[1]);
+    return TRY_AGAIN;
+  }
+  void complete_step() {
+    nst++;
+    hu = h;
+    for (int i = q; i >= 2; --i) tau[i] = tau[i - 1];
+    if ((q == 1) && (nst > 1)) tau[2] = tau[1];
+    tau[1] = h;
+    qwait--;
+    int save = (qwait == 1) && (q != QMAX);
+    Coef6 c6;
+    for (int j = 0; j < 6; ++j) c6.v[j] = l[j];
+    hipLaunchKernelGGL(k_vec_complete, dim3(nblk(n)), dim3(256), 0, s, n, q, c6, acor, S.d_zn,
+                       save, last);
+    if (save) saved_tq5 = tq[5];
+  }
+  void set_eta() {
+    if (eta < THRESH) {
+      eta = 1.0;
+      hprime = h;
+    } else {
+      eta = std::fmin(eta, etamax);
+      hprime = h * eta;
+    }
+  }
+  void prepare_next_step(double dsm) {
+    if (etamax == 1.0) {
+      qwait = std::max(qwait, 2);
+      qprime = q;
+      hprime = h;
+      eta = 1.0;
+      return;
+    }
+    etaq = 1.0 / (rpowerr(BIAS2 * dsm, 1.0 / L) + ADDON);
+    if (qwait != 0) {
+      eta = etaq;
+      qprime = q;
+      set_eta();
+      return;
+    }
+    qwait = 2;
+    etaqm1 = 0.0;
+    if (q > 1) {
+      double ddn = wrms(zn[q]) * tq[1];
+      etaqm1 = 1.0 / (rpowerr(BIAS1 * ddn, 1.0 / q) + ADDON);
+    }
+    etaqp1 = 0.0;
+    if (q != QMAX && saved_tq5 != 0.0) {
+      double cquot = (tq[5] / saved_tq5) * rpoweri(h / tau[2], L);
+      lin(-cquot, zn[QMAX], 1.0, acor, tempv);
+      double dup = wrms(tempv) * tq[3];
+      etaqp1 = 1.0 / (rpowerr(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+    }
+    double etam = std::fmax(etaqm1, std::fmax(etaq, etaqp1));
+    if (etam < THRESH) {
+      eta = 1.0;
+      qprime = q;
+    } else if (etam == etaq) {
+      eta = etaq;
+      qprime = q;
+    } else if (etam == etaqm1) {
+      eta = etaqm1;
+      qprime = q - 1;
+    } else {
+      eta = etaqp1;
+      qprime = q + 1;
+      copy(zn[QMAX], acor);
+    }
+    set_eta();
+  }
+  int step() {
+    double saved_t = tn, dsm = 0.0;
+    int ncf = 0, nef = 0, nflag = FIRST_CALL;
+    if ((nst > 0) && (hprime != h)) adjust_params();
+    for (;;) {
+      predict();
+      set();
+      nflag = nls(nflag);
+      int kflag = handle_nflag(&nflag, saved_t, &ncf);
+      if (kflag == PREDICT_AGAIN) continue;
+      if (kflag != DO_ERROR_TEST) return kflag;
+      int eflag = do_error_test(&nflag, saved_t, &nef, &dsm);
+      if (eflag == TRY_AGAIN) continue;
+      if (eflag != CV_SUCCESS) return eflag;
+      break;
+    }
+    complete_step();
+    prepare_next_step(dsm);
+    etamax = (nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
+    hipLaunchKernelGGL(k_vec_scale, dim3(nblk(n)), dim3(256), 0, s, n, tq[2], acor);
+    return CV_SUCCESS;
+  }
+  int solve(double t0, double tout) {
+    copy(last, zn[0]);
+    tn = t0;
+    init_h = tout - t0;
+    next_h = 0.0;
+    ewt_set(zn[0]);
+    if (f(zn[0], zn[1])) return CV_FIRST_RHSFUNC_ERR;
+    h = init_h;
+    hipLaunchKernelGGL(k_vec_scale, dim3(nblk(n)), dim3(256), 0, s, n, h, zn[1]);
+    hscale = h;
+    hprime = h;
+    long nstloc = 0;
+    for (;;) {
+      next_h = h;
+      if (nst > 0) ewt_set(zn[0]);
+      if (nstloc >= S.m.max_steps) return CV_TOO_MUCH_WORK;
+      double nrm = wrms(zn[0]);
+      if (2.220446049250313e-16 * nrm > 1.0) return CV_TOO_MUCH_ACC;
+      int kflag = step();
+      if (kflag != CV_SUCCESS) return kflag;
+      nstloc++;
+      if ((tn - tout) * h >= 0.0) {
+        double sv = (tout - tn) / h;
+        hipLaunchKernelGGL(k_vec_interp, dim3(nblk(n)), dim3(256), 0, s, n, q, sv, S.d_zn, y);
+        next_h = hprime;
+        return CV_SUCCESS;
+      }
+    }
+  }
+};
+}  // namespace
+
+int Solver::run_block(double* st, double* env, double t0, double t1) {
+  if (!d_zn) {
+    long n = (long)P.nv * m.nc;
+    d_zn = dev_alloc<double>(6 * n, owned);
+    d_ewt = dev_alloc<double>(n, owned);
+    d_acor = dev_alloc<double>(n, owned);
+    d_y = dev_alloc<double>(n, owned);
+    d_tempv = dev_alloc<double>(n, owned);
+    d_ftemp = dev_alloc<double>(n, owned);
+    d_last = dev_alloc<double>(n, owned);
+    d_acinit = dev_alloc<double>(n, owned);
+    d_gh1 = dev_alloc<double>(n, owned);
+    d_gh2 = dev_alloc<double>(n, owned);
+    d_LU = dev_alloc<double>((size_t)m.nc * P.lu_nnz, owned);
+    d_red = dev_alloc<double>(2 * kRedBlocks, owned);
+    h_red.assign(2 * kRedBlocks, 0.0);
+  }
+  int nc = m.nc;
+  hipLaunchKernelGGL(k_clamp_constants, dim3(nblk((long)nc * P.ns)), dim3(256), 0, stream, P, nc,
+                     d_var_type, st);
+  hipLaunchKernelGGL(k_rates, dim3(nblk((long)nc * m.n_rxn)), dim3(256), 0, stream, P, nc, env,
+                     d_renv);
+  hipLaunchKernelGGL(k_state_gather, dim3(nblk((long)nc * P.nv)), dim3(256), 0, stream, P, nc, st,
+                     d_zn);
+  BlockCv cv(*this, st);
+  HIP_OK(hipEventRecord(ev0, stream));
+  int flag = cv.solve(t0, t1);
+  HIP_OK(hipEventRecord(ev1, stream));
+  if (flag >= 0)
+    hipLaunchKernelGGL(k_state_scatter, dim3(nblk((long)nc * P.nv)), dim3(256), 0, stream, P, nc,
+                       d_y, st);
+  HIP_OK(hipStreamSynchronize(stream));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
+  last_kernel_ms = ms;
+  BlockStats b;
+  b.nst = cv.nst; b.nfe = cv.nfe; b.nsetups = cv.nsetups; b.netf = cv.netf;
+  b.nni = cv.nni; b.ncfn = cv.ncfn; b.nje = cv.nje; b.flag = flag;
+  b.hu = cv.hu; b.next_h = cv.next_h;
+  bstats = b;
+  stats_from_cells = false;
+  return flag < 0 ? CAMP_SOLVER_FAIL : CAMP_SOLVER_SUCCESS;
+}
+
+static Solver* SD(void* p) { return static_cast<Solver*>(p); }
+
+}  // namespace camp
+
+using camp::Solver;
+using camp::SD;
+
+// ---------------------------------------------------------------------------
+// Drop-in boundary
+// ---------------------------------------------------------------------------
+extern "C" void* solver_new(int n_state_var, int n_cells, int* var_type, int n_rxn,
+                            int n_rxn_int_param, int n_rxn_float_param, int n_rxn_env_param,
+                            int n_aero_phase, int, int, int n_aero_rep, int, int, int,
+                            int n_sub_model, int, int, int) {
+  if (n_aero_phase || n_aero_rep || n_sub_model)
+    camp::die("aerosol phases, aerosol representations and sub models are not on this path");
+  if (n_state_var <= 0 || n_cells <= 0 || n_rxn < 0) camp::die("invalid solver_new dimensions");
+  Solver* s = new Solver();
+  camp::Model& m = s->m;
+  m.ns = n_state_var;
+  m.nc = n_cells;
+  m.var_type.assign(var_type, var_type + n_state_var);
+  m.nv = 0;
+  for (int i = 0; i < n_state_var; ++i) m.nv += var_type[i] == camp::kVariable;
+  m.n_rxn = n_rxn;
+  m.rxn_int.assign((size_t)n_rxn_int_param + n_rxn, 0);
+  m.rxn_float.assign(std::max(n_rxn_float_param, 0), 0.0);
+  m.rxn_env.assign((size_t)n_cells * std::max(n_rxn_env_param, 0), 0.0);
+  m.rxn_int_idx.assign(n_rxn + 1, 0);
+  m.rxn_float_idx.assign(n_rxn + 1, 0);
+  m.rxn_env_idx.assign(n_rxn + 1, 0);
+  return s;
+}
+
+extern "C" void rxn_add_condensed_data(int rxn_type, int n_int_param, int n_float_param,
+                                       int n_env_param, int* int_param, double* float_param,
+                                       void* solver_data) {
+  Solver* s = SD(solver_data);
+  if (s->m.n_added >= s->m.n_rxn) camp::die("more reactions added than declared in solver_new");
+  s->m.add_rxn(rxn_type, n_int_param, n_float_param, n_env_param, int_param, float_param);
+}
+
+extern "C" void solver_initialize(void* solver_data, double* abs_tol, double rel_tol,
+                                  int max_steps, int max_conv_fails) {
+  Solver* s = SD(solver_data);
+  camp::Model& m = s->m;
+  if ((size_t)m.nc * m.n_env != m.rxn_env.size()) m.rxn_env.assign((size_t)m.nc * m.n_env, 0.0);
+  m.abs_tol.assign(abs_tol, abs_tol + m.ns);
+  m.rel_tol = rel_tol;
+  m.max_steps = max_steps;
+  m.max_conv_fails = max_conv_fails;
+  std::string err = m.initialize();
+  if (!err.empty()) camp::die(err.c_str());
+  s->renv_dirty = true;
+}
+
+extern "C" int solver_run(void* solver_data, double* state, double* env, double t_initial,
+                          double t_final) {
+  Solver* s = SD(solver_data);
+  if (!s->m.initialized) return CAMP_SOLVER_FAIL;
+  s->ensure_device();
+  size_t nst = (size_t)s->m.nc * s->m.ns;
+  HIP_OK(hipMemcpyAsync(s->d_state, state, nst * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipMemcpyAsync(s->d_env, env, (size_t)s->m.nc * 2 * sizeof(double),
+                        hipMemcpyHostToDevice, s->stream));
+  int rc = s->run(s->d_state, s->d_env, t_initial, t_final);
+  HIP_OK(hipMemcpyAsync(state, s->d_state, nst * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return rc;
+}
+
+extern "C" void solver_get_statistics(void* solver_data, int* solver_flag, int* num_steps,
+                                      int* RHS_evals, int* LS_setups, int* error_test_fails,
+                                      int* NLS_iters, int* NLS_convergence_fails,
+                                      int* DLS_Jac_evals, int* DLS_RHS_evals,
+                                      double* last_time_step__s, double* next_time_step__s,
+                                      int* Jac_eval_fails, int* RHS_evals_total,
+                                      int* Jac_evals_total, double* RHS_time__s,
+                                      double* Jac_time__s, double* max_loss_precision) {
+  const camp::BlockStats& b = SD(solver_data)->bstats;
+  *solver_flag = b.flag;
+  *num_steps = (int)b.nst;
+  *RHS_evals = (int)b.nfe;
+  *LS_setups = (int)b.nsetups;
+  *error_test_fails = (int)b.netf;
+  *NLS_iters = (int)b.nni;
+  *NLS_convergence_fails = (int)b.ncfn;
+  *DLS_Jac_evals = (int)b.nje;
+  *DLS_RHS_evals = 0;
+  *last_time_step__s = b.hu;
+  *next_time_step__s = b.next_h;
+  *Jac_eval_fails = 0;
+  *RHS_evals_total = -1;
+  *Jac_evals_total = -1;
+  *RHS_time__s = 0.0;
+  *Jac_time__s = 0.0;
+  *max_loss_precision = 0.0;
+}
+
+extern "C" void solver_reset_timers(void*) {}
+
+extern "C" void solver_free(void* solver_data) { delete SD(solver_data); }
+
+extern "C" void rxn_update_data(int cell_id, int* rxn_id, int update_rxn_type, void* update_data,
+                                void* solver_data) {
+  Solver* s = SD(solver_data);
+  if (cell_id < 0 || cell_id >= s->m.nc) return;
+  int uid;
+  double rate;
+  std::memcpy(&uid, update_data, sizeof(int));
+  std::memcpy(&rate, static_cast<char*>(update_data) + sizeof(int), sizeof(double));
+  if (s->m.update_data(cell_id, rxn_id, update_rxn_type, uid, rate)) s->renv_dirty = true;
+}
+
+extern "C" void rxn_print_data(void* solver_data) {
+  Solver* s = SD(solver_data);
+  std::printf("\n\nReaction data\n\nnumber of reactions: %d\n\n", s->m.n_rxn);
+  for (int r = 0; r < s->m.n_rxn; ++r) std::printf("reaction %d type %d\n", r, s->m.rint(r)[0]);
+  std::fflush(stdout);
+}
+
+extern "C" void rxn_free_update_data(void* update_data) { std::free(update_data); }
+
+static void* new_rate_payload() {
+  void* p = std::malloc(sizeof(int) + sizeof(double));
+  if (!p) camp::die("allocating space for rate update data");
+  return p;
+}
+static void set_rate_payload(void* p, int id, double rate) {
+  std::memcpy(p, &id, sizeof(int));
+  std::memcpy(static_cast<char*>(p) + sizeof(int), &rate, sizeof(double));
+}
+extern "C" void* rxn_photolysis_create_rate_update_data(void) { return new_rate_payload(); }
+extern "C" void rxn_photolysis_set_rate_update_data(void* u, int photo_id, double base_rate) {
+  set_rate_payload(u, photo_id, base_rate);
+}
+extern "C" void* rxn_emission_create_rate_update_data(void) { return new_rate_payload(); }
+extern "C" void rxn_emission_set_rate_update_data(void* u, int rxn_id, double base_rate) {
+  set_rate_payload(u, rxn_id, base_rate);
+}
+extern "C" void* rxn_first_order_loss_create_rate_update_data(void) { return new_rate_payload(); }
+extern "C" void rxn_first_order_loss_set_rate_update_data(void* u, int rxn_id, double base_rate) {
+  set_rate_payload(u, rxn_id, base_rate);
+}
+
+extern "C" void aero_phase_add_condensed_data(int, int, int*, double*, void*) {
+  camp::die("aerosol phases are not on this path");
+}
+extern "C" void aero_rep_add_condensed_data(int, int, int, int, int*, double*, void*) {
+  camp::die("aerosol representations are not on this path");
+}
+extern "C" void sub_model_add_condensed_data(int, int, int, int, int*, double*, void*) {
+  camp::die("sub models are not on this path");
+}
+
+// ---------------------------------------------------------------------------
+// Extensions
+// ---------------------------------------------------------------------------
+extern "C" int camp_solver_set_mode(void* solver_data, int mode) {
+  Solver* s = SD(solver_data);
+  int old = s->mode;
+  if (mode == CAMP_MODE_PER_CELL || mode == CAMP_MODE_BLOCK) s->mode = mode;
+  return old;
+}
+
+extern "C" int camp_solver_set_cells_per_block(void* solver_data, int cpb) {
+  Solver* s = SD(solver_data);
+  if (cpb == 0) cpb = 1;
+  if (cpb == 1 || cpb == 2 || cpb == 4) s->cells_per_block = cpb;
+  return s->cells_per_block;
+}
+
+extern "C" void camp_solver_set_guess_helper(void* solver_data, int on) { SD(solver_data)->guess_on = on != 0; }
+
+extern "C" int camp_solver_run_device(void* solver_data, double* d_state, double* d_env,
+                                      double t_initial, double t_final) {
+  Solver* s = SD(solver_data);
+  if (!s->m.initialized) return CAMP_SOLVER_FAIL;
+  s->ensure_device();
+  int rc = s->run(d_state, d_env, t_initial, t_final);
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return rc;
+}
+
+extern "C" int camp_solver_set_base_rates(void* solver_data, int rxn_type, int unique_id,
+                                          const double* rates) {
+  Solver* s = SD(solver_data);
+  int found = 0;
+  for (int c = 0; c < s->m.nc; ++c) {
+    int rid = 0;
+    found += s->m.update_data(c, &rid, rxn_type, unique_id, rates[c]);
+  }
+  s->renv_dirty = true;
+  return found == s->m.nc ? 0 : 1;
+}
+
+extern "C" int camp_solver_dims(void* solver_data, int* n_dep, int* nnz, int* lu_nnz,
+                                int* nnz_rxn) {
+  Solver* s = SD(solver_data);
+  *n_dep = s->m.prog.nv;
+  *nnz = s->m.prog.nnz;
+  *lu_nnz = s->m.prog.lu_nnz;
+  *nnz_rxn = s->m.rjac.nnz();
+  return 0;
+}
+
+extern "C" void camp_solver_get_pattern(void* solver_data, int* rxn_colptr, int* rxn_rowval,
+                                        int* colptr, int* rowval, int* deriv_ids,
+                                        int* rxn_int_data) {
+  camp::Model& m = SD(solver_data)->m;
+  std::copy(m.rjac.colptr.begin(), m.rjac.colptr.end(), rxn_colptr);
+  std::copy(m.rjac.rowval.begin(), m.rjac.rowval.end(), rxn_rowval);
+  std::copy(m.prog.colptr.begin(), m.prog.colptr.end(), colptr);
+  std::copy(m.prog.rowval.begin(), m.prog.rowval.end(), rowval);
+  std::copy(m.deriv_ids.begin(), m.deriv_ids.end(), deriv_ids);
+  std::copy(m.rxn_int.begin(), m.rxn_int.end(), rxn_int_data);
+}
+
+static void eval_prepare(Solver* s, const double* state, const double* env, double* d_y) {
+  int nc = s->m.nc;
+  const camp::DevProgram& P = s->P;
+  HIP_OK(hipMemcpyAsync(s->d_state, state, (size_t)nc * P.ns * sizeof(double),
+                        hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipMemcpyAsync(s->d_env, env, (size_t)nc * 2 * sizeof(double), hipMemcpyHostToDevice,
+                        s->stream));
+  s->upload_renv();
+  hipLaunchKernelGGL(camp::k_rates, dim3(camp::nblk((long)nc * s->m.n_rxn)), dim3(256), 0,
+                     s->stream, P, nc, s->d_env, s->d_renv);
+  // the solver variables, taken from the state as given (no clamping)
+  std::vector<double> y((size_t)nc * P.nv);
+  for (int c = 0; c < nc; ++c)
+    for (int d = 0; d < P.nv; ++d)
+      y[(size_t)c * P.nv + d] = state[(size_t)c * P.ns + s->m.prog.dep_state[d]];
+  HIP_OK(hipMemcpyAsync(d_y, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+}
+
+extern "C" int camp_eval_deriv(void* solver_data, const double* state, const double* env,
+                               double time_step, int use_deriv_est, double* deriv) {
+  Solver* s = SD(solver_data);
+  s->ensure_device();
+  int nc = s->m.nc;
+  const camp::DevProgram& P = s->P;
+  std::vector<void*> tmp;
+  double* d_y = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
+  double* d_f = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
+  eval_prepare(s, state, env, d_y);
+  HIP_OK(hipMemsetAsync(s->d_fail, 0, sizeof(int), s->stream));
+  size_t lds = 4 * (size_t)(P.ns + P.n_env) * sizeof(double);
+  hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, 4), 4096)), dim3(256), lds,
+                     s->stream, P, nc, d_y, s->d_state, s->d_renv, s->d_Js, s->d_Jy, s->d_Jf, d_f,
+                     time_step, use_deriv_est, s->d_fail);
+  int fail = 0;
+  HIP_OK(hipMemcpyAsync(&fail, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(deriv, d_f, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (void* p : tmp) hipFree(p);
+  return fail;
+}
+
+extern "C" int camp_eval_jac(void* solver_data, const double* state, const double* env,
+                             double time_step, double* jac) {
+  Solver* s = SD(solver_data);
+  s->ensure_device();
+  int nc = s->m.nc;
+  const camp::DevProgram& P = s->P;
+  std::vector<void*> tmp;
+  double* d_y = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
+  eval_prepare(s, state, env, d_y);
+  HIP_OK(hipMemsetAsync(s->d_fail, 0, sizeof(int), s->stream));
+  size_t lds = 4 * (size_t)(P.ns + P.n_env) * sizeof(double);
+  hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, 4), 4096)), dim3(256), lds,
+                     s->stream, P, nc, d_y, s->d_state, s->d_renv, s->d_Js, s->d_Jy, s->d_Jf,
+                     s->d_Jf, time_step, 0, s->d_fail);
+  hipLaunchKernelGGL(camp::k_jac, dim3(std::min(camp::nblk(nc, 4), 4096)), dim3(256), lds,
+                     s->stream, P, nc, d_y, s->d_state, s->d_renv, s->d_Js, time_step);
+  HIP_OK(hipMemcpyAsync(s->d_Jy, d_y, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToDevice,
+                        s->stream));
+  int fail = 0;
+  HIP_OK(hipMemcpyAsync(&fail, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(jac, s->d_Js, (size_t)nc * P.nnz * sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (void* p : tmp) hipFree(p);
+  return fail;
+}
+
+extern "C" int camp_eval_lu_solve(void* solver_data, const double* jac, double gamma, double* b) {
+  Solver* s = SD(solver_data);
+  s->ensure_device();
+  int nc = s->m.nc;
+  const camp::DevProgram& P = s->P;
+  std::vector<void*> tmp;
+  double* d_J = camp::dev_alloc<double>((size_t)nc * P.nnz, tmp);
+  double* d_LU = camp::dev_alloc<double>((size_t)nc * P.lu_nnz, tmp);
+  double* d_b = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
+  HIP_OK(hipMemcpyAsync(d_J, jac, (size_t)nc * P.nnz * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipMemcpyAsync(d_b, b, (size_t)nc * P.nv * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipMemsetAsync(s->d_fail, 0, sizeof(int), s->stream));
+  hipLaunchKernelGGL(camp::k_lu_factor, dim3(std::min(nc, 16384)), dim3(64),
+                     (size_t)P.lu_nnz * sizeof(double), s->stream, P, nc, d_J, gamma, d_LU, s->d_fail);
+  hipLaunchKernelGGL(camp::k_lu_solve, dim3(std::min(nc, 16384)), dim3(64),
+                     ((size_t)P.lu_nnz + 2 * (size_t)P.nv) * sizeof(double), s->stream, P, nc, d_LU,
+                     d_b, 1.0);
+  int fail = 0;
+  HIP_OK(hipMemcpyAsync(&fail, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(b, d_b, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (void* p : tmp) hipFree(p);
+  return fail;
+}
+
+extern "C" void camp_solver_get_cell_stats(void* solver_data, double* stats) {
+  Solver* s = SD(solver_data);
+  s->ensure_device();
+  HIP_OK(hipMemcpy(stats, s->d_stats, (size_t)s->m.nc * camp::kCellStats * sizeof(double),
+                   hipMemcpyDeviceToHost));
+}
+
+extern "C" double camp_solver_last_kernel_ms(void* solver_data) { return SD(solver_data)->last_kernel_ms; }
+
+extern "C" void camp_solver_work_model(void* solver_data, double* out8) {
+  const camp::Program& G = SD(solver_data)->m.prog;
+  double f = 0.0, j = 0.0;
+  for (int i = 0; i < G.nv; ++i)
+    for (int q = G.f_ptr[i]; q < G.f_ptr[i + 1]; ++q) f += G.t_nreact[G.f_term[q]] + 2;
+  f += 4.0 * G.nv + 2.0 * G.nnz + G.nv;  // blend + Jacobian-estimate matvec
+  for (int e = 0; e < G.nnz; ++e)
+    for (int q = G.j_ptr[e]; q < G.j_ptr[e + 1]; ++q) j += G.t_nreact[G.j_term[q]] + 1;
+  double lu = 2.0 * G.nnz + (double)G.l_pos.size() + 2.0 * G.up_dst.size();
+  double sv = 2.0 * (G.lu_nnz - G.nv) + G.nv;
+  double nr = G.n_terms;
+  out8[0] = f;
+  out8[1] = j;
+  out8[2] = lu;
+  out8[3] = sv;
+  out8[4] = 8.0 * (2.0 * G.nv + nr);          // f(y):  read y, k; write f
+  out8[5] = 8.0 * (G.nv + nr + G.nnz);        // J(y):  read y, k; write J
+  out8[6] = 8.0 * (G.nnz + G.lu_nnz);         // LU:    read J; write LU
+  out8[7] = 8.0 * (G.lu_nnz + 2.0 * G.nv);    // solve: read LU, b; write x
+}
+
+extern "C" int camp_solver_time_fj(void* solver_data, const double* d_state, const double* d_env,
+                                   int reps, double* ms_deriv, double* ms_jac) {
+  Solver* s = SD(solver_data);
+  s->ensure_device();
+  int nc = s->m.nc;
+  const camp::DevProgram& P = s->P;
+  std::vector<void*> tmp;
+  double* d_y = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
+  double* d_f = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
+  double* d_J = camp::dev_alloc<double>((size_t)nc * P.nnz, tmp);
+  s->upload_renv();
+  hipLaunchKernelGGL(camp::k_rates, dim3(camp::nblk((long)nc * s->m.n_rxn)), dim3(256), 0,
+                     s->stream, P, nc, d_env, s->d_renv);
+  hipLaunchKernelGGL(camp::k_state_gather, dim3(camp::nblk((long)nc * P.nv)), dim3(256), 0,
+                     s->stream, P, nc, d_state, d_y);
+  size_t lds = 4 * (size_t)(P.ns + P.n_env) * sizeof(double);
+  int grid = std::min(camp::nblk(nc, 4), 1 << 20);
+  float ms = 0.f;
+  HIP_OK(hipEventRecord(s->ev0, s->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(camp::k_deriv, dim3(grid), dim3(256), lds, s->stream, P, nc, d_y, d_state,
+                       s->d_renv, s->d_Js, s->d_Jy, s->d_Jf, d_f, 1.0, 0, s->d_fail);
+  HIP_OK(hipEventRecord(s->ev1, s->stream));
+  HIP_OK(hipEventSynchronize(s->ev1));
+  HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  *ms_deriv = ms / reps;
+  HIP_OK(hipEventRecord(s->ev0, s->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(camp::k_jac, dim3(grid), dim3(256), lds, s->stream, P, nc, d_y, d_state,
+                       s->d_renv, d_J, 1.0);
+  HIP_OK(hipEventRecord(s->ev1, s->stream));
+  HIP_OK(hipEventSynchronize(s->ev1));
+  HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  *ms_jac = ms / reps;
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (void* p : tmp) hipFree(p);
+  return 0;
+}
+
+extern "C" void camp_solver_get_rxn_env(void* solver_data, double* rxn_env) {
+  Solver* s = SD(solver_data);
+  s->ensure_device();
+  HIP_OK(hipMemcpy(rxn_env, s->d_renv, s->m.rxn_env.size() * sizeof(double),
+                   hipMemcpyDeviceToHost));
+}

@@ -1,0 +1,194 @@
+/* C ABI of the MI355X-native CAMP chemistry solver (libcamp_mi355x.so).
+ *
+ * The first block of entry points is the drop-in boundary: the symbols the
+ * reference's Fortran layer binds with bind(c) (src/camp_solver_data.F90:
+ * 43-363), with the reference signatures, argument meaning and error
+ * convention (setup errors print and exit(EXIT_FAILURE); solver_run
+ * returns 0 = CAMP_SOLVER_SUCCESS or 1 = CAMP_SOLVER_FAIL).  The state array
+ * is n_state_var * n_cells doubles, cell-major; env is 2 * n_cells doubles
+ * [temperature K, pressure Pa] per cell; species ids inside the packed int
+ * parameters are 1-based.
+ *
+ * The second block has no counterpart in the reference: mode selection,
+ * device-resident runs, bulk rate updates and the test/inspection hooks.
+ */
+#ifndef CAMP_MI355X_SOLVER_H
+#define CAMP_MI355X_SOLVER_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAMP_SOLVER_SUCCESS 0 /* camp_solver.c:49 */
+#define CAMP_SOLVER_FAIL 1    /* camp_solver.c:50 */
+
+/* ---- drop-in boundary ---------------------------------------------------- */
+
+/* replaces solver_new, src/camp_solver.c:83-362 (camp_solver.h:16-24) */
+void *solver_new(int n_state_var, int n_cells, int *var_type, int n_rxn,
+                 int n_rxn_int_param, int n_rxn_float_param,
+                 int n_rxn_env_param, int n_aero_phase,
+                 int n_aero_phase_int_param, int n_aero_phase_float_param,
+                 int n_aero_rep, int n_aero_rep_int_param,
+                 int n_aero_rep_float_param, int n_aero_rep_env_param,
+                 int n_sub_model, int n_sub_model_int_param,
+                 int n_sub_model_float_param, int n_sub_model_env_param);
+
+/* replaces solver_initialize, src/camp_solver.c:374-494 */
+void solver_initialize(void *solver_data, double *abs_tol, double rel_tol,
+                       int max_steps, int max_conv_fails);
+
+/* replaces solver_run, src/camp_solver.c:544-693 */
+int solver_run(void *solver_data, double *state, double *env, double t_initial,
+               double t_final);
+
+/* replaces solver_get_statistics, src/camp_solver.c:725-791 */
+void solver_get_statistics(void *solver_data, int *solver_flag, int *num_steps,
+                           int *RHS_evals, int *LS_setups,
+                           int *error_test_fails, int *NLS_iters,
+                           int *NLS_convergence_fails, int *DLS_Jac_evals,
+                           int *DLS_RHS_evals, double *last_time_step__s,
+                           double *next_time_step__s, int *Jac_eval_fails,
+                           int *RHS_evals_total, int *Jac_evals_total,
+                           double *RHS_time__s, double *Jac_time__s,
+                           double *max_loss_precision);
+
+/* replaces solver_reset_timers, src/camp_solver.c:1764-1775 */
+void solver_reset_timers(void *solver_data);
+
+/* replaces solver_free, src/camp_solver.c:1832-1873 */
+void solver_free(void *solver_data);
+
+/* replaces rxn_add_condensed_data, src/rxn_solver.c:658-693 */
+void rxn_add_condensed_data(int rxn_type, int n_int_param, int n_float_param,
+                            int n_env_param, int *int_param,
+                            double *float_param, void *solver_data);
+
+/* replaces rxn_update_data, src/rxn_solver.c:708-763; update_data is the
+ * {int id; double rate} payload of the *_create_rate_update_data helpers */
+void rxn_update_data(int cell_id, int *rxn_id, int update_rxn_type,
+                     void *update_data, void *solver_data);
+
+/* replaces rxn_print_data, src/rxn_solver.c:769-845 */
+void rxn_print_data(void *solver_data);
+
+/* replaces rxn_free_update_data, src/rxn_solver.c:851 */
+void rxn_free_update_data(void *update_data);
+
+/* replace the bind(c) helpers of rxn_photolysis.F90:135-155,
+ * rxn_emission.F90:118-138 and rxn_first_order_loss.F90:121-141, whose C
+ * bodies are rxn_photolysis.c:262-283, rxn_emission.c:192-213 and
+ * rxn_first_order_loss.c:208-229 */
+void *rxn_photolysis_create_rate_update_data(void);
+void rxn_photolysis_set_rate_update_data(void *update_data, int photo_id,
+                                         double base_rate);
+void *rxn_emission_create_rate_update_data(void);
+void rxn_emission_set_rate_update_data(void *update_data, int rxn_id,
+                                       double base_rate);
+void *rxn_first_order_loss_create_rate_update_data(void);
+void rxn_first_order_loss_set_rate_update_data(void *update_data, int rxn_id,
+                                               double base_rate);
+
+/* replace aero_phase_add_condensed_data (aero_phase_solver.c:175),
+ * aero_rep_add_condensed_data (aero_rep_solver.c:443) and
+ * sub_model_add_condensed_data (sub_model_solver.c:413), bound at
+ * camp_solver_data.F90:201-354.  The gas-phase path accepts none: a call
+ * prints an error and exits, the reference's setup-error convention. */
+void aero_phase_add_condensed_data(int n_int_param, int n_float_param,
+                                   int *int_param, double *float_param,
+                                   void *solver_data);
+void aero_rep_add_condensed_data(int aero_rep_type, int n_int_param,
+                                 int n_float_param, int n_env_param,
+                                 int *int_param, double *float_param,
+                                 void *solver_data);
+void sub_model_add_condensed_data(int sub_model_type, int n_int_param,
+                                  int n_float_param, int n_env_param,
+                                  int *int_param, double *float_param,
+                                  void *solver_data);
+
+/* ---- extensions (no counterpart in the reference) ------------------------ */
+
+#define CAMP_MODE_PER_CELL 0 /* each cell its own BDF control (default) */
+#define CAMP_MODE_BLOCK 1    /* one BDF over all cells, launch per f/J/LU/solve */
+
+/* Select the integration mode; returns the previous mode. */
+int camp_solver_set_mode(void *solver_data, int mode);
+
+/* Cells per workgroup of the persistent kernel (1, 2, 4; one wavefront per
+ * cell); 0 selects the built-in default.  Returns the value in effect. */
+int camp_solver_set_cells_per_block(void *solver_data, int cells_per_block);
+
+/* Enable (1, default) or disable (0) the negative-prediction guess helper
+ * (camp_solver.c:1321-1408). */
+void camp_solver_set_guess_helper(void *solver_data, int on);
+
+/* solver_run on device-resident arrays (HIP device pointers, same layout
+ * as solver_run's host arrays); ordered on the solver's stream and
+ * synchronised before returning. */
+int camp_solver_run_device(void *solver_data, double *d_state, double *d_env,
+                           double t_initial, double t_final);
+
+/* Set the base rate of updatable reaction `unique_id` (photolysis,
+ * emission, first-order loss) for every cell from rates[n_cells]; the bulk
+ * equivalent of one rxn_update_data call per cell. */
+int camp_solver_set_base_rates(void *solver_data, int rxn_type, int unique_id,
+                               const double *rates);
+
+/* Number of solver variables, solver-Jacobian nonzeros, LU nonzeros and
+ * reaction-Jacobian nonzeros per cell (after solver_initialize). */
+int camp_solver_dims(void *solver_data, int *n_dep, int *nnz, int *lu_nnz,
+                     int *nnz_rxn);
+
+/* Copies of the CAMP sparsity pattern and ids built by solver_initialize:
+ * reaction Jacobian (CSC over state variables), per-cell solver Jacobian
+ * (CSC over solver variables), deriv_ids[n_state], and the packed int
+ * parameters after the id update (n_rxn_int_param + n_rxn entries). */
+void camp_solver_get_pattern(void *solver_data, int *rxn_colptr,
+                             int *rxn_rowval, int *colptr, int *rowval,
+                             int *deriv_ids, int *rxn_int_data);
+
+/* Test hooks: f(y) and J(y) of all cells on the GPU for the given host state
+ * (n_cells x n_state_var) and env; rate constants are computed first.
+ * deriv is n_cells x n_dep, jac n_cells x nnz (CSC element order).
+ * use_deriv_est selects the Jacobian-extrapolated blend against the stored
+ * J history; camp_eval_jac also stores the history, like Jac().  Return 0,
+ * or 1 when a cell has y < -1e-30. */
+int camp_eval_deriv(void *solver_data, const double *state, const double *env,
+                    double time_step, int use_deriv_est, double *deriv);
+int camp_eval_jac(void *solver_data, const double *state, const double *env,
+                  double time_step, double *jac);
+/* Batched LU solve test hook: factor I - gamma*J per cell (jac n_cells x
+ * nnz) and solve for b (n_cells x n_dep) in place. Returns 0 or 1. */
+int camp_eval_lu_solve(void *solver_data, const double *jac, double gamma,
+                       double *b);
+
+/* Per-cell statistics of the last per-cell run: n_cells x 10 doubles
+ * [num_steps, RHS_evals, LS_setups, error_test_fails, NLS_iters,
+ *  NLS_convergence_fails, Jac_evals, flag, last_h, next_h]. */
+void camp_solver_get_cell_stats(void *solver_data, double *stats);
+
+/* Duration (ms) of the last persistent-kernel launch, from HIP events on
+ * the launch stream. */
+double camp_solver_last_kernel_ms(void *solver_data);
+
+/* Algorithmic work per cell and evaluation, for roofline accounting:
+ * out8 = [flops of f(y) (with the Jacobian-estimate blend), of J(y), of one
+ * LU factorisation, of one triangular solve, then the HBM bytes the same
+ * four operations must move when run as separate kernels]. */
+void camp_solver_work_model(void *solver_data, double *out8);
+
+/* Measurement hook: average duration (ms, HIP events on the solver's
+ * stream) of the plain-path f(y) and J(y) kernels over all cells for the
+ * device-resident state/env, each launched `reps` times. */
+int camp_solver_time_fj(void *solver_data, const double *d_state,
+                        const double *d_env, int reps, double *ms_deriv,
+                        double *ms_jac);
+
+/* Copy of the device-side per-cell reaction environment data (n_cells x
+ * n_rxn_env_param: rate constants and base rates) as last computed. */
+void camp_solver_get_rxn_env(void *solver_data, double *rxn_env);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

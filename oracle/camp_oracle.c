@@ -122,7 +122,7 @@ typedef struct {
   int *rxn_int, *rxn_int_idx, *rxn_float_idx, *rxn_env_idx, n_env;
   double *rxn_float, *rxn_env;
   double *abs_tol, rtol;
-  int max_steps, max_conv_fails, initialized, guess_helper_on;
+  int max_steps, max_conv_fails, initialized, guess_helper_on, neg_mode;
   Csc rjac, pjac, sjac;          /* rxn, param, solver Jacobians (state ids) */
   int nnz, *colptr, *rowval;     /* per-cell solver pattern (dep ids)   */
   int n_mapped, *map_solver, *map_rxn, *map_param;
@@ -161,6 +161,7 @@ void *oracle_new(int n_state_var, int n_cells, const int *var_type, int n_rxn,
   o->n_int_total = n_rxn_int_param + n_rxn;
   o->n_float_total = n_rxn_float_param;
   o->guess_helper_on = 1;
+  o->neg_mode = 1;
   return o;
 }
 
@@ -1033,6 +1034,15 @@ static int cv_newton_iteration(Cv *cv) {
     if (m > 0) cv->crate = fmax(CRDOWN * cv->crate, del / delp);
     dcon = del * fmin(1.0, cv->crate) / cv->tq[4];
     if (dcon <= 1.0) {
+      /* Non-negativity safeguard (not in CVODE): CAMP's f() rejects
+       * y < -SMALL (camp_solver.c:816-822), so a converged iterate below
+       * that would make every later f() at the prediction fail; it is
+       * treated as the recoverable f() failure it implies. */
+      if (cv->o->neg_mode == 1) {
+        double mn = cv->y[0];
+        for (int i = 1; i < cv->n; ++i) mn = cv->y[i] < mn ? cv->y[i] : mn;
+        if (mn < -SMALL) return cv->jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+      }
       cv->acnrm = (m == 0) ? del : wrms(cv, cv->acor);
       cv->jcur = 0;
       return CV_SUCCESS;
@@ -1436,6 +1446,7 @@ void oracle_get_stats(void *om, long *counters8, double *h2) {
 }
 
 void oracle_set_guess_helper(void *om, int on) { ((Oracle *)om)->guess_helper_on = on; }
+void oracle_set_neg_mode(void *om, int mode) { ((Oracle *)om)->neg_mode = mode; }
 
 /* --------------------------------------------------------------------- */
 /* Direct f/J evaluation for parity tests                                 */
@@ -1575,4 +1586,65 @@ int oracle_lu_solve_dense(void *om, const double *jac, double gamma,
   }
   free(A);
   return 0;
+}
+
+/* --------------------------------------------------------------------- */
+/* Known-answer hooks for the pattern builder and the accumulators        */
+/* --------------------------------------------------------------------- */
+int oracle_kat_pattern(int n, int n_pairs, const int *dep, const int *ind,
+                       int *colptr, int *rowval) {
+  ColElems c;
+  cols_init(&c, n);
+  for (int k = 0; k < n_pairs; ++k) cols_register(&c, dep[k], ind[k]);
+  Csc m = cols_build(&c);
+  memcpy(colptr, m.colptr, (n + 1) * sizeof(int));
+  memcpy(rowval, m.rowval, m.nnz * sizeof(int));
+  int nnz = m.nnz;
+  free(m.colptr);
+  free(m.rowval);
+  return nnz;
+}
+
+/* jacobian_add_value / jacobian_output (Jacobian.c:223-241): flag 0 adds to
+ * the production sum, 1 to the loss sum; out = (double)(prod - loss). */
+void oracle_kat_jac_accum(int nnz, int n_adds, const int *elem, const int *flag,
+                          const double *val, double *out) {
+  long double *p = xcalloc(nnz, sizeof(long double));
+  long double *l = xcalloc(nnz, sizeof(long double));
+  for (int k = 0; k < n_adds; ++k) {
+    if (flag[k] == 0) p[elem[k]] += val[k];
+    if (flag[k] == 1) l[elem[k]] += val[k];
+  }
+  for (int e = 0; e < nnz; ++e) out[e] = p[e] - l[e];
+  free(p);
+  free(l);
+}
+
+/* time_derivative_add_value / time_derivative_output with the oracle's
+ * accumulators (time_derivative.c:46-97). */
+void oracle_kat_time_derivative(int n_spec, int n_contrib, const int *spec,
+                                const double *contrib, const double *est,
+                                double *out) {
+  Oracle o;
+  memset(&o, 0, sizeof o);
+  o.nv = n_spec;
+  o.td_p = xcalloc(n_spec, sizeof(long double));
+  o.td_l = xcalloc(n_spec, sizeof(long double));
+  for (int k = 0; k < n_contrib; ++k) td_add(&o, spec[k], (long double)contrib[k]);
+  for (int i = 0; i < n_spec; ++i) {
+    long double rp = o.td_p[i], rl = o.td_l[i];
+    if (rp + rl != 0.0) {
+      if (est) {
+        long double sf = 1.0 / (rp + rl) /
+                         (1.0 / (rp + rl) + MAX_PRECISION_LOSS / fabsl(rp - rl));
+        out[i] = sf * (rp - rl) + (1.0 - sf) * est[i];
+      } else {
+        out[i] = rp - rl;
+      }
+    } else {
+      out[i] = 0.0;
+    }
+  }
+  free(o.td_p);
+  free(o.td_l);
 }

@@ -89,6 +89,9 @@ int oracle_solver_run_cells(void *om, double *state, const double *env,
  * and next step size of the last problem solved. */
 void oracle_get_stats(void *om, long *counters8, double *h2);
 void oracle_set_guess_helper(void *om, int on);
+/* 1 (default): reject a converged Newton iterate with y < -SMALL as a
+ * recoverable f() failure; 0: accept it (plain CVODE behaviour). */
+void oracle_set_neg_mode(void *om, int mode);
 
 /* Linear-algebra checks used by tests: factor M = I - gamma*J (solver
  * pattern, one cell) with the oracle's sparse LU and with dense partial
@@ -98,6 +101,17 @@ int oracle_lu_solve_sparse(void *om, const double *jac, double gamma,
 int oracle_lu_solve_dense(void *om, const double *jac, double gamma,
                           const double *b, double *x);
 int oracle_lu_nnz(void *om);
+
+/* Known-answer hooks: the pattern builder on explicit (dep, ind)
+ * registrations (returns nnz), the long-double Jacobian accumulators, and
+ * the time-derivative accumulators + output. */
+int oracle_kat_pattern(int n, int n_pairs, const int *dep, const int *ind,
+                       int *colptr, int *rowval);
+void oracle_kat_jac_accum(int nnz, int n_adds, const int *elem, const int *flag,
+                          const double *val, double *out);
+void oracle_kat_time_derivative(int n_spec, int n_contrib, const int *spec,
+                                const double *contrib, const double *est,
+                                double *out);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,11 @@ def lib():
         L.oracle_lu_solve_sparse.restype = _i
         L.oracle_lu_solve_dense.argtypes = [_p, _dp, _d, _dp, _dp]
         L.oracle_lu_solve_dense.restype = _i
+        L.oracle_set_neg_mode.argtypes = [_p, _i]
+        L.oracle_kat_pattern.argtypes = [_i, _i, _ip, _ip, _ip, _ip]
+        L.oracle_kat_pattern.restype = _i
+        L.oracle_kat_jac_accum.argtypes = [_i, _i, _ip, _ip, _dp, _dp]
+        L.oracle_kat_time_derivative.argtypes = [_i, _i, _ip, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -300,4 +305,31 @@ def ref_time_derivative(n_spec, spec, contrib, est=None):
     e = None if est is None else np.ascontiguousarray(est, dtype=np.float64)
     reflib().ref_time_derivative(n_spec, len(spec), _ip_(spec), _dp_(contrib),
                                  None if e is None else _dp_(e), _dp_(out))
+    return out
+
+
+def kat_pattern(n, pairs):
+    dep = np.ascontiguousarray([p[0] for p in pairs], dtype=np.int32)
+    ind = np.ascontiguousarray([p[1] for p in pairs], dtype=np.int32)
+    cp = np.zeros(n + 1, np.int32)
+    rv = np.zeros(max(len(pairs), 1), np.int32)
+    nnz = lib().oracle_kat_pattern(n, len(pairs), _ip_(dep), _ip_(ind), _ip_(cp), _ip_(rv))
+    return cp, rv[:nnz]
+
+
+def kat_jac_accum(nnz, adds):
+    elem = np.ascontiguousarray([a[0] for a in adds], dtype=np.int32)
+    flag = np.ascontiguousarray([a[1] for a in adds], dtype=np.int32)
+    val = np.ascontiguousarray([a[2] for a in adds], dtype=np.float64)
+    out = np.zeros(nnz)
+    lib().oracle_kat_jac_accum(nnz, len(adds), _ip_(elem), _ip_(flag), _dp_(val), _dp_(out))
+    return out
+
+
+def kat_time_derivative(n_spec, spec, contrib, est=None):
+    spec = np.ascontiguousarray(spec, dtype=np.int32)
+    contrib = np.ascontiguousarray(contrib, dtype=np.float64)
+    out = np.zeros(n_spec)
+    e = None if est is None else _dp_(np.ascontiguousarray(est, dtype=np.float64))
+    lib().oracle_kat_time_derivative(n_spec, len(spec), _ip_(spec), _dp_(contrib), e, _dp_(out))
     return out

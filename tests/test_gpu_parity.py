@@ -1,0 +1,164 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: rate constants, f(y) and J(y) within 1e-12 relative; LU solves to
+1e-10 relative (normwise); final concentrations within 1e-4 relative,
+atol 1e-8 ppm per species per cell.
+"""
+import numpy as np
+import pytest
+
+from camp_amd.core import CampSolver, MODE_BLOCK, MODE_PER_CELL
+from camp_amd.mechanisms import cb05_scale_mechanism, synthetic_cells, box_mechanism
+from tests.helpers import cb05_inputs, box_inputs
+
+pytestmark = pytest.mark.gpu
+
+F_RTOL = 1e-12
+SOLVE_RTOL = 1e-4
+SOLVE_ATOL = 1e-8
+
+
+def _random_states(mech, n, seed=7):
+    rng = np.random.default_rng(seed)
+    st = np.exp(rng.uniform(np.log(1e-6), np.log(1e2), size=(n, mech.n_state)))
+    st[1, :5] = 0.0            # exact zeros
+    st[2, 5:9] = 1e-40          # tiny values
+    return st
+
+
+def _set_rates_both(sol, orc, rates):
+    sol.set_rates(rates)
+    orc.set_rates(rates)
+
+
+def test_rate_constants_match_oracle(oracle_lib):
+    mech, cells = cb05_inputs(256)
+    sol = CampSolver(mech, 256)
+    orc = oracle_lib.Oracle(mech, 256)
+    _set_rates_both(sol, orc, cells.rates)
+    rc, _ = sol.eval_deriv(cells.state, cells.env)
+    assert rc == 0
+    orc.update_env(cells.env)
+    a, b = sol.rxn_env(), orc.rxn_env()
+    nz = b != 0
+    assert np.array_equal(a == 0, b == 0)
+    assert np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz])) <= F_RTOL
+
+
+@pytest.mark.parametrize("use_est", [False, True])
+def test_deriv_matches_oracle(oracle_lib, use_est):
+    n = 128
+    mech, cells = cb05_inputs(n)
+    sol = CampSolver(mech, n)
+    orc = oracle_lib.Oracle(mech, n)
+    _set_rates_both(sol, orc, cells.rates)
+    st = _random_states(mech, n)
+    if use_est:
+        # J history at a nearby state, then f at the state with the blend
+        st0 = st * 1.01
+        assert sol.eval_jac(st0, cells.env, 10.0)[0] == 0
+        orc.update_env(cells.env)
+        for c in range(n):
+            orc.eval_jac(c, st0[c], 10.0)
+    rc, f_gpu = sol.eval_deriv(st, cells.env, 10.0, use_est)
+    assert rc == 0
+    orc.update_env(cells.env)
+    f_ref = np.stack([orc.eval_f(c, st[c], 10.0, use_est)[1] for c in range(n)])
+    err = np.abs(f_gpu - f_ref)
+    scale = np.abs(f_ref)
+    bad = err > F_RTOL * scale
+    assert not bad.any(), "max rel err %g" % np.max(err / np.maximum(scale, 1e-300))
+
+
+def test_jac_matches_oracle(oracle_lib):
+    n = 128
+    mech, cells = cb05_inputs(n)
+    sol = CampSolver(mech, n)
+    orc = oracle_lib.Oracle(mech, n)
+    _set_rates_both(sol, orc, cells.rates)
+    st = _random_states(mech, n)
+    rc, j_gpu = sol.eval_jac(st, cells.env, 10.0)
+    assert rc == 0
+    orc.update_env(cells.env)
+    j_ref = np.stack([orc.eval_jac(c, st[c], 10.0)[1] for c in range(n)])
+    err = np.abs(j_gpu - j_ref)
+    assert not (err > F_RTOL * np.abs(j_ref)).any(), "max abs err %g" % err.max()
+
+
+def test_negative_state_fails_like_reference(oracle_lib):
+    mech, cells = cb05_inputs(4)
+    sol = CampSolver(mech, 4)
+    st = cells.state.copy()
+    st[2, 3] = -1e-20
+    rc, _ = sol.eval_deriv(st, cells.env)
+    assert rc == 1
+    orc = oracle_lib.Oracle(mech, 4)
+    assert orc.eval_f(2, st[2])[0] == 1
+
+
+def test_lu_solve_matches_dense_oracle(oracle_lib):
+    n = 64
+    mech, cells = cb05_inputs(n)
+    sol = CampSolver(mech, n)
+    orc = oracle_lib.Oracle(mech, n)
+    _set_rates_both(sol, orc, cells.rates)
+    rc, jac = sol.eval_jac(cells.state, cells.env, 10.0)
+    assert rc == 0
+    rng = np.random.default_rng(3)
+    b = rng.normal(size=(n, sol.nv))
+    for gamma in (0.1, 10.0):
+        rc, x = sol.eval_lu_solve(jac, gamma, b)
+        assert rc == 0
+        for c in range(n):
+            rc2, xr = orc.lu_solve(jac[c], gamma, b[c], dense=True)
+            assert rc2 == 0
+            assert np.linalg.norm(x[c] - xr) <= 1e-10 * np.linalg.norm(xr)
+
+
+def _compare_final(a, b):
+    ok = np.abs(a - b) <= SOLVE_RTOL * np.abs(b) + SOLVE_ATOL
+    return ok.all(), float(np.max(np.abs(a - b) / (np.abs(b) + SOLVE_ATOL)))
+
+
+def test_box_model_per_cell_matches_oracle(oracle_lib):
+    mech, cells = box_inputs(1)
+    sol = CampSolver(mech, 1)
+    orc = oracle_lib.Oracle(mech, 1)
+    a = cells.state.copy()
+    b = cells.state.copy()
+    for _ in range(60):
+        assert sol.solve(a, cells.env, 0.0, 60.0) == 0
+        assert orc.run(b, cells.env, 0.0, 60.0, mode=1) == 0
+    ok, worst = _compare_final(a, b)
+    assert ok, worst
+
+
+@pytest.mark.parametrize("mode", [MODE_PER_CELL, MODE_BLOCK])
+def test_cb05_solve_matches_oracle(oracle_lib, mode):
+    n = 32 if mode == MODE_PER_CELL else 8
+    mech, cells = cb05_inputs(n)
+    sol = CampSolver(mech, n, mode=mode)
+    orc = oracle_lib.Oracle(mech, n)
+    _set_rates_both(sol, orc, cells.rates)
+    a = cells.state.copy()
+    b = cells.state.copy()
+    for _ in range(2):
+        assert sol.solve(a, cells.env, 0.0, 180.0) == 0
+        assert orc.run(b, cells.env, 0.0, 180.0, mode=0 if mode == MODE_BLOCK else 1) == 0
+    ok, worst = _compare_final(a, b)
+    print("mode", mode, "worst scaled diff", worst, sol.stats(), orc.stats())
+    assert ok, worst
+
+
+def test_cell_alone_equals_cell_in_mix():
+    mech, cells = cb05_inputs(16, night_fraction=0.5, stressed_fraction=0.2)
+    mix = CampSolver(mech, 16)
+    mix.set_rates(cells.rates)
+    a = cells.state.copy()
+    assert mix.solve(a, cells.env, 0.0, 180.0) == 0
+    for c in (0, 5, 11):
+        one = CampSolver(mech, 1)
+        one.set_rates({k: v[c:c + 1] for k, v in cells.rates.items()})
+        s1 = cells.state[c:c + 1].copy()
+        assert one.solve(s1, cells.env[c:c + 1], 0.0, 180.0) == 0
+        assert np.array_equal(s1[0], a[c])

@@ -1,0 +1,100 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every declared symbol, solver_initialize builds CAMP's pattern and ids
+bit-exactly on the host, the packer follows the reference's Fortran
+packers, and setup errors follow the reference's convention."""
+import json
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from camp_amd.mechanisms import (CONV, box_mechanism, cb05_scale_mechanism, load_camp_json, pack,
+                                 synthetic_cells, RXN_TROE)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "camp_solver.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", text)) - {"void", "int", "double"})
+
+
+def test_library_exports_every_declared_symbol():
+    from camp_amd import _lib
+    L = _lib.lib()
+    names = declared_functions()
+    assert "solver_run" in names and "rxn_add_condensed_data" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+@pytest.mark.parametrize("mech", [box_mechanism(1), cb05_scale_mechanism(1),
+                                  load_camp_json(json.load(open(os.path.join(
+                                      HERE, "golden", "rxn_arrhenius.json"))))],
+                         ids=["box", "cb05", "arrhenius_json"])
+def test_product_pattern_and_ids_match_oracle(oracle_lib, mech):
+    from camp_amd.core import CampSolver
+    s = CampSolver(mech, 3)          # solver_initialize runs on the host
+    o = oracle_lib.Oracle(mech, 3)
+    ps, po = s.pattern(), o.pattern()
+    for k in ("rxn_colptr", "rxn_rowval", "colptr", "rowval", "deriv_ids"):
+        assert np.array_equal(ps[k], po[k]), k
+    assert np.array_equal(ps["rxn_int_data"], o.int_data())
+    # invariants of the CAMP pattern: sorted rows per column, full diagonal
+    cp, rv = ps["colptr"], ps["rowval"]
+    for j in range(s.nv):
+        col = rv[cp[j]:cp[j + 1]]
+        assert np.all(np.diff(col) > 0) and j in col
+
+
+def test_cb05_scale_dimensions():
+    from camp_amd.core import CampSolver
+    s = CampSolver(cb05_scale_mechanism(1), 2)
+    assert (s.nv, s.ns) == (72, 72)
+    assert 600 < s.nnz < 900
+    assert s.lu_nnz >= s.nnz
+    types = [r.type for r in cb05_scale_mechanism(1).reactions]
+    assert {t: types.count(t) for t in set(types)} == {
+        "ARRHENIUS": 133, "TROE": 28, "PHOTOLYSIS": 19, "EMISSION": 5, "FIRST_ORDER_LOSS": 5}
+
+
+def test_troe_packing_follows_fortran_packer():
+    """rxn_troe.F90:196-245: k0_A x 1e6, time unit MIN -> SCALING 1/60, CONV."""
+    mech = load_camp_json(json.load(open(os.path.join(HERE, "golden", "test_troe.json"))))
+    (t0, i0, f0, e0), (t1, i1, f1, e1) = pack(mech)
+    assert t0 == RXN_TROE and e0 == 1
+    assert i0 == [1, 1, 1, 2, 0, 0, 0, 0]
+    assert f0[0] == 4.0e-18 * 1.0e6 and f0[6] == 0.6 and f0[7] == 1.0 and f0[8] == 1.0
+    assert f0[9] == CONV
+    assert f1[:9] == [1.2e-12 * 1.0e6, 167, 3, 136, 5, 24, 0.9, 0.8, 1.0 / 60.0]
+
+
+def test_seeded_inputs_are_reproducible_and_batch_independent():
+    mech = cb05_scale_mechanism(1)
+    a = synthetic_cells(mech, 50, seed=2, night_fraction=0.5, stressed_fraction=0.1)
+    b = synthetic_cells(mech, 10, seed=2, cell_offset=20, night_fraction=0.5,
+                        stressed_fraction=0.1)
+    assert np.array_equal(a.state[20:30], b.state)
+    assert np.array_equal(a.env[20:30], b.env)
+    for k in a.rates:
+        assert np.array_equal(a.rates[k][20:30], b.rates[k])
+    assert cb05_scale_mechanism(1).reactions[5].params == cb05_scale_mechanism(1).reactions[5].params
+
+
+def test_setup_error_exits_like_the_reference():
+    """camp_solver.c:93-96: setup errors print and exit(EXIT_FAILURE)."""
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from camp_amd._lib import lib\n"
+            "import numpy as np, ctypes as C\n"
+            "vt = np.ones(3, np.int32)\n"
+            "lib().solver_new(3, 1, vt.ctypes.data_as(C.POINTER(C.c_int)), 0, 0, 0, 0,"
+            " 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    assert "ERROR" in r.stderr
