@@ -45,16 +45,18 @@ __device__ __forceinline__ double rpoweri(double b, int e) {
 struct CellCv {
   DevProgram P;
   int lane, nv;
-  double *s_state, *s_k, *zn[6], *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2,
+  double *s_state, *s_k, *znb, *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2,
       *w, *lu;
   double *Js, *Jy, *Jf;
   double tn, h, hscale, hprime, next_h, eta, etamax, etaq, etaqm1, etaqp1, hu, init_h, rtol;
-  double l[6], tq[6], tau[8];
+  double *l, *tq, *tau;  // small coefficient arrays, in LDS (uniform values)
   double rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
   int q, qprime, L, qwait, jcur, use_acinit, guess_on, maxcf, max_steps;
   int nst, nfe, nsetups, nni, ncfn, netf, nje, nstlp, nstlj;
 
-  __device__ double wrms(const double* x) {
+  __device__ __forceinline__ double* zn(int j) const { return znb + (size_t)j * nv; }
+
+  __device__ __forceinline__ double wrms(const double* x) {
     double s = 0.0;
     for (int i = lane; i < nv; i += kWave) {
       double p = x[i] * ewt[i];
@@ -64,14 +66,14 @@ struct CellCv {
     return sqrt(s / nv);
   }
 
-  __device__ void ewt_set(const double* ycur) {
+  __device__ __forceinline__ void ewt_set(const double* ycur) {
     for (int i = lane; i < nv; i += kWave)
       ewt[i] = 1.0 / (rtol * fabs(ycur[i]) + P.abs_tol_dep[i]);
     wsync();
   }
 
   // CAMP f(): camp_solver.c:843-958 for one cell (use_deriv_est = 1)
-  __device__ int f(const double* yv, double* out) {
+  __device__ __forceinline__ int f(const double* yv, double* out) {
     nfe++;
     double ts = next_h > 0.0 ? next_h : init_h;
     if (cell_set_state(P, lane, yv, s_state)) return 1;
@@ -81,7 +83,7 @@ struct CellCv {
   }
 
   // CAMP Jac(): pure f into fy, J into the cell's J_solver, save history
-  __device__ int jac(const double* yv, double* fy) {
+  __device__ __forceinline__ int jac(const double* yv, double* fy) {
     double ts = next_h > 0.0 ? next_h : init_h;
     if (cell_set_state(P, lane, yv, s_state)) return 1;
     cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy);
@@ -95,10 +97,10 @@ struct CellCv {
     return 0;
   }
 
-  __device__ void rescale() {
+  __device__ __forceinline__ void rescale() {
     double factor = eta;
     for (int j = 1; j <= q; ++j) {
-      for (int i = lane; i < nv; i += kWave) zn[j][i] *= factor;
+      for (int i = lane; i < nv; i += kWave) zn(j)[i] *= factor;
       factor *= eta;
     }
     h = hscale * eta;
@@ -107,23 +109,23 @@ struct CellCv {
     wsync();
   }
 
-  __device__ void predict() {
+  __device__ __forceinline__ void predict() {
     tn += h;
     for (int i = lane; i < nv; i += kWave)
       for (int k = 1; k <= q; ++k)
-        for (int j = q; j >= k; --j) zn[j - 1][i] += zn[j][i];
+        for (int j = q; j >= k; --j) zn(j - 1)[i] += zn(j)[i];
     wsync();
   }
 
-  __device__ void restore(double saved_t) {
+  __device__ __forceinline__ void restore(double saved_t) {
     tn = saved_t;
     for (int i = lane; i < nv; i += kWave)
       for (int k = 1; k <= q; ++k)
-        for (int j = q; j >= k; --j) zn[j - 1][i] -= zn[j][i];
+        for (int j = q; j >= k; --j) zn(j - 1)[i] -= zn(j)[i];
     wsync();
   }
 
-  __device__ void set_tq(double hsum, double alpha0, double alpha0_hat, double xi_inv,
+  __device__ __forceinline__ void set_tq(double hsum, double alpha0, double alpha0_hat, double xi_inv,
                          double xistar_inv) {
     double A1 = 1.0 - alpha0_hat + alpha0;
     double A2 = 1.0 + q * A1;
@@ -149,7 +151,7 @@ struct CellCv {
     tq[4] = NLSCOEF / tq[2];
   }
 
-  __device__ void set() {
+  __device__ __forceinline__ void set() {
     double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
     l[0] = l[1] = xi_inv = xistar_inv = 1.0;
     for (int z = 2; z <= q; ++z) l[z] = 0.0;
@@ -176,7 +178,7 @@ struct CellCv {
     gamrat = (nst > 0) ? gamma / gammap : 1.0;
   }
 
-  __device__ void increase_bdf() {
+  __device__ __forceinline__ void increase_bdf() {
     double alpha0, alpha1, prod, xi, xiold, hsum, A1;
     for (int j = 0; j <= QMAX; ++j) l[j] = 0.0;
     l[2] = alpha1 = prod = xiold = 1.0;
@@ -195,14 +197,14 @@ struct CellCv {
     }
     A1 = (-alpha0 - alpha1) / prod;
     for (int i = lane; i < nv; i += kWave) {
-      double v = A1 * zn[QMAX][i];
-      zn[L][i] = v;
-      for (int j = 2; j <= q; ++j) zn[j][i] += l[j] * v;
+      double v = A1 * zn(QMAX)[i];
+      zn(L)[i] = v;
+      for (int j = 2; j <= q; ++j) zn(j)[i] += l[j] * v;
     }
     wsync();
   }
 
-  __device__ void decrease_bdf() {
+  __device__ __forceinline__ void decrease_bdf() {
     double hsum, xi;
     for (int j = 0; j <= QMAX; ++j) l[j] = 0.0;
     l[2] = 1.0;
@@ -213,11 +215,11 @@ struct CellCv {
       for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xi + l[i - 1];
     }
     for (int i = lane; i < nv; i += kWave)
-      for (int j = 2; j < q; ++j) zn[j][i] += (-l[j]) * zn[q][i];
+      for (int j = 2; j < q; ++j) zn(j)[i] += (-l[j]) * zn(q)[i];
     wsync();
   }
 
-  __device__ void adjust_params() {
+  __device__ __forceinline__ void adjust_params() {
     if (qprime != q) {
       int dq = qprime - q;
       if (dq == 1) increase_bdf();
@@ -230,7 +232,7 @@ struct CellCv {
   }
 
   // guess_helper, camp_solver.c:1321-1408 (called with h_n = h > 0)
-  __device__ int guess_helper(double t_n, double h_n, const double* y_n, const double* y_n1,
+  __device__ __forceinline__ int guess_helper(double t_n, double h_n, const double* y_n, const double* y_n1,
                               const double* hf, double* tmp1, double* corr) {
     double mn = 1.0e308;
     for (int i = lane; i < nv; i += kWave) mn = fmin(mn, y_n[i]);
@@ -270,7 +272,7 @@ struct CellCv {
     return 1;
   }
 
-  __device__ int lsetup(int convfail, const double* ypred, double* fpred) {
+  __device__ __forceinline__ int lsetup(int convfail, const double* ypred, double* fpred) {
     double dgamma = fabs(gamma / gammap - 1.0);
     int jbad = (nst == 0) || (nst > nstlj + MSBJ) ||
                ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_JBAD)) ||
@@ -286,7 +288,7 @@ struct CellCv {
     return cell_lu_factor(P, lane, Js, gamma, lu);
   }
 
-  __device__ void lsolve(double* b) {
+  __device__ __forceinline__ void lsolve(double* b) {
     cell_lu_solve(P, lane, lu, b, w);
     if (gamrat != 1.0) {
       double s = 2.0 / (1.0 + gamrat);
@@ -295,12 +297,12 @@ struct CellCv {
     }
   }
 
-  __device__ int newton_iteration() {
+  __device__ __forceinline__ int newton_iteration() {
     int m = 0;
     double del = 0.0, delp = 0.0;
     for (;;) {
       for (int i = lane; i < nv; i += kWave) {
-        double t = rl1 * zn[1][i] + acor[i];
+        double t = rl1 * zn(1)[i] + acor[i];
         tempv[i] = gamma * ftemp[i] - t;
       }
       wsync();
@@ -310,7 +312,7 @@ struct CellCv {
       for (int i = lane; i < nv; i += kWave) {
         double a = acor[i] + tempv[i];
         acor[i] = a;
-        y[i] = zn[0][i] + a;
+        y[i] = zn(0)[i] + a;
       }
       wsync();
       if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
@@ -334,15 +336,15 @@ struct CellCv {
     }
   }
 
-  __device__ int nls(int nflag) {
+  __device__ __forceinline__ int nls(int nflag) {
     int convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? CV_NO_FAILURES : CV_FAIL_OTHER;
     int callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (nst == 0) ||
                     (nst >= nstlp + MSBP) || (fabs(gamrat - 1.0) > DGMAX);
     use_acinit = 0;
     if (guess_on) {
-      for (int i = lane; i < nv; i += kWave) acinit[i] = zn[0][i] - last[i];
+      for (int i = lane; i < nv; i += kWave) acinit[i] = zn(0)[i] - last[i];
       wsync();
-      int gh = guess_helper(tn, h, zn[0], last, acinit, gh1, gh2);
+      int gh = guess_helper(tn, h, zn(0), last, acinit, gh1, gh2);
       if (gh == 1) {
         for (int i = lane; i < nv; i += kWave) acinit[i] = gh2[i];
         wsync();
@@ -350,7 +352,7 @@ struct CellCv {
       }
     }
     for (;;) {
-      for (int i = lane; i < nv; i += kWave) y[i] = use_acinit ? zn[0][i] + acinit[i] : zn[0][i];
+      for (int i = lane; i < nv; i += kWave) y[i] = use_acinit ? zn(0)[i] + acinit[i] : zn(0)[i];
       wsync();
       if (f(y, ftemp)) return RHSFUNC_RECVR;
       if (callSetup) {
@@ -371,7 +373,7 @@ struct CellCv {
     }
   }
 
-  __device__ int handle_nflag(int* nflagPtr, double saved_t, int* ncfPtr) {
+  __device__ __forceinline__ int handle_nflag(int* nflagPtr, double saved_t, int* ncfPtr) {
     int nflag = *nflagPtr;
     if (nflag == CV_SUCCESS) return DO_ERROR_TEST;
     ncfn++;
@@ -389,7 +391,7 @@ struct CellCv {
     return PREDICT_AGAIN;
   }
 
-  __device__ int do_error_test(int* nflagPtr, double saved_t, int* nefPtr, double* dsmPtr) {
+  __device__ __forceinline__ int do_error_test(int* nflagPtr, double saved_t, int* nefPtr, double* dsmPtr) {
     double dsm = acnrm * tq[2];
     *dsmPtr = dsm;
     if (dsm <= 1.0) return CV_SUCCESS;
@@ -420,13 +422,13 @@ struct CellCv {
     next_h = h;
     hscale = h;
     qwait = LONG_WAIT;
-    if (f(zn[0], tempv)) return CV_UNREC_RHSFUNC_ERR;
-    for (int i = lane; i < nv; i += kWave) zn[1][i] = h * tempv[i];
+    if (f(zn(0), tempv)) return CV_UNREC_RHSFUNC_ERR;
+    for (int i = lane; i < nv; i += kWave) zn(1)[i] = h * tempv[i];
     wsync();
     return TRY_AGAIN;
   }
 
-  __device__ void complete_step() {
+  __device__ __forceinline__ void complete_step() {
     nst++;
     hu = h;
     for (int i = q; i >= 2; --i) tau[i] = tau[i - 1];
@@ -436,15 +438,15 @@ struct CellCv {
     bool save = (qwait == 1) && (q != QMAX);
     for (int i = lane; i < nv; i += kWave) {
       double a = acor[i];
-      for (int j = 0; j <= q; ++j) zn[j][i] += l[j] * a;
-      if (save) zn[QMAX][i] = a;
-      last[i] = zn[0][i];
+      for (int j = 0; j <= q; ++j) zn(j)[i] += l[j] * a;
+      if (save) zn(QMAX)[i] = a;
+      last[i] = zn(0)[i];
     }
     if (save) saved_tq5 = tq[5];
     wsync();
   }
 
-  __device__ void set_eta() {
+  __device__ __forceinline__ void set_eta() {
     if (eta < THRESH) {
       eta = 1.0;
       hprime = h;
@@ -454,7 +456,7 @@ struct CellCv {
     }
   }
 
-  __device__ void prepare_next_step(double dsm) {
+  __device__ __forceinline__ void prepare_next_step(double dsm) {
     if (etamax == 1.0) {
       qwait = qwait > 2 ? qwait : 2;
       qprime = q;
@@ -472,13 +474,13 @@ struct CellCv {
     qwait = 2;
     etaqm1 = 0.0;
     if (q > 1) {
-      double ddn = wrms(zn[q]) * tq[1];
+      double ddn = wrms(zn(q)) * tq[1];
       etaqm1 = 1.0 / (rpowerr(BIAS1 * ddn, 1.0 / q) + ADDON);
     }
     etaqp1 = 0.0;
     if (q != QMAX && saved_tq5 != 0.0) {
       double cquot = (tq[5] / saved_tq5) * rpoweri(h / tau[2], L);
-      for (int i = lane; i < nv; i += kWave) tempv[i] = -cquot * zn[QMAX][i] + acor[i];
+      for (int i = lane; i < nv; i += kWave) tempv[i] = -cquot * zn(QMAX)[i] + acor[i];
       wsync();
       double dup = wrms(tempv) * tq[3];
       etaqp1 = 1.0 / (rpowerr(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
@@ -496,13 +498,13 @@ struct CellCv {
     } else {
       eta = etaqp1;
       qprime = q + 1;
-      for (int i = lane; i < nv; i += kWave) zn[QMAX][i] = acor[i];
+      for (int i = lane; i < nv; i += kWave) zn(QMAX)[i] = acor[i];
       wsync();
     }
     set_eta();
   }
 
-  __device__ int step() {
+  __device__ __forceinline__ int step() {
     double saved_t = tn, dsm = 0.0;
     int ncf = 0, nef = 0, nflag = FIRST_CALL;
     if ((nst > 0) && (hprime != h)) adjust_params();
@@ -526,10 +528,10 @@ struct CellCv {
     return CV_SUCCESS;
   }
 
-  // CVodeReInit + CVodeSetInitStep + CVode(tout, CV_NORMAL); zn[0] holds y0
+  // CVodeReInit + CVodeSetInitStep + CVode(tout, CV_NORMAL); zn(0) holds y0
   // on entry, y holds y(tout) on successful exit.
-  __device__ int solve(double t0, double tout) {
-    for (int i = lane; i < nv; i += kWave) last[i] = zn[0][i];
+  __device__ __forceinline__ int solve(double t0, double tout) {
+    for (int i = lane; i < nv; i += kWave) last[i] = zn(0)[i];
     tn = t0;
     q = 1;
     L = 2;
@@ -549,19 +551,19 @@ struct CellCv {
     init_h = tout - t0;
     next_h = 0.0;
     wsync();
-    ewt_set(zn[0]);
-    if (f(zn[0], zn[1])) return CV_FIRST_RHSFUNC_ERR;
+    ewt_set(zn(0));
+    if (f(zn(0), zn(1))) return CV_FIRST_RHSFUNC_ERR;
     h = init_h;
-    for (int i = lane; i < nv; i += kWave) zn[1][i] *= h;
+    for (int i = lane; i < nv; i += kWave) zn(1)[i] *= h;
     wsync();
     hscale = h;
     hprime = h;
     int nstloc = 0;
     for (;;) {
       next_h = h;
-      if (nst > 0) ewt_set(zn[0]);
+      if (nst > 0) ewt_set(zn(0));
       if (nstloc >= max_steps) return CV_TOO_MUCH_WORK;
-      double nrm = wrms(zn[0]);
+      double nrm = wrms(zn(0));
       if (2.220446049250313e-16 * nrm > 1.0) return CV_TOO_MUCH_ACC;
       int kflag = step();
       if (kflag != CV_SUCCESS) return kflag;
@@ -569,8 +571,8 @@ struct CellCv {
       if ((tn - tout) * h >= 0.0) {
         double s = (tout - tn) / h;
         for (int i = lane; i < nv; i += kWave) {
-          double v = zn[q][i];
-          for (int j = q - 1; j >= 0; --j) v = zn[j][i] + s * v;
+          double v = zn(q)[i];
+          for (int j = q - 1; j >= 0; --j) v = zn(j)[i] + s * v;
           y[i] = v;
         }
         wsync();
@@ -583,7 +585,7 @@ struct CellCv {
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) k_cell_bdf(CellBdfArgs a) {
+__global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
   extern __shared__ double smem[];
   const DevProgram& P = a.P;
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
@@ -597,7 +599,7 @@ __global__ void __launch_bounds__(256) k_cell_bdf(CellBdfArgs a) {
   cv.s_state = base;
   cv.s_k = cv.s_state + P.ns;
   double* v = cv.s_k + P.n_env;
-  for (int j = 0; j < 6; ++j) cv.zn[j] = v + (size_t)j * nv;
+  cv.znb = v;
   v += 6 * (size_t)nv;
   cv.ewt = v; v += nv;
   cv.acor = v; v += nv;
@@ -610,6 +612,10 @@ __global__ void __launch_bounds__(256) k_cell_bdf(CellBdfArgs a) {
   cv.gh2 = v; v += nv;
   cv.w = v; v += nv;
   cv.lu = v;
+  v += P.lu_nnz;
+  cv.l = v;
+  cv.tq = v + 6;
+  cv.tau = v + 12;
   cv.rtol = a.rtol;
   cv.guess_on = a.guess_on;
   cv.maxcf = a.max_conv_fails;
@@ -644,7 +650,7 @@ __global__ void __launch_bounds__(256) k_cell_bdf(CellBdfArgs a) {
     for (int i = lane; i < P.n_env; i += kWave) renv_c[i] = cv.s_k[i];
     for (int i = lane; i < nv; i += kWave) {
       double s = cv.s_state[P.dep_state[i]];
-      cv.zn[0][i] = s > 1.0e-60 ? s : 1.0e-60;
+      cv.zn(0)[i] = s > 1.0e-60 ? s : 1.0e-60;
     }
     wsync();
     int flag = cv.solve(a.t0, a.t1);

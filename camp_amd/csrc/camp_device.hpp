@@ -29,6 +29,11 @@ struct DevProgram {
   const int *perm, *m_to_lu, *diag_pos, *e_diag;
   const int *l_ptr, *l_pos, *l_row, *u_ptr, *u_pos, *u_row, *up_ptr;
   const uint16_t *up_dst, *up_l, *up_u;
+  const int *lu_rowptr, *lu_col;
+  const uint8_t* row_off;  // nullptr when rows exceed 254 entries or nv > 128
+  const int4 *fe_a, *je_a;  // packed gather records (see Program in camp_model.hpp)
+  const int *fe_b, *je_b;
+  const double *fe_c, *je_c;
   // rate-constant program (rxn_update_env_state)
   const int *rx_type, *rx_fidx, *rx_eidx, *rx_nreact;
   const double* rx_float;
@@ -70,8 +75,13 @@ __device__ __forceinline__ dd dd_add(dd a, dd b) {
 // ---------------------------------------------------------------------
 // Wavefront helpers
 // ---------------------------------------------------------------------
+// Orders the LDS (and global) accesses of one wavefront across lanes.  A
+// wavefront's memory instructions are processed in program order, so a
+// wavefront-scope fence needs no s_waitcnt: it only stops the compiler from
+// moving memory operations across it, and outstanding global prefetches
+// stay in flight.
 __device__ __forceinline__ void wsync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 __device__ __forceinline__ double wave_sum(double v) {
@@ -145,11 +155,22 @@ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const 
     dd D = {0.0, 0.0};
     double S = 0.0;
     for (int q = P.f_ptr[i]; q < P.f_ptr[i + 1]; ++q) {
-      int t = P.f_term[q];
-      double c = P.f_coef[q];
-      int g = P.f_guard[q];
-      dd rate = term_rate(P, t, s_state, s_k);
-      if ((P.t_flags[t] & 1) && rate.hi == 0.0) continue;
+      int4 a = P.fe_a[q];
+      int b = P.fe_b[q];
+      double c = P.fe_c[q];
+      int flags = b >> 24;
+      int g = (b << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
+      double k = s_k[a.x];
+      dd rate;
+      if (flags & 2) {
+        rate = {k * s_state[a.y], 0.0};
+      } else {
+        rate = {k, 0.0};
+        if (a.y >= 0) rate = dd_mul_d(rate, s_state[a.y]);
+        if (a.z >= 0) rate = dd_mul_d(rate, s_state[a.z]);
+        if (a.w >= 0) rate = dd_mul_d(rate, s_state[a.w]);
+      }
+      if ((flags & 1) && rate.hi == 0.0) continue;
       if (g >= 0 && !(-rate.hi * c * h <= s_state[g])) continue;
       dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
       D = dd_add(D, contrib);
@@ -201,18 +222,16 @@ __device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const do
   for (int e = lane; e < P.nnz; e += kWave) {
     dd D = {0.0, 0.0};
     for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q) {
-      int t = P.j_term[q];
-      int skip = P.j_skip[q];
-      double c = P.j_coef[q];
-      double v = s_k[P.t_kidx[t]];
-      int nr = P.t_nreact[t];
-      for (int j = 0; j < nr; ++j)
-        if (j != skip) v *= s_state[P.t_react[3 * t + j]];
+      int4 a = P.je_a[q];
+      double c = P.je_c[q];
+      double v = s_k[a.x];
+      if (a.y >= 0) v *= s_state[a.y];
+      if (a.z >= 0) v *= s_state[a.z];
       if (c == -1.0) {
         D = dd_add(D, dd{-v, 0.0});
       } else {
-        int g = P.j_guard[q];
-        if (g >= 0 && !(-v * s_state[P.j_rind[q]] * c * h <= s_state[g])) continue;
+        int g = P.je_b[q];
+        if (g >= 0 && !(-v * s_state[a.w] * c * h <= s_state[g])) continue;
         D = dd_add(D, dd{c * v, 0.0});
       }
     }
@@ -235,27 +254,112 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     lu[P.m_to_lu[e]] = v;
   }
   wsync();
+  // The index lists of pivot k+1 (first 64 entries of each) are fetched
+  // from global memory while pivot k is being eliminated, so the chain of
+  // 2*nv dependent LDS phases does not also wait on global-memory latency.
+  int lq0 = P.l_ptr[0], lq1 = P.l_ptr[1], uq0 = P.up_ptr[0], uq1 = P.up_ptr[1];
+  int l_p = (lq0 + lane < lq1) ? P.l_pos[lq0 + lane] : -1;
+  int u_d = -1, u_l = 0, u_u = 0;
+  if (uq0 + lane < uq1) {
+    u_d = P.up_dst[uq0 + lane];
+    u_l = P.up_l[uq0 + lane];
+    u_u = P.up_u[uq0 + lane];
+  }
+  int dpos = P.diag_pos[0];
   for (int k = 0; k < P.nv; ++k) {
-    double piv = lu[P.diag_pos[k]];
+    int nlq0 = lq1, nlq1 = 0, nuq0 = uq1, nuq1 = 0, n_lp = -1, n_ud = -1, n_ul = 0, n_uu = 0,
+        ndpos = 0;
+    if (k + 1 < P.nv) {
+      nlq1 = P.l_ptr[k + 2];
+      nuq1 = P.up_ptr[k + 2];
+      ndpos = P.diag_pos[k + 1];
+      if (nlq0 + lane < nlq1) n_lp = P.l_pos[nlq0 + lane];
+      if (nuq0 + lane < nuq1) {
+        n_ud = P.up_dst[nuq0 + lane];
+        n_ul = P.up_l[nuq0 + lane];
+        n_uu = P.up_u[nuq0 + lane];
+      }
+    }
+    double piv = lu[dpos];
     if (piv == 0.0 || !isfinite(piv)) return 1;
-    for (int q = P.l_ptr[k] + lane; q < P.l_ptr[k + 1]; q += kWave) {
+    if (l_p >= 0) lu[l_p] = lu[l_p] / piv;
+    for (int q = lq0 + kWave + lane; q < lq1; q += kWave) {
       int p = P.l_pos[q];
       lu[p] = lu[p] / piv;
     }
     wsync();
-    int q1 = P.up_ptr[k + 1];
-    for (int q = P.up_ptr[k] + lane; q < q1; q += kWave) {
+    if (u_d >= 0) lu[u_d] = lu[u_d] - lu[u_l] * lu[u_u];
+    for (int q = uq0 + kWave + lane; q < uq1; q += kWave) {
       int d = P.up_dst[q];
       lu[d] = lu[d] - lu[P.up_l[q]] * lu[P.up_u[q]];
     }
     wsync();
+    lq0 = nlq0; lq1 = nlq1; uq0 = nuq0; uq1 = nuq1;
+    l_p = n_lp; u_d = n_ud; u_l = n_ul; u_u = n_uu; dpos = ndpos;
   }
   return 0;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Triangular solves with the solution vector in registers (nv <= 128):
+// lane t owns permuted rows t and t+64.  Pivot k's value is broadcast with
+// v_readlane (no LDS round trip, no barrier), so the dependency chain per
+// pivot is one readlane and one FMA; the LU value and the row offset each
+// lane needs do not depend on the solution and are loaded off the chain.
+__device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane, const double* lu,
+                                                  double* b) {
+  const int nv = P.nv;
+  const int r0 = lane, r1 = lane + kWave;
+  const bool has0 = r0 < nv, has1 = r1 < nv;
+  double x0 = has0 ? b[P.perm[r0]] : 0.0;
+  double x1 = has1 ? b[P.perm[r1]] : 0.0;
+  const double* row0 = lu + (has0 ? P.lu_rowptr[r0] : 0);
+  const double* row1 = lu + (has1 ? P.lu_rowptr[r1] : 0);
+  const uint8_t* off0 = P.row_off + (size_t)(has0 ? r0 : 0) * nv;
+  const uint8_t* off1 = P.row_off + (size_t)(has1 ? r1 : 0) * nv;
+#pragma unroll 4
+  for (int k = 0; k < nv - 1; ++k) {
+    int o0 = off0[k], o1 = off1[k];
+    bool a0 = has0 && r0 > k && o0 != 255, a1 = has1 && r1 > k && o1 != 255;
+    double l0 = a0 ? row0[o0] : 0.0;
+    double l1 = a1 ? row1[o1] : 0.0;
+    double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
+    x0 = a0 ? x0 - l0 * xk : x0;
+    x1 = a1 ? x1 - l1 * xk : x1;
+  }
+#pragma unroll 4
+  for (int k = nv - 1; k >= 0; --k) {
+    int o0 = off0[k], o1 = off1[k];
+    bool a0 = has0 && r0 < k && o0 != 255, a1 = has1 && r1 < k && o1 != 255;
+    double u0 = a0 ? row0[o0] : 0.0;
+    double u1 = a1 ? row1[o1] : 0.0;
+    double xk = (k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave)) / lu[P.diag_pos[k]];
+    if (k < kWave) {
+      if (lane == k) x0 = xk;
+    } else if (lane == k - kWave) {
+      x1 = xk;
+    }
+    x0 = a0 ? x0 - u0 * xk : x0;
+    x1 = a1 ? x1 - u1 * xk : x1;
+  }
+  wsync();
+  if (has0) b[P.perm[r0]] = x0;
+  if (has1) b[P.perm[r1]] = x1;
+  wsync();
 }
 
 // Solve (LU) x = b in place; b in dep order, w a scratch vector (nv).
 __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, const double* lu,
                                               double* b, double* w) {
+  if (P.row_off) {
+    cell_lu_solve_reg(P, lane, lu, b);
+    return;
+  }
   for (int k = lane; k < P.nv; k += kWave) w[k] = b[P.perm[k]];
   wsync();
   for (int k = 0; k < P.nv; ++k) {

@@ -11,9 +11,10 @@ namespace camp {
 constexpr int kCellStats = 10;  // nst nfe nsetups netf nni ncfn nje flag hu next_h
 
 // LDS doubles one wavefront needs for a cell in the persistent kernel:
-// state, rate data, Nordsieck history (6 vectors), 10 work vectors, LU.
+// state, rate data, Nordsieck history (6 vectors), 10 work vectors, LU and
+// 24 slots for the BDF coefficient arrays l, tq and tau.
 __host__ __device__ inline size_t cell_bdf_lds_doubles(int ns, int nv, int n_env, int lu_nnz) {
-  return (size_t)ns + n_env + 16 * (size_t)nv + lu_nnz;
+  return (size_t)ns + n_env + 16 * (size_t)nv + lu_nnz + 24;
 }
 
 struct CellBdfArgs {

@@ -338,6 +338,31 @@ std::string Model::initialize() {
     P.j_ptr.push_back((int)P.j_term.size());
     P.max_j_len = std::max(P.max_j_len, (int)j_term[e].size());
   }
+  for (size_t q = 0; q < P.f_term.size(); ++q) {
+    int t = P.f_term[q];
+    P.fe_a.push_back(P.t_kidx[t]);
+    for (int j = 0; j < 3; ++j) P.fe_a.push_back(j < P.t_nreact[t] ? P.t_react[3 * t + j] : -1);
+    P.fe_b.push_back((P.f_guard[q] & 0xFFFFFF) | (P.t_flags[t] << 24));
+    P.fe_c.push_back(P.f_coef[q]);
+  }
+  for (size_t q = 0; q < P.j_term.size(); ++q) {
+    int t = P.j_term[q], skip = P.j_skip[q];
+    int others[2] = {-1, -1}, n_o = 0;
+    for (int j = 0; j < P.t_nreact[t]; ++j)
+      if (j != skip) others[n_o++] = P.t_react[3 * t + j];
+    P.je_a.push_back(P.t_kidx[t]);
+    P.je_a.push_back(others[0]);
+    P.je_a.push_back(others[1]);
+    P.je_a.push_back(P.j_rind[q]);
+    P.je_b.push_back(P.j_guard[q]);
+    P.je_c.push_back(P.j_coef[q]);
+  }
+  if (P.fe_c.empty()) {
+    P.fe_a.assign(4, -1); P.fe_b.push_back(-1); P.fe_c.push_back(0.0);
+  }
+  if (P.je_c.empty()) {
+    P.je_a.assign(4, -1); P.je_b.push_back(-1); P.je_c.push_back(0.0);
+  }
   if (P.f_term.empty()) { P.f_term.push_back(0); P.f_coef.push_back(0); P.f_guard.push_back(-1); }
   if (P.j_term.empty()) {
     P.j_term.push_back(0); P.j_skip.push_back(0); P.j_coef.push_back(0);
@@ -389,6 +414,23 @@ std::string Model::initialize() {
     }
   P.diag_pos.resize(nv);
   for (int k = 0; k < nv; ++k) P.diag_pos[k] = pos[(size_t)k * nv + k];
+  P.lu_rowptr.assign(nv + 1, 0);
+  P.lu_col.assign(P.lu_nnz, 0);
+  P.row_off.assign((size_t)nv * nv, 255);
+  for (int i = 0; i < nv; ++i) {
+    int first = -1, cnt = 0;
+    for (int j = 0; j < nv; ++j) {
+      int q = pos[(size_t)i * nv + j];
+      if (q < 0) continue;
+      if (first < 0) first = q;
+      P.lu_col[q] = j;
+      if (cnt < 255) P.row_off[(size_t)i * nv + j] = (uint8_t)(q - first);
+      ++cnt;
+    }
+    P.lu_rowptr[i] = first;
+    if (cnt >= 255) P.row_off.clear();  // rows too long for 8-bit offsets
+  }
+  P.lu_rowptr[nv] = P.lu_nnz;
   P.l_ptr.assign(1, 0);
   P.u_ptr.assign(1, 0);
   P.up_ptr.assign(1, 0);

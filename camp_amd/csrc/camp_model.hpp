@@ -56,6 +56,15 @@ struct Program {
   std::vector<double> f_coef;
   std::vector<int> j_ptr, j_term, j_skip, j_guard, j_rind;  // per element
   std::vector<double> j_coef;
+  // The same gather lists as self-contained records, one vector load each:
+  //  f entry: fe_a = {k index, reactant state ids (-1 = none) x3},
+  //           fe_b = product state id for the negative-yield guard (-1 =
+  //           none) | flags << 24, fe_c = coefficient
+  //  J entry: je_a = {k index, the other reactants (-1 = none) x2, the
+  //           differentiated reactant}, je_b = guard product state id,
+  //           je_c = coefficient
+  std::vector<int> fe_a, fe_b, je_a, je_b;
+  std::vector<double> fe_c, je_c;
   std::vector<int> colptr, rowval;                  // solver pattern (dep ids)
   std::vector<int> r_ptr, r_elem, r_col;            // row view of the pattern
   // LU (positions in the permuted LU value array)
@@ -65,6 +74,11 @@ struct Program {
   std::vector<int> u_ptr, u_pos, u_row;             // per pivot k: U(:,k) above k
   std::vector<int> up_ptr;                          // per pivot k: update range
   std::vector<uint16_t> up_dst, up_l, up_u;         // LU[dst] -= LU[l] * LU[u]
+  // Row-major LU storage: row i occupies [lu_rowptr[i], lu_rowptr[i+1]) in
+  // column order; row_off[i*nv + k] is the offset of (i, k) in its row, or
+  // 255 when (i, k) is structurally zero (requires rows of < 255 entries).
+  std::vector<int> lu_rowptr, lu_col;
+  std::vector<uint8_t> row_off;
   long n_updates() const { return (long)up_dst.size(); }
 };
 

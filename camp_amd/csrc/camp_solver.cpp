@@ -160,6 +160,15 @@ void Solver::setup_device() {
   P.up_dst = dev_upload(G.up_dst, owned);
   P.up_l = dev_upload(G.up_l, owned);
   P.up_u = dev_upload(G.up_u, owned);
+  P.fe_a = reinterpret_cast<const int4*>(dev_upload(G.fe_a, owned));
+  P.fe_b = dev_upload(G.fe_b, owned);
+  P.fe_c = dev_upload(G.fe_c, owned);
+  P.je_a = reinterpret_cast<const int4*>(dev_upload(G.je_a, owned));
+  P.je_b = dev_upload(G.je_b, owned);
+  P.je_c = dev_upload(G.je_c, owned);
+  P.lu_rowptr = dev_upload(G.lu_rowptr, owned);
+  P.lu_col = dev_upload(G.lu_col, owned);
+  P.row_off = (G.nv <= 2 * kWave && !G.row_off.empty()) ? dev_upload(G.row_off, owned) : nullptr;
   std::vector<int> rx_type(m.n_rxn), rx_fidx(m.n_rxn), rx_eidx(m.n_rxn), rx_nreact(m.n_rxn);
   for (int r = 0; r < m.n_rxn; ++r) {
     int* id = m.rint(r);
