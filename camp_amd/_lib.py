@@ -12,8 +12,9 @@ LIB_PATH = os.path.join(HERE, "libcamp_mi355x.so")
 
 _i, _d, _p = C.c_int, C.c_double, C.c_void_p
 _ip, _dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+_l, _lp = C.c_long, C.POINTER(C.c_long)
 
-# name -> (restype, argtypes); every symbol declared in include/camp_solver.h
+# name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     "solver_new": (_p, [_i, _i, _ip, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i]),
     "solver_initialize": (None, [_p, _dp, _d, _i, _i]),
@@ -47,9 +48,22 @@ SIGNATURES = {
     "camp_eval_lu_solve": (_i, [_p, _dp, _d, _dp]),
     "camp_solver_get_cell_stats": (None, [_p, _dp]),
     "camp_solver_last_kernel_ms": (_d, [_p]),
+    "camp_solver_set_profiling": (None, [_p, _i]),
+    "camp_solver_get_cell_profile": (_i, [_p, _dp]),
     "camp_solver_get_rxn_env": (None, [_p, _dp]),
     "camp_solver_time_fj": (_i, [_p, _p, _p, _i, _dp, _dp]),
     "camp_solver_work_model": (None, [_p, _dp]),
+    # include/camp_server.h
+    "camp_server_new": (_p, [_i, _l]),
+    "camp_server_add_mechanism": (_i, [_p, _p, _l]),
+    "camp_server_capacity": (_l, [_p, _i]),
+    "camp_solver_rate_layout": (_i, [_p, _ip, _ip]),
+    "camp_server_submit": (_l, [_p, _i, _i, _p, _p, _p, _d, _i, _p, _p]),
+    "camp_server_wait": (_i, [_p, _l, _dp]),
+    "camp_server_set_pool": (_i, [_p, _i, _l, _dp, _dp, _dp]),
+    "camp_server_replay": (_i, [_p, _i, _dp, _ip, _ip, _lp, _dp, _ip, _dp, _dp]),
+    "camp_server_counters": (None, [_p, _dp]),
+    "camp_server_free": (None, [_p]),
 }
 
 _lib = None

@@ -150,6 +150,16 @@ class CampSolver:
     def last_kernel_ms(self):
         return lib().camp_solver_last_kernel_ms(self.h)
 
+    def set_profiling(self, on=True):
+        lib().camp_solver_set_profiling(self.h, int(bool(on)))
+
+    def cell_profile(self):
+        """Shader cycles per cell: [f, Jac, LU factor, LU solve, whole solve]."""
+        out = np.zeros((self.n_cells, 5))
+        if lib().camp_solver_get_cell_profile(self.h, _dp(out)) != 0:
+            raise RuntimeError("no profiled per-cell run yet")
+        return out
+
     def work_model(self):
         """Per-cell algorithmic flops and HBM bytes of f, J, LU, solve."""
         out = np.zeros(8)

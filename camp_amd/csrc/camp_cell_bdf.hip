@@ -53,6 +53,10 @@ struct CellCv {
   double rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
   int q, qprime, L, qwait, jcur, use_acinit, guess_on, maxcf, max_steps;
   int nst, nfe, nsetups, nni, ncfn, netf, nje, nstlp, nstlj;
+  // optional shader-clock counters (cycles in f, Jac, LU factor, LU solve)
+  bool prof;
+  long long c_f, c_j, c_lu, c_sol;
+  __device__ __forceinline__ long long tick() const { return prof ? clock64() : 0; }
 
   __device__ __forceinline__ double* zn(int j) const { return znb + (size_t)j * nv; }
 
@@ -75,15 +79,18 @@ struct CellCv {
   // CAMP f(): camp_solver.c:843-958 for one cell (use_deriv_est = 1)
   __device__ __forceinline__ int f(const double* yv, double* out) {
     nfe++;
+    long long t0 = tick();
     double ts = next_h > 0.0 ? next_h : init_h;
-    if (cell_set_state(P, lane, yv, s_state)) return 1;
-    cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, true, ts, out);
+    int bad = cell_set_state(P, lane, yv, s_state);
+    if (!bad) cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, true, ts, out);
     wsync();
-    return 0;
+    if (prof) c_f += clock64() - t0;
+    return bad;
   }
 
   // CAMP Jac(): pure f into fy, J into the cell's J_solver, save history
   __device__ __forceinline__ int jac(const double* yv, double* fy) {
+    long long t0 = tick();
     double ts = next_h > 0.0 ? next_h : init_h;
     if (cell_set_state(P, lane, yv, s_state)) return 1;
     cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy);
@@ -94,6 +101,7 @@ struct CellCv {
     }
     __threadfence_block();
     wsync();
+    if (prof) c_j += clock64() - t0;
     return 0;
   }
 
@@ -285,11 +293,16 @@ struct CellCv {
       jcur = 1;
       if (jac(ypred, fpred)) return 1;
     }
-    return cell_lu_factor(P, lane, Js, gamma, lu);
+    long long t0 = tick();
+    int rc = cell_lu_factor(P, lane, Js, gamma, lu);
+    if (prof) c_lu += clock64() - t0;
+    return rc;
   }
 
   __device__ __forceinline__ void lsolve(double* b) {
+    long long t0 = tick();
     cell_lu_solve(P, lane, lu, b, w);
+    if (prof) c_sol += clock64() - t0;
     if (gamrat != 1.0) {
       double s = 2.0 / (1.0 + gamrat);
       for (int i = lane; i < nv; i += kWave) b[i] *= s;
@@ -620,12 +633,15 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
   cv.guess_on = a.guess_on;
   cv.maxcf = a.max_conv_fails;
   cv.max_steps = a.max_steps;
+  cv.prof = a.prof != nullptr;
 
   for (;;) {
-    int cell = 0;
-    if (lane == 0) cell = atomicAdd(a.counter, 1);
-    cell = __shfl(cell, 0, kWave);
-    if (cell >= a.nc) break;
+    int q = 0;
+    if (lane == 0) q = atomicAdd(a.counter, 1);
+    q = __shfl(q, 0, kWave);
+    if (q >= a.nc) break;
+    int cell = a.cell_list ? a.cell_list[q] : q;
+    double t1 = a.cell_t1 ? a.cell_t1[cell] : a.t1;
     double* state_c = a.state + (size_t)cell * P.ns;
     double* renv_c = a.rxn_env + (size_t)cell * P.n_env;
     cv.Js = a.Js + (size_t)cell * P.nnz;
@@ -653,7 +669,9 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
       cv.zn(0)[i] = s > 1.0e-60 ? s : 1.0e-60;
     }
     wsync();
-    int flag = cv.solve(a.t0, a.t1);
+    cv.c_f = cv.c_j = cv.c_lu = cv.c_sol = 0;
+    long long c_start = cv.tick();
+    int flag = cv.solve(a.t0, t1);
     if (flag >= 0) {
       for (int i = lane; i < nv; i += kWave) {
         double yv = cv.y[i];
@@ -665,6 +683,12 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
       st[0] = cv.nst; st[1] = cv.nfe; st[2] = cv.nsetups; st[3] = cv.netf;
       st[4] = cv.nni; st[5] = cv.ncfn; st[6] = cv.nje; st[7] = flag;
       st[8] = cv.hu; st[9] = cv.next_h;
+      if (a.fail_count && flag < 0) a.fail_count[cell] += 1;
+      if (cv.prof) {
+        double* pr = a.prof + (size_t)cell * kCellProf;
+        pr[0] = (double)cv.c_f; pr[1] = (double)cv.c_j; pr[2] = (double)cv.c_lu;
+        pr[3] = (double)cv.c_sol; pr[4] = (double)(clock64() - c_start);
+      }
     }
     wsync();
   }

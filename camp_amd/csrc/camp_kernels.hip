@@ -304,4 +304,18 @@ __global__ void k_state_scatter(DevProgram P, int nc, const double* __restrict__
   state[(size_t)c * P.ns + P.dep_state[d]] = v > 0.0 ? v : 0.0;
 }
 
+__global__ void k_scatter_rates(int n_cells, int n_upd, const int* upd_off, const double* rates,
+                                int n_env, double* renv) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n_cells * n_upd) return;
+  long c = i / n_upd;
+  int u = (int)(i - c * n_upd);
+  renv[c * n_env + upd_off[u]] = rates[i];
+}
+
+__global__ void k_fill(long n, double v, double* x) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
 }  // namespace camp

@@ -9,6 +9,7 @@
 namespace camp {
 
 constexpr int kCellStats = 10;  // nst nfe nsetups netf nni ncfn nje flag hu next_h
+constexpr int kCellProf = 5;    // shader cycles: f, Jac, LU factor, LU solve, whole solve
 
 // LDS doubles one wavefront needs for a cell in the persistent kernel:
 // state, rate data, Nordsieck history (6 vectors), 10 work vectors, LU and
@@ -29,6 +30,13 @@ struct CellBdfArgs {
   int max_steps, max_conv_fails, guess_on;
   int* counter;         // work queue head (zeroed before launch)
   double* stats;        // [nc][kCellStats]
+  // Optional indirection used by the request server: queue position q
+  // solves cell cell_list[q] (nullptr: cell q) over [t0, cell_t1[cell]]
+  // (nullptr: [t0, t1]).  nc is then the length of cell_list.
+  const int* cell_list = nullptr;
+  const double* cell_t1 = nullptr;
+  double* prof = nullptr;  // [cells][kCellProf] when phase timing is on
+  int* fail_count = nullptr;  // per cell: incremented on each failed solve
 };
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);
@@ -64,6 +72,10 @@ __global__ void k_vec_wsq(long n, const double* x, const double* w, double* out)
 __global__ void k_vec_min(long n, const double* x, double* out);
 __global__ void k_vec_tstar(long n, const double* tmp1, const double* corr, double h_j,
                             double* out);
+__global__ void k_fill(long n, double v, double* x);
+// renv[c][upd_off[u]] = rates[c][u] for c < n_cells, u < n_upd
+__global__ void k_scatter_rates(int n_cells, int n_upd, const int* upd_off, const double* rates,
+                                int n_env, double* renv);
 __global__ void k_clamp_constants(DevProgram P, int nc, const int* var_type, double* state);
 __global__ void k_state_gather(DevProgram P, int nc, const double* state, double* y);
 __global__ void k_state_scatter(DevProgram P, int nc, const double* y, double* state);

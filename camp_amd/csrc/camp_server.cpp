@@ -1,0 +1,571 @@
+// Request server: continuous batching of CAMP chemistry solves on one GPU.
+//
+// Per mechanism an HBM arena holds cell slots (state, env, rate data,
+// Jacobian history, statistics, end time, failure counter).  A request is
+// admitted when a contiguous range of slots is free; its inputs are copied
+// in on the upload stream.  Every scheduling round, each idle batch stream
+// takes the ready requests of one mechanism (oldest first, up to
+// max_batch_cells), builds the slot list and launches the persistent
+// per-cell kernel over it; requests on their last step copy their results
+// out on the same stream.  When a batch's event completes its requests
+// either become ready for their next timestep (state stays in the arena)
+// or complete and release their slots.  One thread makes every HIP call.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "../../include/camp_server.h"
+#include "camp_solver_impl.hpp"
+
+namespace camp {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum ReqState { kQueued, kReady, kRunning, kDone };
+
+struct Request {
+  long id = 0;
+  int mech = 0, n = 0, steps_left = 0;
+  double dt = 0.0;
+  const double *in_state = nullptr, *in_env = nullptr, *in_rates = nullptr;
+  double *out_state = nullptr, *out_stats = nullptr;
+  Clock::time_point t_submit, t_done;
+  long slot0 = -1;
+  hipEvent_t up_ev = nullptr;
+  int state = kQueued;
+  bool failed = false;
+};
+
+struct Arena {
+  Solver* S = nullptr;
+  long cap = 0;
+  int ns = 0, n_env = 0, nnz = 0, nv = 0, n_upd = 0;
+  double *state = nullptr, *env = nullptr, *renv = nullptr, *Js = nullptr, *Jy = nullptr,
+         *Jf = nullptr, *stats = nullptr, *t1 = nullptr, *rates = nullptr;
+  int *fail = nullptr, *upd_off = nullptr;
+  int* h_fail = nullptr;  // pinned [cap]
+  std::vector<void*> owned;
+  std::map<long, long> free_ranges;  // start -> length
+  long resident = 0;
+  // replay pool (pinned host memory)
+  long pool_n = 0;
+  double *pool_state = nullptr, *pool_env = nullptr, *pool_rates = nullptr, *pool_out = nullptr;
+
+  long alloc(long n) {
+    for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it) {
+      if (it->second < n) continue;
+      long s = it->first, len = it->second;
+      free_ranges.erase(it);
+      if (len > n) free_ranges[s + n] = len - n;
+      resident += n;
+      return s;
+    }
+    return -1;
+  }
+  void release(long s, long n) {
+    resident -= n;
+    auto next = free_ranges.lower_bound(s);
+    if (next != free_ranges.end() && next->first == s + n) {
+      n += next->second;
+      free_ranges.erase(next);
+    }
+    if (!free_ranges.empty()) {
+      auto prev = free_ranges.lower_bound(s);
+      if (prev != free_ranges.begin()) {
+        --prev;
+        if (prev->first + prev->second == s) {
+          prev->second += n;
+          return;
+        }
+      }
+    }
+    free_ranges[s] = n;
+  }
+};
+
+struct Lane {
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  int* d_counter = nullptr;
+  int* d_list = nullptr;
+  int* h_list = nullptr;  // pinned
+  long list_cap = 0;
+  bool busy = false;
+  long cells = 0;
+  std::vector<Request*> batch;
+};
+
+struct Server {
+  int dev = 0;
+  long max_batch = 0;
+  std::vector<std::unique_ptr<Arena>> mechs;
+  std::vector<Lane> lanes;
+  hipStream_t up_st = nullptr;
+  std::thread thr;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  long next_id = 0;
+  std::deque<Request*> inbox;                        // guarded by mu
+  std::unordered_map<long, std::unique_ptr<Request>> reqs;  // guarded by mu
+  // scheduler-thread state
+  std::deque<Request*> pending, ready;
+  // counters (guarded by mu)
+  double n_done = 0, n_failed = 0, cell_steps = 0, batches = 0, batch_cells = 0, peak = 0;
+
+  void loop();
+  bool poll_lanes();
+  bool admit();
+  bool launch();
+  void upload(Arena& A, Request* r);
+};
+
+void Server::upload(Arena& A, Request* r) {
+  long s = r->slot0, n = r->n;
+  size_t d = sizeof(double);
+  HIP_OK(hipMemcpyAsync(A.state + s * A.ns, r->in_state, n * A.ns * d, hipMemcpyHostToDevice, up_st));
+  HIP_OK(hipMemcpyAsync(A.env + s * 2, r->in_env, n * 2 * d, hipMemcpyHostToDevice, up_st));
+  if (A.n_upd > 0) {
+    HIP_OK(hipMemcpyAsync(A.rates + s * A.n_upd, r->in_rates, n * A.n_upd * d,
+                          hipMemcpyHostToDevice, up_st));
+    long m = n * A.n_upd;
+    hipLaunchKernelGGL(k_scatter_rates, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, up_st,
+                       (int)n, A.n_upd, A.upd_off, A.rates + s * A.n_upd, A.n_env,
+                       A.renv + s * A.n_env);
+  }
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, up_st, n, r->dt,
+                     A.t1 + s);
+  // a fresh request starts with an empty Jacobian history, as a new solver
+  HIP_OK(hipMemsetAsync(A.Js + s * A.nnz, 0, n * A.nnz * d, up_st));
+  HIP_OK(hipMemsetAsync(A.Jy + s * A.nv, 0, n * A.nv * d, up_st));
+  HIP_OK(hipMemsetAsync(A.Jf + s * A.nv, 0, n * A.nv * d, up_st));
+  HIP_OK(hipMemsetAsync(A.fail + s, 0, n * sizeof(int), up_st));
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventCreateWithFlags(&r->up_ev, hipEventDisableTiming));
+  HIP_OK(hipEventRecord(r->up_ev, up_st));
+}
+
+bool Server::admit() {
+  bool any = false;
+  // strict arrival order per mechanism: a request that does not fit yet
+  // holds back later requests of its mechanism (no starvation of large ones)
+  std::vector<char> blocked(mechs.size(), 0);
+  for (auto it = pending.begin(); it != pending.end();) {
+    Request* r = *it;
+    Arena& A = *mechs[r->mech];
+    if (blocked[r->mech]) {
+      ++it;
+      continue;
+    }
+    long s = A.alloc(r->n);
+    if (s < 0) {
+      blocked[r->mech] = 1;
+      ++it;
+      continue;
+    }
+    r->slot0 = s;
+    upload(A, r);
+    r->state = kReady;
+    ready.push_back(r);
+    it = pending.erase(it);
+    any = true;
+    std::lock_guard<std::mutex> g(mu);
+    peak = std::max(peak, (double)A.resident);
+  }
+  return any;
+}
+
+bool Server::launch() {
+  bool any = false;
+  for (Lane& L : lanes) {
+    if (L.busy || ready.empty()) continue;
+    int mech = ready.front()->mech;
+    Arena& A = *mechs[mech];
+    L.batch.clear();
+    long total = 0;
+    for (auto it = ready.begin(); it != ready.end();) {
+      Request* r = *it;
+      if (r->mech != mech || (total > 0 && total + r->n > max_batch)) {
+        ++it;
+        continue;
+      }
+      L.batch.push_back(r);
+      total += r->n;
+      it = ready.erase(it);
+      if (total >= max_batch) break;
+    }
+    if (total > L.list_cap) {
+      if (L.d_list) (void)hipFree(L.d_list);
+      if (L.h_list) (void)hipHostFree(L.h_list);
+      L.list_cap = std::max(total, 2 * L.list_cap);
+      HIP_OK(hipMalloc(&L.d_list, L.list_cap * sizeof(int)));
+      HIP_OK(hipHostMalloc((void**)&L.h_list, L.list_cap * sizeof(int), hipHostMallocDefault));
+    }
+    long q = 0;
+    for (Request* r : L.batch) {
+      for (long c = 0; c < r->n; ++c) L.h_list[q++] = (int)(r->slot0 + c);
+      if (r->up_ev) {
+        HIP_OK(hipStreamWaitEvent(L.st, r->up_ev, 0));
+      }
+      r->state = kRunning;
+    }
+    HIP_OK(hipMemcpyAsync(L.d_list, L.h_list, total * sizeof(int), hipMemcpyHostToDevice, L.st));
+    HIP_OK(hipMemsetAsync(L.d_counter, 0, sizeof(int), L.st));
+    CellBdfArgs a = A.S->cell_args();
+    a.nc = (int)total;
+    a.state = A.state;
+    a.env = A.env;
+    a.rxn_env = A.renv;
+    a.Js = A.Js;
+    a.Jy = A.Jy;
+    a.Jf = A.Jf;
+    a.t0 = 0.0;
+    a.t1 = 0.0;
+    a.counter = L.d_counter;
+    a.stats = A.stats;
+    a.cell_list = L.d_list;
+    a.cell_t1 = A.t1;
+    a.fail_count = A.fail;
+    int grid = 0, block = 0;
+    size_t lds = 0;
+    A.S->cell_launch_shape(total, &grid, &block, &lds);
+    launch_cell_bdf(a, grid, block, lds, L.st);
+    HIP_OK(hipGetLastError());
+    for (Request* r : L.batch) {
+      if (r->steps_left != 1) continue;
+      size_t d = sizeof(double);
+      HIP_OK(hipMemcpyAsync(r->out_state, A.state + r->slot0 * A.ns, r->n * A.ns * d,
+                            hipMemcpyDeviceToHost, L.st));
+      if (r->out_stats) {
+        HIP_OK(hipMemcpyAsync(r->out_stats, A.stats + r->slot0 * kCellStats,
+                              r->n * kCellStats * d, hipMemcpyDeviceToHost, L.st));
+      }
+      HIP_OK(hipMemcpyAsync(A.h_fail + r->slot0, A.fail + r->slot0, r->n * sizeof(int),
+                            hipMemcpyDeviceToHost, L.st));
+    }
+    HIP_OK(hipEventRecord(L.ev, L.st));
+    L.busy = true;
+    L.cells = total;
+    any = true;
+    std::lock_guard<std::mutex> g(mu);
+    batches += 1;
+    batch_cells += (double)total;
+  }
+  return any;
+}
+
+bool Server::poll_lanes() {
+  bool any = false;
+  for (Lane& L : lanes) {
+    if (!L.busy) continue;
+    hipError_t e = hipEventQuery(L.ev);
+    if (e == hipErrorNotReady) continue;
+    HIP_OK(e);
+    L.busy = false;
+    any = true;
+    auto now = Clock::now();
+    std::vector<Request*> again;
+    {
+      std::lock_guard<std::mutex> g(mu);
+      cell_steps += (double)L.cells;
+      for (Request* r : L.batch) {
+        if (r->up_ev) {
+          (void)hipEventDestroy(r->up_ev);
+          r->up_ev = nullptr;
+        }
+        if (--r->steps_left > 0) {
+          r->state = kReady;
+          again.push_back(r);
+          continue;
+        }
+        Arena& A = *mechs[r->mech];
+        for (long c = 0; c < r->n; ++c) r->failed |= A.h_fail[r->slot0 + c] != 0;
+        A.release(r->slot0, r->n);
+        r->t_done = now;
+        r->state = kDone;
+        n_done += 1;
+        n_failed += r->failed ? 1 : 0;
+      }
+    }
+    // requests between timesteps rejoin the ready queue in arrival order
+    for (Request* r : again) {
+      auto pos = ready.begin();
+      while (pos != ready.end() && (*pos)->id < r->id) ++pos;
+      ready.insert(pos, r);
+    }
+    L.batch.clear();
+    cv_done.notify_all();
+  }
+  return any;
+}
+
+void Server::loop() {
+  HIP_OK(hipSetDevice(dev));
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> g(mu);
+      while (!inbox.empty()) {
+        pending.push_back(inbox.front());
+        inbox.pop_front();
+      }
+      bool idle = pending.empty() && ready.empty();
+      for (Lane& L : lanes) idle = idle && !L.busy;
+      if (idle) {
+        if (stop) break;
+        cv_work.wait_for(g, std::chrono::milliseconds(2));
+        continue;
+      }
+    }
+    bool progress = poll_lanes();
+    progress |= admit();
+    progress |= launch();
+    if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+Server* SV(void* p) { return static_cast<Server*>(p); }
+
+}  // namespace
+}  // namespace camp
+
+using namespace camp;
+
+extern "C" void* camp_server_new(int n_streams, long max_batch_cells) {
+  if (n_streams < 1 || n_streams > 8) die("camp_server_new: n_streams must be 1..8");
+  if (max_batch_cells < 1) die("camp_server_new: max_batch_cells must be positive");
+  Server* s = new Server();
+  HIP_OK(hipGetDevice(&s->dev));
+  s->max_batch = max_batch_cells;
+  HIP_OK(hipStreamCreateWithFlags(&s->up_st, hipStreamNonBlocking));
+  s->lanes.resize(n_streams);
+  for (Lane& L : s->lanes) {
+    HIP_OK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&L.ev, hipEventDisableTiming));
+    HIP_OK(hipMalloc(&L.d_counter, sizeof(int)));
+  }
+  s->thr = std::thread([s] { s->loop(); });
+  return s;
+}
+
+extern "C" int camp_solver_rate_layout(void* solver_data, int* types, int* uids) {
+  Model& m = static_cast<Solver*>(solver_data)->m;
+  int n = 0;
+  for (int r = 0; r < m.n_rxn; ++r) {
+    int* id = m.rint(r);
+    int t = id[0], uid;
+    if (t == kRxnPhotolysis) uid = id[3];
+    else if (t == kRxnEmission || t == kRxnFirstOrderLoss) uid = id[1];
+    else continue;
+    if (types) types[n] = t;
+    if (uids) uids[n] = uid;
+    ++n;
+  }
+  return n;
+}
+
+extern "C" int camp_server_add_mechanism(void* server, void* solver_data, long max_cells) {
+  Server* s = SV(server);
+  Solver* S = static_cast<Solver*>(solver_data);
+  if (!S->m.initialized) die("camp_server_add_mechanism: solver is not initialised");
+  S->ensure_device();
+  std::unique_ptr<Arena> A(new Arena());
+  A->S = S;
+  A->ns = S->m.prog.ns;
+  A->n_env = S->m.prog.n_env;
+  A->nnz = S->m.prog.nnz;
+  A->nv = S->m.prog.nv;
+  std::vector<int> off;
+  for (int r = 0; r < S->m.n_rxn; ++r) {
+    int t = S->m.rint(r)[0];
+    if (t == kRxnPhotolysis || t == kRxnEmission || t == kRxnFirstOrderLoss)
+      off.push_back(S->m.rxn_env_idx[r] + 1);  // BASE_RATE slot
+  }
+  A->n_upd = (int)off.size();
+  size_t per_cell = sizeof(double) * ((size_t)A->ns + 2 + A->n_env + A->nnz + 2 * A->nv +
+                                      kCellStats + 1 + A->n_upd) +
+                    2 * sizeof(int) + s->lanes.size() * sizeof(int);
+  if (max_cells <= 0) {
+    size_t fr = 0, tot = 0;
+    HIP_OK(hipMemGetInfo(&fr, &tot));
+    max_cells = (long)std::min<size_t>(fr / 2 / per_cell, (size_t)1 << 24);
+  }
+  if (max_cells < 1) die("camp_server_add_mechanism: no device memory for a cell arena");
+  long n = max_cells;
+  A->cap = n;
+  A->state = dev_alloc<double>((size_t)n * A->ns, A->owned);
+  A->env = dev_alloc<double>((size_t)n * 2, A->owned);
+  A->renv = dev_alloc<double>((size_t)n * A->n_env, A->owned);
+  A->Js = dev_alloc<double>((size_t)n * A->nnz, A->owned);
+  A->Jy = dev_alloc<double>((size_t)n * A->nv, A->owned);
+  A->Jf = dev_alloc<double>((size_t)n * A->nv, A->owned);
+  A->stats = dev_alloc<double>((size_t)n * kCellStats, A->owned);
+  A->t1 = dev_alloc<double>((size_t)n, A->owned);
+  A->rates = dev_alloc<double>((size_t)n * std::max(A->n_upd, 1), A->owned);
+  A->fail = dev_alloc<int>((size_t)n, A->owned);
+  A->upd_off = dev_upload(off, A->owned);
+  HIP_OK(hipHostMalloc((void**)&A->h_fail, (size_t)n * sizeof(int), hipHostMallocDefault));
+  A->free_ranges[0] = n;
+  std::lock_guard<std::mutex> g(s->mu);
+  s->mechs.push_back(std::move(A));
+  return (int)s->mechs.size() - 1;
+}
+
+extern "C" long camp_server_capacity(void* server, int mech) {
+  Server* s = SV(server);
+  std::lock_guard<std::mutex> g(s->mu);
+  if (mech < 0 || mech >= (int)s->mechs.size()) return -1;
+  return s->mechs[mech]->cap;
+}
+
+extern "C" long camp_server_submit(void* server, int mech, int n_cells, const double* state,
+                                   const double* env, const double* base_rates, double dt,
+                                   int n_steps, double* out_state, double* out_stats) {
+  Server* s = SV(server);
+  std::lock_guard<std::mutex> g(s->mu);
+  if (mech < 0 || mech >= (int)s->mechs.size()) return -1;
+  Arena& A = *s->mechs[mech];
+  if (n_cells < 1 || n_cells > A.cap || n_steps < 1 || !(dt > 0.0)) return -1;
+  if (A.n_upd > 0 && !base_rates) return -1;
+  std::unique_ptr<Request> r(new Request());
+  r->id = s->next_id++;
+  r->mech = mech;
+  r->n = n_cells;
+  r->dt = dt;
+  r->steps_left = n_steps;
+  r->in_state = state;
+  r->in_env = env;
+  r->in_rates = base_rates;
+  r->out_state = out_state;
+  r->out_stats = out_stats;
+  r->t_submit = Clock::now();
+  s->inbox.push_back(r.get());
+  long id = r->id;
+  s->reqs[id] = std::move(r);
+  s->cv_work.notify_one();
+  return id;
+}
+
+extern "C" int camp_server_wait(void* server, long request, double* latency_s) {
+  Server* s = SV(server);
+  std::unique_lock<std::mutex> g(s->mu);
+  auto it = s->reqs.find(request);
+  if (it == s->reqs.end()) return 1;
+  Request* r = it->second.get();
+  s->cv_done.wait(g, [r] { return r->state == kDone; });
+  if (latency_s) latency_s[0] = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+  int rc = r->failed ? 1 : 0;
+  s->reqs.erase(it);
+  return rc;
+}
+
+extern "C" int camp_server_set_pool(void* server, int mech, long n_cells, const double* state,
+                                    const double* env, const double* base_rates) {
+  Server* s = SV(server);
+  Arena* A;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (mech < 0 || mech >= (int)s->mechs.size() || n_cells < 1) return 1;
+    A = s->mechs[mech].get();
+  }
+  if (A->pool_n) return 1;
+  size_t d = sizeof(double);
+  HIP_OK(hipHostMalloc((void**)&A->pool_state, n_cells * A->ns * d, hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&A->pool_out, n_cells * A->ns * d, hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&A->pool_env, n_cells * 2 * d, hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&A->pool_rates, n_cells * std::max(A->n_upd, 1) * d,
+                       hipHostMallocDefault));
+  std::memcpy(A->pool_state, state, n_cells * A->ns * d);
+  std::memcpy(A->pool_env, env, n_cells * 2 * d);
+  if (A->n_upd > 0) std::memcpy(A->pool_rates, base_rates, n_cells * A->n_upd * d);
+  A->pool_n = n_cells;
+  return 0;
+}
+
+extern "C" int camp_server_replay(void* server, int n_requests, const double* t_arrive,
+                                  const int* mech, const int* n_cells, const long* offset,
+                                  const double* dt, const int* n_steps, double* latency_s,
+                                  double* makespan_s) {
+  Server* s = SV(server);
+  std::vector<long> ids(n_requests, -1);
+  auto t0 = Clock::now();
+  for (int i = 0; i < n_requests; ++i) {
+    Arena* A;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      if (mech[i] < 0 || mech[i] >= (int)s->mechs.size()) return -1;
+      A = s->mechs[mech[i]].get();
+    }
+    if (!A->pool_n || offset[i] < 0 || offset[i] + n_cells[i] > A->pool_n) return -1;
+    std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
+                                           std::chrono::duration<double>(t_arrive[i])));
+    long o = offset[i];
+    ids[i] = camp_server_submit(server, mech[i], n_cells[i], A->pool_state + o * A->ns,
+                                A->pool_env + o * 2, A->pool_rates + o * std::max(A->n_upd, 1),
+                                dt[i], n_steps[i], A->pool_out + o * A->ns, nullptr);
+    if (ids[i] < 0) return -1;
+  }
+  int failed = 0;
+  Clock::time_point last = t0;
+  for (int i = 0; i < n_requests; ++i) {
+    std::unique_lock<std::mutex> g(s->mu);
+    Request* r = s->reqs[ids[i]].get();
+    s->cv_done.wait(g, [r] { return r->state == kDone; });
+    latency_s[i] = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+    last = std::max(last, r->t_done);
+    failed += r->failed ? 1 : 0;
+    s->reqs.erase(ids[i]);
+  }
+  if (makespan_s) makespan_s[0] = std::chrono::duration<double>(last - t0).count();
+  return failed;
+}
+
+extern "C" void camp_server_counters(void* server, double* out6) {
+  Server* s = SV(server);
+  std::lock_guard<std::mutex> g(s->mu);
+  out6[0] = s->n_done;
+  out6[1] = s->n_failed;
+  out6[2] = s->cell_steps;
+  out6[3] = s->batches;
+  out6[4] = s->batches > 0 ? s->batch_cells / s->batches : 0.0;
+  out6[5] = s->peak;
+}
+
+extern "C" void camp_server_free(void* server) {
+  Server* s = SV(server);
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  s->thr.join();
+  for (Lane& L : s->lanes) {
+    (void)hipStreamSynchronize(L.st);
+    (void)hipStreamDestroy(L.st);
+    (void)hipEventDestroy(L.ev);
+    (void)hipFree(L.d_counter);
+    if (L.d_list) (void)hipFree(L.d_list);
+    if (L.h_list) (void)hipHostFree(L.h_list);
+  }
+  (void)hipStreamSynchronize(s->up_st);
+  (void)hipStreamDestroy(s->up_st);
+  for (auto& A : s->mechs) {
+    for (void* p : A->owned) (void)hipFree(p);
+    (void)hipHostFree(A->h_fail);
+    if (A->pool_n) {
+      (void)hipHostFree(A->pool_state);
+      (void)hipHostFree(A->pool_out);
+      (void)hipHostFree(A->pool_env);
+      (void)hipHostFree(A->pool_rates);
+    }
+  }
+  delete s;
+}

@@ -23,94 +23,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/camp_solver.h"
-#include "camp_kernels.hpp"
-#include "camp_model.hpp"
+#include "camp_solver_impl.hpp"
 
 namespace camp {
-
-[[noreturn]] static void die(const char* what) {
-  std::fprintf(stderr, "\n\nERROR %s\n\n", what);
-  std::fflush(stderr);
-  std::exit(EXIT_FAILURE);
-}
-
-#define HIP_OK(x)                                                                         \
-  do {                                                                                    \
-    hipError_t e_ = (x);                                                                  \
-    if (e_ != hipSuccess) {                                                               \
-      std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
-      std::exit(EXIT_FAILURE);                                                            \
-    }                                                                                     \
-  } while (0)
-
-template <class T>
-static T* dev_upload(const std::vector<T>& v, std::vector<void*>& owned) {
-  T* p = nullptr;
-  size_t n = std::max<size_t>(v.size(), 1);
-  HIP_OK(hipMalloc(&p, n * sizeof(T)));
-  if (!v.empty()) HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  owned.push_back(p);
-  return p;
-}
-
-template <class T>
-static T* dev_alloc(size_t n, std::vector<void*>& owned) {
-  T* p = nullptr;
-  HIP_OK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
-  HIP_OK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
-  owned.push_back(p);
-  return p;
-}
-
-struct BlockStats {
-  long nst = 0, nfe = 0, nsetups = 0, netf = 0, nni = 0, ncfn = 0, nje = 0;
-  int flag = 0;
-  double hu = 0.0, next_h = 0.0;
-};
-
-struct Solver {
-  Model m;
-  int mode = CAMP_MODE_PER_CELL;
-  int cells_per_block = 1;
-  int guess_on = 1;
-  bool renv_dirty = true;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_kernel_ms = 0.0;
-  std::vector<void*> owned;  // device allocations
-  DevProgram P{};
-  // per-cell persistent buffers
-  double *d_state = nullptr, *d_env = nullptr, *d_renv = nullptr;
-  double *d_Js = nullptr, *d_Jy = nullptr, *d_Jf = nullptr, *d_stats = nullptr;
-  int *d_counter = nullptr, *d_fail = nullptr, *d_var_type = nullptr;
-  // block-path buffers
-  double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_y = nullptr,
-         *d_tempv = nullptr, *d_ftemp = nullptr, *d_last = nullptr, *d_acinit = nullptr,
-         *d_gh1 = nullptr, *d_gh2 = nullptr, *d_LU = nullptr, *d_red = nullptr,
-         *d_jtmp = nullptr;
-  std::vector<double> h_red;
-  BlockStats bstats;
-  bool stats_from_cells = false;
-
-  ~Solver() {
-    for (void* p : owned) hipFree(p);
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
-    if (stream) hipStreamDestroy(stream);
-  }
-
-  void setup_device();
-  // Device buffers are created on first use, so that solver_initialize (and
-  // the pattern getters) work on a host without a GPU.
-  void ensure_device() {
-    if (!stream) setup_device();
-  }
-  void upload_renv();
-  int run(double* d_state_in, double* d_env_in, double t0, double t1);
-  int run_cells(double* st, double* env, double t0, double t1);
-  int run_block(double* st, double* env, double t0, double t1);
-};
 
 static inline int nblk(long n, int b = 256) { return (int)((n + b - 1) / b); }
 
@@ -212,9 +127,31 @@ int Solver::run(double* st, double* env, double t0, double t1) {
   return mode == CAMP_MODE_BLOCK ? run_block(st, env, t0, t1) : run_cells(st, env, t0, t1);
 }
 
-int Solver::run_cells(double* st, double* env, double t0, double t1) {
+CellBdfArgs Solver::cell_args() const {
   CellBdfArgs a;
   a.P = P;
+  a.var_type = d_var_type;
+  a.rtol = m.rel_tol;
+  a.max_steps = m.max_steps;
+  a.max_conv_fails = m.max_conv_fails;
+  a.guess_on = guess_on;
+  return a;
+}
+
+void Solver::cell_launch_shape(long nq, int* grid, int* block, size_t* lds) const {
+  int cpb = cells_per_block;
+  *block = kWave * cpb;
+  *lds = (size_t)cpb * cell_bdf_lds_doubles(P.ns, P.nv, P.n_env, P.lu_nnz) * sizeof(double);
+  int dev = 0, ncu = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int per_cu = cell_bdf_blocks_per_cu(*block, *lds);
+  if (per_cu <= 0) die("per-cell kernel does not fit on a compute unit (LDS)");
+  *grid = (int)std::max<long>(1, std::min<long>((long)per_cu * ncu, (nq + cpb - 1) / cpb));
+}
+
+int Solver::run_cells(double* st, double* env, double t0, double t1) {
+  CellBdfArgs a = cell_args();
   a.nc = m.nc;
   a.state = st;
   a.env = env;
@@ -222,24 +159,19 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   a.Js = d_Js;
   a.Jy = d_Jy;
   a.Jf = d_Jf;
-  a.var_type = d_var_type;
-  a.rtol = m.rel_tol;
   a.t0 = t0;
   a.t1 = t1;
-  a.max_steps = m.max_steps;
-  a.max_conv_fails = m.max_conv_fails;
-  a.guess_on = guess_on;
   a.counter = d_counter;
   a.stats = d_stats;
-  int cpb = cells_per_block;
-  int block = kWave * cpb;
-  size_t lds = (size_t)cpb * cell_bdf_lds_doubles(P.ns, P.nv, P.n_env, P.lu_nnz) * sizeof(double);
-  int dev = 0, ncu = 0;
+  if (profiling) {
+    if (!d_prof) d_prof = dev_alloc<double>((size_t)m.nc * kCellProf, owned);
+    a.prof = d_prof;
+  }
+  int grid = 0, block = 0;
+  size_t lds = 0;
+  cell_launch_shape(m.nc, &grid, &block, &lds);
+  int dev = 0;
   HIP_OK(hipGetDevice(&dev));
-  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  int per_cu = cell_bdf_blocks_per_cu(block, lds);
-  if (per_cu <= 0) die("per-cell kernel does not fit on a compute unit (LDS)");
-  int grid = std::min<long>((long)per_cu * ncu, (m.nc + cpb - 1) / cpb);
   HIP_OK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
   HIP_OK(hipEventRecord(ev0, stream));
   launch_cell_bdf(a, grid, block, lds, stream);
@@ -266,6 +198,19 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   }
   bstats = s;
   stats_from_cells = true;
+  if (profiling) {
+    std::vector<double> pr((size_t)m.nc * kCellProf);
+    HIP_OK(hipMemcpy(pr.data(), d_prof, pr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    int khz = 0;
+    HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, dev));
+    double cf = 0.0, cj = 0.0;
+    for (int c = 0; c < m.nc; ++c) {
+      cf += pr[(size_t)c * kCellProf];
+      cj += pr[(size_t)c * kCellProf + 1];
+    }
+    rhs_time_s = khz > 0 ? cf / (khz * 1.0e3) : 0.0;
+    jac_time_s = khz > 0 ? cj / (khz * 1.0e3) : 0.0;
+  }
   return fail ? CAMP_SOLVER_FAIL : CAMP_SOLVER_SUCCESS;
 }
 
@@ -915,12 +860,15 @@ extern "C" void solver_get_statistics(void* solver_data, int* solver_flag, int* 
   *Jac_eval_fails = 0;
   *RHS_evals_total = -1;
   *Jac_evals_total = -1;
-  *RHS_time__s = 0.0;
-  *Jac_time__s = 0.0;
+  *RHS_time__s = SD(solver_data)->rhs_time_s;
+  *Jac_time__s = SD(solver_data)->jac_time_s;
   *max_loss_precision = 0.0;
 }
 
-extern "C" void solver_reset_timers(void*) {}
+extern "C" void solver_reset_timers(void* solver_data) {
+  SD(solver_data)->rhs_time_s = 0.0;
+  SD(solver_data)->jac_time_s = 0.0;
+}
 
 extern "C" void solver_free(void* solver_data) { delete SD(solver_data); }
 
@@ -1078,7 +1026,7 @@ extern "C" int camp_eval_deriv(void* solver_data, const double* state, const dou
   HIP_OK(hipMemcpyAsync(deriv, d_f, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToHost,
                         s->stream));
   HIP_OK(hipStreamSynchronize(s->stream));
-  for (void* p : tmp) hipFree(p);
+  for (void* p : tmp) (void)hipFree(p);
   return fail;
 }
 
@@ -1105,7 +1053,7 @@ extern "C" int camp_eval_jac(void* solver_data, const double* state, const doubl
   HIP_OK(hipMemcpyAsync(jac, s->d_Js, (size_t)nc * P.nnz * sizeof(double), hipMemcpyDeviceToHost,
                         s->stream));
   HIP_OK(hipStreamSynchronize(s->stream));
-  for (void* p : tmp) hipFree(p);
+  for (void* p : tmp) (void)hipFree(p);
   return fail;
 }
 
@@ -1130,7 +1078,7 @@ extern "C" int camp_eval_lu_solve(void* solver_data, const double* jac, double g
   HIP_OK(hipMemcpyAsync(&fail, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipMemcpyAsync(b, d_b, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipStreamSynchronize(s->stream));
-  for (void* p : tmp) hipFree(p);
+  for (void* p : tmp) (void)hipFree(p);
   return fail;
 }
 
@@ -1142,6 +1090,18 @@ extern "C" void camp_solver_get_cell_stats(void* solver_data, double* stats) {
 }
 
 extern "C" double camp_solver_last_kernel_ms(void* solver_data) { return SD(solver_data)->last_kernel_ms; }
+
+extern "C" void camp_solver_set_profiling(void* solver_data, int on) {
+  SD(solver_data)->profiling = on != 0;
+}
+
+extern "C" int camp_solver_get_cell_profile(void* solver_data, double* out) {
+  Solver* s = SD(solver_data);
+  if (!s->d_prof) return 1;
+  HIP_OK(hipMemcpy(out, s->d_prof, (size_t)s->m.nc * camp::kCellProf * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  return 0;
+}
 
 extern "C" void camp_solver_work_model(void* solver_data, double* out8) {
   const camp::Program& G = SD(solver_data)->m.prog;
@@ -1199,7 +1159,7 @@ extern "C" int camp_solver_time_fj(void* solver_data, const double* d_state, con
   HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   *ms_jac = ms / reps;
   HIP_OK(hipStreamSynchronize(s->stream));
-  for (void* p : tmp) hipFree(p);
+  for (void* p : tmp) (void)hipFree(p);
   return 0;
 }
 

@@ -1,0 +1,112 @@
+// Host-side solver object shared by the C ABI (camp_solver.cpp) and the
+// request server (camp_server.cpp): the Model, its kernel program uploaded to
+// HBM, per-cell persistent buffers and the launch of the per-cell kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../include/camp_solver.h"
+#include "camp_kernels.hpp"
+#include "camp_model.hpp"
+
+namespace camp {
+
+[[noreturn]] inline void die(const char* what) {
+  std::fprintf(stderr, "\n\nERROR %s\n\n", what);
+  std::fflush(stderr);
+  std::exit(EXIT_FAILURE);
+}
+
+#define HIP_OK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      std::exit(EXIT_FAILURE);                                                            \
+    }                                                                                     \
+  } while (0)
+
+template <class T>
+inline T* dev_upload(const std::vector<T>& v, std::vector<void*>& owned) {
+  T* p = nullptr;
+  size_t n = std::max<size_t>(v.size(), 1);
+  HIP_OK(hipMalloc(&p, n * sizeof(T)));
+  if (!v.empty()) HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  owned.push_back(p);
+  return p;
+}
+
+template <class T>
+inline T* dev_alloc(size_t n, std::vector<void*>& owned) {
+  T* p = nullptr;
+  HIP_OK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  HIP_OK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  owned.push_back(p);
+  return p;
+}
+
+struct BlockStats {
+  long nst = 0, nfe = 0, nsetups = 0, netf = 0, nni = 0, ncfn = 0, nje = 0;
+  int flag = 0;
+  double hu = 0.0, next_h = 0.0;
+};
+
+struct Solver {
+  Model m;
+  int mode = CAMP_MODE_PER_CELL;
+  int cells_per_block = 1;
+  int guess_on = 1;
+  bool renv_dirty = true;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_kernel_ms = 0.0;
+  std::vector<void*> owned;  // device allocations
+  DevProgram P{};
+  // per-cell persistent buffers
+  double *d_state = nullptr, *d_env = nullptr, *d_renv = nullptr;
+  double *d_Js = nullptr, *d_Jy = nullptr, *d_Jf = nullptr, *d_stats = nullptr;
+  int *d_counter = nullptr, *d_fail = nullptr, *d_var_type = nullptr;
+  // block-path buffers
+  double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_y = nullptr,
+         *d_tempv = nullptr, *d_ftemp = nullptr, *d_last = nullptr, *d_acinit = nullptr,
+         *d_gh1 = nullptr, *d_gh2 = nullptr, *d_LU = nullptr, *d_red = nullptr,
+         *d_jtmp = nullptr;
+  std::vector<double> h_red;
+  BlockStats bstats;
+  bool stats_from_cells = false;
+  // per-cell phase timing (per-cell mode): shader cycles, and the summed
+  // seconds reported as RHS_time__s / Jac_time__s
+  bool profiling = false;
+  double* d_prof = nullptr;
+  double rhs_time_s = 0.0, jac_time_s = 0.0;
+
+  ~Solver() {
+    for (void* p : owned) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void setup_device();
+  // Device buffers are created on first use, so that solver_initialize (and
+  // the pattern getters) work on a host without a GPU.
+  void ensure_device() {
+    if (!stream) setup_device();
+  }
+  void upload_renv();
+  // Per-cell kernel arguments with the solver's program and controls filled
+  // in (buffers left to the caller), and the launch shape for nq queued cells.
+  CellBdfArgs cell_args() const;
+  void cell_launch_shape(long nq, int* grid, int* block, size_t* lds) const;
+  int run(double* d_state_in, double* d_env_in, double t0, double t1);
+  int run_cells(double* st, double* env, double t0, double t1);
+  int run_block(double* st, double* env, double t0, double t1);
+};
+
+
+}  // namespace camp

@@ -171,6 +171,16 @@ void camp_solver_get_cell_stats(void *solver_data, double *stats);
  * the launch stream. */
 double camp_solver_last_kernel_ms(void *solver_data);
 
+/* Per-cell phase timing in per-cell mode (the device-side counterpart of the
+ * RHS/Jac timers that solver_get_statistics reports, camp_solver.c:725-791):
+ * when on, each solver_run records per cell the shader-clock cycles spent in
+ * f(y), J(y), LU factorisation, triangular solves and the whole integration,
+ * and solver_get_statistics reports the f and J sums as RHS_time__s and
+ * Jac_time__s.  get_cell_profile copies [n_cells][5] cycles; returns 1 if no
+ * profiled run has happened. */
+void camp_solver_set_profiling(void *solver_data, int on);
+int camp_solver_get_cell_profile(void *solver_data, double *out);
+
 /* Algorithmic work per cell and evaluation, for roofline accounting:
  * out8 = [flops of f(y) (with the Jacobian-estimate blend), of J(y), of one
  * LU factorisation, of one triangular solve, then the HBM bytes the same

@@ -133,21 +133,56 @@ def test_box_model_per_cell_matches_oracle(oracle_lib):
     assert ok, worst
 
 
-@pytest.mark.parametrize("mode", [MODE_PER_CELL, MODE_BLOCK])
-def test_cb05_solve_matches_oracle(oracle_lib, mode):
-    n = 32 if mode == MODE_PER_CELL else 8
-    mech, cells = cb05_inputs(n)
-    sol = CampSolver(mech, n, mode=mode)
-    orc = oracle_lib.Oracle(mech, n)
+def _cb05_pair(oracle_lib, n, mode, rel_tol, abs_tol, nsteps=2, **cell_kw):
+    mech, cells = cb05_inputs(n, **cell_kw)
+    atol = np.full(mech.n_state, abs_tol)
+    sol = CampSolver(mech, n, mode=mode, rel_tol=rel_tol, abs_tol=atol)
+    orc = oracle_lib.Oracle(mech, n, rel_tol=rel_tol, abs_tol=atol)
     _set_rates_both(sol, orc, cells.rates)
     a = cells.state.copy()
     b = cells.state.copy()
-    for _ in range(2):
+    for _ in range(nsteps):
         assert sol.solve(a, cells.env, 0.0, 180.0) == 0
         assert orc.run(b, cells.env, 0.0, 180.0, mode=0 if mode == MODE_BLOCK else 1) == 0
+    return mech, cells, a, b
+
+
+# At rtol 1e-4 the 1e-4 bar is below the integration error itself: the
+# oracle against itself with T perturbed by 1e-15 (relative) moves by up to
+# 80x the bar on these cells, so no two BDF implementations can meet it
+# there.  At rtol 1e-8 / atol 1e-12 the same perturbation moves the oracle
+# by < 0.01 of the bar, so the bar pins the solution both paths converge to.
+@pytest.mark.parametrize("mode", [MODE_PER_CELL, MODE_BLOCK])
+def test_cb05_solve_matches_oracle(oracle_lib, mode):
+    n = 32 if mode == MODE_PER_CELL else 8
+    # the shared-step block mode cannot finish stressed cells in max_steps
+    kw = dict(night_fraction=0.5, stressed_fraction=0.1) if mode == MODE_PER_CELL else {}
+    _, _, a, b = _cb05_pair(oracle_lib, n, mode, 1.0e-8, 1.0e-12, **kw)
     ok, worst = _compare_final(a, b)
-    print("mode", mode, "worst scaled diff", worst, sol.stats(), orc.stats())
+    print("mode", mode, "worst scaled diff", worst)
     assert ok, worst
+
+
+@pytest.mark.parametrize("mode", [MODE_PER_CELL, MODE_BLOCK])
+def test_cb05_rtol_1e4_error_no_worse_than_oracle(oracle_lib, mode):
+    """At the workload tolerance both paths are compared with a converged
+    solution (oracle at rtol 1e-10): the HIP path's worst error, in units of
+    the 1e-4 / 1e-8 bar, may not exceed twice the oracle's own."""
+    n = 32 if mode == MODE_PER_CELL else 8
+    mech, cells, a, b = _cb05_pair(oracle_lib, n, mode, 1.0e-4, 1.0e-8)
+    atol = np.full(mech.n_state, 1.0e-14)
+    tight = oracle_lib.Oracle(mech, n, rel_tol=1.0e-10, abs_tol=atol, max_steps=200000)
+    tight.set_rates(cells.rates)
+    t = cells.state.copy()
+    for _ in range(2):
+        assert tight.run(t, cells.env, 0.0, 180.0, mode=1) == 0
+
+    def err(x):
+        return float(np.max(np.abs(x - t) / (SOLVE_RTOL * np.abs(t) + SOLVE_ATOL)))
+
+    e_gpu, e_orc = err(a), err(b)
+    print("mode", mode, "error in bar units: hip %.3g oracle %.3g" % (e_gpu, e_orc))
+    assert e_gpu <= 2.0 * max(e_orc, 1.0)
 
 
 def test_cell_alone_equals_cell_in_mix():
