@@ -6,9 +6,15 @@ loss; mechanism seed 1) over 100,000 cells (cell seed 2) of uneven
 stiffness: 50% night cells (photolysis J = 0) and 10% with concentrations
 scaled by 1e3; FP64, rtol 1e-4, atol 1e-8 ppm.  One step = one 180 s
 solver_run over all cells, each cell with its own BDF order, step size and
-Newton control, on state already resident in HBM.  K consecutive steps are
-timed from the seeded initial state (the first step carries the stiff
-start-up transient); warmup runs the same steps on a separate solver.
+Newton control, on state already resident in HBM.  W warmup steps from the
+seeded initial state (the first carries the stiff start-up transient) run
+untimed on the measured solver, then K consecutive steps are timed.
+
+Serving leg: the same mechanism behind the request server.  Requests of
+1..50,000 cells (log-uniform), dt uniform 60..600 s and 1..4 timesteps draw
+their cells from the spun-up 100k-cell pool.  Capacity is measured with all
+requests arriving at once; then Poisson arrivals at 70% of that capacity
+give p50/p99 request latency (submission to results on the host).
 
 Prints one JSON line.  `roofline` is for the dominant kernel (k_cell_bdf,
 the persistent per-cell kernel): FP64 algorithmic flops (per-evaluation
@@ -16,7 +22,7 @@ counts of f, J, LU and triangular solves, times the evaluation counts the
 kernel itself reports) over the kernel's HIP-event duration.
 `roofline_fj` gives the plain-path f(y) and J(y) kernels against HBM.
 `cpu_baseline` is the CPU oracle (a C restatement of the reference path)
-on a bounded sample of the same cells, on this host's cores.
+on a bounded sample of the same cells and steps, on this host's cores.
 """
 import argparse
 import json
@@ -42,37 +48,92 @@ def make_inputs(n_cells):
 
 
 def _cpu_worker(args):
-    n_cells, c0, c1, steps, dt = args
+    n_cells, c0, c1, warm, steps, dt = args
     from oracle.oracle import Oracle
-    mech, cells = make_inputs(n_cells)
-    sub = {k: v[c0:c1] for k, v in cells.rates.items()}
+    from camp_amd.mechanisms import cb05_scale_mechanism, synthetic_cells
+    mech = cb05_scale_mechanism(1)
+    # cells c0..c1-1 of the workload (the generator is batch-independent)
+    cells = synthetic_cells(mech, c1 - c0, seed=2, cell_offset=c0, night_fraction=0.5,
+                            stressed_fraction=0.1)
     o = Oracle(mech, c1 - c0)
-    o.set_rates(sub)
-    st = np.ascontiguousarray(cells.state[c0:c1])
-    env = np.ascontiguousarray(cells.env[c0:c1])
+    o.set_rates(cells.rates)
+    st = np.ascontiguousarray(cells.state)
+    env = np.ascontiguousarray(cells.env)
+    for _ in range(warm):
+        o.run(st, env, 0.0, dt, mode=1)
     t = time.perf_counter()
     for _ in range(steps):
         o.run(st, env, 0.0, dt, mode=1)
     return time.perf_counter() - t
 
 
-def cpu_baseline(n_cells, steps, dt, cores, per_core):
-    """Oracle per-cell solve of the first cores*per_core cells, K steps."""
+def cpu_baseline(n_cells, warm, steps, dt, cores, per_core):
+    """Oracle per-cell solve of cells spread over the workload: the same W
+    untimed and K timed steps as the GPU; rate = cells x K / slowest worker."""
     from oracle import oracle
     oracle.build(ref=False)
-    jobs = [(n_cells, i * per_core, (i + 1) * per_core, steps, dt) for i in range(cores)]
-    t = time.perf_counter()
+    jobs = []
+    for i in range(cores):
+        c0 = i * (n_cells // cores)
+        jobs.append((n_cells, c0, c0 + per_core, warm, steps, dt))
     if cores == 1:
-        _cpu_worker(jobs[0])
+        times = [_cpu_worker(jobs[0])]
     else:
         with mp.get_context("spawn").Pool(cores) as pool:
-            pool.map(_cpu_worker, jobs)
-    wall = time.perf_counter() - t
+            times = pool.map(_cpu_worker, jobs)
+    wall = max(times)
     return {"value": cores * per_core * steps / wall, "unit": "cell-timesteps/s", "cores": cores,
             "kind": "port",
-            "sample": "%d cells (cells 0..%d of the same workload) x %d steps of %g s, "
-                      "per-cell CVODE-algorithm BDF, %d processes" % (
-                          cores * per_core, cores * per_core - 1, steps, dt, cores)}
+            "sample": "%d cells (%d blocks of %d consecutive cells spread over the %d-cell "
+                      "workload) x %d timed steps of %g s after %d untimed, per-cell "
+                      "CVODE-algorithm BDF, one process per core" % (
+                          cores * per_core, cores, per_core, n_cells, steps, dt, warm)}
+
+
+def serving(mech, solver, pool_state, env, rates, n_requests, seed=3):
+    """Capacity at saturation, then Poisson arrivals at 70% of it."""
+    from camp_amd.core import CampSolver
+    from camp_amd.server import CampServer, base_rate_matrix
+    n_pool = pool_state.shape[0]
+    rng = np.random.default_rng(seed)
+    srv = CampServer(n_streams=4, max_batch_cells=1 << 17)
+    mid = srv.add_mechanism(CampSolver(mech, 1))
+    srv.set_pool(mid, pool_state, env, base_rate_matrix(solver, rates, n_pool))
+
+    def draw(k):
+        sizes = np.minimum(np.exp(rng.uniform(0.0, np.log(50000.0), size=k)), n_pool)
+        sizes = np.maximum(sizes.astype(np.int32), 1)
+        off = (rng.random(k) * (n_pool - sizes + 1)).astype(np.int64)
+        dts = rng.uniform(60.0, 600.0, size=k)
+        steps = rng.integers(1, 5, size=k).astype(np.int32)
+        return sizes, off, dts, steps
+
+    k_cap = max(8, n_requests // 4)
+    sizes, off, dts, steps = draw(k_cap)
+    lat, span, failed_cap = srv.replay(np.zeros(k_cap), np.zeros(k_cap, np.int32), sizes, off,
+                                       dts, steps)
+    capacity = float(np.sum(sizes * steps)) / span
+    sizes, off, dts, steps = draw(n_requests)
+    mean_work = (50000.0 - 1.0) / np.log(50000.0) * 2.5
+    rate = 0.7 * capacity / mean_work
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+    lat, span, failed = srv.replay(t_arr, np.zeros(n_requests, np.int32), sizes, off, dts, steps)
+    c = srv.counters()
+    srv.close()
+    return {"requests": int(n_requests), "load": 0.7,
+            "capacity_cell_timesteps_per_s": capacity,
+            "throughput_cell_timesteps_per_s": float(np.sum(sizes * steps)) / span,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+            "failed_requests": int(failed + failed_cap),
+            "mean_batch_cells": c["mean_batch_cells"], "batches": c["batches"],
+            "request_mix": "1..50000 cells log-uniform, dt U(60,600) s, 1..4 steps; gas "
+                           "mechanism only (the multi-phase mechanism is not built)"}
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def main():
@@ -83,9 +144,11 @@ def main():
     ap.add_argument("--cells", type=int, default=100000)
     ap.add_argument("--dt", type=float, default=180.0)
     ap.add_argument("--cells-per-block", type=int, default=0)
-    ap.add_argument("--cpu-cores", type=int, default=8)
-    ap.add_argument("--cpu-cells-per-core", type=int, default=8)
+    ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-cells-per-core", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=-1,
+                    help="requests in the serving leg (-1: sized to about 40 s, 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (cells shard "
@@ -107,30 +170,28 @@ def main():
             s.set_cells_per_block(args.cells_per_block)
         return s
 
-    # warmup: same steps on a separate solver instance (kernels loaded, HBM paged)
-    if args.warmup > 0:
-        w = fresh()
-        st = state0.clone()
-        torch.cuda.synchronize()
-        for _ in range(args.warmup):
-            rc = w.solve_device(st.data_ptr(), env.data_ptr(), 0.0, args.dt)
-            assert rc == 0, "warmup solve failed"
-        w.close()
-
     solver = fresh()
     st = state0.clone()
     torch.cuda.synchronize()
+    # warmup: the first W steps from the seeded state, untimed, same solver
+    warm_failed = 0
+    for _ in range(args.warmup):
+        rc = solver.solve_device(st.data_ptr(), env.data_ptr(), 0.0, args.dt)
+        warm_failed += int(np.sum(solver.cell_stats()[:, 7] < 0))
+        progress("warmup step done (%.0f ms kernel)" % solver.last_kernel_ms())
+    torch.cuda.synchronize()
     step_ms, kern_ms, stats = [], [], []
+    failed_cell_steps = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
         rc = solver.solve_device(st.data_ptr(), env.data_ptr(), 0.0, args.dt)
         step_ms.append(1e3 * (time.perf_counter() - ts))
         kern_ms.append(solver.last_kernel_ms())
-        stats.append(solver.cell_stats()[:, :7].sum(axis=0))
-        if rc != 0:
-            print("solve failed", file=sys.stderr)
-            sys.exit(1)
+        cs = solver.cell_stats()
+        stats.append(cs[:, :7].sum(axis=0))
+        failed_cell_steps += int(np.sum(cs[:, 7] < 0))
+        progress("timed step done (%.0f ms)" % step_ms[-1])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = args.cells * args.steps / elapsed
@@ -167,9 +228,22 @@ def main():
                 "frac": gbs_j / HBM_PEAK_GBS, "avg_launch_ms": ms_j,
                 "bytes_per_cell": bytes_["jac"]}}
 
+    serve = None
+    if args.serve_requests != 0:
+        n_req = args.serve_requests
+        if n_req < 0:
+            # about 40 s of work at the measured throughput, within 50..10000
+            n_req = int(np.clip(40.0 * value / ((50000.0 - 1.0) / np.log(50000.0) * 2.5),
+                                50, 10000))
+        progress("serving leg: %d requests" % n_req)
+        serve = serving(mech, solver, st.cpu().numpy(), cells.env, cells.rates, n_req)
+        progress("serving leg done")
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.cells, args.steps, args.dt, args.cpu_cores, args.cpu_cells_per_core)
+        progress("cpu baseline on %d cores" % args.cpu_cores)
+        cpu = cpu_baseline(args.cells, args.warmup, args.steps, args.dt, args.cpu_cores,
+                           args.cpu_cells_per_core)
 
     out = {
         "metric": "cell-timesteps/s at 100k cells, CB05-scale FP64; p50/p99 request latency",
@@ -178,13 +252,18 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "cb05scale72x190 batched per-cell BDF: %d cells (50%% night, "
-                               "10%% x1e3), %d x %g s steps from the seeded initial state, "
-                               "rtol 1e-4, atol 1e-8" % (args.cells, args.steps, args.dt),
+                               "10%% x1e3), %d x %g s steps after %d untimed from the seeded "
+                               "initial state, rtol 1e-4, atol 1e-8" % (
+                                   args.cells, args.steps, args.dt, args.warmup),
                    "n_cells": args.cells, "dt_s": args.dt, "mechanism_seed": 1, "cell_seed": 2,
                    "parallelism": "1 GPU"},
-        "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
-                       "p99": float(np.percentile(step_ms, 99)),
-                       "what": "one 180 s step over all cells"},
+        "latency_ms": ({"p50": serve["p50_ms"], "p99": serve["p99_ms"],
+                        "what": "serving request latency at 70% load"} if serve else None),
+        "step_ms": {"p50": float(np.percentile(step_ms, 50)),
+                    "p99": float(np.percentile(step_ms, 99)),
+                    "what": "one 180 s step over all cells"},
+        "failed_cell_steps": {"timed": failed_cell_steps, "warmup": warm_failed},
+        "serving": serve,
         "solver_stats_per_cell_step": {"steps": float(tot[0] / args.cells / args.steps),
                                        "rhs_evals": float(nfe / args.cells / args.steps),
                                        "lu_setups": float(nsetups / args.cells / args.steps),

@@ -138,6 +138,14 @@ typedef struct {
   /* stats */
   long st[8];
   double h_last, h_next;
+  /* aerosol phases, representations and sub models (camp_solver.c:200-349) */
+  int n_phase, n_added_phase, *ph_int, *ph_int_idx, *ph_flt_idx;
+  double *ph_flt;
+  int n_rep, n_added_rep, *rep_int, *rep_int_idx, *rep_flt_idx, *rep_env_idx, n_rep_env;
+  double *rep_flt, *rep_env;
+  int n_sub, n_added_sub, *sub_int, *sub_int_idx, *sub_flt_idx, *sub_env_idx, n_sub_env;
+  double *sub_flt, *sub_env;
+  double *J_params; /* [pjac.nnz], one cell */
 } Oracle;
 
 void *oracle_new(int n_state_var, int n_cells, const int *var_type, int n_rxn,
@@ -204,6 +212,964 @@ static int is_mass_action(int type) {
   return type == RXN_ARRHENIUS || type == RXN_TROE || type == RXN_PHOTOLYSIS;
 }
 
+static void td_add(Oracle *o, int i, long double c);
+/* ===================================================================== */
+/* Multi-phase chemistry: aerosol phases (aero_phase_solver.c), the       */
+/* modal/binned-mass representation (aero_reps/aero_rep_modal_binned_     */
+/* mass.c), SIMPOL phase transfer (rxns/rxn_SIMPOL_phase_transfer.c),     */
+/* aqueous equilibrium (rxns/rxn_aqueous_equilibrium.c) and the ZSR       */
+/* aerosol-water sub model (sub_models/sub_model_ZSR_aerosol_water.c).    */
+/* Like the reference, these keep per-cell intermediate values in their   */
+/* shared float arrays, so every use is preceded by the cell's update.    */
+/* ===================================================================== */
+#define RXN_AQUEOUS_EQUILIBRIUM 7
+#define RXN_SIMPOL_PHASE_TRANSFER 10
+#define AERO_REP_MODAL_BINNED_MASS 2
+#define SUB_MODEL_ZSR_AEROSOL_WATER 2
+#define CONSTANT 2
+#define PSSA 3
+#define UNIV_GAS_CONST 8.314472               /* util.h:17 */
+#define PI_ 3.14159265358979323846
+#define MINIMUM_MASS 1.0e-25L                 /* aero_phase_solver.c:19-24 */
+#define MINIMUM_MW 0.1L
+#define MINIMUM_DENSITY 1800.0L
+
+#define PH_INT(o, p) (&(o)->ph_int[(o)->ph_int_idx[p]])
+#define PH_FLT(o, p) (&(o)->ph_flt[(o)->ph_flt_idx[p]])
+
+static int phase_spec_counts(int t) { return t == VARIABLE || t == CONSTANT || t == PSSA; }
+
+/* aero_phase_get_used_jac_elem, aero_phase_solver.c:40-63 */
+static int phase_used_jac_elem(Oracle *o, int p, int state_var_id, char *flags) {
+  int *id = PH_INT(o, p);
+  int n = 0;
+  for (int i = 0; i < id[0]; ++i)
+    if (phase_spec_counts(id[1 + i])) {
+      flags[state_var_id + i] = 1;
+      ++n;
+    }
+  return n;
+}
+
+/* aero_phase_get_mass__kg_m3, aero_phase_solver.c:78-113 */
+static void phase_get_mass(Oracle *o, int p, const double *state, double *mass, double *mw,
+                           double *jm, double *jmw) {
+  int *id = PH_INT(o, p);
+  double *fd = PH_FLT(o, p);
+  long double l_mass = MINIMUM_MASS;
+  long double moles = MINIMUM_MASS / MINIMUM_MW;
+  int ij = 0;
+  for (int i = 0; i < id[0]; ++i) {
+    if (!phase_spec_counts(id[1 + i])) continue;
+    l_mass += state[i];
+    moles += state[i] / (long double)fd[i];
+    if (jm) jm[ij] = 1.0L;
+    if (jmw) jmw[ij] = 1.0L / fd[i];
+    ij++;
+  }
+  *mw = (double)l_mass / moles;
+  if (jmw)
+    for (int j = 0; j < ij; ++j) jmw[j] = (moles - jmw[j] * l_mass) / (moles * moles);
+  *mass = (double)l_mass;
+}
+
+/* aero_phase_get_volume__m3_m3, aero_phase_solver.c:128-150 */
+static void phase_get_volume(Oracle *o, int p, const double *state, double *volume,
+                             double *je) {
+  int *id = PH_INT(o, p);
+  double *fd = PH_FLT(o, p);
+  int n = id[0];
+  *volume = MINIMUM_MASS / MINIMUM_DENSITY;
+  int ij = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!phase_spec_counts(id[1 + i])) continue;
+    *volume += state[i] / fd[n + i];
+    if (je) je[ij++] = 1.0 / fd[n + i];
+  }
+}
+
+/* modal/binned-mass layout, aero_rep_modal_binned_mass.c:27-81 (d: int data
+ * after the type slot, f: float data, e: the cell's env data) */
+#define MB_BINNED 1
+#define MB_MODAL 2
+#define MB_NSEC(d) ((d)[0])
+#define MB_ID(d) ((d)[3])
+#define MB_ILOC(d, x) ((d)[4 + (x)] - 1)
+#define MB_FLOC(d, x) ((d)[4 + MB_NSEC(d) + (x)] - 1)
+#define MB_TYPE(d, x) ((d)[MB_ILOC(d, x)])
+#define MB_NBIN(d, x) ((d)[MB_ILOC(d, x) + 1])
+#define MB_NPH(d, x) ((d)[MB_ILOC(d, x) + 2])
+#define MB_PSTATE(d, x, y, b) ((d)[MB_ILOC(d, x) + 3 + (b) * MB_NPH(d, x) + (y)] - 1)
+#define MB_PMODEL(d, x, y, b) \
+  ((d)[MB_ILOC(d, x) + 3 + MB_NBIN(d, x) * MB_NPH(d, x) + (b) * MB_NPH(d, x) + (y)] - 1)
+#define MB_PNJAC(d, x, y, b) \
+  ((d)[MB_ILOC(d, x) + 3 + 2 * MB_NBIN(d, x) * MB_NPH(d, x) + (b) * MB_NPH(d, x) + (y)])
+#define MB_DP(d, f, x, b) ((f)[MB_FLOC(d, x) + (b) * 3])
+#define MB_GMD(d, e, x) ((e)[x])
+#define MB_GSD(d, e, x) ((e)[MB_NSEC(d) + (x)])
+#define MB_NCONC(d, f, x, b) ((f)[MB_FLOC(d, x) + (b) * 3 + 1])
+#define MB_RAD(d, f, x, b) ((f)[MB_FLOC(d, x) + (b) * 3 + 2])
+#define MB_PMASS(d, f, x, y, b) ((f)[MB_FLOC(d, x) + 3 * MB_NBIN(d, x) + (b) * MB_NPH(d, x) + (y)])
+#define MB_PMW(d, f, x, y, b) \
+  ((f)[MB_FLOC(d, x) + (3 + MB_NPH(d, x)) * MB_NBIN(d, x) + (b) * MB_NPH(d, x) + (y)])
+
+typedef struct {
+  int *d;
+  double *f, *e;
+} RepData;
+
+static RepData rep_data(Oracle *o, int r, int cell) {
+  RepData x;
+  int *id = &o->rep_int[o->rep_int_idx[r]];
+  if (id[0] != AERO_REP_MODAL_BINNED_MASS) {
+    fprintf(stderr, "oracle: unsupported aerosol representation type %d\n", id[0]);
+    exit(EXIT_FAILURE);
+  }
+  x.d = id + 1;
+  x.f = &o->rep_flt[o->rep_flt_idx[r]];
+  x.e = &o->rep_env[(size_t)cell * o->n_rep_env + o->rep_env_idx[r]];
+  return x;
+}
+
+/* aero_rep_modal_binned_mass_get_used_jac_elem, :84-113 */
+static int rep_used_jac_elem(Oracle *o, int r, int aero_phase_idx, char *flags) {
+  RepData x = rep_data(o, r, 0);
+  int *d = x.d;
+  int n = 0;
+  for (int s = 0; s < MB_NSEC(d) && aero_phase_idx >= 0; s++)
+    for (int b = 0; b < MB_NBIN(d, s) && aero_phase_idx >= 0; b++)
+      for (int p = 0; p < MB_NPH(d, s) && aero_phase_idx >= 0; p++) {
+        if (aero_phase_idx == 0)
+          for (int j = 0; j < MB_NPH(d, s); j++) {
+            MB_PNJAC(d, s, j, b) =
+                phase_used_jac_elem(o, MB_PMODEL(d, s, j, b), MB_PSTATE(d, s, j, b), flags);
+            n += MB_PNJAC(d, s, j, b);
+          }
+        aero_phase_idx -= 1;
+      }
+  return n;
+}
+
+/* aero_rep_modal_binned_mass_update_state, :183-266 */
+static void rep_update_state(Oracle *o, int cell, const double *state) {
+  for (int r = 0; r < o->n_rep; ++r) {
+    RepData x = rep_data(o, r, cell);
+    int *d = x.d;
+    double *f = x.f, *e = x.e;
+    for (int s = 0; s < MB_NSEC(d); s++) {
+      double volume, mass;
+      switch (MB_TYPE(d, s)) {
+        case MB_MODAL:
+          volume = 0.0;
+          for (int p = 0; p < MB_NPH(d, s); p++) {
+            const double *st = state + MB_PSTATE(d, s, p, 0);
+            phase_get_mass(o, MB_PMODEL(d, s, p, 0), st, &MB_PMASS(d, f, s, p, 0),
+                           &MB_PMW(d, f, s, p, 0), NULL, NULL);
+            double pv = 0.0;
+            phase_get_volume(o, MB_PMODEL(d, s, p, 0), st, &pv, NULL);
+            volume += pv;
+          }
+          MB_NCONC(d, f, s, 0) = volume * 6.0 /
+                                 (PI_ * pow(MB_GMD(d, e, s), 3) *
+                                  exp(9.0 / 2.0 * pow(log(MB_GSD(d, e, s)), 2)));
+          break;
+        case MB_BINNED:
+          for (int b = 0; b < MB_NBIN(d, s); b++) {
+            volume = 0.0;
+            for (int p = 0; p < MB_NPH(d, s); p++) {
+              const double *st = state + MB_PSTATE(d, s, p, b);
+              phase_get_mass(o, MB_PMODEL(d, s, p, b), st, &MB_PMASS(d, f, s, p, b),
+                             &MB_PMW(d, f, s, p, b), NULL, NULL);
+              double pv = 0.0;
+              phase_get_volume(o, MB_PMODEL(d, s, p, b), st, &pv, NULL);
+              volume += pv;
+            }
+            MB_NCONC(d, f, s, b) =
+                volume * 3.0 / (4.0 * PI_) / pow(MB_DP(d, f, s, b) / 2.0, 3);
+          }
+          break;
+      }
+      (void)mass;
+    }
+  }
+}
+
+/* aero_rep_modal_binned_mass_get_effective_radius__m, :282-309 */
+static void rep_eff_radius(Oracle *o, int r, int cell, int aero_phase_idx, double *radius,
+                           double *partial) {
+  RepData x = rep_data(o, r, cell);
+  int *d = x.d;
+  double *f = x.f;
+  for (int s = 0; s < MB_NSEC(d); s++)
+    for (int b = 0; b < MB_NBIN(d, s); b++) {
+      aero_phase_idx -= MB_NPH(d, s);
+      if (aero_phase_idx < 0) {
+        *radius = MB_RAD(d, f, s, b);
+        if (partial)
+          for (int p = 0; p < MB_NPH(d, s); ++p)
+            for (int k = 0; k < MB_PNJAC(d, s, p, b); ++k) *(partial++) = 0.0;
+        s = MB_NSEC(d);
+        break;
+      }
+    }
+}
+
+/* aero_rep_modal_binned_mass_get_number_conc__n_m3, :325-372 */
+static void rep_number_conc(Oracle *o, int r, int cell, const double *state,
+                            int aero_phase_idx, double *nconc, double *partial) {
+  RepData x = rep_data(o, r, cell);
+  int *d = x.d;
+  double *f = x.f, *e = x.e;
+  for (int s = 0; s < MB_NSEC(d) && aero_phase_idx >= 0; s++)
+    for (int b = 0; b < MB_NBIN(d, s) && aero_phase_idx >= 0; b++) {
+      aero_phase_idx -= MB_NPH(d, s);
+      if (aero_phase_idx < 0) {
+        *nconc = MB_NCONC(d, f, s, b);
+        if (partial)
+          for (int p = 0; p < MB_NPH(d, s); ++p) {
+            const double *st = state + MB_PSTATE(d, s, p, b);
+            double pv = 0.0;
+            phase_get_volume(o, MB_PMODEL(d, s, p, b), st, &pv, partial);
+            for (int k = 0; k < MB_PNJAC(d, s, p, b); ++k) {
+              switch (MB_TYPE(d, s)) {
+                case MB_MODAL:
+                  *(partial++) *= 6.0 / (PI_ * pow(MB_GMD(d, e, s), 3) *
+                                         exp(9.0 / 2.0 * pow(log(MB_GSD(d, e, s)), 2)));
+                  break;
+                case MB_BINNED:
+                  *(partial++) *= 3.0 / (4.0 * PI_) / pow(MB_DP(d, f, s, b) / 2.0, 3);
+                  break;
+              }
+            }
+          }
+        s = MB_NSEC(d);
+        break;
+      }
+    }
+}
+
+/* get_aero_phase_mass (:402-446) and get_aero_phase_avg_MW (:459-503):
+ * which = 0 -> mass, 1 -> average MW */
+static void rep_phase_prop(Oracle *o, int r, int cell, const double *state, int aero_phase_idx,
+                           int which, double *value, double *partial) {
+  RepData x = rep_data(o, r, cell);
+  int *d = x.d;
+  double *f = x.f;
+  for (int s = 0; s < MB_NSEC(d) && aero_phase_idx >= 0; ++s)
+    for (int b = 0; b < MB_NBIN(d, s) && aero_phase_idx >= 0; ++b) {
+      if (aero_phase_idx < 0 || aero_phase_idx >= MB_NPH(d, s)) {
+        aero_phase_idx -= MB_NPH(d, s);
+        continue;
+      }
+      for (int p = 0; p < MB_NPH(d, s); ++p) {
+        if (aero_phase_idx == 0) {
+          *value = which == 0 ? MB_PMASS(d, f, s, p, b) : MB_PMW(d, f, s, p, b);
+          if (partial) {
+            const double *st = state + MB_PSTATE(d, s, p, b);
+            double mass, mw;
+            phase_get_mass(o, MB_PMODEL(d, s, p, b), st, &mass, &mw,
+                           which == 0 ? partial : NULL, which == 1 ? partial : NULL);
+            partial += MB_PNJAC(d, s, p, b);
+          }
+        } else if (partial) {
+          for (int k = 0; k < MB_PNJAC(d, s, p, b); ++k) *(partial++) = 0.0;
+        }
+        aero_phase_idx -= 1;
+      }
+    }
+}
+
+/* util.h:19-68 */
+static double mean_speed(double T, double mw) {
+  return sqrt(8.0 * UNIV_GAS_CONST * T / (PI_ * mw));
+}
+static double mean_free_path(double D, double T, double mw) {
+  return 3.0 * D / mean_speed(T, mw);
+}
+static double transition_corr(double mfp, double r, double alpha) {
+  double Kn = mfp / r;
+  return (0.75 * alpha * (1.0 + Kn)) / (Kn * Kn + (1.0 + 0.283 * alpha) * Kn + 0.75 * alpha);
+}
+static double d_transition_corr_d_radius(double mfp, double r, double alpha) {
+  double Kn = mfp / r;
+  return (0.75 * alpha * mfp * (Kn * Kn + 2.0 * Kn + 1.0 + (0.283 - 0.75) * alpha)) /
+         pow(r * (Kn * Kn + (1.0 + 0.283 * alpha) * Kn + 0.75 * alpha), 2);
+}
+static double transition_rate(double D, double mfp, double r, double alpha) {
+  return 4.0 * PI_ * r * D * transition_corr(mfp, r, alpha);
+}
+static double d_transition_rate_d_radius(double D, double mfp, double r, double alpha) {
+  return 4.0 * PI_ * D * (transition_corr(mfp, r, alpha) + r * d_transition_corr_d_radius(mfp, r, alpha));
+}
+
+/* SIMPOL layout, rxn_SIMPOL_phase_transfer.c:31-74 (d after the type slot) */
+#define SP_N(d) ((d)[0])
+#define SP_GAS(d) ((d)[1] - 1)
+#define SP_AERO(d, x) ((d)[2 + (x)] - 1)
+#define SP_ACT(d, x) ((d)[2 + SP_N(d) + (x)] - 1)
+#define SP_PHASE(d, x) ((d)[2 + 2 * SP_N(d) + (x)] - 1)
+#define SP_REP(d, x) ((d)[2 + 3 * SP_N(d) + (x)] - 1)
+#define SP_DERIV(d, x) ((d)[2 + 4 * SP_N(d) + (x)])
+#define SP_GAS_ACT_JAC(d, x) ((d)[3 + 5 * SP_N(d) + (x)])
+#define SP_AERO_ACT_JAC(d, x) ((d)[3 + 6 * SP_N(d) + (x)])
+#define SP_JAC(d, x) ((d)[3 + 7 * SP_N(d) + (x)])
+#define SP_ILOC(d, x) ((d)[4 + 10 * SP_N(d) + (x)] - 1)
+#define SP_FLOC(d, x) ((d)[4 + 11 * SP_N(d) + (x)] - 1)
+#define SP_NJAC(d, x) ((d)[SP_ILOC(d, x)])
+#define SP_PJAC(d, x, s, k) ((d)[SP_ILOC(d, x) + 1 + (s) * SP_NJAC(d, x) + (k)])
+#define SP_RAD_J(d, f, x, k) ((f)[SP_FLOC(d, x) + (k)])
+#define SP_NUM_J(d, f, x, k) ((f)[SP_FLOC(d, x) + SP_NJAC(d, x) + (k)])
+#define SP_MASS_J(d, f, x, k) ((f)[SP_FLOC(d, x) + 2 * SP_NJAC(d, x) + (k)])
+#define SP_MW_J(d, f, x, k) ((f)[SP_FLOC(d, x) + 3 * SP_NJAC(d, x) + (k)])
+#define SP_JAC_GAS 0
+#define SP_JAC_AERO 1
+/* float: DELTA_H, DELTA_S, DIFF_COEFF, PRE_C_AVG, B1..B4, CONV, MW;
+ * env: MFP_M, ALPHA, EQUIL_CONST, KGM3_TO_PPM */
+
+/* rxn_SIMPOL_phase_transfer_get_used_jac_elem, :86-149 */
+static void simpol_used_jac_elem(Oracle *o, int *d, ColElems *jac) {
+  char *flags = xcalloc(o->ns, 1);
+  cols_register(jac, SP_GAS(d), SP_GAS(d));
+  for (int i = 0; i < SP_N(d); i++) {
+    cols_register(jac, SP_AERO(d, i), SP_GAS(d));
+    cols_register(jac, SP_GAS(d), SP_AERO(d, i));
+    cols_register(jac, SP_AERO(d, i), SP_AERO(d, i));
+    if (SP_ACT(d, i) > 0) {
+      cols_register(jac, SP_GAS(d), SP_ACT(d, i));
+      cols_register(jac, SP_AERO(d, i), SP_ACT(d, i));
+    }
+    memset(flags, 0, o->ns);
+    int n_elem = rep_used_jac_elem(o, SP_REP(d, i), SP_PHASE(d, i), flags);
+    if (n_elem > SP_NJAC(d, i)) {
+      fprintf(stderr, "oracle: SIMPOL got more Jacobian elements than expected\n");
+      exit(EXIT_FAILURE);
+    }
+    int used = 0;
+    for (int k = 0; k < o->ns; ++k)
+      if (flags[k]) {
+        cols_register(jac, SP_GAS(d), k);
+        cols_register(jac, SP_AERO(d, i), k);
+        SP_PJAC(d, i, SP_JAC_GAS, used) = k;
+        SP_PJAC(d, i, SP_JAC_AERO, used) = k;
+        ++used;
+      }
+    for (; used < SP_NJAC(d, i); ++used) {
+      SP_PJAC(d, i, SP_JAC_GAS, used) = -1;
+      SP_PJAC(d, i, SP_JAC_AERO, used) = -1;
+    }
+  }
+  free(flags);
+}
+
+/* rxn_SIMPOL_phase_transfer_update_ids, :160-207 */
+static void simpol_update_ids(Oracle *o, int *d) {
+  SP_DERIV(d, 0) = o->deriv_ids[SP_GAS(d)];
+  for (int i = 0; i < SP_N(d); i++) SP_DERIV(d, i + 1) = o->deriv_ids[SP_AERO(d, i)];
+  int j = 0;
+  SP_JAC(d, j++) = csc_find(&o->rjac, SP_GAS(d), SP_GAS(d));
+  for (int i = 0; i < SP_N(d); i++) {
+    SP_JAC(d, j++) = csc_find(&o->rjac, SP_AERO(d, i), SP_GAS(d));
+    SP_JAC(d, j++) = csc_find(&o->rjac, SP_GAS(d), SP_AERO(d, i));
+    SP_JAC(d, j++) = csc_find(&o->rjac, SP_AERO(d, i), SP_AERO(d, i));
+    if (SP_ACT(d, i) > 0) {
+      SP_GAS_ACT_JAC(d, i) = csc_find(&o->rjac, SP_GAS(d), SP_ACT(d, i));
+      SP_AERO_ACT_JAC(d, i) = csc_find(&o->rjac, SP_AERO(d, i), SP_ACT(d, i));
+    } else {
+      SP_GAS_ACT_JAC(d, i) = -1;
+      SP_AERO_ACT_JAC(d, i) = -1;
+    }
+    for (int k = 0; k < SP_NJAC(d, i); ++k) {
+      if (SP_PJAC(d, i, SP_JAC_GAS, k) > 0)
+        SP_PJAC(d, i, SP_JAC_GAS, k) = csc_find(&o->rjac, SP_GAS(d), SP_PJAC(d, i, SP_JAC_GAS, k));
+      if (SP_PJAC(d, i, SP_JAC_AERO, k) > 0)
+        SP_PJAC(d, i, SP_JAC_AERO, k) =
+            csc_find(&o->rjac, SP_AERO(d, i), SP_PJAC(d, i, SP_JAC_AERO, k));
+    }
+  }
+}
+
+/* rxn_SIMPOL_phase_transfer_update_env_state, :220-266 */
+static void simpol_env(const double *f, double *e, double T, double P) {
+  e[1] = 0.1;
+  if (f[0] != 0.0 || f[1] != 0.0) {
+    double del_G = f[0] - T * f[1];
+    e[1] = exp(-del_G / (UNIV_GAS_CONST * T));
+    e[1] = e[1] / (1.0 + e[1]);
+  }
+  e[0] = mean_free_path(f[2], T, f[9]);
+  double vp = f[4] / T + f[5] + f[6] * T + f[7] * log(T);
+  vp = 101325.0 * pow(10, vp);
+  e[3] = f[8] * T / P;
+  e[2] = vp / P / f[9] * 1.0e6;
+}
+
+/* rxn_SIMPOL_phase_transfer_calc_deriv_contrib, :279-402 */
+static void simpol_deriv(Oracle *o, int cell, const double *state, int *d, const double *fl,
+                         const double *e) {
+  for (int i = 0; i < SP_N(d); i++) {
+    double radius, number_conc, mass, avg_mw;
+    rep_eff_radius(o, SP_REP(d, i), cell, SP_PHASE(d, i), &radius, NULL);
+    int conc_type = 1; /* modal/binned mass: TOTAL_PARTICLE_MASS (:375-385) */
+    rep_number_conc(o, SP_REP(d, i), cell, state, SP_PHASE(d, i), &number_conc, NULL);
+    rep_phase_prop(o, SP_REP(d, i), cell, state, SP_PHASE(d, i), 0, &mass, NULL);
+    rep_phase_prop(o, SP_REP(d, i), cell, state, SP_PHASE(d, i), 1, &avg_mw, NULL);
+    long double cond = transition_rate(fl[2], e[0], radius, e[1]);
+    long double evap = cond * (e[2] * avg_mw / mass);
+    long double act = 1.0;
+    if (SP_ACT(d, i) > -1) act = state[SP_ACT(d, i)];
+    evap *= act;
+    cond *= state[SP_GAS(d)];
+    evap *= state[SP_AERO(d, i)];
+    if (conc_type == 0) {
+      if (SP_DERIV(d, 0) >= 0) {
+        td_add(o, SP_DERIV(d, 0), number_conc * evap);
+        td_add(o, SP_DERIV(d, 0), -number_conc * cond);
+      }
+      if (SP_DERIV(d, 1 + i) >= 0) {
+        td_add(o, SP_DERIV(d, 1 + i), -evap / e[3]);
+        td_add(o, SP_DERIV(d, 1 + i), cond / e[3]);
+      }
+    } else {
+      if (SP_DERIV(d, 0) >= 0) {
+        td_add(o, SP_DERIV(d, 0), number_conc * evap);
+        td_add(o, SP_DERIV(d, 0), -number_conc * cond);
+      }
+      if (SP_DERIV(d, 1 + i) >= 0) {
+        td_add(o, SP_DERIV(d, 1 + i), -number_conc * evap / e[3]);
+        td_add(o, SP_DERIV(d, 1 + i), number_conc * cond / e[3]);
+      }
+    }
+  }
+}
+
+static void jac_add(Oracle *o, int elem, int production, long double v) {
+  if (production) o->jp[elem] += v;
+  else o->jl[elem] += v;
+}
+
+/* rxn_SIMPOL_phase_transfer_calc_jac_contrib, :415-~740 */
+static void simpol_jac(Oracle *o, int cell, const double *state, int *d, double *fl,
+                       const double *e) {
+  for (int i = 0; i < SP_N(d); i++) {
+    double radius, number_conc, mass, avg_mw;
+    rep_eff_radius(o, SP_REP(d, i), cell, SP_PHASE(d, i), &radius, &SP_RAD_J(d, fl, i, 0));
+    int conc_type = 1;
+    rep_number_conc(o, SP_REP(d, i), cell, state, SP_PHASE(d, i), &number_conc,
+                    &SP_NUM_J(d, fl, i, 0));
+    rep_phase_prop(o, SP_REP(d, i), cell, state, SP_PHASE(d, i), 0, &mass, &SP_MASS_J(d, fl, i, 0));
+    rep_phase_prop(o, SP_REP(d, i), cell, state, SP_PHASE(d, i), 1, &avg_mw, &SP_MW_J(d, fl, i, 0));
+    long double cond = transition_rate(fl[2], e[0], radius, e[1]);
+    long double evap = cond * (e[2] * avg_mw / mass);
+    long double act = 1.0;
+    if (SP_ACT(d, i) > -1) act = state[SP_ACT(d, i)];
+    const double kp = e[3];
+    if (conc_type == 0) {
+      if (SP_JAC(d, 1 + i * 3 + 1) >= 0)
+        jac_add(o, SP_JAC(d, 1 + i * 3 + 1), 1, number_conc * evap * act);
+      if (SP_JAC(d, 0) >= 0) jac_add(o, SP_JAC(d, 0), 0, number_conc * cond);
+      if (SP_JAC(d, 1 + i * 3) >= 0) jac_add(o, SP_JAC(d, 1 + i * 3), 1, cond / kp);
+      if (SP_JAC(d, 1 + i * 3 + 2) >= 0)
+        jac_add(o, SP_JAC(d, 1 + i * 3 + 2), 0, evap * act / kp);
+      if (SP_GAS_ACT_JAC(d, i) > 0)
+        jac_add(o, SP_GAS_ACT_JAC(d, i), 1, number_conc * evap * state[SP_AERO(d, i)]);
+      if (SP_AERO_ACT_JAC(d, i) > 0)
+        jac_add(o, SP_AERO_ACT_JAC(d, i), 0, evap / kp * state[SP_AERO(d, i)]);
+    } else {
+      if (SP_JAC(d, 1 + i * 3 + 1) >= 0)
+        jac_add(o, SP_JAC(d, 1 + i * 3 + 1), 1, number_conc * evap * act);
+      if (SP_JAC(d, 0) >= 0) jac_add(o, SP_JAC(d, 0), 0, number_conc * cond);
+      if (SP_JAC(d, 1 + i * 3) >= 0) jac_add(o, SP_JAC(d, 1 + i * 3), 1, number_conc * cond / kp);
+      if (SP_JAC(d, 1 + i * 3 + 2) >= 0)
+        jac_add(o, SP_JAC(d, 1 + i * 3 + 2), 0, number_conc * evap * act / kp);
+      if (SP_GAS_ACT_JAC(d, i) > 0)
+        jac_add(o, SP_GAS_ACT_JAC(d, i), 1, number_conc * evap * state[SP_AERO(d, i)]);
+      if (SP_AERO_ACT_JAC(d, i) > 0)
+        jac_add(o, SP_AERO_ACT_JAC(d, i), 0, number_conc * evap / kp * state[SP_AERO(d, i)]);
+    }
+    evap *= act;
+    cond *= state[SP_GAS(d)];
+    evap *= state[SP_AERO(d, i)];
+    double d_cond_d_radius =
+        d_transition_rate_d_radius(fl[2], e[0], radius, e[1]) * state[SP_GAS(d)];
+    double d_evap_d_radius = d_cond_d_radius / state[SP_GAS(d)] * e[2] * avg_mw / mass *
+                             state[SP_AERO(d, i)];
+    double d_evap_d_mass = -evap / mass;
+    double d_evap_d_mw = evap / avg_mw;
+    for (int k = 0; k < SP_NJAC(d, i); ++k) {
+      int g = SP_PJAC(d, i, SP_JAC_GAS, k), a = SP_PJAC(d, i, SP_JAC_AERO, k);
+      double rj = SP_RAD_J(d, fl, i, k), nj = SP_NUM_J(d, fl, i, k);
+      double mj = SP_MASS_J(d, fl, i, k), wj = SP_MW_J(d, fl, i, k);
+      if (g > 0) {
+        jac_add(o, g, 1, number_conc * d_evap_d_radius * rj);
+        jac_add(o, g, 0, number_conc * d_cond_d_radius * rj);
+        jac_add(o, g, 1, evap * nj);
+        jac_add(o, g, 0, cond * nj);
+        jac_add(o, g, 1, number_conc * d_evap_d_mass * mj);
+        jac_add(o, g, 1, number_conc * d_evap_d_mw * wj);
+      }
+      if (a > 0) {
+        if (conc_type == 0) {
+          jac_add(o, a, 0, d_evap_d_radius / kp * rj);
+          jac_add(o, a, 1, d_cond_d_radius / kp * rj);
+          jac_add(o, a, 0, d_evap_d_mass / kp * mj);
+          jac_add(o, a, 0, d_evap_d_mw / kp * wj);
+        } else {
+          jac_add(o, a, 0, number_conc * d_evap_d_radius / kp * rj);
+          jac_add(o, a, 1, number_conc * d_cond_d_radius / kp * rj);
+          jac_add(o, a, 0, evap / kp * nj);
+          jac_add(o, a, 1, cond / kp * nj);
+          jac_add(o, a, 0, number_conc * d_evap_d_mass / kp * mj);
+          jac_add(o, a, 0, number_conc * d_evap_d_mw / kp * wj);
+        }
+      }
+    }
+  }
+}
+
+/* aqueous equilibrium layout, rxn_aqueous_equilibrium.c:27-68 */
+#define AQ_NR(d) ((d)[0])
+#define AQ_NP(d) ((d)[1])
+#define AQ_NPH(d) ((d)[2])
+#define AQ_REACT(d, x) ((d)[3 + (x)] - 1)
+#define AQ_PROD(d, x) ((d)[3 + AQ_NR(d) * AQ_NPH(d) + (x)] - 1)
+#define AQ_WATER(d, x) ((d)[3 + (AQ_NR(d) + AQ_NP(d)) * AQ_NPH(d) + (x)] - 1)
+#define AQ_ACT(d, x) ((d)[3 + (AQ_NR(d) + AQ_NP(d) + 1) * AQ_NPH(d) + (x)] - 1)
+#define AQ_DERIV(d, x) ((d)[3 + (AQ_NR(d) + AQ_NP(d) + 2) * AQ_NPH(d) + (x)])
+#define AQ_JAC(d, x) ((d)[3 + (2 * (AQ_NR(d) + AQ_NP(d)) + 2) * AQ_NPH(d) + (x)])
+#define AQ_A(f) ((f)[0])
+#define AQ_C(f) ((f)[1])
+#define AQ_KREV(f) ((f)[2])
+#define AQ_WATER_CONC(f) ((f)[3])
+#define AQ_ACT_VAL(f) ((f)[4])
+#define AQ_MFTM(d, f, x) ((f)[5 + (x)])
+#define AQ_REACT_CONC(d, f, x) ((f)[5 + AQ_NR(d) + AQ_NP(d) + (x)])
+#define AQ_PROD_CONC(d, f, x) ((f)[5 + 2 * AQ_NR(d) + AQ_NP(d) + (x)])
+#define AQ_SMALL_WATER(d, f, x) ((f)[5 + 2 * AQ_NR(d) + 2 * AQ_NP(d) + (x)])
+#define AQ_SMALL_CONC(d, f, x) ((f)[5 + 2 * AQ_NR(d) + 2 * AQ_NP(d) + AQ_NPH(d) + (x)])
+#define MIN_WATER 1.0e-4
+
+/* rxn_aqueous_equilibrium_get_used_jac_elem, :80-126 */
+static void aqeq_used_jac_elem(int *d, ColElems *jac) {
+  int nr = AQ_NR(d), np = AQ_NP(d);
+  for (int p = 0; p < AQ_NPH(d); p++) {
+    for (int ri = p * nr; ri < (p + 1) * nr; ri++) {
+      for (int rd = p * nr; rd < (p + 1) * nr; rd++) cols_register(jac, AQ_REACT(d, rd), AQ_REACT(d, ri));
+      for (int pd = p * np; pd < (p + 1) * np; pd++) cols_register(jac, AQ_PROD(d, pd), AQ_REACT(d, ri));
+    }
+    for (int pi = p * np; pi < (p + 1) * np; pi++) {
+      for (int rd = p * nr; rd < (p + 1) * nr; rd++) cols_register(jac, AQ_REACT(d, rd), AQ_PROD(d, pi));
+      for (int pd = p * np; pd < (p + 1) * np; pd++) cols_register(jac, AQ_PROD(d, pd), AQ_PROD(d, pi));
+    }
+    for (int rd = p * nr; rd < (p + 1) * nr; rd++) cols_register(jac, AQ_REACT(d, rd), AQ_WATER(d, p));
+    for (int pd = p * np; pd < (p + 1) * np; pd++) cols_register(jac, AQ_PROD(d, pd), AQ_WATER(d, p));
+    if (AQ_ACT(d, p) < 0) continue;
+    for (int rd = p * nr; rd < (p + 1) * nr; ++rd) cols_register(jac, AQ_REACT(d, rd), AQ_ACT(d, p));
+    for (int pd = p * np; pd < (p + 1) * np; ++pd) cols_register(jac, AQ_PROD(d, pd), AQ_ACT(d, p));
+  }
+}
+
+/* rxn_aqueous_equilibrium_update_ids, :138-230 */
+static void aqeq_update_ids(Oracle *o, int *d, double *f) {
+  int nr = AQ_NR(d), np = AQ_NP(d);
+  for (int p = 0, k = 0; p < AQ_NPH(d); p++) {
+    for (int r = 0; r < nr; r++) AQ_DERIV(d, k++) = o->deriv_ids[AQ_REACT(d, p * nr + r)];
+    for (int q = 0; q < np; q++) AQ_DERIV(d, k++) = o->deriv_ids[AQ_PROD(d, p * np + q)];
+  }
+  const Csc *J = &o->rjac;
+  for (int p = 0, j = 0; p < AQ_NPH(d); p++) {
+    for (int ri = p * nr; ri < (p + 1) * nr; ri++) {
+      for (int rd = p * nr; rd < (p + 1) * nr; rd++) AQ_JAC(d, j++) = csc_find(J, AQ_REACT(d, rd), AQ_REACT(d, ri));
+      for (int pd = p * np; pd < (p + 1) * np; pd++) AQ_JAC(d, j++) = csc_find(J, AQ_PROD(d, pd), AQ_REACT(d, ri));
+    }
+    for (int pi = p * np; pi < (p + 1) * np; pi++) {
+      for (int rd = p * nr; rd < (p + 1) * nr; rd++) AQ_JAC(d, j++) = csc_find(J, AQ_REACT(d, rd), AQ_PROD(d, pi));
+      for (int pd = p * np; pd < (p + 1) * np; pd++) AQ_JAC(d, j++) = csc_find(J, AQ_PROD(d, pd), AQ_PROD(d, pi));
+    }
+    for (int rd = p * nr; rd < (p + 1) * nr; rd++) AQ_JAC(d, j++) = csc_find(J, AQ_REACT(d, rd), AQ_WATER(d, p));
+    for (int pd = p * np; pd < (p + 1) * np; pd++) AQ_JAC(d, j++) = csc_find(J, AQ_PROD(d, pd), AQ_WATER(d, p));
+    if (AQ_ACT(d, p) < 0) {
+      for (int rd = p * nr; rd < (p + 1) * nr; ++rd) AQ_JAC(d, j++) = -1;
+      for (int pd = p * np; pd < (p + 1) * np; ++pd) AQ_JAC(d, j++) = -1;
+    } else {
+      for (int rd = p * nr; rd < (p + 1) * nr; ++rd) AQ_JAC(d, j++) = csc_find(J, AQ_REACT(d, rd), AQ_ACT(d, p));
+      for (int pd = p * np; pd < (p + 1) * np; ++pd) AQ_JAC(d, j++) = csc_find(J, AQ_PROD(d, pd), AQ_ACT(d, p));
+    }
+  }
+  for (int p = 0; p < AQ_NPH(d); p++) {
+    AQ_SMALL_CONC(d, f, p) = 99999.0;
+    for (int r = 0; r < nr; r++)
+      if (AQ_SMALL_CONC(d, f, p) > o->abs_tol[AQ_REACT(d, p * nr + r)] / 100.0)
+        AQ_SMALL_CONC(d, f, p) = o->abs_tol[AQ_REACT(d, p * nr + r)] / 100.0;
+    for (int q = 0; q < np; q++)
+      if (AQ_SMALL_CONC(d, f, p) > o->abs_tol[AQ_PROD(d, p * np + q)] / 100.0)
+        AQ_SMALL_CONC(d, f, p) = o->abs_tol[AQ_PROD(d, p * np + q)] / 100.0;
+  }
+  for (int p = 0; p < AQ_NPH(d); p++) AQ_SMALL_WATER(d, f, p) = o->abs_tol[AQ_WATER(d, p)] / 10.0;
+}
+
+/* rxn_aqueous_equilibrium_update_env_state, :244-265 */
+static void aqeq_env(const double *f, double *e, double T) {
+  double K = AQ_C(f) == 0.0 ? AQ_A(f) : AQ_A(f) * exp(AQ_C(f) * (1.0 / T - 1.0 / 298.0));
+  e[0] = K * AQ_KREV(f);
+}
+
+/* calc_standard_rate, :278-310 */
+static long double aqeq_standard_rate(int *d, double *f, const double *e, int water_partial,
+                                      long double *rf, long double *rr) {
+  long double water = AQ_WATER_CONC(f);
+  long double react = (long double)AQ_REACT_CONC(d, f, 0) * AQ_MFTM(d, f, 0);
+  for (int r = 1; r < AQ_NR(d); r++) react *= AQ_REACT_CONC(d, f, r) * AQ_MFTM(d, f, r) / water;
+  long double prod = (long double)AQ_PROD_CONC(d, f, 0) * AQ_MFTM(d, f, AQ_NR(d));
+  prod *= (long double)AQ_ACT_VAL(f);
+  for (int q = 1; q < AQ_NP(d); q++)
+    prod *= AQ_PROD_CONC(d, f, q) * AQ_MFTM(d, f, AQ_NR(d) + q) / water;
+  *rf = e[0] * react;
+  *rr = AQ_KREV(f) * prod;
+  if (water_partial) return *rf * (AQ_NR(d) - 1) - *rr * (AQ_NP(d) - 1);
+  return *rf - *rr;
+}
+
+/* rxn_aqueous_equilibrium_calc_deriv_contrib, :323-390 */
+static void aqeq_deriv(Oracle *o, const double *state, int *d, double *f, const double *e) {
+  int nr = AQ_NR(d), np = AQ_NP(d);
+  for (int p = 0, k = 0; p < AQ_NPH(d); p++) {
+    long double water = state[AQ_WATER(d, p)];
+    if (water < MIN_WATER * AQ_SMALL_WATER(d, f, p)) {
+      k += nr + np;
+      continue;
+    }
+    for (int r = 0; r < nr; ++r) AQ_REACT_CONC(d, f, r) = state[AQ_REACT(d, p * nr + r)];
+    for (int q = 0; q < np; ++q) AQ_PROD_CONC(d, f, q) = state[AQ_PROD(d, p * np + q)];
+    AQ_WATER_CONC(f) = state[AQ_WATER(d, p)];
+    AQ_ACT_VAL(f) = AQ_ACT(d, p) >= 0 ? state[AQ_ACT(d, p)] : 1.0;
+    long double rf, rr;
+    long double rate = aqeq_standard_rate(d, f, e, 0, &rf, &rr);
+    if (rate == 0.0) {
+      k += nr + np;
+      continue;
+    }
+    for (int r = 0; r < nr; r++) {
+      if (AQ_DERIV(d, k) < 0) {
+        k++;
+        continue;
+      }
+      td_add(o, AQ_DERIV(d, k), -rf / AQ_MFTM(d, f, r));
+      td_add(o, AQ_DERIV(d, k++), rr / AQ_MFTM(d, f, r));
+    }
+    for (int q = 0; q < np; q++) {
+      if (AQ_DERIV(d, k) < 0) {
+        k++;
+        continue;
+      }
+      td_add(o, AQ_DERIV(d, k), rf / AQ_MFTM(d, f, nr + q));
+      td_add(o, AQ_DERIV(d, k++), -rr / AQ_MFTM(d, f, nr + q));
+    }
+  }
+}
+
+/* rxn_aqueous_equilibrium_calc_jac_contrib, :403-~548 */
+static void aqeq_jac(Oracle *o, const double *state, int *d, double *f, const double *e) {
+  int nr = AQ_NR(d), np = AQ_NP(d);
+  for (int p = 0, j = 0; p < AQ_NPH(d); p++) {
+    long double water = state[AQ_WATER(d, p)];
+    if (water < MIN_WATER * AQ_SMALL_WATER(d, f, p)) {
+      j += (nr + np) * (nr + np + 2);
+      continue;
+    }
+    long double fwd = e[0];
+    for (int r = 0; r < nr; r++) fwd *= state[AQ_REACT(d, p * nr + r)] * AQ_MFTM(d, f, r) / water;
+    long double rev = AQ_KREV(f);
+    for (int q = 0; q < np; q++) rev *= state[AQ_PROD(d, p * np + q)] * AQ_MFTM(d, f, nr + q) / water;
+    if (AQ_ACT(d, p) >= 0) rev *= state[AQ_ACT(d, p)];
+    for (int ri = 0; ri < nr; ri++) {
+      for (int rd = 0; rd < nr; rd++) {
+        if (AQ_JAC(d, j) < 0 || fwd == 0.0) {
+          j++;
+          continue;
+        }
+        jac_add(o, AQ_JAC(d, j++), 0,
+                fwd / state[AQ_REACT(d, p * nr + ri)] / AQ_MFTM(d, f, rd) * water);
+      }
+      for (int pd = 0; pd < np; pd++) {
+        if (AQ_JAC(d, j) < 0 || fwd == 0.0) {
+          j++;
+          continue;
+        }
+        jac_add(o, AQ_JAC(d, j++), 1,
+                fwd / state[AQ_REACT(d, p * nr + ri)] / AQ_MFTM(d, f, nr + pd) * water);
+      }
+    }
+    for (int pi = 0; pi < np; pi++) {
+      for (int rd = 0; rd < nr; rd++) {
+        if (AQ_JAC(d, j) < 0 || rev == 0.0) {
+          j++;
+          continue;
+        }
+        jac_add(o, AQ_JAC(d, j++), 1,
+                rev / state[AQ_PROD(d, p * np + pi)] / AQ_MFTM(d, f, rd) * water);
+      }
+      for (int pd = 0; pd < np; pd++) {
+        if (AQ_JAC(d, j) < 0 || rev == 0.0) {
+          j++;
+          continue;
+        }
+        jac_add(o, AQ_JAC(d, j++), 0,
+                rev / state[AQ_PROD(d, p * np + pi)] / AQ_MFTM(d, f, nr + pd) * water);
+      }
+    }
+    for (int rd = 0; rd < nr; rd++) {
+      if (AQ_JAC(d, j) < 0) {
+        j++;
+        continue;
+      }
+      jac_add(o, AQ_JAC(d, j), 0, -fwd / AQ_MFTM(d, f, rd) * (nr - 1));
+      jac_add(o, AQ_JAC(d, j++), 1, -rev / AQ_MFTM(d, f, rd) * (np - 1));
+    }
+    for (int pd = 0; pd < np; pd++) {
+      if (AQ_JAC(d, j) < 0) {
+        j++;
+        continue;
+      }
+      jac_add(o, AQ_JAC(d, j), 1, -fwd / AQ_MFTM(d, f, nr + pd) * (nr - 1));
+      jac_add(o, AQ_JAC(d, j++), 0, -rev / AQ_MFTM(d, f, nr + pd) * (np - 1));
+    }
+    if (AQ_ACT(d, p) < 0) {
+      j += nr + np;
+      continue;
+    }
+    for (int rd = 0; rd < nr; rd++) {
+      if (AQ_JAC(d, j) < 0) {
+        j++;
+        continue;
+      }
+      jac_add(o, AQ_JAC(d, j++), 1, rev / state[AQ_ACT(d, p)] / AQ_MFTM(d, f, rd) * water);
+    }
+    for (int pd = 0; pd < np; pd++) {
+      if (AQ_JAC(d, j) < 0) {
+        j++;
+        continue;
+      }
+      jac_add(o, AQ_JAC(d, j++), 0, rev / state[AQ_ACT(d, p)] / AQ_MFTM(d, f, nr + pd) * water);
+    }
+  }
+}
+
+/* ZSR layout, sub_model_ZSR_aerosol_water.c:30-68 (d after the type slot) */
+#define ZS_NPH(d) ((d)[0])
+#define ZS_GAS_WATER(d) ((d)[1] - 1)
+#define ZS_NPAIR(d) ((d)[2])
+#define ZS_PHASE(d, p) ((d)[5 + (p)] - 1)
+#define ZS_PILOC(d, x) ((d)[5 + ZS_NPH(d) + (x)] - 1)
+#define ZS_PFLOC(d, x) ((d)[5 + ZS_NPH(d) + ZS_NPAIR(d) + (x)] - 1)
+#define ZS_TYPE(d, x) ((d)[ZS_PILOC(d, x)])
+#define ZS_J_NCAT(d, x) ((d)[ZS_PILOC(d, x) + 1])
+#define ZS_J_NAN(d, x) ((d)[ZS_PILOC(d, x) + 2])
+#define ZS_J_CAT(d, x) ((d)[ZS_PILOC(d, x) + 3])
+#define ZS_J_AN(d, x) ((d)[ZS_PILOC(d, x) + 4])
+#define ZS_J_NY(d, x) ((d)[ZS_PILOC(d, x) + 5])
+#define ZS_J_GW_JAC(d, p, x) ((d)[ZS_PILOC(d, x) + 6 + (p)])
+#define ZS_J_CAT_JAC(d, p, x) ((d)[ZS_PILOC(d, x) + 6 + ZS_NPH(d) + (p)])
+#define ZS_J_AN_JAC(d, p, x) ((d)[ZS_PILOC(d, x) + 6 + 2 * ZS_NPH(d) + (p)])
+#define ZS_J_LOW_RH(d, f, x) ((f)[ZS_PFLOC(d, x)])
+#define ZS_J_CAT_MW(d, f, x) ((f)[ZS_PFLOC(d, x) + 1])
+#define ZS_J_AN_MW(d, f, x) ((f)[ZS_PFLOC(d, x) + 2])
+#define ZS_J_Y(d, f, x, y) ((f)[ZS_PFLOC(d, x) + 3 + (y)])
+#define ZS_ALPHA (-100.0)
+#define ZS_MIN_WATER 1.0e-30L
+#define ZS_JACOBSON 1
+
+static void zsr_check_type(int *d, int x) {
+  if (ZS_TYPE(d, x) != ZS_JACOBSON) {
+    fprintf(stderr, "oracle: ZSR ion-pair type %d is not restated\n", ZS_TYPE(d, x));
+    exit(EXIT_FAILURE);
+  }
+}
+
+/* sub_model_ZSR_aerosol_water_get_used_jac_elem, :90-122 */
+static void zsr_used_jac_elem(int *d, ColElems *jac) {
+  for (int p = 0; p < ZS_NPH(d); ++p) {
+    cols_register(jac, ZS_PHASE(d, p), ZS_GAS_WATER(d));
+    for (int x = 0; x < ZS_NPAIR(d); ++x) {
+      zsr_check_type(d, x);
+      cols_register(jac, ZS_PHASE(d, p), ZS_PHASE(d, p) + ZS_J_CAT(d, x));
+      cols_register(jac, ZS_PHASE(d, p), ZS_PHASE(d, p) + ZS_J_AN(d, x));
+    }
+  }
+}
+
+/* sub_model_ZSR_aerosol_water_update_ids, :134-173 */
+static void zsr_update_ids(Oracle *o, int *d) {
+  for (int p = 0; p < ZS_NPH(d); ++p)
+    for (int x = 0; x < ZS_NPAIR(d); ++x) {
+      ZS_J_GW_JAC(d, p, x) = csc_find(&o->pjac, ZS_PHASE(d, p), ZS_GAS_WATER(d));
+      ZS_J_CAT_JAC(d, p, x) = csc_find(&o->pjac, ZS_PHASE(d, p), ZS_PHASE(d, p) + ZS_J_CAT(d, x));
+      ZS_J_AN_JAC(d, p, x) = csc_find(&o->pjac, ZS_PHASE(d, p), ZS_PHASE(d, p) + ZS_J_AN(d, x));
+    }
+}
+
+/* sub_model_ZSR_aerosol_water_update_env_state, :189-207 */
+static void zsr_env(double *e, double T, double P) {
+  double t_steam = 373.15;
+  double a = 1.0 - t_steam / T;
+  a = (((-0.1299 * a - 0.6445) * a - 1.976) * a + 13.3185) * a;
+  double water_vp = 101325.0 * exp(a);
+  e[0] = P / water_vp / 1.0e6;
+}
+
+/* sub_model_ZSR_aerosol_water_calculate, :216-301 (JACOBSON pairs) */
+static void zsr_calculate(int *d, double *f, const double *e, double *state) {
+  long double a_w = e[0] * state[ZS_GAS_WATER(d)];
+  for (int p = 0; p < ZS_NPH(d); p++) {
+    double *water = &state[ZS_PHASE(d, p)];
+    *water = ZS_MIN_WATER;
+    for (int x = 0; x < ZS_NPAIR(d); x++) {
+      long double molality, conc;
+      long double j_aw = a_w > ZS_J_LOW_RH(d, f, x) ? a_w : ZS_J_LOW_RH(d, f, x);
+      molality = 0.0;
+      for (int k = 0; k < ZS_J_NY(d, x); k++) molality += ZS_J_Y(d, f, x, k) * pow(j_aw, k);
+      molality *= molality;
+      long double cation = state[ZS_PHASE(d, p) + ZS_J_CAT(d, x)] / ZS_J_NCAT(d, x) /
+                           ZS_J_CAT_MW(d, f, x) / 1000.0;
+      long double anion = state[ZS_PHASE(d, p) + ZS_J_AN(d, x)] / ZS_J_NAN(d, x) /
+                          ZS_J_AN_MW(d, f, x) / 1000.0;
+      long double e_ac = exp(ZS_ALPHA * cation);
+      long double e_aa = exp(ZS_ALPHA * anion);
+      conc = (cation * e_ac + anion * e_aa) / (e_ac + e_aa);
+      *water += conc / molality * 1000.0;
+    }
+  }
+}
+
+/* sub_model_ZSR_aerosol_water_get_jac_contrib, :314-~440 (JACOBSON pairs) */
+static void zsr_jac(int *d, double *f, const double *e, const double *state, double *J) {
+  long double a_w = e[0] * state[ZS_GAS_WATER(d)];
+  long double d_aw_d_wg = e[0];
+  for (int p = 0; p < ZS_NPH(d); p++)
+    for (int x = 0; x < ZS_NPAIR(d); x++) {
+      long double molality, d_molal_d_wg, conc;
+      long double j_aw = a_w > ZS_J_LOW_RH(d, f, x) ? a_w : ZS_J_LOW_RH(d, f, x);
+      long double d_jaw_d_wg = a_w > ZS_J_LOW_RH(d, f, x) ? d_aw_d_wg : 0.0;
+      molality = ZS_J_Y(d, f, x, 0);
+      d_molal_d_wg = 0.0;
+      for (int k = 1; k < ZS_J_NY(d, x); k++) {
+        molality += ZS_J_Y(d, f, x, k) * pow(j_aw, k);
+        d_molal_d_wg += ZS_J_Y(d, f, x, k) * k * pow(j_aw, (k - 1));
+      }
+      d_molal_d_wg *= d_jaw_d_wg;
+      long double cation = state[ZS_PHASE(d, p) + ZS_J_CAT(d, x)] / ZS_J_NCAT(d, x) /
+                           ZS_J_CAT_MW(d, f, x) / 1000.0;
+      long double d_cation_d_C = 1.0 / ZS_J_NCAT(d, x) / ZS_J_CAT_MW(d, f, x) / 1000.0;
+      long double anion = state[ZS_PHASE(d, p) + ZS_J_AN(d, x)] / ZS_J_NAN(d, x) /
+                          ZS_J_AN_MW(d, f, x) / 1000.0;
+      long double d_anion_d_A = 1.0 / ZS_J_NAN(d, x) / ZS_J_AN_MW(d, f, x) / 1000.0;
+      long double e_ac = exp(ZS_ALPHA * cation);
+      long double e_aa = exp(ZS_ALPHA * anion);
+      conc = (cation * e_ac + anion * e_aa) / (e_ac + e_aa);
+      long double denom = (e_ac + e_aa) * (e_ac + e_aa);
+      long double d_conc_d_cation =
+          (e_ac * e_ac + e_ac * e_aa * (1.0 - ZS_ALPHA * anion + ZS_ALPHA * cation)) / denom;
+      long double d_conc_d_anion =
+          (e_aa * e_aa + e_ac * e_aa * (1.0 - ZS_ALPHA * cation + ZS_ALPHA * anion)) / denom;
+      J[ZS_J_GW_JAC(d, p, x)] += -2.0 * conc / pow(molality, 3) * 1000.0 * d_molal_d_wg;
+      J[ZS_J_AN_JAC(d, p, x)] += 1.0 / pow(molality, 2) * 1000.0 * d_conc_d_anion * d_anion_d_A;
+      J[ZS_J_CAT_JAC(d, p, x)] += 1.0 / pow(molality, 2) * 1000.0 * d_conc_d_cation * d_cation_d_C;
+    }
+}
+
+/* aero_phase/aero_rep/sub_model _add_condensed_data (aero_phase_solver.c:
+ * 165-194, aero_rep_solver.c:443-482, sub_model_solver.c:413-453), with
+ * the array sizes solver_new takes (camp_solver.c:200-349). */
+void oracle_set_aero_sizes(void *om, int n_phase, int n_phase_int, int n_phase_float,
+                           int n_rep, int n_rep_int, int n_rep_float, int n_rep_env,
+                           int n_sub, int n_sub_int, int n_sub_float, int n_sub_env) {
+  Oracle *o = om;
+  o->n_phase = n_phase;
+  o->ph_int = xcalloc(n_phase_int + 1, sizeof(int));
+  o->ph_flt = xcalloc(n_phase_float + 1, sizeof(double));
+  o->ph_int_idx = xcalloc(n_phase + 1, sizeof(int));
+  o->ph_flt_idx = xcalloc(n_phase + 1, sizeof(int));
+  o->n_rep = n_rep;
+  o->rep_int = xcalloc(n_rep_int + n_rep + 1, sizeof(int));
+  o->rep_flt = xcalloc(n_rep_float + 1, sizeof(double));
+  o->rep_env = xcalloc((size_t)o->nc * n_rep_env + 1, sizeof(double));
+  o->rep_int_idx = xcalloc(n_rep + 1, sizeof(int));
+  o->rep_flt_idx = xcalloc(n_rep + 1, sizeof(int));
+  o->rep_env_idx = xcalloc(n_rep + 1, sizeof(int));
+  o->n_sub = n_sub;
+  o->sub_int = xcalloc(n_sub_int + n_sub + 1, sizeof(int));
+  o->sub_flt = xcalloc(n_sub_float + 1, sizeof(double));
+  o->sub_env = xcalloc((size_t)o->nc * n_sub_env + 1, sizeof(double));
+  o->sub_int_idx = xcalloc(n_sub + 1, sizeof(int));
+  o->sub_flt_idx = xcalloc(n_sub + 1, sizeof(int));
+  o->sub_env_idx = xcalloc(n_sub + 1, sizeof(int));
+}
+
+void oracle_add_aero_phase(void *om, int n_int, int n_float, const int *ip, const double *fp) {
+  Oracle *o = om;
+  int k = o->n_added_phase++;
+  o->ph_int_idx[k + 1] = o->ph_int_idx[k] + n_int;
+  o->ph_flt_idx[k + 1] = o->ph_flt_idx[k] + n_float;
+  memcpy(&o->ph_int[o->ph_int_idx[k]], ip, n_int * sizeof(int));
+  memcpy(&o->ph_flt[o->ph_flt_idx[k]], fp, n_float * sizeof(double));
+}
+
+void oracle_add_aero_rep(void *om, int type, int n_int, int n_float, int n_env, const int *ip,
+                         const double *fp) {
+  Oracle *o = om;
+  int k = o->n_added_rep++;
+  o->rep_int_idx[k + 1] = o->rep_int_idx[k] + n_int + 1;
+  o->rep_flt_idx[k + 1] = o->rep_flt_idx[k] + n_float;
+  o->rep_env_idx[k + 1] = o->rep_env_idx[k] + n_env;
+  int *id = &o->rep_int[o->rep_int_idx[k]];
+  id[0] = type;
+  memcpy(id + 1, ip, n_int * sizeof(int));
+  memcpy(&o->rep_flt[o->rep_flt_idx[k]], fp, n_float * sizeof(double));
+  o->n_rep_env += n_env;
+}
+
+void oracle_add_sub_model(void *om, int type, int n_int, int n_float, int n_env, const int *ip,
+                          const double *fp) {
+  Oracle *o = om;
+  int k = o->n_added_sub++;
+  o->sub_int_idx[k + 1] = o->sub_int_idx[k] + n_int + 1;
+  o->sub_flt_idx[k + 1] = o->sub_flt_idx[k] + n_float;
+  o->sub_env_idx[k + 1] = o->sub_env_idx[k] + n_env;
+  int *id = &o->sub_int[o->sub_int_idx[k]];
+  id[0] = type;
+  memcpy(id + 1, ip, n_int * sizeof(int));
+  memcpy(&o->sub_flt[o->sub_flt_idx[k]], fp, n_float * sizeof(double));
+  o->n_sub_env += n_env;
+}
+
+#define SUB_INT(o, k) (&(o)->sub_int[(o)->sub_int_idx[k]])
+#define SUB_FLT(o, k) (&(o)->sub_flt[(o)->sub_flt_idx[k]])
+#define SUB_ENV(o, k, cell) (&(o)->sub_env[(size_t)(cell) * (o)->n_sub_env + (o)->sub_env_idx[k]])
+
+static void sub_check(int *id) {
+  if (id[0] != SUB_MODEL_ZSR_AEROSOL_WATER) {
+    fprintf(stderr, "oracle: unsupported sub model type %d\n", id[0]);
+    exit(EXIT_FAILURE);
+  }
+}
+
+/* sub_model_calculate, sub_model_solver.c:308-345 */
+static void sub_calculate(Oracle *o, int cell, double *state) {
+  for (int k = 0; k < o->n_sub; ++k) {
+    int *id = SUB_INT(o, k);
+    sub_check(id);
+    zsr_calculate(id + 1, SUB_FLT(o, k), SUB_ENV(o, k, cell), state);
+  }
+}
+
+/* sub_model_get_jac_contrib, sub_model_solver.c:354-395 */
+static void sub_jac(Oracle *o, int cell, const double *state, double *J) {
+  for (int k = 0; k < o->n_sub; ++k) {
+    int *id = SUB_INT(o, k);
+    sub_check(id);
+    zsr_jac(id + 1, SUB_FLT(o, k), SUB_ENV(o, k, cell), state, J);
+  }
+}
+
 /* rxn_get_used_jac_elem, rxn_solver.c:42-129 */
 static void used_jac_elem(Oracle *o, ColElems *jac) {
   for (int r = 0; r < o->n_rxn; ++r) {
@@ -218,6 +1184,10 @@ static void used_jac_elem(Oracle *o, ColElems *jac) {
       }
     } else if (t == RXN_FIRST_ORDER_LOSS) {
       cols_register(jac, id[1] - 1, id[1] - 1);
+    } else if (t == RXN_AQUEOUS_EQUILIBRIUM) {
+      aqeq_used_jac_elem(id, jac);
+    } else if (t == RXN_SIMPOL_PHASE_TRANSFER) {
+      simpol_used_jac_elem(o, id, jac);
     }
   }
 }
@@ -246,6 +1216,10 @@ static void update_ids(Oracle *o) {
     } else if (t == RXN_FIRST_ORDER_LOSS) {
       id[2] = o->deriv_ids[id[1] - 1];
       id[3] = csc_find(&o->rjac, id[1] - 1, id[1] - 1);
+    } else if (t == RXN_AQUEOUS_EQUILIBRIUM) {
+      aqeq_update_ids(o, id, RX_FLT(o, r));
+    } else if (t == RXN_SIMPOL_PHASE_TRANSFER) {
+      simpol_update_ids(o, id);
     }
   }
 }
@@ -282,11 +1256,19 @@ void oracle_initialize(void *om, const double *abs_tol, double rel_tol,
   }
   update_ids(o);
 
-  /* sub-model parameter Jacobian: only the dummy (0,0) element */
+  /* sub-model parameter Jacobian: the dummy (0,0) element, then the sub
+   * models' elements (camp_solver.c:1509-1550) */
   ColElems pj;
   cols_init(&pj, ns);
   cols_register(&pj, 0, 0);
+  for (int k = 0; k < o->n_sub; ++k) {
+    int *id = SUB_INT(o, k);
+    sub_check(id);
+    zsr_used_jac_elem(id + 1, &pj);
+  }
   o->pjac = cols_build(&pj);
+  for (int k = 0; k < o->n_sub; ++k) zsr_update_ids(o, SUB_INT(o, k) + 1);
+  o->J_params = xcalloc(o->pjac.nnz, sizeof(double));
 
   /* solver Jacobian structure and number of mapped values */
   ColElems sj;
@@ -375,6 +1357,11 @@ void oracle_free(void *om) {
   free(o->rxn_int); free(o->rxn_float); free(o->rxn_env);
   free(o->rxn_int_idx); free(o->rxn_float_idx); free(o->rxn_env_idx);
   free(o->abs_tol);
+  free(o->ph_int); free(o->ph_flt); free(o->ph_int_idx); free(o->ph_flt_idx);
+  free(o->rep_int); free(o->rep_flt); free(o->rep_env); free(o->rep_int_idx);
+  free(o->rep_flt_idx); free(o->rep_env_idx);
+  free(o->sub_int); free(o->sub_flt); free(o->sub_env); free(o->sub_int_idx);
+  free(o->sub_flt_idx); free(o->sub_env_idx);
   if (o->initialized) {
     free(o->rjac.colptr); free(o->rjac.rowval);
     free(o->pjac.colptr); free(o->pjac.rowval);
@@ -386,6 +1373,7 @@ void oracle_free(void *om) {
     free(o->J_solver); free(o->J_state); free(o->J_deriv);
     free(o->td_p); free(o->td_l); free(o->jp); free(o->jl); free(o->J_rxn);
     free(o->est); free(o->tmp1); free(o->tmp2); free(o->lu_w);
+    free(o->J_params);
   }
   free(o);
 }
@@ -398,6 +1386,12 @@ void oracle_free(void *om) {
  * rxn_emission.c:109-119, rxn_first_order_loss.c:119-129 */
 static void update_env_cell(Oracle *o, int cell, const double *env_data) {
   double T = env_data[0], P = env_data[1];
+  /* aero_rep_update_env_state (nothing for modal/binned mass, :137-146) and
+   * sub_model_update_env_state (camp_solver.c:599-601) */
+  for (int k = 0; k < o->n_sub; ++k) {
+    sub_check(SUB_INT(o, k));
+    zsr_env(SUB_ENV(o, k, cell), T, P);
+  }
   for (int r = 0; r < o->n_rxn; ++r) {
     int *id = RX_INT(o, r);
     int t = *(id++);
@@ -424,6 +1418,12 @@ static void update_env_cell(Oracle *o, int cell, const double *env_data) {
       case RXN_EMISSION:
       case RXN_FIRST_ORDER_LOSS:
         e[0] = f[0] * e[1];
+        break;
+      case RXN_AQUEOUS_EQUILIBRIUM:
+        aqeq_env(f, e, T);
+        break;
+      case RXN_SIMPOL_PHASE_TRANSFER:
+        simpol_env(f, e, T, P);
         break;
       default:
         fprintf(stderr, "oracle: unsupported reaction type %d\n", t);
@@ -468,8 +1468,11 @@ static void td_add(Oracle *o, int i, long double c) {
 }
 
 /* one cell; state = the cell's (already updated) state vector */
-static void cell_deriv(Oracle *o, int cell, const double *state, double *deriv,
+static void cell_deriv(Oracle *o, int cell, double *state, double *deriv,
                        const double *est, double time_step) {
+  /* aero_rep_update_state and sub_model_calculate (camp_solver.c:913-917) */
+  rep_update_state(o, cell, state);
+  sub_calculate(o, cell, state);
   for (int i = 0; i < o->nv; ++i) o->td_p[i] = o->td_l[i] = 0.0;
   const double *env = &o->rxn_env[(size_t)cell * o->n_env];
   for (int r = 0; r < o->n_rxn; ++r) {
@@ -497,6 +1500,10 @@ static void cell_deriv(Oracle *o, int cell, const double *state, double *deriv,
     } else if (t == RXN_FIRST_ORDER_LOSS) {
       long double rate = e[0] * state[id[1] - 1];
       if (id[2] >= 0) td_add(o, id[2], -rate);
+    } else if (t == RXN_AQUEOUS_EQUILIBRIUM) {
+      aqeq_deriv(o, state, id, f, e);
+    } else if (t == RXN_SIMPOL_PHASE_TRANSFER) {
+      simpol_deriv(o, cell, state, id, f, e);
     }
   }
   /* time_derivative_output, time_derivative.c:46-88 */
@@ -558,9 +1565,15 @@ static int range_f(Oracle *o, int c0, int nc, const double *y, double *deriv,
 }
 
 /* J(y): camp_solver.c:972-1126 with Jacobian.c:216-241 and calc_jac */
-static void cell_jac(Oracle *o, int cell, const double *state, double *J,
+static void cell_jac(Oracle *o, int cell, double *state, double *J,
                      double time_step) {
+  for (int e = 0; e < o->pjac.nnz; ++e) o->J_params[e] = 0.0;
   for (int e = 0; e < o->rjac.nnz; ++e) o->jp[e] = o->jl[e] = 0.0;
+  /* aero_rep_update_state, sub_model_calculate, sub-model Jacobian
+   * (camp_solver.c:1065-1072) */
+  rep_update_state(o, cell, state);
+  sub_calculate(o, cell, state);
+  sub_jac(o, cell, state, o->J_params);
   const double *env = &o->rxn_env[(size_t)cell * o->n_env];
   for (int r = 0; r < o->n_rxn; ++r) {
     int *id = RX_INT(o, r);
@@ -586,13 +1599,18 @@ static void cell_jac(Oracle *o, int cell, const double *state, double *J,
       }
     } else if (t == RXN_FIRST_ORDER_LOSS) {
       if (id[3] >= 0) o->jl[id[3]] += e[0];
+    } else if (t == RXN_AQUEOUS_EQUILIBRIUM) {
+      aqeq_jac(o, state, id, f, e);
+    } else if (t == RXN_SIMPOL_PHASE_TRANSFER) {
+      simpol_jac(o, cell, state, id, f, e);
     }
   }
   for (int e = 0; e < o->rjac.nnz; ++e) o->J_rxn[e] = o->jp[e] - o->jl[e];
   for (int k = 0; k < o->nnz; ++k) J[k] = 0.0;
-  /* J_params[0] = 1 is the only parameter value without sub models */
+  /* J += J_rxn * J_params over the map, J_params[0] = 1 (:1100-1106) */
+  o->J_params[0] = 1.0;
   for (int m = 0; m < o->n_mapped; ++m)
-    J[o->map_solver[m]] += o->J_rxn[o->map_rxn[m]] * (o->map_param[m] == 0 ? 1.0 : 0.0);
+    J[o->map_solver[m]] += o->J_rxn[o->map_rxn[m]] * o->J_params[o->map_param[m]];
 }
 
 static int range_jac(Oracle *o, int c0, int nc, const double *y, double *fy,
@@ -1384,6 +2402,10 @@ static int run_problem(Oracle *o, Cv *cv, int c0, int nc, double t0, double t1) 
       for (int d = 0; d < nv; ++d)
         o->state[(size_t)(c0 + c) * ns + o->dep_state[d]] =
             y[c * nv + d] > 0.0 ? y[c * nv + d] : 0.0;
+    /* the post-solve sub_model_calculate runs on the cell the last f()/Jac()
+     * loop left current, the problem's last cell (camp_solver.c:685-687) */
+    if (o->n_rxn > 0)
+      sub_calculate(o, c0 + nc - 1, &o->state[(size_t)(c0 + nc - 1) * ns]);
   }
   free(y);
   return flag;
@@ -1647,4 +2669,69 @@ void oracle_kat_time_derivative(int n_spec, int n_contrib, const int *spec,
   }
   free(o.td_p);
   free(o.td_l);
+}
+
+/* --------------------------------------------------------------------- */
+/* Multi-phase introspection for the parity tests                         */
+/* --------------------------------------------------------------------- */
+int oracle_param_pattern(void *om, int *colptr, int *rowval) {
+  Oracle *o = om;
+  if (colptr) memcpy(colptr, o->pjac.colptr, (o->ns + 1) * sizeof(int));
+  if (rowval) memcpy(rowval, o->pjac.rowval, o->pjac.nnz * sizeof(int));
+  return o->pjac.nnz;
+}
+
+/* which: 0 rxn float data, 1 rep int data (with type slots), 2 rep float
+ * data, 3 sub-model int data (with type slots), 4 sub-model env data */
+void oracle_get_arrays(void *om, int which, void *dst, int n) {
+  Oracle *o = om;
+  switch (which) {
+    case 0: memcpy(dst, o->rxn_float, n * sizeof(double)); break;
+    case 1: memcpy(dst, o->rep_int, n * sizeof(int)); break;
+    case 2: memcpy(dst, o->rep_flt, n * sizeof(double)); break;
+    case 3: memcpy(dst, o->sub_int, n * sizeof(int)); break;
+    case 4: memcpy(dst, o->sub_env, n * sizeof(double)); break;
+  }
+}
+
+/* aero_rep_update_state + sub_model_calculate on one cell's state */
+void oracle_cell_update(void *om, int cell, double *state) {
+  Oracle *o = om;
+  rep_update_state(o, cell, state);
+  sub_calculate(o, cell, state);
+}
+
+/* which: 0 radius, 1 number concentration, 2 phase mass, 3 phase avg MW */
+void oracle_rep_get(void *om, int rep, int cell, const double *state, int phase, int which,
+                    double *value, double *partial) {
+  Oracle *o = om;
+  switch (which) {
+    case 0: rep_eff_radius(o, rep, cell, phase, value, partial); break;
+    case 1: rep_number_conc(o, rep, cell, state, phase, value, partial); break;
+    case 2: rep_phase_prop(o, rep, cell, state, phase, 0, value, partial); break;
+    case 3: rep_phase_prop(o, rep, cell, state, phase, 1, value, partial); break;
+  }
+}
+
+/* The two factors of the solver Jacobian for one cell: the reaction
+ * Jacobian (rjac order) and the sub-model parameter Jacobian (pjac order,
+ * dummy element included), as Jac() forms them (camp_solver.c:1052-1106). */
+int oracle_eval_jac_parts(void *om, int cell, const double *state_cell, double *j_rxn,
+                          double *j_params) {
+  Oracle *o = om;
+  double *st = xmalloc((size_t)o->ns * sizeof(double));
+  double *y = xmalloc((size_t)(o->nv ? o->nv : 1) * sizeof(double));
+  double *fy = xmalloc((size_t)(o->nv ? o->nv : 1) * sizeof(double));
+  double *jac = xmalloc((size_t)(o->nnz ? o->nnz : 1) * sizeof(double));
+  memcpy(st, state_cell, o->ns * sizeof(double));
+  for (int d = 0; d < o->nv; ++d) y[d] = state_cell[o->dep_state[d]];
+  int rc = update_cell_state(o, st, y);
+  if (!rc) {
+    cell_deriv(o, cell, st, fy, NULL, 1.0);
+    cell_jac(o, cell, st, jac, 1.0);
+    memcpy(j_rxn, o->J_rxn, o->rjac.nnz * sizeof(double));
+    memcpy(j_params, o->J_params, o->pjac.nnz * sizeof(double));
+  }
+  free(st); free(y); free(fy); free(jac);
+  return rc;
 }

@@ -87,8 +87,43 @@ def lib():
         L.oracle_kat_pattern.restype = _i
         L.oracle_kat_jac_accum.argtypes = [_i, _i, _ip, _ip, _dp, _dp]
         L.oracle_kat_time_derivative.argtypes = [_i, _i, _ip, _dp, _dp, _dp]
+        L.oracle_set_aero_sizes.argtypes = [_p] + [_i] * 11
+        L.oracle_add_aero_phase.argtypes = [_p, _i, _i, _ip, _dp]
+        L.oracle_add_aero_rep.argtypes = [_p, _i, _i, _i, _i, _ip, _dp]
+        L.oracle_add_sub_model.argtypes = [_p, _i, _i, _i, _i, _ip, _dp]
+        L.oracle_param_pattern.argtypes = [_p, _ip, _ip]
+        L.oracle_param_pattern.restype = _i
+        L.oracle_get_arrays.argtypes = [_p, _i, _p, _i]
+        L.oracle_cell_update.argtypes = [_p, _i, _dp]
+        L.oracle_rep_get.argtypes = [_p, _i, _i, _dp, _i, _i, _dp, _dp]
+        L.oracle_eval_jac_parts.argtypes = [_p, _i, _dp, _dp, _dp]
+        L.oracle_eval_jac_parts.restype = _i
         _lib = L
     return _lib
+
+
+def packed_parts(mech):
+    """{'rxn', 'phase', 'rep', 'sub'} packed arrays of a gas or multi-phase
+    mechanism, in the order the Fortran layer adds them."""
+    if hasattr(mech, "rep"):
+        from camp_amd.multiphase import pack_multiphase
+        return pack_multiphase(mech)
+    from camp_amd.mechanisms import pack
+    return {"rxn": pack(mech), "phase": [], "rep": [], "sub": []}
+
+
+def aero_sizes(parts):
+    """solver_new's aerosol/sub-model size arguments for packed parts."""
+    ph, rp, sb = parts["phase"], parts["rep"], parts["sub"]
+    return [len(ph), sum(len(p[0]) for p in ph), sum(len(p[1]) for p in ph),
+            len(rp), sum(len(p[1]) for p in rp), sum(len(p[2]) for p in rp),
+            sum(p[3] for p in rp),
+            len(sb), sum(len(p[1]) for p in sb), sum(len(p[2]) for p in sb),
+            sum(p[3] for p in sb)]
+
+
+def _arr(v, dt):
+    return np.asarray(v if len(v) else [0], dtype=dt)
 
 
 class Oracle:
@@ -96,21 +131,33 @@ class Oracle:
 
     def __init__(self, mech, n_cells, abs_tol=None, rel_tol=None,
                  max_steps=10000, max_conv_fails=1000):
-        from camp_amd.mechanisms import pack
         L = lib()
         self.mech = mech
         self.n_cells = n_cells
         self.ns = mech.n_state
-        packed = pack(mech)
+        parts = packed_parts(mech)
+        packed = parts["rxn"]
         n_int = sum(len(p[1]) for p in packed)
         n_float = sum(len(p[2]) for p in packed)
         n_env = sum(p[3] for p in packed)
         vt = np.asarray(mech.var_type, dtype=np.int32)
         self.h = L.oracle_new(self.ns, n_cells, _ip_(vt), len(packed), n_int, n_float, n_env)
+        self.sizes = aero_sizes(parts)
+        L.oracle_set_aero_sizes(self.h, *self.sizes)
+        for ints, floats in parts["phase"]:
+            L.oracle_add_aero_phase(self.h, len(ints), len(floats), _ip_(_arr(ints, np.int32)),
+                                    _dp_(_arr(floats, np.float64)))
+        for t, ints, floats, ne in parts["rep"]:
+            L.oracle_add_aero_rep(self.h, t, len(ints), len(floats), ne,
+                                  _ip_(_arr(ints, np.int32)), _dp_(_arr(floats, np.float64)))
+        for t, ints, floats, ne in parts["sub"]:
+            L.oracle_add_sub_model(self.h, t, len(ints), len(floats), ne,
+                                   _ip_(_arr(ints, np.int32)), _dp_(_arr(floats, np.float64)))
         for t, ints, floats, ne in packed:
             ia = np.asarray(ints, dtype=np.int32)
             fa = np.asarray(floats, dtype=np.float64)
             L.oracle_add_rxn(self.h, t, len(ints), len(floats), ne, _ip_(ia), _dp_(fa))
+        self.n_float = n_float
         at = np.asarray(mech.abs_tol if abs_tol is None else abs_tol, dtype=np.float64)
         rt = mech.rel_tol if rel_tol is None else rel_tol
         L.oracle_initialize(self.h, _dp_(at), rt, max_steps, max_conv_fails)
@@ -222,6 +269,41 @@ class Oracle:
     def lu_nnz(self):
         return lib().oracle_lu_nnz(self.h)
 
+    # multi-phase introspection
+    def param_pattern(self):
+        L = lib()
+        n = L.oracle_param_pattern(self.h, None, None)
+        cp = np.zeros(self.ns + 1, np.int32)
+        rv = np.zeros(max(n, 1), np.int32)
+        L.oracle_param_pattern(self.h, _ip_(cp), _ip_(rv))
+        return cp, rv[:n]
+
+    def arrays(self, which, n):
+        """0 rxn float, 1 rep int, 2 rep float, 3 sub int, 4 sub env data."""
+        a = np.zeros(n, np.int32 if which in (1, 3) else np.float64)
+        lib().oracle_get_arrays(self.h, which, a.ctypes.data, n)
+        return a
+
+    def cell_update(self, cell, state_cell):
+        s = np.array(state_cell, dtype=np.float64)
+        lib().oracle_cell_update(self.h, cell, _dp_(s))
+        return s
+
+    def jac_parts(self, cell, state_cell):
+        """(J_rxn over the reaction pattern, J_params over the parameter pattern)"""
+        s = np.ascontiguousarray(state_cell, dtype=np.float64)
+        jr = np.zeros(lib().oracle_nnz_rxn(self.h))
+        jp = np.zeros(lib().oracle_param_pattern(self.h, None, None))
+        rc = lib().oracle_eval_jac_parts(self.h, cell, _dp_(s), _dp_(jr), _dp_(jp))
+        return rc, jr, jp
+
+    def rep_get(self, rep, cell, state_cell, phase, which, n_partial):
+        s = np.ascontiguousarray(state_cell, dtype=np.float64)
+        v = np.zeros(1)
+        part = np.zeros(max(n_partial, 1))
+        lib().oracle_rep_get(self.h, rep, cell, _dp_(s), phase, which, _dp_(v), _dp_(part))
+        return float(v[0]), part[:n_partial]
+
 
 _ref = None
 
@@ -245,6 +327,16 @@ def reflib():
         L.ref_update_data.argtypes = [_p, _i, _ip, _i, _i, _d]
         L.ref_get_env_data.argtypes = [_p, _dp, _i]
         L.ref_time_derivative.argtypes = [_i, _i, _ip, _dp, _dp, _dp]
+        L.ref_set_aero.argtypes = [_p] + [_i] * 11 + [_dp]
+        L.ref_add_phase.argtypes = [_p, _i, _i, _ip, _dp]
+        L.ref_add_rep.argtypes = [_p, _i, _i, _i, _i, _ip, _dp]
+        L.ref_add_sub.argtypes = [_p, _i, _i, _i, _i, _ip, _dp]
+        L.ref_param_pattern.argtypes = [_p, _ip, _ip, _ip, _i]
+        L.ref_param_pattern.restype = _i
+        L.ref_update_env_all.argtypes = [_p, _i, _dp]
+        L.ref_cell_update.argtypes = [_p, _i, _dp]
+        L.ref_rep_get.argtypes = [_p, _i, _dp, _i, _i, _i, _dp, _dp]
+        L.ref_get_arrays.argtypes = [_p, _i, _p, _i]
         _ref = L
     return _ref
 
@@ -253,18 +345,31 @@ class RefModel:
     """The reference's own rxn_* / Jacobian functions on a packed mechanism."""
 
     def __init__(self, mech, n_cells):
-        from camp_amd.mechanisms import pack
         L = reflib()
         self.mech = mech
         self.n_cells = n_cells
-        packed = pack(mech)
+        parts = packed_parts(mech)
+        packed = parts["rxn"]
         self.n_int = sum(len(p[1]) for p in packed) + len(packed)
         n_float = sum(len(p[2]) for p in packed)
+        self.n_float = n_float
         self.n_env = sum(p[3] for p in packed)
         vt = np.asarray(mech.var_type, dtype=np.int32)
-        self._keep = []
         self.h = L.ref_new(mech.n_state, n_cells, _ip_(vt), len(packed),
                            self.n_int - len(packed), n_float, self.n_env)
+        self.sizes = aero_sizes(parts)
+        at = np.asarray(mech.abs_tol, dtype=np.float64)
+        L.ref_set_aero(self.h, *self.sizes, _dp_(at))
+        # the reference's add functions copy from (and may advance) these
+        for ints, floats in parts["phase"]:
+            L.ref_add_phase(self.h, len(ints), len(floats), _ip_(_arr(ints, np.int32)),
+                            _dp_(_arr(floats, np.float64)))
+        for t, ints, floats, ne in parts["rep"]:
+            L.ref_add_rep(self.h, t, len(ints), len(floats), ne, _ip_(_arr(ints, np.int32)),
+                          _dp_(_arr(floats, np.float64)))
+        for t, ints, floats, ne in parts["sub"]:
+            L.ref_add_sub(self.h, t, len(ints), len(floats), ne, _ip_(_arr(ints, np.int32)),
+                          _dp_(_arr(floats, np.float64)))
         for t, ints, floats, ne in packed:
             ia = np.asarray(ints, dtype=np.int32)
             fa = np.asarray(floats if floats else [0.0], dtype=np.float64)
@@ -296,6 +401,36 @@ class RefModel:
         out = np.zeros(self.n_cells * self.n_env)
         reflib().ref_get_env_data(self.h, _dp_(out), out.size)
         return out.reshape(self.n_cells, self.n_env)
+
+    # multi-phase
+    def param_pattern(self, deriv_ids):
+        ns = self.mech.n_state
+        di = np.ascontiguousarray(deriv_ids, dtype=np.int32)
+        cp = np.zeros(ns + 1, np.int32)
+        rv = np.zeros(ns * ns, np.int32)
+        n = reflib().ref_param_pattern(self.h, _ip_(di), _ip_(cp), _ip_(rv), ns * ns)
+        return cp, rv[:n]
+
+    def update_env_all(self, cell, T, P):
+        e = np.array([T, P], dtype=np.float64)
+        reflib().ref_update_env_all(self.h, cell, _dp_(e))
+
+    def cell_update(self, cell, state_cell):
+        s = np.array(state_cell, dtype=np.float64)
+        reflib().ref_cell_update(self.h, cell, _dp_(s))
+        return s
+
+    def rep_get(self, rep, cell, state_cell, phase, which, n_partial):
+        s = np.ascontiguousarray(state_cell, dtype=np.float64)
+        v = np.zeros(1)
+        part = np.zeros(max(n_partial, 1))
+        reflib().ref_rep_get(self.h, cell, _dp_(s), rep, phase, which, _dp_(v), _dp_(part))
+        return float(v[0]), part[:n_partial]
+
+    def arrays(self, which, n):
+        a = np.zeros(n, np.int32 if which in (1, 3) else np.float64)
+        reflib().ref_get_arrays(self.h, which, a.ctypes.data, n)
+        return a
 
 
 def ref_time_derivative(n_spec, spec, contrib, est=None):
