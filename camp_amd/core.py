@@ -38,6 +38,15 @@ _PAYLOAD = {
 }
 
 
+def packed_parts(mech):
+    """{'rxn', 'phase', 'rep', 'sub'}: packed arrays of a gas-phase or
+    multi-phase mechanism in the reference layouts."""
+    if hasattr(mech, "rep"):
+        from .multiphase import pack_multiphase
+        return pack_multiphase(mech)
+    return {"rxn": pack(mech), "phase": [], "rep": [], "sub": []}
+
+
 class CampSolver:
     """camp_solver_data_t over libcamp_mi355x.so."""
 
@@ -48,17 +57,39 @@ class CampSolver:
         self.mech = mech
         self.n_cells = int(n_cells)
         self.ns = mech.n_state
-        packed = pack(mech)
+        parts = packed_parts(mech)
+        packed = parts["rxn"]
+        self._n_int = sum(len(p[1]) + 1 for p in packed)
         n_int = sum(len(p[1]) for p in packed)
         n_float = sum(len(p[2]) for p in packed)
         n_env = sum(p[3] for p in packed)
         vt = np.ascontiguousarray(mech.var_type, dtype=np.int32)
+        ph, rp, sb = parts["phase"], parts["rep"], parts["sub"]
+        sizes = [len(ph), sum(len(p[0]) for p in ph), sum(len(p[1]) for p in ph),
+                 len(rp), sum(len(p[1]) for p in rp), sum(len(p[2]) for p in rp),
+                 sum(p[3] for p in rp),
+                 len(sb), sum(len(p[1]) for p in sb), sum(len(p[2]) for p in sb),
+                 sum(p[3] for p in sb)]
         self.h = L.solver_new(self.ns, self.n_cells, _ip(vt), len(packed), n_int, n_float, n_env,
-                              0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+                              *sizes)
+        # the order camp_solver_data adds them (camp_solver_data.F90:662-762)
         for t, ints, floats, ne in packed:
             ia = np.ascontiguousarray(ints, dtype=np.int32)
             fa = np.ascontiguousarray(floats if floats else [0.0], dtype=np.float64)
             L.rxn_add_condensed_data(t, len(ints), len(floats), ne, _ip(ia), _dp(fa), self.h)
+        for ints, floats in ph:
+            ia = np.ascontiguousarray(ints, dtype=np.int32)
+            fa = np.ascontiguousarray(floats if floats else [0.0], dtype=np.float64)
+            L.aero_phase_add_condensed_data(len(ints), len(floats), _ip(ia), _dp(fa), self.h)
+        for t, ints, floats, ne in rp:
+            ia = np.ascontiguousarray(ints, dtype=np.int32)
+            fa = np.ascontiguousarray(floats if floats else [0.0], dtype=np.float64)
+            L.aero_rep_add_condensed_data(t, len(ints), len(floats), ne, _ip(ia), _dp(fa), self.h)
+        for t, ints, floats, ne in sb:
+            ia = np.ascontiguousarray(ints, dtype=np.int32)
+            fa = np.ascontiguousarray(floats if floats else [0.0], dtype=np.float64)
+            L.sub_model_add_condensed_data(t, len(ints), len(floats), ne, _ip(ia), _dp(fa),
+                                           self.h)
         at = np.ascontiguousarray(mech.abs_tol if abs_tol is None else abs_tol, dtype=np.float64)
         rt = mech.rel_tol if rel_tol is None else rel_tol
         L.solver_initialize(self.h, _dp(at), rt, max_steps, max_conv_fails)
@@ -142,7 +173,7 @@ class CampSolver:
         return out
 
     def rxn_env(self):
-        n_env = sum(p[3] for p in pack(self.mech))
+        n_env = sum(p[3] for p in packed_parts(self.mech)["rxn"])
         out = np.zeros((self.n_cells, n_env))
         lib().camp_solver_get_rxn_env(self.h, _dp(out))
         return out
@@ -182,11 +213,10 @@ class CampSolver:
         cp = np.zeros(self.nv + 1, np.int32)
         rv = np.zeros(max(self.nnz, 1), np.int32)
         di = np.zeros(self.ns, np.int32)
-        n_int = sum(len(p[1]) + 1 for p in pack(self.mech))
-        ri = np.zeros(max(n_int, 1), np.int32)
+        ri = np.zeros(max(self._n_int, 1), np.int32)
         L.camp_solver_get_pattern(self.h, _ip(rcp), _ip(rrv), _ip(cp), _ip(rv), _ip(di), _ip(ri))
         return dict(rxn_colptr=rcp, rxn_rowval=rrv[:self.nnz_rxn], colptr=cp,
-                    rowval=rv[:self.nnz], deriv_ids=di, rxn_int_data=ri[:n_int])
+                    rowval=rv[:self.nnz], deriv_ids=di, rxn_int_data=ri[:self._n_int])
 
     def eval_deriv(self, state, env, time_step=1.0, use_deriv_est=False):
         state = np.ascontiguousarray(state, dtype=np.float64)

@@ -48,6 +48,8 @@ struct CellCv {
   double *s_state, *s_k, *znb, *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2,
       *w, *lu;
   double *Js, *Jy, *Jf;
+  MpLds mpl;   // multi-phase extras (LDS); unused for gas-phase programs
+  double* Jr;  // reaction Jacobian scratch (global) of multi-phase programs
   double tn, h, hscale, hprime, next_h, eta, etamax, etaq, etaqm1, etaqp1, hu, init_h, rtol;
   double *l, *tq, *tau;  // small coefficient arrays, in LDS (uniform values)
   double rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
@@ -82,7 +84,10 @@ struct CellCv {
     long long t0 = tick();
     double ts = next_h > 0.0 ? next_h : init_h;
     int bad = cell_set_state(P, lane, yv, s_state);
-    if (!bad) cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, true, ts, out);
+    if (!bad) {
+      if (P.mp) mp_f_prologue(P, lane, s_state, s_k, mpl);
+      cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, true, ts, out, mpl.slot);
+    }
     wsync();
     if (prof) c_f += clock64() - t0;
     return bad;
@@ -93,8 +98,15 @@ struct CellCv {
     long long t0 = tick();
     double ts = next_h > 0.0 ? next_h : init_h;
     if (cell_set_state(P, lane, yv, s_state)) return 1;
-    cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy);
-    cell_jac(P, lane, s_state, s_k, ts, Js);
+    if (P.mp) {
+      mp_f_prologue(P, lane, s_state, s_k, mpl);
+      mp_j_prologue(P, lane, s_state, s_k, mpl);
+      cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy, mpl.slot);
+      cell_jac_mp(P, lane, s_state, s_k, ts, mpl, Jr, Js);
+    } else {
+      cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy);
+      cell_jac(P, lane, s_state, s_k, ts, Js);
+    }
     for (int i = lane; i < nv; i += kWave) {
       Jy[i] = yv[i];
       Jf[i] = fy[i];
@@ -603,15 +615,20 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
   const DevProgram& P = a.P;
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
   int nv = P.nv;
-  size_t per_cell = cell_bdf_lds_doubles(P.ns, P.nv, P.n_env, P.lu_nnz);
+  size_t per_cell = cell_bdf_lds_doubles(P);
   double* base = smem + (size_t)wv * per_cell;
+  double* scratch = a.wave_scratch
+                        ? a.wave_scratch + ((size_t)blockIdx.x * (blockDim.x / kWave) + wv) *
+                                               cell_bdf_wave_scratch(P)
+                        : nullptr;
   CellCv cv;
   cv.P = P;
   cv.lane = lane;
   cv.nv = nv;
   cv.s_state = base;
   cv.s_k = cv.s_state + P.ns;
-  double* v = cv.s_k + P.n_env;
+  cv.mpl = mp_lds(P, cv.s_k + P.n_env);
+  double* v = cv.s_k + P.n_env + mp_lds_doubles(P);
   cv.znb = v;
   v += 6 * (size_t)nv;
   cv.ewt = v; v += nv;
@@ -624,8 +641,14 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
   cv.gh1 = v; v += nv;
   cv.gh2 = v; v += nv;
   cv.w = v; v += nv;
-  cv.lu = v;
-  v += P.lu_nnz;
+  if (P.mp) {
+    cv.lu = scratch;
+    cv.Jr = scratch + P.lu_nnz;
+  } else {
+    cv.lu = v;
+    cv.Jr = nullptr;
+    v += P.lu_nnz;
+  }
   cv.l = v;
   cv.tq = v + 6;
   cv.tau = v + 12;
@@ -663,6 +686,7 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
     double T = a.env[2 * cell], Pa = a.env[2 * cell + 1];
     for (int r = lane; r < P.n_rxn; r += kWave) rate_constant(P, r, T, Pa, cv.s_k);
     wsync();
+    if (P.mp) mp_cell_env(P, lane, T, Pa, cv.mpl);
     for (int i = lane; i < P.n_env; i += kWave) renv_c[i] = cv.s_k[i];
     for (int i = lane; i < nv; i += kWave) {
       double s = cv.s_state[P.dep_state[i]];
@@ -676,6 +700,16 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
       for (int i = lane; i < nv; i += kWave) {
         double yv = cv.y[i];
         state_c[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
+        if (P.mp) cv.s_state[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
+      }
+      if (P.mp) {
+        // the post-solve sub_model_calculate (camp_solver.c:685-687) on the
+        // final state; the sub models' outputs go back to the state array
+        wsync();
+        for (int z = lane; z < P.n_zi; z += kWave) {
+          mp_zsr_water(P, z, cv.mpl.sub_env, cv.s_state);
+          state_c[P.zi_water[z]] = cv.s_state[P.zi_water[z]];
+        }
       }
     }
     if (lane == 0) {
@@ -694,11 +728,21 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
   }
 }
 
+// Cells of multi-phase programs need more than 64 KiB of LDS per block;
+// gfx950 provides 160 KiB per CU, opted into per kernel.
+static void allow_lds(size_t lds) {
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)k_cell_bdf,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s) {
+  allow_lds(lds);
   hipLaunchKernelGGL(k_cell_bdf, dim3(grid), dim3(block), lds, s, a);
 }
 
 int cell_bdf_blocks_per_cu(int block, size_t lds) {
+  allow_lds(lds);
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cell_bdf, block, lds) != hipSuccess) return 0;
   return n;

@@ -38,7 +38,25 @@ struct DevProgram {
   const int *rx_type, *rx_fidx, *rx_eidx, *rx_nreact;
   const double* rx_float;
   const double* abs_tol_dep;  // [nv]
+  // multi-phase tables (mp == 0 for gas-phase programs); see Program in
+  // camp_model.hpp for their meaning
+  int mp, rnnz, pnnz, n_pi, n_zi, n_sp, n_aq, n_slots, n_sub_env;
+  double min_mass_hi, min_mass_lo, min_moles_hi, min_moles_lo, min_volume;
+  const int *pi_start, *pi_model, *pm_ptr, *pm_counted;
+  const double *pi_dp, *pm_mw, *pm_dens;
+  const int *zi_water, *zi_gw, *zi_sub, *zi_pair_ptr, *zp_cat, *zp_an, *zp_ncat, *zp_nan, *zp_ny,
+      *zp_jgw, *zp_jcat, *zp_jan;
+  const double *zp_catmw, *zp_anmw, *zp_lowrh, *zp_y;
+  const int *sp_gas, *sp_aero, *sp_dgas, *sp_daero, *sp_pi, *sp_env, *sp_flt, *sp_slot;
+  const double* sp_radius;
+  const int *aq_nr, *aq_np, *aq_sp, *aq_water, *aq_env, *aq_flt, *aq_slot;
+  const double* aq_small_water;
+  const int* js_ptr;
+  const int4* js_a;
+  const double* js_c;
+  const int *jm_ptr, *jm_rxn, *jm_param;
 };
+
 
 // ---------------------------------------------------------------------
 // Double-double arithmetic: the accumulators of f and J use it in place of
@@ -97,6 +115,38 @@ __device__ __forceinline__ int wave_or(int v) {
   return __shfl(v, 0, kWave);
 }
 
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kGasConst = 8.314472;  // util.h:17
+
+// --- environment: SIMPOL and aqueous equilibrium (rxn env slots) ----------
+// rxn_SIMPOL_phase_transfer_update_env_state (:220-266):
+// e = [MFP_M, ALPHA, EQUIL_CONST, KGM3_TO_PPM]
+__device__ __forceinline__ void mp_simpol_env(const double* f, double* e, double T, double Pa) {
+  double alpha = 0.1;
+  if (f[0] != 0.0 || f[1] != 0.0) {
+    double del_g = f[0] - T * f[1];
+    alpha = exp(-del_g / (kGasConst * T));
+    alpha = alpha / (1.0 + alpha);
+  }
+  e[1] = alpha;
+  e[0] = 3.0 * f[2] / sqrt(8.0 * kGasConst * T / (kPi * f[9]));
+  double vp = f[4] / T + f[5] + f[6] * T + f[7] * log(T);
+  vp = 101325.0 * pow(10.0, vp);
+  e[3] = f[8] * T / Pa;
+  e[2] = vp / Pa / f[9] * 1.0e6;
+}
+// rxn_aqueous_equilibrium_update_env_state (:244-265): e = [k_forward]
+__device__ __forceinline__ void mp_aqeq_env(const double* f, double* e, double T) {
+  double k = f[1] == 0.0 ? f[0] : f[0] * exp(f[1] * (1.0 / T - 1.0 / 298.0));
+  e[0] = k * f[2];
+}
+// sub_model_ZSR_aerosol_water_update_env_state (:189-207): PPM_TO_RH
+__device__ __forceinline__ double mp_ppm_to_rh(double T, double Pa) {
+  double a = 1.0 - 373.15 / T;
+  a = (((-0.1299 * a - 0.6445) * a - 1.976) * a + 13.3185) * a;
+  return Pa / (101325.0 * exp(a)) / 1.0e6;
+}
+
 // ---------------------------------------------------------------------
 // Rate constants of one cell: rxn_update_env_state (rxn_solver.c:224-313)
 // with rxn_arrhenius.c:103-118, rxn_troe.c:106-130, rxn_photolysis.c:
@@ -121,6 +171,10 @@ __device__ __forceinline__ void rate_constant(const DevProgram& P, int r, double
     double lg = log10(kinf) / f[7];
     e[0] = (k0 / (1.0 + kinf)) * pow(f[6], (1.0 / (1.0 + pow(lg, 2.0)))) *
            pow(conv, (double)(nr - 1)) * f[8];
+  } else if (t == 10) {  // SIMPOL_PHASE_TRANSFER
+    mp_simpol_env(f, e, T, Pa);
+  } else if (t == 7) {  // AQUEOUS_EQUILIBRIUM
+    mp_aqeq_env(f, e, T);
   } else {  // PHOTOLYSIS, EMISSION, FIRST_ORDER_LOSS: SCALING * BASE_RATE
     e[0] = f[0] * e[1];
   }
@@ -150,7 +204,7 @@ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const 
                                            const double* s_k, const double* y,
                                            const double* Js, const double* Jy,
                                            const double* Jf, bool use_est, double h,
-                                           double* out) {
+                                           double* out, const double* s_slot = nullptr) {
   for (int i = lane; i < P.nv; i += kWave) {
     dd D = {0.0, 0.0};
     double S = 0.0;
@@ -160,6 +214,12 @@ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const 
       double c = P.fe_c[q];
       int flags = b >> 24;
       int g = (b << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
+      if (flags & 4) {  // multi-phase contribution slot (double-double)
+        dd v = {s_slot[2 * a.x], s_slot[2 * a.x + 1]};
+        D = dd_add(D, v);
+        S += fabs(v.hi);
+        continue;
+      }
       double k = s_k[a.x];
       dd rate;
       if (flags & 2) {
@@ -217,6 +277,28 @@ __device__ __forceinline__ int cell_set_state(const DevProgram& P, int lane, con
 // calc_jac_contrib + jacobian_output + jac_map (camp_solver.c:1052-1106,
 // Jacobian.c:223-241).  Lane e gathers the contributions of element e.
 // ---------------------------------------------------------------------
+// Mass-action contributions to Jacobian element e of the gather space
+// (solver elements, or reaction-pattern elements for multi-phase programs).
+__device__ __forceinline__ dd jac_elem_mass_action(const DevProgram& P, int e, const double* s_state,
+                                                   const double* s_k, double h) {
+  dd D = {0.0, 0.0};
+  for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q) {
+    int4 a = P.je_a[q];
+    double c = P.je_c[q];
+    double v = s_k[a.x];
+    if (a.y >= 0) v *= s_state[a.y];
+    if (a.z >= 0) v *= s_state[a.z];
+    if (c == -1.0) {
+      D = dd_add(D, dd{-v, 0.0});
+    } else {
+      int g = P.je_b[q];
+      if (g >= 0 && !(-v * s_state[a.w] * c * h <= s_state[g])) continue;
+      D = dd_add(D, dd{c * v, 0.0});
+    }
+  }
+  return D;
+}
+
 __device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const double* s_state,
                                          const double* s_k, double h, double* J) {
   for (int e = lane; e < P.nnz; e += kWave) {

@@ -40,6 +40,7 @@ __device__ __forceinline__ void stage_cell(const DevProgram& P, int lane, const 
 // ---------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const double* __restrict__ y,
                                                const double* __restrict__ state,
+                                               const double* __restrict__ env,
                                                const double* __restrict__ rxn_env,
                                                const double* __restrict__ Js,
                                                const double* __restrict__ Jy,
@@ -49,8 +50,9 @@ __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const doubl
   extern __shared__ double smem[];
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
   int wpb = blockDim.x / kWave;
-  double* s_state = smem + (size_t)wv * (P.ns + P.n_env);
+  double* s_state = smem + (size_t)wv * (P.ns + P.n_env + mp_lds_doubles(P));
   double* s_k = s_state + P.ns;
+  MpLds L = mp_lds(P, s_k + P.n_env);
   for (int c = blockIdx.x * wpb + wv; c < nc; c += gridDim.x * wpb) {
     stage_cell(P, lane, state + (size_t)c * P.ns, rxn_env + (size_t)c * P.n_env, s_state, s_k);
     const double* yc = y + (size_t)c * P.nv;
@@ -58,8 +60,12 @@ __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const doubl
       if (lane == 0) atomicOr(fail, 1);
       continue;
     }
+    if (P.mp) {
+      mp_cell_env(P, lane, env[2 * c], env[2 * c + 1], L);
+      mp_f_prologue(P, lane, s_state, s_k, L);
+    }
     cell_deriv(P, lane, s_state, s_k, yc, Js + (size_t)c * P.nnz, Jy + (size_t)c * P.nv,
-               Jf + (size_t)c * P.nv, use_est != 0, h, fout + (size_t)c * P.nv);
+               Jf + (size_t)c * P.nv, use_est != 0, h, fout + (size_t)c * P.nv, L.slot);
     wsync();
   }
 }
@@ -67,17 +73,27 @@ __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const doubl
 // Plain path: J(y) for all cells into J (nnz per cell, CSC element order).
 __global__ void __launch_bounds__(256) k_jac(DevProgram P, int nc, const double* __restrict__ y,
                                              const double* __restrict__ state,
+                                             const double* __restrict__ env,
                                              const double* __restrict__ rxn_env,
-                                             double* __restrict__ J, double h) {
+                                             double* __restrict__ J, double h,
+                                             double* __restrict__ Jr) {
   extern __shared__ double smem[];
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
   int wpb = blockDim.x / kWave;
-  double* s_state = smem + (size_t)wv * (P.ns + P.n_env);
+  double* s_state = smem + (size_t)wv * (P.ns + P.n_env + mp_lds_doubles(P));
   double* s_k = s_state + P.ns;
+  MpLds L = mp_lds(P, s_k + P.n_env);
   for (int c = blockIdx.x * wpb + wv; c < nc; c += gridDim.x * wpb) {
     stage_cell(P, lane, state + (size_t)c * P.ns, rxn_env + (size_t)c * P.n_env, s_state, s_k);
     if (cell_set_state(P, lane, y + (size_t)c * P.nv, s_state)) continue;
-    cell_jac(P, lane, s_state, s_k, h, J + (size_t)c * P.nnz);
+    if (P.mp) {
+      mp_cell_env(P, lane, env[2 * c], env[2 * c + 1], L);
+      mp_f_prologue(P, lane, s_state, s_k, L);
+      mp_j_prologue(P, lane, s_state, s_k, L);
+      cell_jac_mp(P, lane, s_state, s_k, h, L, Jr + (size_t)c * P.rnnz, J + (size_t)c * P.nnz);
+    } else {
+      cell_jac(P, lane, s_state, s_k, h, J + (size_t)c * P.nnz);
+    }
     wsync();
   }
 }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "camp_device.hpp"
+#include "camp_mp_device.hpp"
 
 namespace camp {
 
@@ -12,10 +13,17 @@ constexpr int kCellStats = 10;  // nst nfe nsetups netf nni ncfn nje flag hu nex
 constexpr int kCellProf = 5;    // shader cycles: f, Jac, LU factor, LU solve, whole solve
 
 // LDS doubles one wavefront needs for a cell in the persistent kernel:
-// state, rate data, Nordsieck history (6 vectors), 10 work vectors, LU and
-// 24 slots for the BDF coefficient arrays l, tq and tau.
-__host__ __device__ inline size_t cell_bdf_lds_doubles(int ns, int nv, int n_env, int lu_nnz) {
-  return (size_t)ns + n_env + 16 * (size_t)nv + lu_nnz + 24;
+// state, rate data, the multi-phase extras, Nordsieck history (6 vectors),
+// 10 work vectors, the LU factors (in global scratch for multi-phase
+// programs) and 24 slots for the BDF coefficient arrays l, tq and tau.
+__host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P) {
+  return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 16 * (size_t)P.nv +
+         (P.mp ? 0 : (size_t)P.lu_nnz) + 24;
+}
+// Global scratch doubles per wavefront of multi-phase programs: LU factors
+// and the reaction Jacobian.
+__host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P) {
+  return P.mp ? (size_t)P.lu_nnz + P.rnnz : 0;
 }
 
 struct CellBdfArgs {
@@ -37,6 +45,7 @@ struct CellBdfArgs {
   const double* cell_t1 = nullptr;
   double* prof = nullptr;  // [cells][kCellProf] when phase timing is on
   int* fail_count = nullptr;  // per cell: incremented on each failed solve
+  double* wave_scratch = nullptr;  // [grid waves][cell_bdf_wave_scratch] (multi-phase)
 };
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);
@@ -44,10 +53,18 @@ int cell_bdf_blocks_per_cu(int block, size_t lds);
 
 __global__ void k_rates(DevProgram P, int nc, const double* env, double* rxn_env);
 __global__ void k_deriv(DevProgram P, int nc, const double* y, const double* state,
-                        const double* rxn_env, const double* Js, const double* Jy,
-                        const double* Jf, double* fout, double h, int use_est, int* fail);
+                        const double* env, const double* rxn_env, const double* Js,
+                        const double* Jy, const double* Jf, double* fout, double h, int use_est,
+                        int* fail);
+// Jr: reaction-Jacobian scratch [nc][rnnz] of multi-phase programs (else unused)
 __global__ void k_jac(DevProgram P, int nc, const double* y, const double* state,
-                      const double* rxn_env, double* J, double h);
+                      const double* env, const double* rxn_env, double* J, double h, double* Jr);
+// LDS doubles per wavefront of k_deriv and k_jac, and wavefronts per block
+// (one for multi-phase programs, whose per-cell extras are large)
+inline size_t fj_lds_doubles(const DevProgram& P) {
+  return (size_t)P.ns + P.n_env + mp_lds_doubles(P);
+}
+inline int fj_waves_per_block(const DevProgram& P) { return P.mp ? 1 : 4; }
 __global__ void k_lu_factor(DevProgram P, int nc, const double* J, double gamma, double* LU,
                             int* fail);
 __global__ void k_lu_solve(DevProgram P, int nc, const double* LU, double* b, double s);

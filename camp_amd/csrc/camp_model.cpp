@@ -2,6 +2,8 @@
 // compilation into the flat tables the HIP kernels read.
 #include "camp_model.hpp"
 
+#include "camp_aero.hpp"
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -59,7 +61,107 @@ struct MassActionView {
   double yield(int x) const { return f[npf() + x]; }
 };
 
+// aero_phase_get_used_jac_elem (aero_phase_solver.c:40-63)
+int phase_used_jac_elem(const Model& m, int p, int state_id, std::vector<char>& flags) {
+  PhaseView v{&m.ph_int[m.ph_int_idx[p]], &m.ph_flt[m.ph_flt_idx[p]]};
+  int n = 0;
+  for (int i = 0; i < v.n(); ++i)
+    if (v.counted(i)) {
+      flags[state_id + i] = 1;
+      ++n;
+    }
+  return n;
+}
+
+BinnedRepView rep_view(Model& m, int r) {
+  return BinnedRepView{&m.rep_int[m.rep_int_idx[r]] + 1, &m.rep_flt[m.rep_flt_idx[r]]};
+}
+
+// aero_rep_modal_binned_mass_get_used_jac_elem (:84-113): flags the species
+// of every phase in the bin that holds phase instance aero_phase_idx and
+// records each phase's count in the representation's int data
+int rep_used_jac_elem(Model& m, int r, int aero_phase_idx, std::vector<char>& flags) {
+  BinnedRepView v = rep_view(m, r);
+  int n = 0;
+  for (int s = 0; s < v.nsec() && aero_phase_idx >= 0; s++)
+    for (int b = 0; b < v.nbin(s) && aero_phase_idx >= 0; b++)
+      for (int p = 0; p < v.nph(s) && aero_phase_idx >= 0; p++) {
+        if (aero_phase_idx == 0)
+          for (int j = 0; j < v.nph(s); j++) {
+            v.phase_njac(s, j, b) =
+                phase_used_jac_elem(m, v.phase_model(s, j, b), v.phase_state(s, j, b), flags);
+            n += v.phase_njac(s, j, b);
+          }
+        aero_phase_idx -= 1;
+      }
+  return n;
+}
+
 }  // namespace
+
+void Model::size_aero(int n_ph, int n_ph_int, int n_ph_flt, int n_rp, int n_rp_int,
+                      int n_rp_flt, int n_sb, int n_sb_int, int n_sb_flt) {
+  n_phase = n_ph;
+  ph_int.assign(std::max(n_ph_int, 0), 0);
+  ph_flt.assign(std::max(n_ph_flt, 0), 0.0);
+  ph_int_idx.assign(n_ph + 1, 0);
+  ph_flt_idx.assign(n_ph + 1, 0);
+  n_rep = n_rp;
+  rep_int.assign(std::max(n_rp_int, 0) + n_rp, 0);
+  rep_flt.assign(std::max(n_rp_flt, 0), 0.0);
+  rep_int_idx.assign(n_rp + 1, 0);
+  rep_flt_idx.assign(n_rp + 1, 0);
+  rep_env_idx.assign(n_rp + 1, 0);
+  n_sub = n_sb;
+  sub_int.assign(std::max(n_sb_int, 0) + n_sb, 0);
+  sub_flt.assign(std::max(n_sb_flt, 0), 0.0);
+  sub_int_idx.assign(n_sb + 1, 0);
+  sub_flt_idx.assign(n_sb + 1, 0);
+  sub_env_idx.assign(n_sb + 1, 0);
+}
+
+// aero_phase_solver.c:165-194
+void Model::add_phase(int n_int, int n_float, const int* ip, const double* fp) {
+  int k = n_added_phase;
+  if (k >= n_phase) return;
+  ph_int_idx[k + 1] = ph_int_idx[k] + n_int;
+  ph_flt_idx[k + 1] = ph_flt_idx[k] + n_float;
+  std::memcpy(&ph_int[ph_int_idx[k]], ip, sizeof(int) * n_int);
+  if (n_float) std::memcpy(&ph_flt[ph_flt_idx[k]], fp, sizeof(double) * n_float);
+  ++n_added_phase;
+}
+
+// aero_rep_solver.c:443-482
+void Model::add_rep(int type, int n_int, int n_float, int n_env_param, const int* ip,
+                    const double* fp) {
+  int k = n_added_rep;
+  if (k >= n_rep) return;
+  rep_int_idx[k + 1] = rep_int_idx[k] + n_int + 1;
+  rep_flt_idx[k + 1] = rep_flt_idx[k] + n_float;
+  rep_env_idx[k + 1] = rep_env_idx[k] + n_env_param;
+  int* id = &rep_int[rep_int_idx[k]];
+  id[0] = type;
+  std::memcpy(id + 1, ip, sizeof(int) * n_int);
+  if (n_float) std::memcpy(&rep_flt[rep_flt_idx[k]], fp, sizeof(double) * n_float);
+  n_rep_env += n_env_param;
+  ++n_added_rep;
+}
+
+// sub_model_solver.c:413-453
+void Model::add_sub(int type, int n_int, int n_float, int n_env_param, const int* ip,
+                    const double* fp) {
+  int k = n_added_sub;
+  if (k >= n_sub) return;
+  sub_int_idx[k + 1] = sub_int_idx[k] + n_int + 1;
+  sub_flt_idx[k + 1] = sub_flt_idx[k] + n_float;
+  sub_env_idx[k + 1] = sub_env_idx[k] + n_env_param;
+  int* id = &sub_int[sub_int_idx[k]];
+  id[0] = type;
+  std::memcpy(id + 1, ip, sizeof(int) * n_int);
+  if (n_float) std::memcpy(&sub_flt[sub_flt_idx[k]], fp, sizeof(double) * n_float);
+  n_sub_env += n_env_param;
+  ++n_added_sub;
+}
 
 void Model::add_rxn(int type, int n_int, int n_float, int n_env_param,
                     const int* int_param, const double* float_param) {
@@ -143,6 +245,56 @@ std::string Model::initialize() {
       }
     } else if (t == kRxnFirstOrderLoss) {
       rb.add(id[2] - 1, id[2] - 1);
+    } else if (t == kRxnAqueousEquilibrium) {
+      // rxn_aqueous_equilibrium_get_used_jac_elem (:80-126)
+      AqEqView v{id + 1, rflt(r)};
+      int nr = v.nr(), np = v.np();
+      for (int p = 0; p < v.nph(); p++) {
+        for (int ri = p * nr; ri < (p + 1) * nr; ri++) {
+          for (int rd = p * nr; rd < (p + 1) * nr; rd++) rb.add(v.react(rd), v.react(ri));
+          for (int pd = p * np; pd < (p + 1) * np; pd++) rb.add(v.prod(pd), v.react(ri));
+        }
+        for (int pi = p * np; pi < (p + 1) * np; pi++) {
+          for (int rd = p * nr; rd < (p + 1) * nr; rd++) rb.add(v.react(rd), v.prod(pi));
+          for (int pd = p * np; pd < (p + 1) * np; pd++) rb.add(v.prod(pd), v.prod(pi));
+        }
+        for (int rd = p * nr; rd < (p + 1) * nr; rd++) rb.add(v.react(rd), v.water(p));
+        for (int pd = p * np; pd < (p + 1) * np; pd++) rb.add(v.prod(pd), v.water(p));
+        if (v.act(p) < 0) continue;
+        for (int rd = p * nr; rd < (p + 1) * nr; ++rd) rb.add(v.react(rd), v.act(p));
+        for (int pd = p * np; pd < (p + 1) * np; ++pd) rb.add(v.prod(pd), v.act(p));
+      }
+    } else if (t == kRxnSimpolPhaseTransfer) {
+      // rxn_SIMPOL_phase_transfer_get_used_jac_elem (:86-149)
+      SimpolView v{id + 1, rflt(r)};
+      std::vector<char> flags(ns);
+      rb.add(v.gas(), v.gas());
+      for (int i = 0; i < v.n(); i++) {
+        rb.add(v.aero(i), v.gas());
+        rb.add(v.gas(), v.aero(i));
+        rb.add(v.aero(i), v.aero(i));
+        if (v.act(i) > 0) {
+          rb.add(v.gas(), v.act(i));
+          rb.add(v.aero(i), v.act(i));
+        }
+        std::fill(flags.begin(), flags.end(), 0);
+        if (v.rep(i) < 0 || v.rep(i) >= n_rep || rep_int[rep_int_idx[v.rep(i)]] !=
+                                                      kAeroRepModalBinnedMass)
+          return "SIMPOL phase transfer needs a modal/binned-mass aerosol representation";
+        int n_elem = rep_used_jac_elem(*this, v.rep(i), v.phase(i), flags);
+        if (n_elem > v.njac(i))
+          return "SIMPOL phase transfer: more Jacobian elements than expected";
+        int used = 0;
+        for (int k = 0; k < ns; ++k)
+          if (flags[k]) {
+            rb.add(v.gas(), k);
+            rb.add(v.aero(i), k);
+            v.phase_jac(i, 0, used) = k;
+            v.phase_jac(i, 1, used) = k;
+            ++used;
+          }
+        for (; used < v.njac(i); ++used) v.phase_jac(i, 0, used) = v.phase_jac(i, 1, used) = -1;
+      }
     } else if (t != kRxnEmission) {
       char buf[96];
       std::snprintf(buf, sizeof buf, "unsupported reaction type %d", t);
@@ -177,13 +329,89 @@ std::string Model::initialize() {
     } else if (t == kRxnFirstOrderLoss) {
       id[3] = deriv_ids[id[2] - 1];
       id[4] = rjac.find(id[2] - 1, id[2] - 1);
+    } else if (t == kRxnAqueousEquilibrium) {
+      // rxn_aqueous_equilibrium_update_ids (:138-230)
+      AqEqView v{id + 1, rflt(r)};
+      int nr = v.nr(), np = v.np();
+      for (int p = 0, k = 0; p < v.nph(); p++) {
+        for (int q = 0; q < nr; q++) v.deriv_id(k++) = deriv_ids[v.react(p * nr + q)];
+        for (int q = 0; q < np; q++) v.deriv_id(k++) = deriv_ids[v.prod(p * np + q)];
+      }
+      for (int p = 0, j = 0; p < v.nph(); p++) {
+        for (int ri = p * nr; ri < (p + 1) * nr; ri++) {
+          for (int rd = p * nr; rd < (p + 1) * nr; rd++) v.jac_id(j++) = rjac.find(v.react(rd), v.react(ri));
+          for (int pd = p * np; pd < (p + 1) * np; pd++) v.jac_id(j++) = rjac.find(v.prod(pd), v.react(ri));
+        }
+        for (int pi = p * np; pi < (p + 1) * np; pi++) {
+          for (int rd = p * nr; rd < (p + 1) * nr; rd++) v.jac_id(j++) = rjac.find(v.react(rd), v.prod(pi));
+          for (int pd = p * np; pd < (p + 1) * np; pd++) v.jac_id(j++) = rjac.find(v.prod(pd), v.prod(pi));
+        }
+        for (int rd = p * nr; rd < (p + 1) * nr; rd++) v.jac_id(j++) = rjac.find(v.react(rd), v.water(p));
+        for (int pd = p * np; pd < (p + 1) * np; pd++) v.jac_id(j++) = rjac.find(v.prod(pd), v.water(p));
+        for (int rd = p * nr; rd < (p + 1) * nr; ++rd)
+          v.jac_id(j++) = v.act(p) < 0 ? -1 : rjac.find(v.react(rd), v.act(p));
+        for (int pd = p * np; pd < (p + 1) * np; ++pd)
+          v.jac_id(j++) = v.act(p) < 0 ? -1 : rjac.find(v.prod(pd), v.act(p));
+      }
+      for (int p = 0; p < v.nph(); p++) {
+        v.small_conc(p) = 99999.0;
+        for (int q = 0; q < nr; q++)
+          if (v.small_conc(p) > abs_tol[v.react(p * nr + q)] / 100.0)
+            v.small_conc(p) = abs_tol[v.react(p * nr + q)] / 100.0;
+        for (int q = 0; q < np; q++)
+          if (v.small_conc(p) > abs_tol[v.prod(p * np + q)] / 100.0)
+            v.small_conc(p) = abs_tol[v.prod(p * np + q)] / 100.0;
+      }
+      for (int p = 0; p < v.nph(); p++) v.small_water(p) = abs_tol[v.water(p)] / 10.0;
+    } else if (t == kRxnSimpolPhaseTransfer) {
+      // rxn_SIMPOL_phase_transfer_update_ids (:160-207)
+      SimpolView v{id + 1, rflt(r)};
+      v.deriv_id(0) = deriv_ids[v.gas()];
+      for (int i = 0; i < v.n(); i++) v.deriv_id(i + 1) = deriv_ids[v.aero(i)];
+      int j = 0;
+      v.jac_id(j++) = rjac.find(v.gas(), v.gas());
+      for (int i = 0; i < v.n(); i++) {
+        v.jac_id(j++) = rjac.find(v.aero(i), v.gas());
+        v.jac_id(j++) = rjac.find(v.gas(), v.aero(i));
+        v.jac_id(j++) = rjac.find(v.aero(i), v.aero(i));
+        v.gas_act_jac(i) = v.act(i) > 0 ? rjac.find(v.gas(), v.act(i)) : -1;
+        v.aero_act_jac(i) = v.act(i) > 0 ? rjac.find(v.aero(i), v.act(i)) : -1;
+        for (int k = 0; k < v.njac(i); ++k) {
+          if (v.phase_jac(i, 0, k) > 0) v.phase_jac(i, 0, k) = rjac.find(v.gas(), v.phase_jac(i, 0, k));
+          if (v.phase_jac(i, 1, k) > 0)
+            v.phase_jac(i, 1, k) = rjac.find(v.aero(i), v.phase_jac(i, 1, k));
+        }
+      }
     }
   }
 
-  // --- parameter (sub-model) pattern: only the dummy (0,0) element -------
+  // --- parameter (sub-model) pattern: the dummy (0,0) element, then the
+  // sub models' elements (camp_solver.c:1509-1552) --------------------------
   PatternBuilder pb(ns);
   pb.add(0, 0);
+  for (int k = 0; k < n_sub; ++k) {
+    int* id = &sub_int[sub_int_idx[k]];
+    if (id[0] != kSubModelZsrAerosolWater) return "unsupported sub model type";
+    ZsrView z{id + 1, &sub_flt[sub_flt_idx[k]]};
+    for (int p = 0; p < z.nph(); ++p) {
+      pb.add(z.phase(p), z.gas_water());
+      for (int x = 0; x < z.npair(); ++x) {
+        if (z.type(x) != kZsrJacobson) return "ZSR ion pairs other than JACOBSON are not supported";
+        pb.add(z.phase(p), z.phase(p) + z.cation(x));
+        pb.add(z.phase(p), z.phase(p) + z.anion(x));
+      }
+    }
+  }
   pjac = pb.build();
+  for (int k = 0; k < n_sub; ++k) {
+    ZsrView z{&sub_int[sub_int_idx[k]] + 1, &sub_flt[sub_flt_idx[k]]};
+    for (int p = 0; p < z.nph(); ++p)
+      for (int x = 0; x < z.npair(); ++x) {
+        z.gw_jac(p, x) = pjac.find(z.phase(p), z.gas_water());
+        z.cation_jac(p, x) = pjac.find(z.phase(p), z.phase(p) + z.cation(x));
+        z.anion_jac(p, x) = pjac.find(z.phase(p), z.phase(p) + z.anion(x));
+      }
+  }
 
   // --- solver Jacobian pattern and jac_map --------------------------------
   PatternBuilder sb(ns);
@@ -218,8 +446,9 @@ std::string Model::initialize() {
           map_param.push_back(pjac.find(i_ind, j_ind));
         }
     }
-  for (int p : map_param)
-    if (p != 0) return "sub-model Jacobian parameters are not supported on this path";
+  bool chained = false;
+  for (int p : map_param) chained |= p != 0;
+  if (chained && !multiphase()) return "sub-model Jacobian parameters without sub models";
 
   // --- compile the kernel program ----------------------------------------
   Program& P = prog;
@@ -235,8 +464,23 @@ std::string Model::initialize() {
   P.colptr[nv] = P.nnz;
   P.rowval.resize(P.nnz);
   for (int e = 0; e < P.nnz; ++e) P.rowval[e] = deriv_ids[sjac.rowval[e]];
+  // Jacobian gather space: solver elements directly when the map is the
+  // identity (gas phase), reaction-pattern elements otherwise (the solver
+  // Jacobian is then J_rxn * J_params over the map, camp_solver.c:1100-1106)
+  const bool mp = multiphase();
+  const int jn = mp ? rjac.nnz() : P.nnz;
   std::vector<int> rxn_to_solver(rjac.nnz(), -1);
-  for (size_t m = 0; m < map_rxn.size(); ++m) rxn_to_solver[map_rxn[m]] = map_solver[m];
+  if (mp) {
+    for (int e = 0; e < rjac.nnz(); ++e) rxn_to_solver[e] = e;
+  } else {
+    for (size_t m = 0; m < map_rxn.size(); ++m) rxn_to_solver[map_rxn[m]] = map_solver[m];
+  }
+  // multi-phase instances: slots of f contributions per solver variable
+  std::vector<std::vector<int>> f_slots(nv);
+  if (mp) {
+    std::string err = compile_multiphase(f_slots);
+    if (!err.empty()) return err;
+  }
 
   // rate terms: one per reaction
   P.n_terms = n_rxn;
@@ -244,9 +488,9 @@ std::string Model::initialize() {
   P.t_react.assign(3 * n_rxn, -1);
   P.t_nreact.resize(n_rxn);
   P.t_flags.resize(n_rxn);
-  std::vector<std::vector<int>> f_term(nv), f_guard(nv), j_term(P.nnz), j_skip(P.nnz),
-      j_guard(P.nnz), j_rind(P.nnz);
-  std::vector<std::vector<double>> f_coef(nv), j_coef(P.nnz);
+  std::vector<std::vector<int>> f_term(nv), f_guard(nv), j_term(jn), j_skip(jn), j_guard(jn),
+      j_rind(jn);
+  std::vector<std::vector<double>> f_coef(nv), j_coef(jn);
   for (int r = 0; r < n_rxn; ++r) {
     int* id = rint(r);
     int t = id[0];
@@ -320,6 +564,13 @@ std::string Model::initialize() {
       }
     }
   }
+  // slot records (term -1 - slot): contributions of the multi-phase types
+  for (int i = 0; i < nv; ++i)
+    for (int sl : f_slots[i]) {
+      f_term[i].push_back(-1 - sl);
+      f_coef[i].push_back(1.0);
+      f_guard[i].push_back(-1);
+    }
   P.f_ptr.assign(1, 0);
   for (int i = 0; i < nv; ++i) {
     P.f_term.insert(P.f_term.end(), f_term[i].begin(), f_term[i].end());
@@ -329,7 +580,7 @@ std::string Model::initialize() {
     P.max_f_len = std::max(P.max_f_len, (int)f_term[i].size());
   }
   P.j_ptr.assign(1, 0);
-  for (int e = 0; e < P.nnz; ++e) {
+  for (int e = 0; e < jn; ++e) {
     P.j_term.insert(P.j_term.end(), j_term[e].begin(), j_term[e].end());
     P.j_skip.insert(P.j_skip.end(), j_skip[e].begin(), j_skip[e].end());
     P.j_coef.insert(P.j_coef.end(), j_coef[e].begin(), j_coef[e].end());
@@ -340,6 +591,13 @@ std::string Model::initialize() {
   }
   for (size_t q = 0; q < P.f_term.size(); ++q) {
     int t = P.f_term[q];
+    if (t < 0) {  // slot record
+      P.fe_a.push_back(-1 - t);
+      for (int j = 0; j < 3; ++j) P.fe_a.push_back(-1);
+      P.fe_b.push_back(0xFFFFFF | (kTermSlot << 24));
+      P.fe_c.push_back(1.0);
+      continue;
+    }
     P.fe_a.push_back(P.t_kidx[t]);
     for (int j = 0; j < 3; ++j) P.fe_a.push_back(j < P.t_nreact[t] ? P.t_react[3 * t + j] : -1);
     P.fe_b.push_back((P.f_guard[q] & 0xFFFFFF) | (P.t_flags[t] << 24));
@@ -462,6 +720,210 @@ std::string Model::initialize() {
   if (P.u_pos.empty()) { P.u_pos.push_back(0); P.u_row.push_back(0); }
   if (P.up_dst.empty()) { P.up_dst.push_back(0); P.up_l.push_back(0); P.up_u.push_back(0); }
   initialized = true;
+  return std::string();
+}
+
+// ---------------------------------------------------------------------------
+// Multi-phase kernel tables: aerosol phase instances, ZSR items, SIMPOL and
+// aqueous-equilibrium instances with their f-contribution slots, the special
+// Jacobian records of those instances per reaction-pattern element, and the
+// J_rxn * J_params map per solver element (camp_solver.c:1100-1106).
+// ---------------------------------------------------------------------------
+std::string Model::compile_multiphase(std::vector<std::vector<int>>& f_slots) {
+  Program& P = prog;
+  P.mp = true;
+  P.rnnz = rjac.nnz();
+  P.pnnz = pjac.nnz();
+  P.pm_ptr.assign(1, 0);
+  for (int p = 0; p < n_phase; ++p) {
+    PhaseView v{&ph_int[ph_int_idx[p]], &ph_flt[ph_flt_idx[p]]};
+    for (int i = 0; i < v.n(); ++i) {
+      P.pm_counted.push_back(v.counted(i) ? 1 : 0);
+      P.pm_mw.push_back(v.mw(i));
+      P.pm_dens.push_back(v.density(i));
+    }
+    P.pm_ptr.push_back((int)P.pm_counted.size());
+  }
+  // phase instances in the representation's phase-index order
+  std::vector<int> rep_pi_base(n_rep, 0);
+  std::vector<double> pi_radius;
+  for (int r = 0; r < n_rep; ++r) {
+    if (rep_int[rep_int_idx[r]] != kAeroRepModalBinnedMass)
+      return "unsupported aerosol representation type";
+    rep_pi_base[r] = (int)P.pi_start.size();
+    BinnedRepView v = rep_view(*this, r);
+    for (int s = 0; s < v.nsec(); ++s) {
+      if (v.type(s) != kBinned) return "MODAL sections are not supported by the multi-phase kernels";
+      if (v.nph(s) != 1) return "bins holding more than one phase are not supported by the multi-phase kernels";
+      for (int b = 0; b < v.nbin(s); ++b) {
+        P.pi_start.push_back(v.phase_state(s, 0, b));
+        P.pi_model.push_back(v.phase_model(s, 0, b));
+        P.pi_dp.push_back(v.bin_dp(s, b));
+        pi_radius.push_back(v.radius(s, b));
+      }
+    }
+  }
+  // ZSR items
+  P.zi_pair_ptr.assign(1, 0);
+  for (int k = 0; k < n_sub; ++k) {
+    ZsrView z{&sub_int[sub_int_idx[k]] + 1, &sub_flt[sub_flt_idx[k]]};
+    for (int p = 0; p < z.nph(); ++p) {
+      P.zi_water.push_back(z.phase(p));
+      P.zi_gw.push_back(z.gas_water());
+      P.zi_sub.push_back(k);
+      for (int x = 0; x < z.npair(); ++x) {
+        if (z.ny(x) > 8) return "ZSR molality polynomials of more than 8 terms";
+        P.zp_cat.push_back(z.phase(p) + z.cation(x));
+        P.zp_an.push_back(z.phase(p) + z.anion(x));
+        P.zp_ncat.push_back(z.n_cation(x));
+        P.zp_nan.push_back(z.n_anion(x));
+        P.zp_ny.push_back(z.ny(x));
+        P.zp_jgw.push_back(z.gw_jac(p, x));
+        P.zp_jcat.push_back(z.cation_jac(p, x));
+        P.zp_jan.push_back(z.anion_jac(p, x));
+        P.zp_catmw.push_back(z.cation_mw(x));
+        P.zp_anmw.push_back(z.anion_mw(x));
+        P.zp_lowrh.push_back(z.low_rh(x));
+        for (int q = 0; q < 8; ++q) P.zp_y.push_back(q < z.ny(x) ? z.y(x, q) : 0.0);
+      }
+      P.zi_pair_ptr.push_back((int)P.zp_cat.size());
+    }
+  }
+  P.n_sub_env = n_sub;
+
+  std::vector<std::vector<int>> ja(rjac.nnz());
+  std::vector<std::vector<double>> jc(rjac.nnz());
+  auto addj = [&](int e, int kind, int inst, int p1, int p2, double c) {
+    if (e < 0) return;
+    ja[e].insert(ja[e].end(), {kind, inst, p1, p2});
+    jc[e].push_back(c);
+  };
+  int slot = 0;
+  for (int r = 0; r < n_rxn; ++r) {
+    int* id = rint(r);
+    int t = id[0];
+    if (t == kRxnSimpolPhaseTransfer) {
+      SimpolView v{id + 1, rflt(r)};
+      for (int i = 0; i < v.n(); ++i) {
+        if (v.act(i) > -1) return "SIMPOL activity coefficients are not supported by the multi-phase kernels";
+        if (v.rep(i) < 0 || v.rep(i) >= n_rep) return "SIMPOL: bad aerosol representation id";
+        int inst = (int)P.sp_gas.size();
+        int pi = rep_pi_base[v.rep(i)] + v.phase(i);
+        P.sp_gas.push_back(v.gas());
+        P.sp_aero.push_back(v.aero(i));
+        P.sp_dgas.push_back(v.deriv_id(0));
+        P.sp_daero.push_back(v.deriv_id(1 + i));
+        P.sp_pi.push_back(pi);
+        P.sp_env.push_back(rxn_env_idx[r]);
+        P.sp_flt.push_back(rxn_float_idx[r]);
+        P.sp_radius.push_back(pi_radius[pi]);
+        P.sp_slot.push_back(slot);
+        // rxn_SIMPOL_phase_transfer.c:355-399 (total particle mass): gas
+        // +N evap, -N cond; aerosol -N evap/kp, +N cond/kp
+        if (v.deriv_id(0) >= 0) {
+          f_slots[v.deriv_id(0)].push_back(slot);
+          f_slots[v.deriv_id(0)].push_back(slot + 1);
+        }
+        if (v.deriv_id(1 + i) >= 0) {
+          f_slots[v.deriv_id(1 + i)].push_back(slot + 2);
+          f_slots[v.deriv_id(1 + i)].push_back(slot + 3);
+        }
+        slot += 4;
+        addj(v.jac_id(0), kJsSimpolGG, inst, 0, 0, 0.0);
+        addj(v.jac_id(1 + 3 * i), kJsSimpolAG, inst, 0, 0, 0.0);
+        addj(v.jac_id(1 + 3 * i + 1), kJsSimpolGA, inst, 0, 0, 0.0);
+        addj(v.jac_id(1 + 3 * i + 2), kJsSimpolAA, inst, 0, 0, 0.0);
+        // phase element k = k-th counted species of the bin's phase
+        int pm = P.pi_model[pi], k = 0;
+        for (int q = P.pm_ptr[pm]; q < P.pm_ptr[pm + 1]; ++q) {
+          if (!P.pm_counted[q]) continue;
+          if (k < v.njac(i)) {
+            if (v.phase_jac(i, 0, k) > 0) addj(v.phase_jac(i, 0, k), kJsSimpolGP, inst, q, 0, 0.0);
+            if (v.phase_jac(i, 1, k) > 0) addj(v.phase_jac(i, 1, k), kJsSimpolAP, inst, q, 0, 0.0);
+          }
+          ++k;
+        }
+      }
+    } else if (t == kRxnAqueousEquilibrium) {
+      AqEqView v{id + 1, rflt(r)};
+      int nr = v.nr(), np = v.np();
+      if (nr > 3 || np > 3) return "aqueous equilibria with more than 3 reactants or products";
+      for (int p = 0; p < v.nph(); ++p) {
+        if (v.act(p) >= 0) return "aqueous-equilibrium activity coefficients are not supported by the multi-phase kernels";
+        int inst = (int)P.aq_nr.size();
+        P.aq_nr.push_back(nr);
+        P.aq_np.push_back(np);
+        for (int q = 0; q < 3; ++q) P.aq_sp.push_back(q < nr ? v.react(p * nr + q) : -1);
+        for (int q = 0; q < 3; ++q) P.aq_sp.push_back(q < np ? v.prod(p * np + q) : -1);
+        P.aq_water.push_back(v.water(p));
+        P.aq_env.push_back(rxn_env_idx[r]);
+        P.aq_flt.push_back(rxn_float_idx[r]);
+        P.aq_small_water.push_back(v.small_water(p));
+        P.aq_slot.push_back(slot);
+        int d0 = p * (nr + np);
+        for (int q = 0; q < 6; ++q) P.aq_deriv.push_back(q < nr + np ? v.deriv_id(d0 + q) : -1);
+        // rxn_aqueous_equilibrium.c:362-386: -rf/m, +rr/m per reactant;
+        // +rf/m, -rr/m per product
+        for (int q = 0; q < nr + np; ++q)
+          if (v.deriv_id(d0 + q) >= 0) {
+            f_slots[v.deriv_id(d0 + q)].push_back(slot + 2 * q);
+            f_slots[v.deriv_id(d0 + q)].push_back(slot + 2 * q + 1);
+          }
+        slot += 2 * (nr + np);
+        // Jacobian ids in the reference's order (rxn_aqueous_equilibrium.c:418-545)
+        int j = p * (nr + np) * (nr + np + 2);
+        for (int ri = 0; ri < nr; ++ri) {
+          for (int rd = 0; rd < nr; ++rd) addj(v.jac_id(j++), kJsAqRR, inst, ri, rd, v.mftm(rd));
+          for (int pd = 0; pd < np; ++pd)
+            addj(v.jac_id(j++), kJsAqRP, inst, ri, nr + pd, v.mftm(nr + pd));
+        }
+        for (int pj = 0; pj < np; ++pj) {
+          for (int rd = 0; rd < nr; ++rd) addj(v.jac_id(j++), kJsAqPR, inst, 3 + pj, rd, v.mftm(rd));
+          for (int pd = 0; pd < np; ++pd)
+            addj(v.jac_id(j++), kJsAqPP, inst, 3 + pj, nr + pd, v.mftm(nr + pd));
+        }
+        for (int rd = 0; rd < nr; ++rd) addj(v.jac_id(j++), kJsAqRW, inst, 0, rd, v.mftm(rd));
+        for (int pd = 0; pd < np; ++pd)
+          addj(v.jac_id(j++), kJsAqPW, inst, 0, nr + pd, v.mftm(nr + pd));
+      }
+    }
+  }
+  P.n_slots = slot;
+  P.n_pi = (int)P.pi_start.size();
+  P.n_zi = (int)P.zi_water.size();
+  P.n_sp = (int)P.sp_gas.size();
+  P.n_aq = (int)P.aq_nr.size();
+  P.js_ptr.assign(1, 0);
+  for (int e = 0; e < rjac.nnz(); ++e) {
+    P.js_a.insert(P.js_a.end(), ja[e].begin(), ja[e].end());
+    P.js_c.insert(P.js_c.end(), jc[e].begin(), jc[e].end());
+    P.js_ptr.push_back((int)P.js_c.size());
+  }
+  if (P.js_c.empty()) {
+    P.js_a.assign(4, 0);
+    P.js_c.push_back(0.0);
+  }
+  // solver element <- (reaction element, parameter element) in map order
+  std::vector<std::vector<int>> mr(P.nnz), mpar(P.nnz);
+  for (size_t m = 0; m < map_solver.size(); ++m) {
+    mr[map_solver[m]].push_back(map_rxn[m]);
+    mpar[map_solver[m]].push_back(map_param[m]);
+  }
+  P.jm_ptr.assign(1, 0);
+  for (int e = 0; e < P.nnz; ++e) {
+    P.jm_rxn.insert(P.jm_rxn.end(), mr[e].begin(), mr[e].end());
+    P.jm_param.insert(P.jm_param.end(), mpar[e].begin(), mpar[e].end());
+    P.jm_ptr.push_back((int)P.jm_rxn.size());
+  }
+  for (auto* v : {&P.pi_start, &P.pi_model, &P.zi_water, &P.zi_gw, &P.zi_sub, &P.zp_cat, &P.zp_an,
+                  &P.zp_ncat, &P.zp_nan, &P.zp_ny, &P.zp_jgw, &P.zp_jcat, &P.zp_jan, &P.sp_gas,
+                  &P.sp_aero, &P.sp_dgas, &P.sp_daero, &P.sp_pi, &P.sp_env, &P.sp_flt, &P.sp_slot,
+                  &P.aq_nr, &P.aq_np, &P.aq_sp, &P.aq_water, &P.aq_env, &P.aq_flt, &P.aq_deriv,
+                  &P.aq_slot, &P.jm_rxn, &P.jm_param, &P.pm_counted})
+    if (v->empty()) v->push_back(0);
+  for (auto* v : {&P.pi_dp, &P.pm_mw, &P.pm_dens, &P.zp_catmw, &P.zp_anmw, &P.zp_lowrh, &P.zp_y,
+                  &P.sp_radius, &P.aq_small_water})
+    if (v->empty()) v->push_back(0.0);
   return std::string();
 }
 

@@ -34,6 +34,7 @@ constexpr int kRxnEmission = 13;
 // Flags of a rate term.
 constexpr int kTermMassAction = 1;   // skip contributions when the rate is 0
 constexpr int kTermRoundRate = 2;    // rate product rounded to double (loss)
+constexpr int kTermSlot = 4;         // contribution read from a per-cell slot
 
 // Sparse matrix with per-column sorted rows (CSC), as built by Jacobian.c.
 struct Csc {
@@ -80,7 +81,37 @@ struct Program {
   std::vector<int> lu_rowptr, lu_col;
   std::vector<uint8_t> row_off;
   long n_updates() const { return (long)up_dst.size(); }
+
+  // ---- multi-phase tables (empty for gas-phase models) -------------------
+  bool mp = false;
+  int rnnz = 0;                          // reaction-pattern elements (J gather space)
+  int n_pi = 0, n_zi = 0, n_sp = 0, n_aq = 0;  // instance counts of the tables below
+  // aerosol phase instances (one per bin and phase): first state id, phase
+  // model, bin diameter; phase models: counted flag, MW, density per species
+  std::vector<int> pi_start, pi_model;
+  std::vector<double> pi_dp;
+  std::vector<int> pm_ptr, pm_counted;
+  std::vector<double> pm_mw, pm_dens;
+  // ZSR items (one per sub model and phase) and their JACOBSON ion pairs
+  std::vector<int> zi_water, zi_gw, zi_sub, zi_pair_ptr;
+  std::vector<int> zp_cat, zp_an, zp_ncat, zp_nan, zp_ny, zp_jgw, zp_jcat, zp_jan;
+  std::vector<double> zp_catmw, zp_anmw, zp_lowrh, zp_y;   // zp_y: 8 per pair
+  int n_sub_env = 0;
+  // SIMPOL instances (one per reaction and phase)
+  std::vector<int> sp_gas, sp_aero, sp_dgas, sp_daero, sp_pi, sp_env, sp_flt, sp_slot;
+  std::vector<double> sp_radius;
+  // aqueous-equilibrium instances: species (3 + 3 slots), water, ids, slots
+  std::vector<int> aq_nr, aq_np, aq_sp, aq_water, aq_env, aq_flt, aq_deriv, aq_slot;
+  std::vector<double> aq_small_water;
+  int n_slots = 0;
+  // special Jacobian records per reaction element: {kind, instance, p1, p2}, c
+  std::vector<int> js_ptr, js_a;
+  std::vector<double> js_c;
+  // solver element e = sum over [jm_ptr[e], jm_ptr[e+1]) of J_rxn[jm_rxn] * J_params[jm_param]
+  std::vector<int> jm_ptr, jm_rxn, jm_param;
+  int pnnz = 0;
 };
+
 
 struct Model {
   // solver_new arguments (camp_solver.c:83-362)
@@ -90,6 +121,13 @@ struct Model {
   std::vector<int> rxn_int, rxn_int_idx, rxn_float_idx, rxn_env_idx;
   std::vector<double> rxn_float;
   std::vector<double> rxn_env;  // [nc][n_env], calloc-zeroed as in CAMP
+  // aerosol phases, representations and sub models (camp_solver.c:200-349);
+  // reps and sub models carry a type slot like reactions
+  int n_phase = 0, n_added_phase = 0, n_rep = 0, n_added_rep = 0, n_sub = 0, n_added_sub = 0;
+  int n_rep_env = 0, n_sub_env = 0;
+  std::vector<int> ph_int, ph_int_idx, ph_flt_idx, rep_int, rep_int_idx, rep_flt_idx, rep_env_idx,
+      sub_int, sub_int_idx, sub_flt_idx, sub_env_idx;
+  std::vector<double> ph_flt, rep_flt, sub_flt;
   // solver_initialize arguments
   std::vector<double> abs_tol;  // [ns]
   double rel_tol = 1.0e-8;
@@ -102,6 +140,17 @@ struct Model {
 
   void add_rxn(int type, int n_int, int n_float, int n_env_param,
                const int* int_param, const double* float_param);
+  void size_aero(int n_phase, int n_phase_int, int n_phase_float, int n_rep, int n_rep_int,
+                 int n_rep_float, int n_sub, int n_sub_int, int n_sub_float);
+  void add_phase(int n_int, int n_float, const int* int_param, const double* float_param);
+  void add_rep(int type, int n_int, int n_float, int n_env_param, const int* int_param,
+               const double* float_param);
+  void add_sub(int type, int n_int, int n_float, int n_env_param, const int* int_param,
+               const double* float_param);
+  bool multiphase() const { return n_phase + n_rep + n_sub > 0; }
+  // multi-phase tables of the kernel program; f_slots[i] receives the
+  // contribution slots that feed solver variable i
+  std::string compile_multiphase(std::vector<std::vector<int>>& f_slots);
   // Builds pattern, ids, the kernel program and the LU ordering.
   // Returns an empty string on success, else an error message.
   std::string initialize();

@@ -94,6 +94,8 @@ struct Arena {
 
 struct Lane {
   hipStream_t st = nullptr;
+  std::vector<double*> scratch;       // per mechanism: multi-phase wave scratch
+  std::vector<size_t> scratch_waves;
   hipEvent_t ev = nullptr;
   int* d_counter = nullptr;
   int* d_list = nullptr;
@@ -238,6 +240,19 @@ bool Server::launch() {
     int grid = 0, block = 0;
     size_t lds = 0;
     A.S->cell_launch_shape(total, &grid, &block, &lds);
+    if (A.S->P.mp) {
+      size_t waves = (size_t)grid * (block / kWave);
+      if (L.scratch.size() < mechs.size()) {
+        L.scratch.resize(mechs.size(), nullptr);
+        L.scratch_waves.resize(mechs.size(), 0);
+      }
+      if (L.scratch_waves[mech] < waves) {
+        if (L.scratch[mech]) HIP_OK(hipFree(L.scratch[mech]));
+        HIP_OK(hipMalloc(&L.scratch[mech], waves * cell_bdf_wave_scratch(A.S->P) * sizeof(double)));
+        L.scratch_waves[mech] = waves;
+      }
+      a.wave_scratch = L.scratch[mech];
+    }
     launch_cell_bdf(a, grid, block, lds, L.st);
     HIP_OK(hipGetLastError());
     for (Request* r : L.batch) {
@@ -554,6 +569,8 @@ extern "C" void camp_server_free(void* server) {
     (void)hipFree(L.d_counter);
     if (L.d_list) (void)hipFree(L.d_list);
     if (L.h_list) (void)hipHostFree(L.h_list);
+    for (double* p : L.scratch)
+      if (p) (void)hipFree(p);
   }
   (void)hipStreamSynchronize(s->up_st);
   (void)hipStreamDestroy(s->up_st);

@@ -100,6 +100,71 @@ void Solver::setup_device() {
   std::vector<double> atol_dep(G.nv);
   for (int d = 0; d < G.nv; ++d) atol_dep[d] = m.abs_tol[G.dep_state[d]];
   P.abs_tol_dep = dev_upload(atol_dep, owned);
+  P.mp = G.mp ? 1 : 0;
+  if (G.mp) {
+    P.rnnz = G.rnnz;
+    P.pnnz = G.pnnz;
+    P.n_pi = G.n_pi;
+    P.n_zi = G.n_zi;
+    P.n_sp = G.n_sp;
+    P.n_aq = G.n_aq;
+    P.n_slots = G.n_slots;
+    P.n_sub_env = G.n_sub_env;
+    // aero_phase_solver.c:19-24 minimum mass, moles and volume, as
+    // double-double renderings of the long double constants
+    long double mm = 1.0e-25L, mo = 1.0e-25L / 0.1L, mv = 1.0e-25L / 1800.0L;
+    P.min_mass_hi = (double)mm;
+    P.min_mass_lo = (double)(mm - (long double)P.min_mass_hi);
+    P.min_moles_hi = (double)mo;
+    P.min_moles_lo = (double)(mo - (long double)P.min_moles_hi);
+    P.min_volume = (double)mv;
+    P.pi_start = dev_upload(G.pi_start, owned);
+    P.pi_model = dev_upload(G.pi_model, owned);
+    P.pi_dp = dev_upload(G.pi_dp, owned);
+    P.pm_ptr = dev_upload(G.pm_ptr, owned);
+    P.pm_counted = dev_upload(G.pm_counted, owned);
+    P.pm_mw = dev_upload(G.pm_mw, owned);
+    P.pm_dens = dev_upload(G.pm_dens, owned);
+    P.zi_water = dev_upload(G.zi_water, owned);
+    P.zi_gw = dev_upload(G.zi_gw, owned);
+    P.zi_sub = dev_upload(G.zi_sub, owned);
+    P.zi_pair_ptr = dev_upload(G.zi_pair_ptr, owned);
+    P.zp_cat = dev_upload(G.zp_cat, owned);
+    P.zp_an = dev_upload(G.zp_an, owned);
+    P.zp_ncat = dev_upload(G.zp_ncat, owned);
+    P.zp_nan = dev_upload(G.zp_nan, owned);
+    P.zp_ny = dev_upload(G.zp_ny, owned);
+    P.zp_jgw = dev_upload(G.zp_jgw, owned);
+    P.zp_jcat = dev_upload(G.zp_jcat, owned);
+    P.zp_jan = dev_upload(G.zp_jan, owned);
+    P.zp_catmw = dev_upload(G.zp_catmw, owned);
+    P.zp_anmw = dev_upload(G.zp_anmw, owned);
+    P.zp_lowrh = dev_upload(G.zp_lowrh, owned);
+    P.zp_y = dev_upload(G.zp_y, owned);
+    P.sp_gas = dev_upload(G.sp_gas, owned);
+    P.sp_aero = dev_upload(G.sp_aero, owned);
+    P.sp_dgas = dev_upload(G.sp_dgas, owned);
+    P.sp_daero = dev_upload(G.sp_daero, owned);
+    P.sp_pi = dev_upload(G.sp_pi, owned);
+    P.sp_env = dev_upload(G.sp_env, owned);
+    P.sp_flt = dev_upload(G.sp_flt, owned);
+    P.sp_slot = dev_upload(G.sp_slot, owned);
+    P.sp_radius = dev_upload(G.sp_radius, owned);
+    P.aq_nr = dev_upload(G.aq_nr, owned);
+    P.aq_np = dev_upload(G.aq_np, owned);
+    P.aq_sp = dev_upload(G.aq_sp, owned);
+    P.aq_water = dev_upload(G.aq_water, owned);
+    P.aq_env = dev_upload(G.aq_env, owned);
+    P.aq_flt = dev_upload(G.aq_flt, owned);
+    P.aq_slot = dev_upload(G.aq_slot, owned);
+    P.aq_small_water = dev_upload(G.aq_small_water, owned);
+    P.js_ptr = dev_upload(G.js_ptr, owned);
+    P.js_a = reinterpret_cast<const int4*>(dev_upload(G.js_a, owned));
+    P.js_c = dev_upload(G.js_c, owned);
+    P.jm_ptr = dev_upload(G.jm_ptr, owned);
+    P.jm_rxn = dev_upload(G.jm_rxn, owned);
+    P.jm_param = dev_upload(G.jm_param, owned);
+  }
 
   size_t nc = m.nc;
   d_state = dev_alloc<double>(nc * G.ns, owned);
@@ -124,6 +189,8 @@ void Solver::upload_renv() {
 int Solver::run(double* st, double* env, double t0, double t1) {
   upload_renv();
   if (m.n_rxn == 0) return CAMP_SOLVER_SUCCESS;  // no_solve, camp_solver.c:240
+  if (mode == CAMP_MODE_BLOCK && P.mp)
+    die("multi-phase mechanisms run in the per-cell mode (CAMP_MODE_PER_CELL)");
   return mode == CAMP_MODE_BLOCK ? run_block(st, env, t0, t1) : run_cells(st, env, t0, t1);
 }
 
@@ -141,7 +208,7 @@ CellBdfArgs Solver::cell_args() const {
 void Solver::cell_launch_shape(long nq, int* grid, int* block, size_t* lds) const {
   int cpb = cells_per_block;
   *block = kWave * cpb;
-  *lds = (size_t)cpb * cell_bdf_lds_doubles(P.ns, P.nv, P.n_env, P.lu_nnz) * sizeof(double);
+  *lds = (size_t)cpb * cell_bdf_lds_doubles(P) * sizeof(double);
   int dev = 0, ncu = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -170,6 +237,7 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   int grid = 0, block = 0;
   size_t lds = 0;
   cell_launch_shape(m.nc, &grid, &block, &lds);
+  a.wave_scratch = wave_scratch((size_t)grid * (block / kWave));
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
@@ -306,9 +374,10 @@ struct BlockCv {
   int deriv(const double* yv, double* out, int use_est) {
     double ts = next_h > 0.0 ? next_h : init_h;
     HIP_OK(hipMemsetAsync(S.d_fail, 0, sizeof(int), s));
-    size_t lds = 4 * (size_t)(S.P.ns + S.P.n_env) * sizeof(double);
+    size_t lds = 4 * fj_lds_doubles(S.P) * sizeof(double);
     hipLaunchKernelGGL(k_deriv, dim3(std::min(nblk(nc, 4), 4096)), dim3(256), lds, s, S.P, nc,
-                       yv, st, S.d_renv, S.d_Js, S.d_Jy, S.d_Jf, out, ts, use_est, S.d_fail);
+                       yv, st, (const double*)nullptr, S.d_renv, S.d_Js, S.d_Jy, S.d_Jf, out, ts,
+                       use_est, S.d_fail);
     return check_fail();
   }
   int f(const double* yv, double* out) {
@@ -318,9 +387,9 @@ struct BlockCv {
   int jac(const double* yv, double* fy) {
     if (deriv(yv, fy, 0)) return 1;
     double ts = next_h > 0.0 ? next_h : init_h;
-    size_t lds = 4 * (size_t)(S.P.ns + S.P.n_env) * sizeof(double);
+    size_t lds = 4 * fj_lds_doubles(S.P) * sizeof(double);
     hipLaunchKernelGGL(k_jac, dim3(std::min(nblk(nc, 4), 4096)), dim3(256), lds, s, S.P, nc, yv,
-                       st, S.d_renv, S.d_Js, ts);
+                       st, (const double*)nullptr, S.d_renv, S.d_Js, ts, (double*)nullptr);
     copy(S.d_Jy, yv);
     copy(S.d_Jf, fy);
     return 0;
@@ -778,11 +847,16 @@ using camp::SD;
 // ---------------------------------------------------------------------------
 extern "C" void* solver_new(int n_state_var, int n_cells, int* var_type, int n_rxn,
                             int n_rxn_int_param, int n_rxn_float_param, int n_rxn_env_param,
-                            int n_aero_phase, int, int, int n_aero_rep, int, int, int,
-                            int n_sub_model, int, int, int) {
-  if (n_aero_phase || n_aero_rep || n_sub_model)
-    camp::die("aerosol phases, aerosol representations and sub models are not on this path");
-  if (n_state_var <= 0 || n_cells <= 0 || n_rxn < 0) camp::die("invalid solver_new dimensions");
+                            int n_aero_phase, int n_aero_phase_int_param,
+                            int n_aero_phase_float_param, int n_aero_rep, int n_aero_rep_int_param,
+                            int n_aero_rep_float_param, int n_aero_rep_env_param,
+                            int n_sub_model, int n_sub_model_int_param,
+                            int n_sub_model_float_param, int n_sub_model_env_param) {
+  if (n_state_var <= 0 || n_cells <= 0 || n_rxn < 0 || n_aero_phase < 0 || n_aero_rep < 0 ||
+      n_sub_model < 0)
+    camp::die("invalid solver_new dimensions");
+  (void)n_aero_rep_env_param;
+  (void)n_sub_model_env_param;
   Solver* s = new Solver();
   camp::Model& m = s->m;
   m.ns = n_state_var;
@@ -797,6 +871,9 @@ extern "C" void* solver_new(int n_state_var, int n_cells, int* var_type, int n_r
   m.rxn_int_idx.assign(n_rxn + 1, 0);
   m.rxn_float_idx.assign(n_rxn + 1, 0);
   m.rxn_env_idx.assign(n_rxn + 1, 0);
+  m.size_aero(n_aero_phase, n_aero_phase_int_param, n_aero_phase_float_param, n_aero_rep,
+              n_aero_rep_int_param, n_aero_rep_float_param, n_sub_model, n_sub_model_int_param,
+              n_sub_model_float_param);
   return s;
 }
 
@@ -914,14 +991,22 @@ extern "C" void rxn_first_order_loss_set_rate_update_data(void* u, int rxn_id, d
   set_rate_payload(u, rxn_id, base_rate);
 }
 
-extern "C" void aero_phase_add_condensed_data(int, int, int*, double*, void*) {
-  camp::die("aerosol phases are not on this path");
+// aero_phase_solver.c:165-194, aero_rep_solver.c:443-482, sub_model_solver.c:413-453
+extern "C" void aero_phase_add_condensed_data(int n_int_param, int n_float_param, int* int_param,
+                                              double* float_param, void* solver_data) {
+  SD(solver_data)->m.add_phase(n_int_param, n_float_param, int_param, float_param);
 }
-extern "C" void aero_rep_add_condensed_data(int, int, int, int, int*, double*, void*) {
-  camp::die("aerosol representations are not on this path");
+extern "C" void aero_rep_add_condensed_data(int aero_rep_type, int n_int_param, int n_float_param,
+                                            int n_env_param, int* int_param, double* float_param,
+                                            void* solver_data) {
+  SD(solver_data)->m.add_rep(aero_rep_type, n_int_param, n_float_param, n_env_param, int_param,
+                             float_param);
 }
-extern "C" void sub_model_add_condensed_data(int, int, int, int, int*, double*, void*) {
-  camp::die("sub models are not on this path");
+extern "C" void sub_model_add_condensed_data(int sub_model_type, int n_int_param,
+                                             int n_float_param, int n_env_param, int* int_param,
+                                             double* float_param, void* solver_data) {
+  SD(solver_data)->m.add_sub(sub_model_type, n_int_param, n_float_param, n_env_param, int_param,
+                             float_param);
 }
 
 // ---------------------------------------------------------------------------
@@ -1017,10 +1102,11 @@ extern "C" int camp_eval_deriv(void* solver_data, const double* state, const dou
   double* d_f = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
   eval_prepare(s, state, env, d_y);
   HIP_OK(hipMemsetAsync(s->d_fail, 0, sizeof(int), s->stream));
-  size_t lds = 4 * (size_t)(P.ns + P.n_env) * sizeof(double);
-  hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, 4), 4096)), dim3(256), lds,
-                     s->stream, P, nc, d_y, s->d_state, s->d_renv, s->d_Js, s->d_Jy, s->d_Jf, d_f,
-                     time_step, use_deriv_est, s->d_fail);
+  int wpb = camp::fj_waves_per_block(P);
+  size_t lds = wpb * camp::fj_lds_doubles(P) * sizeof(double);
+  hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, wpb), 4096)), dim3(64 * wpb), lds,
+                     s->stream, P, nc, d_y, s->d_state, s->d_env, s->d_renv, s->d_Js, s->d_Jy,
+                     s->d_Jf, d_f, time_step, use_deriv_est, s->d_fail);
   int fail = 0;
   HIP_OK(hipMemcpyAsync(&fail, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipMemcpyAsync(deriv, d_f, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToHost,
@@ -1040,12 +1126,15 @@ extern "C" int camp_eval_jac(void* solver_data, const double* state, const doubl
   double* d_y = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
   eval_prepare(s, state, env, d_y);
   HIP_OK(hipMemsetAsync(s->d_fail, 0, sizeof(int), s->stream));
-  size_t lds = 4 * (size_t)(P.ns + P.n_env) * sizeof(double);
-  hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, 4), 4096)), dim3(256), lds,
-                     s->stream, P, nc, d_y, s->d_state, s->d_renv, s->d_Js, s->d_Jy, s->d_Jf,
-                     s->d_Jf, time_step, 0, s->d_fail);
-  hipLaunchKernelGGL(camp::k_jac, dim3(std::min(camp::nblk(nc, 4), 4096)), dim3(256), lds,
-                     s->stream, P, nc, d_y, s->d_state, s->d_renv, s->d_Js, time_step);
+  int wpb = camp::fj_waves_per_block(P);
+  size_t lds = wpb * camp::fj_lds_doubles(P) * sizeof(double);
+  double* d_jr = P.mp ? camp::dev_alloc<double>((size_t)nc * P.rnnz, tmp) : nullptr;
+  hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, wpb), 4096)), dim3(64 * wpb), lds,
+                     s->stream, P, nc, d_y, s->d_state, s->d_env, s->d_renv, s->d_Js, s->d_Jy,
+                     s->d_Jf, s->d_Jf, time_step, 0, s->d_fail);
+  hipLaunchKernelGGL(camp::k_jac, dim3(std::min(camp::nblk(nc, wpb), 4096)), dim3(64 * wpb), lds,
+                     s->stream, P, nc, d_y, s->d_state, s->d_env, s->d_renv, s->d_Js, time_step,
+                     d_jr);
   HIP_OK(hipMemcpyAsync(s->d_Jy, d_y, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToDevice,
                         s->stream));
   int fail = 0;
@@ -1106,11 +1195,21 @@ extern "C" int camp_solver_get_cell_profile(void* solver_data, double* out) {
 extern "C" void camp_solver_work_model(void* solver_data, double* out8) {
   const camp::Program& G = SD(solver_data)->m.prog;
   double f = 0.0, j = 0.0;
+  // mass-action records: the rate product and the signed sum; slot records
+  // (multi-phase contributions, counted with their instances below): a sum
   for (int i = 0; i < G.nv; ++i)
-    for (int q = G.f_ptr[i]; q < G.f_ptr[i + 1]; ++q) f += G.t_nreact[G.f_term[q]] + 2;
+    for (int q = G.f_ptr[i]; q < G.f_ptr[i + 1]; ++q)
+      f += G.f_term[q] >= 0 ? G.t_nreact[G.f_term[q]] + 2 : 2;
   f += 4.0 * G.nv + 2.0 * G.nnz + G.nv;  // blend + Jacobian-estimate matvec
-  for (int e = 0; e < G.nnz; ++e)
+  int jn = G.mp ? G.rnnz : G.nnz;
+  for (int e = 0; e < jn; ++e)
     for (int q = G.j_ptr[e]; q < G.j_ptr[e + 1]; ++q) j += G.t_nreact[G.j_term[q]] + 1;
+  if (G.mp) {
+    // per-instance arithmetic of the multi-phase types (approximate counts
+    // of their double-double operations) and the J_rxn * J_params map
+    f += 40.0 * G.n_pi + 60.0 * G.n_zi + 80.0 * G.n_sp + 60.0 * G.n_aq;
+    j += 12.0 * (G.js_ptr.back()) + 2.0 * G.jm_rxn.size() + 120.0 * G.n_sp + 40.0 * G.n_aq;
+  }
   double lu = 2.0 * G.nnz + (double)G.l_pos.size() + 2.0 * G.up_dst.size();
   double sv = 2.0 * (G.lu_nnz - G.nv) + G.nv;
   double nr = G.n_terms;
@@ -1139,21 +1238,23 @@ extern "C" int camp_solver_time_fj(void* solver_data, const double* d_state, con
                      s->stream, P, nc, d_env, s->d_renv);
   hipLaunchKernelGGL(camp::k_state_gather, dim3(camp::nblk((long)nc * P.nv)), dim3(256), 0,
                      s->stream, P, nc, d_state, d_y);
-  size_t lds = 4 * (size_t)(P.ns + P.n_env) * sizeof(double);
-  int grid = std::min(camp::nblk(nc, 4), 1 << 20);
+  int wpb = camp::fj_waves_per_block(P);
+  size_t lds = wpb * camp::fj_lds_doubles(P) * sizeof(double);
+  int grid = std::min(camp::nblk(nc, wpb), 1 << 20);
+  double* d_jr = P.mp ? camp::dev_alloc<double>((size_t)nc * P.rnnz, tmp) : nullptr;
   float ms = 0.f;
   HIP_OK(hipEventRecord(s->ev0, s->stream));
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(camp::k_deriv, dim3(grid), dim3(256), lds, s->stream, P, nc, d_y, d_state,
-                       s->d_renv, s->d_Js, s->d_Jy, s->d_Jf, d_f, 1.0, 0, s->d_fail);
+    hipLaunchKernelGGL(camp::k_deriv, dim3(grid), dim3(64 * wpb), lds, s->stream, P, nc, d_y, d_state,
+                       d_env, s->d_renv, s->d_Js, s->d_Jy, s->d_Jf, d_f, 1.0, 0, s->d_fail);
   HIP_OK(hipEventRecord(s->ev1, s->stream));
   HIP_OK(hipEventSynchronize(s->ev1));
   HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   *ms_deriv = ms / reps;
   HIP_OK(hipEventRecord(s->ev0, s->stream));
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(camp::k_jac, dim3(grid), dim3(256), lds, s->stream, P, nc, d_y, d_state,
-                       s->d_renv, d_J, 1.0);
+    hipLaunchKernelGGL(camp::k_jac, dim3(grid), dim3(64 * wpb), lds, s->stream, P, nc, d_y, d_state,
+                       d_env, s->d_renv, d_J, 1.0, d_jr);
   HIP_OK(hipEventRecord(s->ev1, s->stream));
   HIP_OK(hipEventSynchronize(s->ev1));
   HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));

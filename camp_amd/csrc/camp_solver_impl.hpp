@@ -83,10 +83,23 @@ struct Solver {
   // seconds reported as RHS_time__s / Jac_time__s
   bool profiling = false;
   double* d_prof = nullptr;
+  // per-wavefront global scratch of multi-phase programs (grow-only)
+  double* d_wave_scratch = nullptr;
+  size_t wave_scratch_waves = 0;
+  double* wave_scratch(size_t waves) {
+    if (!P.mp) return nullptr;
+    if (waves > wave_scratch_waves) {
+      if (d_wave_scratch) HIP_OK(hipFree(d_wave_scratch));
+      HIP_OK(hipMalloc(&d_wave_scratch, waves * cell_bdf_wave_scratch(P) * sizeof(double)));
+      wave_scratch_waves = waves;
+    }
+    return d_wave_scratch;
+  }
   double rhs_time_s = 0.0, jac_time_s = 0.0;
 
   ~Solver() {
     for (void* p : owned) (void)hipFree(p);
+    if (d_wave_scratch) (void)hipFree(d_wave_scratch);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);

@@ -92,8 +92,10 @@ void rxn_first_order_loss_set_rate_update_data(void *update_data, int rxn_id,
 /* replace aero_phase_add_condensed_data (aero_phase_solver.c:175),
  * aero_rep_add_condensed_data (aero_rep_solver.c:443) and
  * sub_model_add_condensed_data (sub_model_solver.c:413), bound at
- * camp_solver_data.F90:201-354.  The gas-phase path accepts none: a call
- * prints an error and exits, the reference's setup-error convention. */
+ * camp_solver_data.F90:201-354: packed data of aerosol phases, aerosol
+ * representations (MODAL_BINNED_MASS) and sub models (ZSR_AEROSOL_WATER),
+ * in the reference's layouts.  Unsupported types print an error and exit
+ * at solver_initialize, the reference's setup-error convention. */
 void aero_phase_add_condensed_data(int n_int_param, int n_float_param,
                                    int *int_param, double *float_param,
                                    void *solver_data);

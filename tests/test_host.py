@@ -96,8 +96,14 @@ def test_setup_error_exits_like_the_reference():
             "from camp_amd._lib import lib\n"
             "import numpy as np, ctypes as C\n"
             "vt = np.ones(3, np.int32)\n"
-            "lib().solver_new(3, 1, vt.ctypes.data_as(C.POINTER(C.c_int)), 0, 0, 0, 0,"
-            " 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)\n" % ROOT)
+            "ip = C.POINTER(C.c_int); dp = C.POINTER(C.c_double)\n"
+            "h = lib().solver_new(3, 1, vt.ctypes.data_as(ip), 1, 4, 1, 1,"
+            " 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)\n"
+            "ints = np.array([1, 1, 0, 0], np.int32); fl = np.zeros(1)\n"
+            "lib().rxn_add_condensed_data(3, 4, 1, 1, ints.ctypes.data_as(ip),"
+            " fl.ctypes.data_as(dp), h)\n"
+            "at = np.full(3, 1e-8)\n"
+            "lib().solver_initialize(h, at.ctypes.data_as(dp), 1e-4, 100, 10)\n" % ROOT)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1
     assert "ERROR" in r.stderr
@@ -122,3 +128,19 @@ def test_server_rate_layout_follows_reaction_order():
     br = base_rate_matrix(s, cells.rates, 5)
     for j, (_, uid) in enumerate(lay):
         assert np.array_equal(br[:, j], cells.rates[uid])
+
+
+def test_multiphase_pattern_ids_and_map_match_oracle(oracle_lib):
+    """Product host model vs oracle on the multi-phase mechanism: reaction
+    and solver patterns, derivative ids and the reaction int data after the
+    id updates (SIMPOL phase Jacobian ids, aqueous-equilibrium ids)."""
+    from camp_amd.core import CampSolver
+    from camp_amd.multiphase import multiphase_mechanism
+    mp = multiphase_mechanism(1, n_bins=3, n_simpol=5, n_aqeq=4)
+    s = CampSolver(mp, 2)
+    o = oracle_lib.Oracle(mp, 2)
+    a, b = s.pattern(), o.pattern()
+    for k in ("rxn_colptr", "rxn_rowval", "colptr", "rowval", "deriv_ids"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["rxn_int_data"], o.int_data())
+    assert s.nv == o.nv and s.nnz == o.nnz
