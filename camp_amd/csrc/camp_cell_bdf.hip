@@ -100,8 +100,10 @@ struct CellCv {
     if (cell_set_state(P, lane, yv, s_state)) return 1;
     if (P.mp) {
       mp_f_prologue(P, lane, s_state, s_k, mpl);
-      mp_j_prologue(P, lane, s_state, s_k, mpl);
       cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy, mpl.slot);
+      wsync();
+      mp_refresh(P, lane, s_state, mpl);
+      mp_j_prologue(P, lane, s_state, s_k, mpl);
       cell_jac_mp(P, lane, s_state, s_k, ts, mpl, Jr, Js);
     } else {
       cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy);

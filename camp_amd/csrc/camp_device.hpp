@@ -67,24 +67,24 @@ struct dd {
   double hi, lo;
 };
 
-__device__ __forceinline__ dd two_sum(double a, double b) {
+__host__ __device__ __forceinline__ dd two_sum(double a, double b) {
   double s = a + b;
   double bb = s - a;
   double e = (a - (s - bb)) + (b - bb);
   return {s, e};
 }
-__device__ __forceinline__ dd fast_two_sum(double a, double b) {
+__host__ __device__ __forceinline__ dd fast_two_sum(double a, double b) {
   double s = a + b;
   double e = b - (s - a);
   return {s, e};
 }
-__device__ __forceinline__ dd dd_mul_d(dd a, double b) {
+__host__ __device__ __forceinline__ dd dd_mul_d(dd a, double b) {
   double p = a.hi * b;
   double e = fma(a.hi, b, -p);
   e = fma(a.lo, b, e);
   return fast_two_sum(p, e);
 }
-__device__ __forceinline__ dd dd_add(dd a, dd b) {
+__host__ __device__ __forceinline__ dd dd_add(dd a, dd b) {
   dd s = two_sum(a.hi, b.hi);
   double lo = s.lo + (a.lo + b.lo);
   return fast_two_sum(s.hi, lo);
@@ -121,7 +121,7 @@ constexpr double kGasConst = 8.314472;  // util.h:17
 // --- environment: SIMPOL and aqueous equilibrium (rxn env slots) ----------
 // rxn_SIMPOL_phase_transfer_update_env_state (:220-266):
 // e = [MFP_M, ALPHA, EQUIL_CONST, KGM3_TO_PPM]
-__device__ __forceinline__ void mp_simpol_env(const double* f, double* e, double T, double Pa) {
+__host__ __device__ __forceinline__ void mp_simpol_env(const double* f, double* e, double T, double Pa) {
   double alpha = 0.1;
   if (f[0] != 0.0 || f[1] != 0.0) {
     double del_g = f[0] - T * f[1];
@@ -136,12 +136,12 @@ __device__ __forceinline__ void mp_simpol_env(const double* f, double* e, double
   e[2] = vp / Pa / f[9] * 1.0e6;
 }
 // rxn_aqueous_equilibrium_update_env_state (:244-265): e = [k_forward]
-__device__ __forceinline__ void mp_aqeq_env(const double* f, double* e, double T) {
+__host__ __device__ __forceinline__ void mp_aqeq_env(const double* f, double* e, double T) {
   double k = f[1] == 0.0 ? f[0] : f[0] * exp(f[1] * (1.0 / T - 1.0 / 298.0));
   e[0] = k * f[2];
 }
 // sub_model_ZSR_aerosol_water_update_env_state (:189-207): PPM_TO_RH
-__device__ __forceinline__ double mp_ppm_to_rh(double T, double Pa) {
+__host__ __device__ __forceinline__ double mp_ppm_to_rh(double T, double Pa) {
   double a = 1.0 - 373.15 / T;
   a = (((-0.1299 * a - 0.6445) * a - 1.976) * a + 13.3185) * a;
   return Pa / (101325.0 * exp(a)) / 1.0e6;
@@ -153,7 +153,7 @@ __device__ __forceinline__ double mp_ppm_to_rh(double T, double Pa) {
 // 137-147, rxn_emission.c:109-119, rxn_first_order_loss.c:119-129.
 // env_cell points at the cell's rxn_env block (k slots written in place).
 // ---------------------------------------------------------------------
-__device__ __forceinline__ void rate_constant(const DevProgram& P, int r, double T, double Pa,
+__host__ __device__ __forceinline__ void rate_constant(const DevProgram& P, int r, double T, double Pa,
                                               double* env_cell) {
   const double* f = P.rx_float + P.rx_fidx[r];
   double* e = env_cell + P.rx_eidx[r];
@@ -200,7 +200,7 @@ __device__ __forceinline__ dd term_rate(const DevProgram& P, int t, const double
   return rate;
 }
 
-__device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const double* s_state,
+__host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const double* s_state,
                                            const double* s_k, const double* y,
                                            const double* Js, const double* Jy,
                                            const double* Jf, bool use_est, double h,
@@ -279,7 +279,7 @@ __device__ __forceinline__ int cell_set_state(const DevProgram& P, int lane, con
 // ---------------------------------------------------------------------
 // Mass-action contributions to Jacobian element e of the gather space
 // (solver elements, or reaction-pattern elements for multi-phase programs).
-__device__ __forceinline__ dd jac_elem_mass_action(const DevProgram& P, int e, const double* s_state,
+__host__ __device__ __forceinline__ dd jac_elem_mass_action(const DevProgram& P, int e, const double* s_state,
                                                    const double* s_k, double h) {
   dd D = {0.0, 0.0};
   for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q) {

@@ -88,7 +88,8 @@ __global__ void __launch_bounds__(256) k_jac(DevProgram P, int nc, const double*
     if (cell_set_state(P, lane, y + (size_t)c * P.nv, s_state)) continue;
     if (P.mp) {
       mp_cell_env(P, lane, env[2 * c], env[2 * c + 1], L);
-      mp_f_prologue(P, lane, s_state, s_k, L);
+      mp_refresh(P, lane, s_state, L);  // the f() that Jac() runs first
+      mp_refresh(P, lane, s_state, L);  // Jac()'s own update
       mp_j_prologue(P, lane, s_state, s_k, L);
       cell_jac_mp(P, lane, s_state, s_k, h, L, Jr + (size_t)c * P.rnnz, J + (size_t)c * P.nnz);
     } else {

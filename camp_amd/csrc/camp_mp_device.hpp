@@ -25,22 +25,22 @@ constexpr int kSpScalars = 13;          // per SIMPOL instance in LDS
 constexpr int kAqScalars = 6;           // per aqueous-equilibrium instance
 
 // --- double-double helpers ------------------------------------------------
-__device__ __forceinline__ dd two_prod(double a, double b) {
+__host__ __device__ __forceinline__ dd two_prod(double a, double b) {
   double p = a * b;
   return {p, fma(a, b, -p)};
 }
-__device__ __forceinline__ dd dd_from(double a) { return {a, 0.0}; }
-__device__ __forceinline__ double dd_round(dd a) { return a.hi + a.lo; }
-__device__ __forceinline__ dd dd_mul(dd a, dd b) {
+__host__ __device__ __forceinline__ dd dd_from(double a) { return {a, 0.0}; }
+__host__ __device__ __forceinline__ double dd_round(dd a) { return a.hi + a.lo; }
+__host__ __device__ __forceinline__ dd dd_mul(dd a, dd b) {
   dd p = two_prod(a.hi, b.hi);
   p.lo = fma(a.hi, b.lo, p.lo);
   p.lo = fma(a.lo, b.hi, p.lo);
   return fast_two_sum(p.hi, p.lo);
 }
-__device__ __forceinline__ dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
-__device__ __forceinline__ dd dd_sub(dd a, dd b) { return dd_add(a, dd_neg(b)); }
+__host__ __device__ __forceinline__ dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
+__host__ __device__ __forceinline__ dd dd_sub(dd a, dd b) { return dd_add(a, dd_neg(b)); }
 // a / b for double-double a, b (one Newton correction of the double quotient)
-__device__ __forceinline__ dd dd_div(dd a, dd b) {
+__host__ __device__ __forceinline__ dd dd_div(dd a, dd b) {
   double q1 = a.hi / b.hi;
   dd r = dd_sub(a, dd_mul(b, dd_from(q1)));
   double q2 = r.hi / b.hi;
@@ -49,20 +49,20 @@ __device__ __forceinline__ dd dd_div(dd a, dd b) {
   dd q = fast_two_sum(q1, q2);
   return dd_add(q, dd_from(q3));
 }
-__device__ __forceinline__ dd dd_div_d(dd a, double b) { return dd_div(a, dd_from(b)); }
-__device__ __forceinline__ bool dd_gt(dd a, double b) {
+__host__ __device__ __forceinline__ dd dd_div_d(dd a, double b) { return dd_div(a, dd_from(b)); }
+__host__ __device__ __forceinline__ bool dd_gt(dd a, double b) {
   return a.hi > b || (a.hi == b && a.lo > 0.0);
 }
 
 // --- transition-regime condensation (util.h:19-68), all double -------------
-__device__ __forceinline__ double mp_transition_corr(double mfp, double r, double alpha) {
+__host__ __device__ __forceinline__ double mp_transition_corr(double mfp, double r, double alpha) {
   double kn = mfp / r;
   return (0.75 * alpha * (1.0 + kn)) / (kn * kn + (1.0 + 0.283 * alpha) * kn + 0.75 * alpha);
 }
-__device__ __forceinline__ double mp_cond_rate(double D, double mfp, double r, double alpha) {
+__host__ __device__ __forceinline__ double mp_cond_rate(double D, double mfp, double r, double alpha) {
   return 4.0 * kPi * r * D * mp_transition_corr(mfp, r, alpha);
 }
-__device__ __forceinline__ double mp_dcond_rate_dr(double D, double mfp, double r, double alpha) {
+__host__ __device__ __forceinline__ double mp_dcond_rate_dr(double D, double mfp, double r, double alpha) {
   double kn = mfp / r;
   double dcorr = (0.75 * alpha * mfp * (kn * kn + 2.0 * kn + 1.0 + (0.283 - 0.75) * alpha)) /
                  pow(r * (kn * kn + (1.0 + 0.283 * alpha) * kn + 0.75 * alpha), 2.0);
@@ -73,7 +73,7 @@ __device__ __forceinline__ double mp_dcond_rate_dr(double D, double mfp, double 
 // s_aero[i*8]: mass, average MW, number concentration (cached, as the
 // representation caches them), then the mass and mole sums (dd) of the
 // current state, which the MW partial derivatives use
-__device__ __forceinline__ void mp_phase_sums(const DevProgram& P, int i, const double* s_state,
+__host__ __device__ __forceinline__ void mp_phase_sums(const DevProgram& P, int i, const double* s_state,
                                               dd* mass_out, dd* moles_out, double* vol_out) {
   int st0 = P.pi_start[i], pm = P.pi_model[i];
   dd mass = {P.min_mass_hi, P.min_mass_lo};
@@ -91,7 +91,7 @@ __device__ __forceinline__ void mp_phase_sums(const DevProgram& P, int i, const 
   *vol_out = vol;
 }
 
-__device__ __forceinline__ void mp_phase_update(const DevProgram& P, int i, const double* s_state,
+__host__ __device__ __forceinline__ void mp_phase_update(const DevProgram& P, int i, const double* s_state,
                                                 double* a) {
   dd mass, moles;
   double vol;
@@ -104,7 +104,7 @@ __device__ __forceinline__ void mp_phase_update(const DevProgram& P, int i, cons
 }
 
 // sums of the state after the sub models ran (the getters' partials see it)
-__device__ __forceinline__ void mp_phase_partial_sums(const DevProgram& P, int i,
+__host__ __device__ __forceinline__ void mp_phase_partial_sums(const DevProgram& P, int i,
                                                       const double* s_state, double* a) {
   dd mass, moles;
   double vol;
@@ -116,7 +116,7 @@ __device__ __forceinline__ void mp_phase_partial_sums(const DevProgram& P, int i
 }
 
 // --- ZSR water of one (sub model, phase) item (:216-301) -------------------
-__device__ __forceinline__ void mp_zsr_water(const DevProgram& P, int z, const double* s_sub_env,
+__host__ __device__ __forceinline__ void mp_zsr_water(const DevProgram& P, int z, const double* s_sub_env,
                                              double* s_state) {
   // a_w = PPM_TO_RH * state: a double product (both operands double)
   dd aw = dd_from(s_sub_env[P.zi_sub[z]] * s_state[P.zi_gw[z]]);
@@ -140,7 +140,7 @@ __device__ __forceinline__ void mp_zsr_water(const DevProgram& P, int z, const d
 }
 
 // --- ZSR parameter Jacobian of one item (:314-~440), added into s_jp -------
-__device__ __forceinline__ void mp_zsr_jac(const DevProgram& P, int z, const double* s_sub_env,
+__host__ __device__ __forceinline__ void mp_zsr_jac(const DevProgram& P, int z, const double* s_sub_env,
                                            const double* s_state, double* s_jp) {
   double ppm = s_sub_env[P.zi_sub[z]];
   dd aw = dd_from(ppm * s_state[P.zi_gw[z]]);
@@ -191,7 +191,7 @@ struct SpInst {
   dd cond, evap;              // k_c*gas and k_c*Keq*MW/mass*act*aero (long double in CAMP)
 };
 
-__device__ __forceinline__ SpInst mp_simpol_inst(const DevProgram& P, int i, const double* s_state,
+__host__ __device__ __forceinline__ SpInst mp_simpol_inst(const DevProgram& P, int i, const double* s_state,
                                                  const double* s_k, const double* s_aero) {
   const double* e = s_k + P.sp_env[i];
   const double* f = P.rx_float + P.sp_flt[i];
@@ -209,7 +209,7 @@ __device__ __forceinline__ SpInst mp_simpol_inst(const DevProgram& P, int i, con
 }
 
 // f slots of one SIMPOL instance (rxn_SIMPOL_phase_transfer.c:355-399)
-__device__ __forceinline__ void mp_simpol_f(const DevProgram& P, int i, const double* s_state,
+__host__ __device__ __forceinline__ void mp_simpol_f(const DevProgram& P, int i, const double* s_state,
                                             const double* s_k, const double* s_aero,
                                             double* s_slot) {
   SpInst s = mp_simpol_inst(P, i, s_state, s_k, s_aero);
@@ -228,7 +228,7 @@ struct AqRates {
   bool skip;
   dd rf, rr;
 };
-__device__ __forceinline__ AqRates mp_aqeq_rates(const DevProgram& P, int i,
+__host__ __device__ __forceinline__ AqRates mp_aqeq_rates(const DevProgram& P, int i,
                                                  const double* s_state, const double* s_k) {
   AqRates r;
   const double* f = P.rx_float + P.aq_flt[i];
@@ -247,7 +247,7 @@ __device__ __forceinline__ AqRates mp_aqeq_rates(const DevProgram& P, int i,
   return r;
 }
 
-__device__ __forceinline__ void mp_aqeq_f(const DevProgram& P, int i, const double* s_state,
+__host__ __device__ __forceinline__ void mp_aqeq_f(const DevProgram& P, int i, const double* s_state,
                                           const double* s_k, double* s_slot) {
   int nr = P.aq_nr[i], np = P.aq_np[i];
   double* o = s_slot + 2 * P.aq_slot[i];
@@ -272,7 +272,7 @@ __device__ __forceinline__ void mp_aqeq_f(const DevProgram& P, int i, const doub
 // 415-~590), kSpScalars doubles: N, kp, kc, evap0 (dd), cond (dd), evap
 // (dd), d_cond_d_radius, d_evap_d_radius, d_evap_d_mass, d_evap_d_MW.  evap0
 // is k_c*Keq*MW/mass before the activity coefficient and the aerosol factor.
-__device__ __forceinline__ void mp_simpol_jscalars(const DevProgram& P, int i,
+__host__ __device__ __forceinline__ void mp_simpol_jscalars(const DevProgram& P, int i,
                                                    const double* s_state, const double* s_k,
                                                    const double* s_aero, double* o) {
   const double* e = s_k + P.sp_env[i];
@@ -304,7 +304,7 @@ __device__ __forceinline__ void mp_simpol_jscalars(const DevProgram& P, int i,
 
 // --- Jacobian scalars of one aqueous-equilibrium instance (:403-~430):
 // forward (dd), reverse (dd), water, skip flag
-__device__ __forceinline__ void mp_aqeq_jscalars(const DevProgram& P, int i,
+__host__ __device__ __forceinline__ void mp_aqeq_jscalars(const DevProgram& P, int i,
                                                  const double* s_state, const double* s_k,
                                                  double* o) {
   const double* f = P.rx_float + P.aq_flt[i];
@@ -328,7 +328,7 @@ __device__ __forceinline__ void mp_aqeq_jscalars(const DevProgram& P, int i,
 // --- special Jacobian records of one reaction element, added to D ---------
 // PRODUCTION contributions add, LOSS contributions subtract (Jacobian.c:
 // 223-241 keeps them in separate long double sums and outputs prod - loss).
-__device__ __forceinline__ dd mp_jac_records(const DevProgram& P, int e, const double* s_state,
+__host__ __device__ __forceinline__ dd mp_jac_records(const DevProgram& P, int e, const double* s_state,
                                              const double* s_aero, const double* s_jsc, dd D) {
   const double* aq0 = s_jsc + kSpScalars * P.n_sp;
   for (int q = P.js_ptr[e]; q < P.js_ptr[e + 1]; ++q) {
@@ -441,21 +441,29 @@ __device__ __forceinline__ void mp_cell_env(const DevProgram& P, int lane, doubl
   wsync();
 }
 
-// The per-cell start of f() and Jac() (camp_solver.c:913-917, 1065-1072):
-// the representation's cached state, then ZSR water, then the contributions
-// of the multi-phase reactions into their f slots.
-__device__ __forceinline__ void mp_f_prologue(const DevProgram& P, int lane, double* s_state,
-                                              const double* s_k, MpLds L) {
+// aero_rep_update_state then sub_model_calculate for one cell
+// (camp_solver.c:913-917 in f(), 1065-1072 in Jac()): the representation
+// caches mass, MW and number concentration from the water currently in the
+// state, then ZSR writes the new water.  Jac() runs f() first, so its own
+// update sees the water that f() left.
+__device__ __forceinline__ void mp_refresh(const DevProgram& P, int lane, double* s_state, MpLds L) {
   for (int i = lane; i < P.n_pi; i += kWave) mp_phase_update(P, i, s_state, L.aero + kAeroSlots * i);
   wsync();
   for (int z = lane; z < P.n_zi; z += kWave) mp_zsr_water(P, z, L.sub_env, s_state);
   wsync();
+}
+
+// The per-cell start of f(): refresh, then the contributions of the
+// multi-phase reactions into their f slots.
+__device__ __forceinline__ void mp_f_prologue(const DevProgram& P, int lane, double* s_state,
+                                              const double* s_k, MpLds L) {
+  mp_refresh(P, lane, s_state, L);
   for (int i = lane; i < P.n_sp; i += kWave) mp_simpol_f(P, i, s_state, s_k, L.aero, L.slot);
   for (int i = lane; i < P.n_aq; i += kWave) mp_aqeq_f(P, i, s_state, s_k, L.slot);
   wsync();
 }
 
-// After mp_f_prologue: the sums the partial derivatives use, the
+// After mp_f_prologue and the refresh of Jac(): the sums the partial derivatives use, the
 // per-instance Jacobian scalars and the ZSR parameter Jacobian, whose
 // element 0 is then set to 1 (the map's dummy, camp_solver.c:1099).
 __device__ __forceinline__ void mp_j_prologue(const DevProgram& P, int lane, const double* s_state,
