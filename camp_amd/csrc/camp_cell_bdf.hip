@@ -53,7 +53,7 @@ struct CellCv {
   double tn, h, hscale, hprime, next_h, eta, etamax, etaq, etaqm1, etaqp1, hu, init_h, rtol;
   double *l, *tq, *tau;  // small coefficient arrays, in LDS (uniform values)
   double rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
-  int q, qprime, L, qwait, jcur, use_acinit, guess_on, maxcf, max_steps;
+  int q, qprime, L, qwait, jcur, use_acinit, guess_on, maxcf, max_steps, projected;
   int nst, nfe, nsetups, nni, ncfn, netf, nje, nstlp, nstlj;
   // optional shader-clock counters (cycles in f, Jac, LU factor, LU solve)
   bool prof;
@@ -347,12 +347,25 @@ struct CellCv {
       if (dcon <= 1.0) {
         // Non-negativity safeguard: CAMP's f() rejects y < -SMALL
         // (camp_solver.c:816-822); a converged iterate below that would make
-        // the next step's first f() fail at every step size, so it is
-        // treated here as the recoverable f() failure it implies.
-        double mn = 1.0e308;
-        for (int i = lane; i < nv; i += kWave) mn = fmin(mn, y[i]);
-        if (wave_min(mn) < -1.0e-30) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
-        acnrm = (m == 0) ? del : wrms(acor);
+        // the next step's first f() fail at every step size.  Components
+        // below -SMALL by at most their tolerance (-y_i * ewt_i <= 1) are
+        // projected to 0 through the correction; a larger excursion is the
+        // recoverable f() failure it implies.
+        double worst = -1.0;
+        for (int i = lane; i < nv; i += kWave)
+          if (y[i] < -1.0e-30) worst = fmax(worst, -y[i] * ewt[i]);
+        worst = -wave_min(-worst);
+        if (worst > 1.0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        if (worst >= 0.0) {
+          for (int i = lane; i < nv; i += kWave)
+            if (y[i] < -1.0e-30) {
+              acor[i] = -zn(0)[i];
+              y[i] = 0.0;
+            }
+          wsync();
+        }
+        projected = worst >= 0.0;
+        acnrm = (m == 0 && worst < 0.0) ? del : wrms(acor);
         jcur = 0;
         return CV_SUCCESS;
       }
@@ -467,6 +480,9 @@ struct CellCv {
       double a = acor[i];
       for (int j = 0; j <= q; ++j) zn(j)[i] += l[j] * a;
       if (save) zn(QMAX)[i] = a;
+      // after a projection, components at 0 restart flat
+      if (projected && zn(0)[i] == 0.0)
+        for (int j = 1; j <= q; ++j) zn(j)[i] = 0.0;
       last[i] = zn(0)[i];
     }
     if (save) saved_tq5 = tq[5];
@@ -572,6 +588,7 @@ struct CellCv {
     gamrat = 1.0;
     gammap = 1.0;
     jcur = 0;
+    projected = 0;
     eta = 1.0;
     for (int i = 0; i < 8; ++i) tau[i] = 0.0;
     for (int i = 0; i < 6; ++i) tq[i] = 0.0;

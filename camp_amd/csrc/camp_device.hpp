@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "camp_dd.hpp"
+
 namespace camp {
 
 constexpr int kWave = 64;
@@ -59,38 +61,6 @@ struct DevProgram {
 
 
 // ---------------------------------------------------------------------
-// Double-double arithmetic: the accumulators of f and J use it in place of
-// CAMP's 80-bit long double production/loss sums (time_derivative.h:21-28,
-// Jacobian.h:31-40).  Error-free transforms: Knuth TwoSum, FMA TwoProd.
-// ---------------------------------------------------------------------
-struct dd {
-  double hi, lo;
-};
-
-__host__ __device__ __forceinline__ dd two_sum(double a, double b) {
-  double s = a + b;
-  double bb = s - a;
-  double e = (a - (s - bb)) + (b - bb);
-  return {s, e};
-}
-__host__ __device__ __forceinline__ dd fast_two_sum(double a, double b) {
-  double s = a + b;
-  double e = b - (s - a);
-  return {s, e};
-}
-__host__ __device__ __forceinline__ dd dd_mul_d(dd a, double b) {
-  double p = a.hi * b;
-  double e = fma(a.hi, b, -p);
-  e = fma(a.lo, b, e);
-  return fast_two_sum(p, e);
-}
-__host__ __device__ __forceinline__ dd dd_add(dd a, dd b) {
-  dd s = two_sum(a.hi, b.hi);
-  double lo = s.lo + (a.lo + b.lo);
-  return fast_two_sum(s.hi, lo);
-}
-
-// ---------------------------------------------------------------------
 // Wavefront helpers
 // ---------------------------------------------------------------------
 // Orders the LDS (and global) accesses of one wavefront across lanes.  A
@@ -125,26 +95,26 @@ __host__ __device__ __forceinline__ void mp_simpol_env(const double* f, double* 
   double alpha = 0.1;
   if (f[0] != 0.0 || f[1] != 0.0) {
     double del_g = f[0] - T * f[1];
-    alpha = exp(-del_g / (kGasConst * T));
+    alpha = cr_exp(-del_g / (kGasConst * T));
     alpha = alpha / (1.0 + alpha);
   }
   e[1] = alpha;
   e[0] = 3.0 * f[2] / sqrt(8.0 * kGasConst * T / (kPi * f[9]));
-  double vp = f[4] / T + f[5] + f[6] * T + f[7] * log(T);
-  vp = 101325.0 * pow(10.0, vp);
+  double vp = f[4] / T + f[5] + f[6] * T + f[7] * cr_log(T);
+  vp = 101325.0 * cr_pow(10.0, vp);
   e[3] = f[8] * T / Pa;
   e[2] = vp / Pa / f[9] * 1.0e6;
 }
 // rxn_aqueous_equilibrium_update_env_state (:244-265): e = [k_forward]
 __host__ __device__ __forceinline__ void mp_aqeq_env(const double* f, double* e, double T) {
-  double k = f[1] == 0.0 ? f[0] : f[0] * exp(f[1] * (1.0 / T - 1.0 / 298.0));
+  double k = f[1] == 0.0 ? f[0] : f[0] * cr_exp(f[1] * (1.0 / T - 1.0 / 298.0));
   e[0] = k * f[2];
 }
 // sub_model_ZSR_aerosol_water_update_env_state (:189-207): PPM_TO_RH
 __host__ __device__ __forceinline__ double mp_ppm_to_rh(double T, double Pa) {
   double a = 1.0 - 373.15 / T;
   a = (((-0.1299 * a - 0.6445) * a - 1.976) * a + 13.3185) * a;
-  return Pa / (101325.0 * exp(a)) / 1.0e6;
+  return Pa / (101325.0 * cr_exp(a)) / 1.0e6;
 }
 
 // ---------------------------------------------------------------------
@@ -160,17 +130,17 @@ __host__ __device__ __forceinline__ void rate_constant(const DevProgram& P, int 
   int t = P.rx_type[r];
   int nr = P.rx_nreact[r];
   if (t == 1) {  // ARRHENIUS
-    e[0] = f[0] * exp(f[2] / T) * (f[1] == 0.0 ? 1.0 : pow(T / f[3], f[1])) *
-           (f[4] == 0.0 ? 1.0 : (1.0 + f[4] * Pa)) * pow(f[5] * Pa / T, (double)(nr - 1));
+    e[0] = f[0] * cr_exp(f[2] / T) * (f[1] == 0.0 ? 1.0 : cr_pow(T / f[3], f[1])) *
+           (f[4] == 0.0 ? 1.0 : (1.0 + f[4] * Pa)) * cr_pow(f[5] * Pa / T, (double)(nr - 1));
   } else if (t == 2) {  // TROE
     double conv = f[9] * Pa / T;
-    double k0 = f[0] * (f[2] == 0.0 ? 1.0 : exp(f[2] / T)) *
-                (f[1] == 0.0 ? 1.0 : pow(T / 300.0, f[1])) * conv;
-    double kinf = k0 / (f[3] * (f[5] == 0.0 ? 1.0 : exp(f[5] / T)) *
-                        (f[4] == 0.0 ? 1.0 : pow(T / 300.0, f[4])));
-    double lg = log10(kinf) / f[7];
-    e[0] = (k0 / (1.0 + kinf)) * pow(f[6], (1.0 / (1.0 + pow(lg, 2.0)))) *
-           pow(conv, (double)(nr - 1)) * f[8];
+    double k0 = f[0] * (f[2] == 0.0 ? 1.0 : cr_exp(f[2] / T)) *
+                (f[1] == 0.0 ? 1.0 : cr_pow(T / 300.0, f[1])) * conv;
+    double kinf = k0 / (f[3] * (f[5] == 0.0 ? 1.0 : cr_exp(f[5] / T)) *
+                        (f[4] == 0.0 ? 1.0 : cr_pow(T / 300.0, f[4])));
+    double lg = cr_log10(kinf) / f[7];
+    e[0] = (k0 / (1.0 + kinf)) * cr_pow(f[6], (1.0 / (1.0 + cr_pow(lg, 2.0)))) *
+           cr_pow(conv, (double)(nr - 1)) * f[8];
   } else if (t == 10) {  // SIMPOL_PHASE_TRANSFER
     mp_simpol_env(f, e, T, Pa);
   } else if (t == 7) {  // AQUEOUS_EQUILIBRIUM

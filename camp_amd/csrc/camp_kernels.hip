@@ -176,14 +176,18 @@ __global__ void k_vec_rescale(long n, int q, double eta, double* __restrict__ zn
   }
 }
 
-// zn[j] += l[j] * acor for j = 0..q
+// zn[j] += l[j] * acor for j = 0..q; after a projection, components at 0
+// restart flat (zn[1..q] cleared)
 __global__ void k_vec_complete(long n, int q, Coef6 l6, const double* __restrict__ acor,
-                               double* __restrict__ zn, int save, double* __restrict__ last) {
+                               double* __restrict__ zn, int save, double* __restrict__ last,
+                               int projected) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double a = acor[i];
   for (int j = 0; j <= q; ++j) zn[j * n + i] += l6.v[j] * a;
   if (save) zn[5 * n + i] = a;
+  if (projected && zn[i] == 0.0)
+    for (int j = 1; j <= q; ++j) zn[j * n + i] = 0.0;
   last[i] = zn[i];
 }
 
@@ -252,6 +256,30 @@ __global__ void __launch_bounds__(256) k_vec_min(long n, const double* __restric
   if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = s;
   __syncthreads();
   if (threadIdx.x == 0) out[blockIdx.x] = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+}
+
+// Per-block max over i with x_i < -SMALL of -x_i * w_i (-1 when none)
+__global__ void __launch_bounds__(256) k_vec_negmax(long n, const double* __restrict__ x,
+                                                    const double* __restrict__ w,
+                                                    double* __restrict__ out) {
+  __shared__ double red[4];
+  double s = -1.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    if (x[i] < -1.0e-30) s = fmax(s, -x[i] * w[i]);
+  for (int off = kWave / 2; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off, kWave));
+  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// y_i < -SMALL -> acor_i = -zn0_i, y_i = 0 (projection onto y >= 0)
+__global__ void k_vec_project(long n, const double* __restrict__ zn0, double* __restrict__ acor,
+                              double* __restrict__ y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && y[i] < -1.0e-30) {
+    acor[i] = -zn0[i];
+    y[i] = 0.0;
+  }
 }
 
 // Guess-helper jump search: per block, min over i of the candidate t_star

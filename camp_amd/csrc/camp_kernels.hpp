@@ -77,7 +77,7 @@ struct Coef6 {
   double v[6];
 };
 __global__ void k_vec_complete(long n, int q, Coef6 l6, const double* acor, double* zn, int save,
-                               double* last);
+                               double* last, int projected);
 __global__ void k_vec_resid(long n, double rl1, const double* zn1, const double* acor,
                             double gamma, const double* ftemp, double* tempv);
 __global__ void k_vec_newton_update(long n, const double* b, double* acor, const double* zn0,
@@ -87,6 +87,8 @@ __global__ void k_vec_ewt(long n, int nv, double rtol, const double* atol, const
 __global__ void k_vec_interp(long n, int q, double s, const double* zn, double* y);
 __global__ void k_vec_wsq(long n, const double* x, const double* w, double* out);
 __global__ void k_vec_min(long n, const double* x, double* out);
+__global__ void k_vec_negmax(long n, const double* x, const double* w, double* out);
+__global__ void k_vec_project(long n, const double* zn0, double* acor, double* y);
 __global__ void k_vec_tstar(long n, const double* tmp1, const double* corr, double h_j,
                             double* out);
 __global__ void k_fill(long n, double v, double* x);

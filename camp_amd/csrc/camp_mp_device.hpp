@@ -24,36 +24,6 @@ constexpr int kAeroSlots = 8;           // per phase instance in LDS
 constexpr int kSpScalars = 13;          // per SIMPOL instance in LDS
 constexpr int kAqScalars = 6;           // per aqueous-equilibrium instance
 
-// --- double-double helpers ------------------------------------------------
-__host__ __device__ __forceinline__ dd two_prod(double a, double b) {
-  double p = a * b;
-  return {p, fma(a, b, -p)};
-}
-__host__ __device__ __forceinline__ dd dd_from(double a) { return {a, 0.0}; }
-__host__ __device__ __forceinline__ double dd_round(dd a) { return a.hi + a.lo; }
-__host__ __device__ __forceinline__ dd dd_mul(dd a, dd b) {
-  dd p = two_prod(a.hi, b.hi);
-  p.lo = fma(a.hi, b.lo, p.lo);
-  p.lo = fma(a.lo, b.hi, p.lo);
-  return fast_two_sum(p.hi, p.lo);
-}
-__host__ __device__ __forceinline__ dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
-__host__ __device__ __forceinline__ dd dd_sub(dd a, dd b) { return dd_add(a, dd_neg(b)); }
-// a / b for double-double a, b (one Newton correction of the double quotient)
-__host__ __device__ __forceinline__ dd dd_div(dd a, dd b) {
-  double q1 = a.hi / b.hi;
-  dd r = dd_sub(a, dd_mul(b, dd_from(q1)));
-  double q2 = r.hi / b.hi;
-  r = dd_sub(r, dd_mul(b, dd_from(q2)));
-  double q3 = r.hi / b.hi;
-  dd q = fast_two_sum(q1, q2);
-  return dd_add(q, dd_from(q3));
-}
-__host__ __device__ __forceinline__ dd dd_div_d(dd a, double b) { return dd_div(a, dd_from(b)); }
-__host__ __device__ __forceinline__ bool dd_gt(dd a, double b) {
-  return a.hi > b || (a.hi == b && a.lo > 0.0);
-}
-
 // --- transition-regime condensation (util.h:19-68), all double -------------
 __host__ __device__ __forceinline__ double mp_transition_corr(double mfp, double r, double alpha) {
   double kn = mfp / r;
@@ -65,7 +35,7 @@ __host__ __device__ __forceinline__ double mp_cond_rate(double D, double mfp, do
 __host__ __device__ __forceinline__ double mp_dcond_rate_dr(double D, double mfp, double r, double alpha) {
   double kn = mfp / r;
   double dcorr = (0.75 * alpha * mfp * (kn * kn + 2.0 * kn + 1.0 + (0.283 - 0.75) * alpha)) /
-                 pow(r * (kn * kn + (1.0 + 0.283 * alpha) * kn + 0.75 * alpha), 2.0);
+                 cr_pow(r * (kn * kn + (1.0 + 0.283 * alpha) * kn + 0.75 * alpha), 2.0);
   return 4.0 * kPi * D * (mp_transition_corr(mfp, r, alpha) + r * dcorr);
 }
 
@@ -100,7 +70,7 @@ __host__ __device__ __forceinline__ void mp_phase_update(const DevProgram& P, in
   a[0] = mass_d;
   a[1] = dd_round(dd_div(dd_from(mass_d), moles));
   double r = P.pi_dp[i] / 2.0;
-  a[2] = vol * 3.0 / (4.0 * kPi) / pow(r, 3.0);
+  a[2] = vol * 3.0 / (4.0 * kPi) / cr_pow(r, 3.0);
 }
 
 // sums of the state after the sub models ran (the getters' partials see it)
@@ -126,13 +96,13 @@ __host__ __device__ __forceinline__ void mp_zsr_water(const DevProgram& P, int z
     dd jaw = dd_gt(aw, low) ? aw : dd_from(low);
     double jaw_d = dd_round(jaw);
     dd mol = dd_from(0.0);
-    for (int k = 0; k < P.zp_ny[x]; ++k) mol = dd_add(mol, dd_from(P.zp_y[8 * x + k] * pow(jaw_d, (double)k)));
+    for (int k = 0; k < P.zp_ny[x]; ++k) mol = dd_add(mol, dd_from(P.zp_y[8 * x + k] * cr_pow(jaw_d, (double)k)));
     mol = dd_mul(mol, mol);
     // ion concentrations are double expressions (state / int / MW / 1000)
     dd cat = dd_from(s_state[P.zp_cat[x]] / (double)P.zp_ncat[x] / P.zp_catmw[x] / 1000.0);
     dd an = dd_from(s_state[P.zp_an[x]] / (double)P.zp_nan[x] / P.zp_anmw[x] / 1000.0);
-    dd e_ac = dd_from(exp(-100.0 * cat.hi));
-    dd e_aa = dd_from(exp(-100.0 * an.hi));
+    dd e_ac = dd_from(cr_exp(-100.0 * cat.hi));
+    dd e_aa = dd_from(cr_exp(-100.0 * an.hi));
     dd conc = dd_div(dd_add(dd_mul(cat, e_ac), dd_mul(an, e_aa)), dd_add(e_ac, e_aa));
     water = dd_round(dd_add(dd_from(water), dd_mul(dd_div(conc, mol), dd_from(1000.0))));
   }
@@ -153,16 +123,16 @@ __host__ __device__ __forceinline__ void mp_zsr_jac(const DevProgram& P, int z, 
     const double* Y = P.zp_y + 8 * x;
     dd mol = dd_from(Y[0]), dmol = dd_from(0.0);
     for (int k = 1; k < P.zp_ny[x]; ++k) {
-      mol = dd_add(mol, dd_from(Y[k] * pow(jaw_d, (double)k)));
-      dmol = dd_add(dmol, dd_from(Y[k] * k * pow(jaw_d, (double)(k - 1))));
+      mol = dd_add(mol, dd_from(Y[k] * cr_pow(jaw_d, (double)k)));
+      dmol = dd_add(dmol, dd_from(Y[k] * k * cr_pow(jaw_d, (double)(k - 1))));
     }
     dmol = dd_mul(dmol, djaw);
     dd cat = dd_from(s_state[P.zp_cat[x]] / (double)P.zp_ncat[x] / P.zp_catmw[x] / 1000.0);
     double dcat = 1.0 / (double)P.zp_ncat[x] / P.zp_catmw[x] / 1000.0;
     dd an = dd_from(s_state[P.zp_an[x]] / (double)P.zp_nan[x] / P.zp_anmw[x] / 1000.0);
     double dan = 1.0 / (double)P.zp_nan[x] / P.zp_anmw[x] / 1000.0;
-    dd e_ac = dd_from(exp(-100.0 * cat.hi));
-    dd e_aa = dd_from(exp(-100.0 * an.hi));
+    dd e_ac = dd_from(cr_exp(-100.0 * cat.hi));
+    dd e_aa = dd_from(cr_exp(-100.0 * an.hi));
     dd conc = dd_div(dd_add(dd_mul(cat, e_ac), dd_mul(an, e_aa)), dd_add(e_ac, e_aa));
     dd den = dd_mul(dd_add(e_ac, e_aa), dd_add(e_ac, e_aa));
     // (1 - ALPHA*anion + ALPHA*cation), ALPHA = -100
@@ -171,7 +141,7 @@ __host__ __device__ __forceinline__ void mp_zsr_jac(const DevProgram& P, int z, 
     dd dcdc = dd_div(dd_add(dd_mul(e_ac, e_ac), dd_mul(dd_mul(e_ac, e_aa), kc)), den);
     dd dcda = dd_div(dd_add(dd_mul(e_aa, e_aa), dd_mul(dd_mul(e_ac, e_aa), ka)), den);
     double mol_d = dd_round(mol);
-    double p3 = pow(mol_d, 3.0), p2 = pow(mol_d, 2.0);
+    double p3 = cr_pow(mol_d, 3.0), p2 = cr_pow(mol_d, 2.0);
     // J[...] += value: each a double += (long double) rounded to double
     // -2.0 * conc / pow(m, 3) * 1000.0 * dm and 1.0 / pow(m, 2) * 1000.0 * dc * dion,
     // evaluated left to right as the reference does
@@ -356,7 +326,7 @@ __host__ __device__ __forceinline__ dd mp_jac_records(const DevProgram& P, int e
           int pi = P.sp_pi[a.y];
           const double* ag = s_aero + kAeroSlots * pi;
           double dens = P.pm_dens[a.z], mw = P.pm_mw[a.z];
-          double factor = 3.0 / (4.0 * kPi) / pow(P.pi_dp[pi] / 2.0, 3.0);
+          double factor = 3.0 / (4.0 * kPi) / cr_pow(P.pi_dp[pi] / 2.0, 3.0);
           double numj = 1.0 / dens * factor;
           double massj = 1.0;
           dd lm = {ag[3], ag[4]}, mo = {ag[5], ag[6]};

@@ -324,7 +324,7 @@ struct BlockCv {
          etaqm1 = 0, etaqp1 = 0, hu = 0, init_h = 0;
   double l[6] = {0}, tq[6] = {0}, tau[8] = {0};
   double rl1 = 0, gamma = 0, gammap = 1, gamrat = 1, crate = 1, acnrm = 0, saved_tq5 = 0;
-  int q = 1, qprime = 1, L = 2, qwait = 2, jcur = 0, use_acinit = 0;
+  int q = 1, qprime = 1, L = 2, qwait = 2, jcur = 0, use_acinit = 0, projected = 0;
   long nst = 0, nfe = 0, nsetups = 0, nni = 0, ncfn = 0, netf = 0, nje = 0, nstlp = 0,
        nstlj = 0;
 
@@ -359,6 +359,16 @@ struct BlockCv {
     HIP_OK(hipStreamSynchronize(s));
     double v = S.h_red[0];
     for (int i = 1; i < kRedBlocks; ++i) v = std::min(v, S.h_red[i]);
+    return v;
+  }
+  // max over i with y_i < -SMALL of -y_i * ewt_i, or -1 when there is none
+  double vnegmax(const double* x) {
+    hipLaunchKernelGGL(k_vec_negmax, dim3(kRedBlocks), dim3(256), 0, s, n, x, ewt, S.d_red);
+    HIP_OK(hipMemcpyAsync(S.h_red.data(), S.d_red, kRedBlocks * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    double v = S.h_red[0];
+    for (int i = 1; i < kRedBlocks; ++i) v = std::max(v, S.h_red[i]);
     return v;
   }
   void ewt_set(const double* ycur) {
@@ -573,9 +583,14 @@ struct BlockCv {
       if (m > 0) crate = std::fmax(CRDOWN * crate, del / delp);
       double dcon = del * std::fmin(1.0, crate) / tq[4];
       if (dcon <= 1.0) {
-        // non-negativity safeguard (same rule as the per-cell kernel)
-        if (vmin(y) < -1.0e-30) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
-        acnrm = (m == 0) ? del : wrms(acor);
+        // non-negativity safeguard (same rule as the per-cell kernel):
+        // project excursions within tolerance to 0, reject larger ones
+        double worst = vnegmax(y);
+        if (worst > 1.0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        if (worst >= 0.0)
+          hipLaunchKernelGGL(k_vec_project, dim3(nblk(n)), dim3(256), 0, s, n, zn[0], acor, y);
+        projected = worst >= 0.0;
+        acnrm = (m == 0 && worst < 0.0) ? del : wrms(acor);
         jcur = 0;
         return CV_SUCCESS;
       }
@@ -680,7 +695,7 @@ struct BlockCv {
     Coef6 c6;
     for (int j = 0; j < 6; ++j) c6.v[j] = l[j];
     hipLaunchKernelGGL(k_vec_complete, dim3(nblk(n)), dim3(256), 0, s, n, q, c6, acor, S.d_zn,
-                       save, last);
+                       save, last, projected);
     if (save) saved_tq5 = tq[5];
   }
   void set_eta() {
@@ -1269,4 +1284,15 @@ extern "C" void camp_solver_get_rxn_env(void* solver_data, double* rxn_env) {
   s->ensure_device();
   HIP_OK(hipMemcpy(rxn_env, s->d_renv, s->m.rxn_env.size() * sizeof(double),
                    hipMemcpyDeviceToHost));
+}
+
+extern "C" void camp_cr_math(int fn, int n, const double* x, const double* y, double* out) {
+  for (int i = 0; i < n; ++i) {
+    switch (fn) {
+      case 0: out[i] = camp::cr_exp(x[i]); break;
+      case 1: out[i] = camp::cr_log(x[i]); break;
+      case 2: out[i] = camp::cr_log10(x[i]); break;
+      default: out[i] = camp::cr_pow(x[i], y[i]); break;
+    }
+  }
 }

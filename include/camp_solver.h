@@ -200,6 +200,12 @@ int camp_solver_time_fj(void *solver_data, const double *d_state,
  * n_rxn_env_param: rate constants and base rates) as last computed. */
 void camp_solver_get_rxn_env(void *solver_data, double *rxn_env);
 
+/* Test hook: the host build of the correctly rounded elementary functions
+ * that the device code uses in place of the C library's in the restated
+ * rate-constant, SIMPOL, aqueous-equilibrium and ZSR formulas.
+ * fn 0 exp(x), 1 log(x), 2 log10(x), 3 pow(x, y); n values. */
+void camp_cr_math(int fn, int n, const double *x, const double *y, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1784,7 +1784,7 @@ typedef struct {
   double tn, h, hscale, hprime, next_h, eta, etamax, etaq, etaqm1, etaqp1, hu;
   double l[L_MAX + 1], tq[6], tau[L_MAX + 2];
   double rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5, init_h;
-  int q, qprime, next_q, L, qwait, qu, indx_acor, jcur, use_acor_init;
+  int q, qprime, next_q, L, qwait, qu, indx_acor, jcur, use_acor_init, projected;
   long nst, nfe, nsetups, nni, ncfn, netf, nje, nstlp, nstlj, nscon;
 } Cv;
 
@@ -2054,14 +2054,30 @@ static int cv_newton_iteration(Cv *cv) {
     if (dcon <= 1.0) {
       /* Non-negativity safeguard (not in CVODE): CAMP's f() rejects
        * y < -SMALL (camp_solver.c:816-822), so a converged iterate below
-       * that would make every later f() at the prediction fail; it is
-       * treated as the recoverable f() failure it implies. */
+       * that would make every later f() at the prediction fail.  Components
+       * below -SMALL by at most their error weight's tolerance
+       * (-y_i * ewt_i <= 1) are projected to 0 through the correction;
+       * a larger excursion is the recoverable f() failure it implies. */
+      int projected = 0;
       if (cv->o->neg_mode == 1) {
-        double mn = cv->y[0];
-        for (int i = 1; i < cv->n; ++i) mn = cv->y[i] < mn ? cv->y[i] : mn;
-        if (mn < -SMALL) return cv->jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        double worst = -1.0;
+        for (int i = 0; i < cv->n; ++i)
+          if (cv->y[i] < -SMALL) {
+            double r = -cv->y[i] * cv->ewt[i];
+            worst = r > worst ? r : worst;
+          }
+        if (worst > 1.0) return cv->jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        if (worst >= 0.0) {
+          for (int i = 0; i < cv->n; ++i)
+            if (cv->y[i] < -SMALL) {
+              cv->acor[i] = -cv->zn[0][i];
+              cv->y[i] = 0.0;
+            }
+          projected = 1;
+        }
       }
-      cv->acnrm = (m == 0) ? del : wrms(cv, cv->acor);
+      cv->projected = projected;
+      cv->acnrm = (m == 0 && !projected) ? del : wrms(cv, cv->acor);
       cv->jcur = 0;
       return CV_SUCCESS;
     }
@@ -2276,6 +2292,12 @@ static int cv_step(Cv *cv) {
     break;
   }
   cv_complete_step(cv);
+  /* a component projected to 0 restarts flat: its history is cleared so
+   * that the next prediction is 0 rather than a negative extrapolation */
+  if (cv->projected)
+    for (int i = 0; i < cv->n; ++i)
+      if (cv->zn[0][i] == 0.0)
+        for (int j = 1; j <= cv->q; ++j) cv->zn[j][i] = 0.0;
   cv_prepare_next_step(cv, dsm);
   cv->etamax = (cv->nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
   for (int i = 0; i < cv->n; ++i) cv->acor[i] *= cv->tq[2];

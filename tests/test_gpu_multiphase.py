@@ -11,6 +11,10 @@ from camp_amd.multiphase import multiphase_cells, multiphase_mechanism
 pytestmark = pytest.mark.gpu
 
 F_RTOL = 1e-12
+# Below ~1e-290 the low word of the device's double-double accumulators is
+# subnormal, so magnitudes there (physically zero) carry fewer digits than
+# the 80-bit long double the solver accumulates in.
+F_FLOOR = 1e-280
 
 
 @pytest.fixture(scope="module", params=["small", "cfg4"])
@@ -68,9 +72,9 @@ def test_multiphase_deriv_matches_oracle(oracle_lib, case, use_est):
     assert rc == 0
     f_ref = np.stack([orc.eval_f(c, st[c], 10.0, use_est)[1] for c in range(n)])
     err = np.abs(f_gpu - f_ref)
-    worst = float(np.max(err / np.maximum(np.abs(f_ref), 1e-300)))
+    worst = float(np.max(err / (np.abs(f_ref) + F_FLOOR / F_RTOL)))
     print("multi-phase f worst rel", worst)
-    assert (err <= F_RTOL * np.abs(f_ref)).all(), worst
+    assert (err <= F_RTOL * np.abs(f_ref) + F_FLOOR).all(), worst
 
 
 def test_multiphase_jac_matches_oracle(oracle_lib, case):
@@ -83,9 +87,9 @@ def test_multiphase_jac_matches_oracle(oracle_lib, case):
     orc.update_env(cells.env[:n])
     j_ref = np.stack([orc.eval_jac(c, st[c], 10.0)[1] for c in range(n)])
     err = np.abs(j_gpu - j_ref)
-    worst = float(np.max(err / np.maximum(np.abs(j_ref), 1e-300)))
+    worst = float(np.max(err / (np.abs(j_ref) + F_FLOOR / F_RTOL)))
     print("multi-phase J worst rel", worst)
-    assert (err <= F_RTOL * np.abs(j_ref)).all(), worst
+    assert (err <= F_RTOL * np.abs(j_ref) + F_FLOOR).all(), worst
 
 
 def test_multiphase_per_cell_solve_matches_oracle(oracle_lib, case):
