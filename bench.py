@@ -132,6 +132,37 @@ def serving(mech, solver, pool_state, env, rates, n_requests, seed=3):
                            "mechanism only (the multi-phase mechanism is not built)"}
 
 
+def plain_path(mech, n_cells, steps, dt, persistent_value):
+    """The plain GPU path: one BDF over all cells (shared step size and
+    order, block-diagonal Newton) driven from the host with one launch per
+    f(y), J(y), LU, solve and vector operation; the first n_cells cells of
+    the same workload, one untimed step from the seeded state, then `steps`
+    timed steps."""
+    import torch
+    from camp_amd.core import CampSolver, MODE_BLOCK
+    _, cells = make_inputs(n_cells)
+    dev = torch.device("cuda:0")
+    st = torch.from_numpy(cells.state).to(dev)
+    env = torch.from_numpy(cells.env).to(dev)
+    s = CampSolver(mech, n_cells, mode=MODE_BLOCK)
+    s.set_rates(cells.rates)
+    progress("plain path: %d cells" % n_cells)
+    s.solve_device(st.data_ptr(), env.data_ptr(), 0.0, dt)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    failed = 0
+    for _ in range(steps):
+        failed += int(s.solve_device(st.data_ptr(), env.data_ptr(), 0.0, dt) != 0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    v = n_cells * steps / el
+    progress("plain path done (%.0f ms/step)" % (1e3 * el / steps))
+    return {"value": v, "unit": "cell-timesteps/s", "n_cells": n_cells, "steps": steps,
+            "ms_per_step": 1e3 * el / steps, "failed_steps": failed,
+            "persistent_over_plain": persistent_value / v,
+            "what": "block mode: one BDF over all cells, one launch per f/J/LU/solve"}
+
+
 def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -147,6 +178,9 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-cells-per-core", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-cells", type=int, default=10000,
+                    help="cells of the plain one-launch-per-RHS leg (0: skip)")
+    ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=-1,
                     help="requests in the serving leg (-1: sized to about 40 s, 0: skip)")
     args = ap.parse_args()
@@ -179,6 +213,17 @@ def main():
         rc = solver.solve_device(st.data_ptr(), env.data_ptr(), 0.0, args.dt)
         warm_failed += int(np.sum(solver.cell_stats()[:, 7] < 0))
         progress("warmup step done (%.0f ms kernel)" % solver.last_kernel_ms())
+    torch.cuda.synchronize()
+    tune = None
+    if not args.cells_per_block:
+        # cells per workgroup picked by the native autotuner on a sample of
+        # the spun-up cells (copies; the workload's cells are untouched)
+        chosen, cand_ms = solver.autotune(st.data_ptr(), env.data_ptr(), 0.0, args.dt,
+                                          n_sample=min(args.cells, 16384))
+        tune = {"cells_per_workgroup": chosen,
+                "candidate_ms": {str(k): v for k, v in cand_ms.items()},
+                "sample_cells": min(args.cells, 16384)}
+        progress("autotuned cells per workgroup: %d %s" % (chosen, cand_ms))
     torch.cuda.synchronize()
     step_ms, kern_ms, stats = [], [], []
     failed_cell_steps = 0
@@ -228,6 +273,10 @@ def main():
                 "frac": gbs_j / HBM_PEAK_GBS, "avg_launch_ms": ms_j,
                 "bytes_per_cell": bytes_["jac"]}}
 
+    plain = None
+    if args.plain_cells > 0:
+        plain = plain_path(mech, args.plain_cells, args.plain_steps, args.dt, value)
+
     serve = None
     if args.serve_requests != 0:
         n_req = args.serve_requests
@@ -269,6 +318,8 @@ def main():
                                        "lu_setups": float(nsetups / args.cells / args.steps),
                                        "newton_iters": float(nni / args.cells / args.steps),
                                        "jac_evals": float(nje / args.cells / args.steps)},
+        "plain_path": plain,
+        "autotune": tune,
         "roofline": roofline,
         "roofline_fj": roofline_fj,
         "cpu_baseline": cpu,

@@ -116,6 +116,16 @@ class CampSolver:
     def set_cells_per_block(self, n):
         return lib().camp_solver_set_cells_per_block(self.h, n)
 
+    def autotune(self, state_ptr, env_ptr, t0, t1, n_sample=0):
+        """Pick cells per workgroup for the per-cell kernel by timing each
+        candidate on a sample of the device-resident cells (copies).
+        Returns (chosen, {candidate: ms or None when it does not fit})."""
+        ms = np.zeros(6)
+        best = lib().camp_solver_autotune(self.h, C.c_void_p(state_ptr), C.c_void_p(env_ptr),
+                                          float(t0), float(t1), int(n_sample), _dp(ms))
+        return best, {c: (float(v) if v >= 0 else None)
+                      for c, v in zip((1, 2, 4, 8, 16, 32), ms)}
+
     def set_guess_helper(self, on):
         lib().camp_solver_set_guess_helper(self.h, int(bool(on)))
 

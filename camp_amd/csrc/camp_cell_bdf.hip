@@ -324,6 +324,30 @@ struct CellCv {
     }
   }
 
+  // Non-negativity safeguard (not in CVODE): CAMP's f() rejects y < -SMALL
+  // (camp_solver.c:816-822), and cutting the step is CVODE's only answer,
+  // so a species driven to ~0 by a fast loss stalls the integration.
+  // Wherever f() would see the iterate y = zn0 + corr, components below
+  // -SMALL by at most their tolerance (-y_i * ewt_i <= 1) are projected to
+  // TINY (solver_run's floor, camp_solver.c:554-567) through corr.
+  // Returns -1 (nothing below -SMALL), 0 (projected) or 1 (a larger
+  // excursion: the recoverable f() failure it implies); wave-uniform.
+  __device__ __forceinline__ int project(double* corr) {
+    double worst = -1.0;
+    for (int i = lane; i < nv; i += kWave)
+      if (y[i] < -1.0e-30) worst = fmax(worst, -y[i] * ewt[i]);
+    worst = -wave_min(-worst);
+    if (worst > 1.0) return 1;
+    if (worst < 0.0) return -1;
+    for (int i = lane; i < nv; i += kWave)
+      if (y[i] < -1.0e-30) {
+        corr[i] = 1.0e-60 - zn(0)[i];
+        y[i] = 1.0e-60;
+      }
+    wsync();
+    return 0;
+  }
+
   __device__ __forceinline__ int newton_iteration() {
     int m = 0;
     double del = 0.0, delp = 0.0;
@@ -345,33 +369,17 @@ struct CellCv {
       if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
       double dcon = del * fmin(1.0, crate) / tq[4];
       if (dcon <= 1.0) {
-        // Non-negativity safeguard: CAMP's f() rejects y < -SMALL
-        // (camp_solver.c:816-822); a converged iterate below that would make
-        // the next step's first f() fail at every step size.  Components
-        // below -SMALL by at most their tolerance (-y_i * ewt_i <= 1) are
-        // projected to 0 through the correction; a larger excursion is the
-        // recoverable f() failure it implies.
-        double worst = -1.0;
-        for (int i = lane; i < nv; i += kWave)
-          if (y[i] < -1.0e-30) worst = fmax(worst, -y[i] * ewt[i]);
-        worst = -wave_min(-worst);
-        if (worst > 1.0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
-        if (worst >= 0.0) {
-          for (int i = lane; i < nv; i += kWave)
-            if (y[i] < -1.0e-30) {
-              acor[i] = -zn(0)[i];
-              y[i] = 0.0;
-            }
-          wsync();
-        }
-        projected = worst >= 0.0;
-        acnrm = (m == 0 && worst < 0.0) ? del : wrms(acor);
+        int pr = project(acor);
+        if (pr > 0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        projected = pr == 0;
+        acnrm = (m == 0 && pr < 0) ? del : wrms(acor);
         jcur = 0;
         return CV_SUCCESS;
       }
       m++;
       if ((m == MAXCOR) || ((m >= 2) && (del > RDIV * delp))) return jcur ? CONV_FAIL : TRY_AGAIN;
       delp = del;
+      if (project(acor) > 0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
       if (f(y, ftemp)) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
     }
   }
@@ -394,6 +402,16 @@ struct CellCv {
     for (;;) {
       for (int i = lane; i < nv; i += kWave) y[i] = use_acinit ? zn(0)[i] + acinit[i] : zn(0)[i];
       wsync();
+      double mn = 1.0e308;
+      for (int i = lane; i < nv; i += kWave) mn = fmin(mn, y[i]);
+      if (wave_min(mn) < -1.0e-30) {
+        if (!use_acinit) {
+          for (int i = lane; i < nv; i += kWave) acinit[i] = 0.0;
+          wsync();
+          use_acinit = 1;
+        }
+        project(acinit);
+      }
       if (f(y, ftemp)) return RHSFUNC_RECVR;
       if (callSetup) {
         int ier = lsetup(convfail, y, ftemp);
@@ -480,9 +498,11 @@ struct CellCv {
       double a = acor[i];
       for (int j = 0; j <= q; ++j) zn(j)[i] += l[j] * a;
       if (save) zn(QMAX)[i] = a;
-      // after a projection, components at 0 restart flat
-      if (projected && zn(0)[i] == 0.0)
+      // after a projection, components at TINY restart flat
+      if (projected && y[i] == 1.0e-60) {
+        zn(0)[i] = 1.0e-60;
         for (int j = 1; j <= q; ++j) zn(j)[i] = 0.0;
+      }
       last[i] = zn(0)[i];
     }
     if (save) saved_tq5 = tq[5];
@@ -629,7 +649,7 @@ struct CellCv {
 
 }  // namespace
 
-__global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
+__global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   extern __shared__ double smem[];
   const DevProgram& P = a.P;
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
@@ -660,6 +680,9 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
   cv.gh1 = v; v += nv;
   cv.gh2 = v; v += nv;
   cv.w = v; v += nv;
+  // Gas-phase cells keep their J history (J_solver, J_state, J_deriv) in
+  // LDS for the whole solve; multi-phase cells use it in place in HBM.
+  double *h_js = nullptr, *h_jy = nullptr, *h_jf = nullptr;
   if (P.mp) {
     cv.lu = scratch;
     cv.Jr = scratch + P.lu_nnz;
@@ -667,6 +690,9 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
     cv.lu = v;
     cv.Jr = nullptr;
     v += P.lu_nnz;
+    h_js = v; v += P.nnz;
+    h_jy = v; v += nv;
+    h_jf = v; v += nv;
   }
   cv.l = v;
   cv.tq = v + 6;
@@ -686,9 +712,23 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
     double t1 = a.cell_t1 ? a.cell_t1[cell] : a.t1;
     double* state_c = a.state + (size_t)cell * P.ns;
     double* renv_c = a.rxn_env + (size_t)cell * P.n_env;
-    cv.Js = a.Js + (size_t)cell * P.nnz;
-    cv.Jy = a.Jy + (size_t)cell * nv;
-    cv.Jf = a.Jf + (size_t)cell * nv;
+    double* g_js = a.Js + (size_t)cell * P.nnz;
+    double* g_jy = a.Jy + (size_t)cell * nv;
+    double* g_jf = a.Jf + (size_t)cell * nv;
+    if (P.mp) {
+      cv.Js = g_js;
+      cv.Jy = g_jy;
+      cv.Jf = g_jf;
+    } else {
+      for (int e = lane; e < P.nnz; e += kWave) h_js[e] = g_js[e];
+      for (int i = lane; i < nv; i += kWave) {
+        h_jy[i] = g_jy[i];
+        h_jf[i] = g_jf[i];
+      }
+      cv.Js = h_js;
+      cv.Jy = h_jy;
+      cv.Jf = h_jf;
+    }
     // stage state (constants clamped in place) and rate data
     for (int i = lane; i < P.ns; i += kWave) {
       double s = state_c[i];
@@ -703,7 +743,7 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
     wsync();
     // rate constants for the cell's environment (rxn_update_env_state)
     double T = a.env[2 * cell], Pa = a.env[2 * cell + 1];
-    for (int r = lane; r < P.n_rxn; r += kWave) rate_constant(P, r, T, Pa, cv.s_k);
+    cell_rate_constants(P, lane, T, Pa, cv.s_k);
     wsync();
     if (P.mp) mp_cell_env(P, lane, T, Pa, cv.mpl);
     for (int i = lane; i < P.n_env; i += kWave) renv_c[i] = cv.s_k[i];
@@ -729,6 +769,14 @@ __global__ void __launch_bounds__(256, 2) k_cell_bdf(CellBdfArgs a) {
           mp_zsr_water(P, z, cv.mpl.sub_env, cv.s_state);
           state_c[P.zi_water[z]] = cv.s_state[P.zi_water[z]];
         }
+      }
+    }
+    if (!P.mp) {
+      wsync();
+      for (int e = lane; e < P.nnz; e += kWave) g_js[e] = h_js[e];
+      for (int i = lane; i < nv; i += kWave) {
+        g_jy[i] = h_jy[i];
+        g_jf[i] = h_jf[i];
       }
     }
     if (lane == 0) {
@@ -761,6 +809,7 @@ void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipS
 }
 
 int cell_bdf_blocks_per_cu(int block, size_t lds) {
+  if (block > 256 || lds > 160 * 1024) return 0;
   allow_lds(lds);
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cell_bdf, block, lds) != hipSuccess) return 0;

@@ -32,7 +32,7 @@ struct DevProgram {
   const int *l_ptr, *l_pos, *l_row, *u_ptr, *u_pos, *u_row, *up_ptr;
   const uint16_t *up_dst, *up_l, *up_u;
   const int *lu_rowptr, *lu_col;
-  const uint8_t* row_off;  // nullptr when rows exceed 254 entries or nv > 128
+  const uint64_t* row_mask;  // 4 per LU row; nullptr when nv > 128
   const int4 *fe_a, *je_a;  // packed gather records (see Program in camp_model.hpp)
   const int *fe_b, *je_b;
   const double *fe_c, *je_c;
@@ -148,6 +148,12 @@ __host__ __device__ __forceinline__ void rate_constant(const DevProgram& P, int 
   } else {  // PHOTOLYSIS, EMISSION, FIRST_ORDER_LOSS: SCALING * BASE_RATE
     e[0] = f[0] * e[1];
   }
+}
+
+// All rate constants of one cell, lanes over reactions.
+__device__ __forceinline__ void cell_rate_constants(const DevProgram& P, int lane, double T, double Pa,
+                                                 double* s_k) {
+  for (int r = lane; r < P.n_rxn; r += kWave) rate_constant(P, r, T, Pa, s_k);
 }
 
 // ---------------------------------------------------------------------
@@ -358,46 +364,73 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
+__device__ __forceinline__ bool bit128(uint64_t lo, uint64_t hi, int k) {
+  return ((k < 64 ? lo : hi) >> (k & 63)) & 1ull;
+}
+
 // Triangular solves with the solution vector in registers (nv <= 128):
 // lane t owns permuted rows t and t+64.  Pivot k's value is broadcast with
 // v_readlane (no LDS round trip, no barrier), so the dependency chain per
-// pivot is one readlane and one FMA; the LU value and the row offset each
-// lane needs do not depend on the solution and are loaded off the chain.
+// pivot is one readlane and one FMA.  Each lane walks its rows' stored
+// entries in order: whether row r has an entry in column k is a bit of the
+// row's 128-bit L or U mask, and the entry's position is a running count,
+// so the LU value each lane needs is an LDS load that depends on neither
+// the solution nor a table lookup and is issued ahead of the chain.  The
+// diagonal is applied as a reciprocal by the pivot's own lane.
 __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane, const double* lu,
                                                   double* b) {
   const int nv = P.nv;
   const int r0 = lane, r1 = lane + kWave;
   const bool has0 = r0 < nv, has1 = r1 < nv;
-  double x0 = has0 ? b[P.perm[r0]] : 0.0;
-  double x1 = has1 ? b[P.perm[r1]] : 0.0;
-  const double* row0 = lu + (has0 ? P.lu_rowptr[r0] : 0);
-  const double* row1 = lu + (has1 ? P.lu_rowptr[r1] : 0);
-  const uint8_t* off0 = P.row_off + (size_t)(has0 ? r0 : 0) * nv;
-  const uint8_t* off1 = P.row_off + (size_t)(has1 ? r1 : 0) * nv;
-#pragma unroll 4
-  for (int k = 0; k < nv - 1; ++k) {
-    int o0 = off0[k], o1 = off1[k];
-    bool a0 = has0 && r0 > k && o0 != 255, a1 = has1 && r1 > k && o1 != 255;
-    double l0 = a0 ? row0[o0] : 0.0;
-    double l1 = a1 ? row1[o1] : 0.0;
-    double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
-    x0 = a0 ? x0 - l0 * xk : x0;
-    x1 = a1 ? x1 - l1 * xk : x1;
+  uint64_t l0a = 0, l0b = 0, u0a = 0, u0b = 0, l1a = 0, l1b = 0, u1a = 0, u1b = 0;
+  int base0 = 0, base1 = 0, len0 = 1, len1 = 1;
+  double x0 = 0.0, x1 = 0.0;
+  if (has0) {
+    const uint64_t* m = P.row_mask + 4 * r0;
+    l0a = m[0]; l0b = m[1]; u0a = m[2]; u0b = m[3];
+    base0 = P.lu_rowptr[r0];
+    len0 = P.lu_rowptr[r0 + 1] - base0;
+    x0 = b[P.perm[r0]];
   }
-#pragma unroll 4
+  if (has1) {
+    const uint64_t* m = P.row_mask + 4 * r1;
+    l1a = m[0]; l1b = m[1]; u1a = m[2]; u1b = m[3];
+    base1 = P.lu_rowptr[r1];
+    len1 = P.lu_rowptr[r1 + 1] - base1;
+    x1 = b[P.perm[r1]];
+  }
+  const double* row0 = lu + base0;
+  const double* row1 = lu + base1;
+  int o0 = 0, o1 = 0;
+#pragma unroll 8
+  for (int k = 0; k < nv - 1; ++k) {
+    bool a0 = bit128(l0a, l0b, k), a1 = bit128(l1a, l1b, k);
+    double v0 = row0[o0], v1 = row1[o1];
+    double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
+    x0 = a0 ? x0 - v0 * xk : x0;
+    x1 = a1 ? x1 - v1 * xk : x1;
+    o0 += a0;
+    o1 += a1;
+  }
+  // o0, o1 now index the diagonal of each row
+  double d0 = has0 ? 1.0 / row0[o0] : 0.0;
+  double d1 = has1 ? 1.0 / row1[o1] : 0.0;
+  o0 = len0 - 1;
+  o1 = len1 - 1;
+#pragma unroll 8
   for (int k = nv - 1; k >= 0; --k) {
-    int o0 = off0[k], o1 = off1[k];
-    bool a0 = has0 && r0 < k && o0 != 255, a1 = has1 && r1 < k && o1 != 255;
-    double u0 = a0 ? row0[o0] : 0.0;
-    double u1 = a1 ? row1[o1] : 0.0;
-    double xk = (k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave)) / lu[P.diag_pos[k]];
     if (k < kWave) {
-      if (lane == k) x0 = xk;
+      if (lane == k) x0 *= d0;
     } else if (lane == k - kWave) {
-      x1 = xk;
+      x1 *= d1;
     }
-    x0 = a0 ? x0 - u0 * xk : x0;
-    x1 = a1 ? x1 - u1 * xk : x1;
+    bool a0 = bit128(u0a, u0b, k), a1 = bit128(u1a, u1b, k);
+    double v0 = row0[o0], v1 = row1[o1];
+    double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
+    x0 = a0 ? x0 - v0 * xk : x0;
+    x1 = a1 ? x1 - v1 * xk : x1;
+    o0 -= a0;
+    o1 -= a1;
   }
   wsync();
   if (has0) b[P.perm[r0]] = x0;
@@ -408,7 +441,7 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
 // Solve (LU) x = b in place; b in dep order, w a scratch vector (nv).
 __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, const double* lu,
                                               double* b, double* w) {
-  if (P.row_off) {
+  if (P.row_mask) {
     cell_lu_solve_reg(P, lane, lu, b);
     return;
   }

@@ -176,18 +176,20 @@ __global__ void k_vec_rescale(long n, int q, double eta, double* __restrict__ zn
   }
 }
 
-// zn[j] += l[j] * acor for j = 0..q; after a projection, components at 0
-// restart flat (zn[1..q] cleared)
+// zn[j] += l[j] * acor for j = 0..q; after a projection, components
+// projected to TINY restart flat there (zn[1..q] cleared)
 __global__ void k_vec_complete(long n, int q, Coef6 l6, const double* __restrict__ acor,
                                double* __restrict__ zn, int save, double* __restrict__ last,
-                               int projected) {
+                               int projected, const double* __restrict__ y) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double a = acor[i];
   for (int j = 0; j <= q; ++j) zn[j * n + i] += l6.v[j] * a;
   if (save) zn[5 * n + i] = a;
-  if (projected && zn[i] == 0.0)
+  if (projected && y[i] == 1.0e-60) {
+    zn[i] = 1.0e-60;
     for (int j = 1; j <= q; ++j) zn[j * n + i] = 0.0;
+  }
   last[i] = zn[i];
 }
 
@@ -272,14 +274,23 @@ __global__ void __launch_bounds__(256) k_vec_negmax(long n, const double* __rest
   if (threadIdx.x == 0) out[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
-// y_i < -SMALL -> acor_i = -zn0_i, y_i = 0 (projection onto y >= 0)
+// y_i < -SMALL -> y_i = TINY through the correction (acor_i = TINY - zn0_i)
 __global__ void k_vec_project(long n, const double* __restrict__ zn0, double* __restrict__ acor,
                               double* __restrict__ y) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && y[i] < -1.0e-30) {
-    acor[i] = -zn0[i];
-    y[i] = 0.0;
+    acor[i] = 1.0e-60 - zn0[i];
+    y[i] = 1.0e-60;
   }
+}
+
+// dst[r][j] = src[r * stride][j] for r < n rows of len doubles
+__global__ void k_gather_rows(long n, int len, long stride, const double* __restrict__ src,
+                              double* __restrict__ dst) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * len) return;
+  long r = i / len, j = i - r * len;
+  dst[i] = src[r * stride * len + j];
 }
 
 // Guess-helper jump search: per block, min over i of the candidate t_star

@@ -18,7 +18,7 @@ constexpr int kCellProf = 5;    // shader cycles: f, Jac, LU factor, LU solve, w
 // programs) and 24 slots for the BDF coefficient arrays l, tq and tau.
 __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P) {
   return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 16 * (size_t)P.nv +
-         (P.mp ? 0 : (size_t)P.lu_nnz) + 24;
+         (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
 }
 // Global scratch doubles per wavefront of multi-phase programs: LU factors
 // and the reaction Jacobian.
@@ -77,7 +77,7 @@ struct Coef6 {
   double v[6];
 };
 __global__ void k_vec_complete(long n, int q, Coef6 l6, const double* acor, double* zn, int save,
-                               double* last, int projected);
+                               double* last, int projected, const double* y);
 __global__ void k_vec_resid(long n, double rl1, const double* zn1, const double* acor,
                             double gamma, const double* ftemp, double* tempv);
 __global__ void k_vec_newton_update(long n, const double* b, double* acor, const double* zn0,
@@ -87,6 +87,7 @@ __global__ void k_vec_ewt(long n, int nv, double rtol, const double* atol, const
 __global__ void k_vec_interp(long n, int q, double s, const double* zn, double* y);
 __global__ void k_vec_wsq(long n, const double* x, const double* w, double* out);
 __global__ void k_vec_min(long n, const double* x, double* out);
+__global__ void k_gather_rows(long n, int len, long stride, const double* src, double* dst);
 __global__ void k_vec_negmax(long n, const double* x, const double* w, double* out);
 __global__ void k_vec_project(long n, const double* zn0, double* acor, double* y);
 __global__ void k_vec_tstar(long n, const double* tmp1, const double* corr, double h_j,

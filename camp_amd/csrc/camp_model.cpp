@@ -674,19 +674,19 @@ std::string Model::initialize() {
   for (int k = 0; k < nv; ++k) P.diag_pos[k] = pos[(size_t)k * nv + k];
   P.lu_rowptr.assign(nv + 1, 0);
   P.lu_col.assign(P.lu_nnz, 0);
-  P.row_off.assign((size_t)nv * nv, 255);
+  P.row_mask.clear();
+  if (nv <= 128) P.row_mask.assign((size_t)4 * nv, 0);
   for (int i = 0; i < nv; ++i) {
-    int first = -1, cnt = 0;
+    int first = -1;
     for (int j = 0; j < nv; ++j) {
       int q = pos[(size_t)i * nv + j];
       if (q < 0) continue;
       if (first < 0) first = q;
       P.lu_col[q] = j;
-      if (cnt < 255) P.row_off[(size_t)i * nv + j] = (uint8_t)(q - first);
-      ++cnt;
+      if (nv <= 128 && j != i)
+        P.row_mask[(size_t)4 * i + (j < i ? 0 : 2) + (j >> 6)] |= 1ull << (j & 63);
     }
     P.lu_rowptr[i] = first;
-    if (cnt >= 255) P.row_off.clear();  // rows too long for 8-bit offsets
   }
   P.lu_rowptr[nv] = P.lu_nnz;
   P.l_ptr.assign(1, 0);

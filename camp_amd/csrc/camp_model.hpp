@@ -76,10 +76,11 @@ struct Program {
   std::vector<int> up_ptr;                          // per pivot k: update range
   std::vector<uint16_t> up_dst, up_l, up_u;         // LU[dst] -= LU[l] * LU[u]
   // Row-major LU storage: row i occupies [lu_rowptr[i], lu_rowptr[i+1]) in
-  // column order; row_off[i*nv + k] is the offset of (i, k) in its row, or
-  // 255 when (i, k) is structurally zero (requires rows of < 255 entries).
+  // column order.  For nv <= 128, row_mask[4i..4i+3] holds the columns of
+  // row i's strictly lower (L) part as a 128-bit mask (low, high word), then
+  // those of its strictly upper (U) part.
   std::vector<int> lu_rowptr, lu_col;
-  std::vector<uint8_t> row_off;
+  std::vector<uint64_t> row_mask;
   long n_updates() const { return (long)up_dst.size(); }
 
   // ---- multi-phase tables (empty for gas-phase models) -------------------

@@ -83,7 +83,7 @@ void Solver::setup_device() {
   P.je_c = dev_upload(G.je_c, owned);
   P.lu_rowptr = dev_upload(G.lu_rowptr, owned);
   P.lu_col = dev_upload(G.lu_col, owned);
-  P.row_off = (G.nv <= 2 * kWave && !G.row_off.empty()) ? dev_upload(G.row_off, owned) : nullptr;
+  P.row_mask = (G.nv <= 2 * kWave && !G.row_mask.empty()) ? dev_upload(G.row_mask, owned) : nullptr;
   std::vector<int> rx_type(m.n_rxn), rx_fidx(m.n_rxn), rx_eidx(m.n_rxn), rx_nreact(m.n_rxn);
   for (int r = 0; r < m.n_rxn; ++r) {
     int* id = m.rint(r);
@@ -569,6 +569,16 @@ struct BlockCv {
                        ((size_t)S.P.lu_nnz + 2 * (size_t)S.P.nv) * sizeof(double), s, S.P, nc,
                        S.d_LU, b, sc);
   }
+  // Non-negativity safeguard, the per-cell kernel's rule: components of y
+  // below -SMALL by at most their tolerance go to TINY through corr.
+  // Returns -1 (nothing below -SMALL), 0 (projected) or 1 (f() would fail).
+  int project(double* corr) {
+    double worst = vnegmax(y);
+    if (worst > 1.0) return 1;
+    if (worst < 0.0) return -1;
+    hipLaunchKernelGGL(k_vec_project, dim3(nblk(n)), dim3(256), 0, s, n, zn[0], corr, y);
+    return 0;
+  }
   int newton_iteration() {
     int m = 0;
     double del = 0.0, delp = 0.0;
@@ -583,20 +593,17 @@ struct BlockCv {
       if (m > 0) crate = std::fmax(CRDOWN * crate, del / delp);
       double dcon = del * std::fmin(1.0, crate) / tq[4];
       if (dcon <= 1.0) {
-        // non-negativity safeguard (same rule as the per-cell kernel):
-        // project excursions within tolerance to 0, reject larger ones
-        double worst = vnegmax(y);
-        if (worst > 1.0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
-        if (worst >= 0.0)
-          hipLaunchKernelGGL(k_vec_project, dim3(nblk(n)), dim3(256), 0, s, n, zn[0], acor, y);
-        projected = worst >= 0.0;
-        acnrm = (m == 0 && worst < 0.0) ? del : wrms(acor);
+        int pr = project(acor);
+        if (pr > 0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        projected = pr == 0;
+        acnrm = (m == 0 && pr < 0) ? del : wrms(acor);
         jcur = 0;
         return CV_SUCCESS;
       }
       m++;
       if ((m == MAXCOR) || ((m >= 2) && (del > RDIV * delp))) return jcur ? CONV_FAIL : TRY_AGAIN;
       delp = del;
+      if (project(acor) > 0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
       if (f(y, ftemp)) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
     }
   }
@@ -615,6 +622,13 @@ struct BlockCv {
     for (;;) {
       if (use_acinit) lin(1.0, zn[0], 1.0, acinit, y);
       else copy(y, zn[0]);
+      if (vmin(y) < -1.0e-30) {
+        if (!use_acinit) {
+          HIP_OK(hipMemsetAsync(acinit, 0, n * sizeof(double), s));
+          use_acinit = 1;
+        }
+        project(acinit);
+      }
       if (f(y, ftemp)) return RHSFUNC_RECVR;
       if (callSetup) {
         int ier = lsetup(convfail, y, ftemp);
@@ -695,7 +709,7 @@ struct BlockCv {
     Coef6 c6;
     for (int j = 0; j < 6; ++j) c6.v[j] = l[j];
     hipLaunchKernelGGL(k_vec_complete, dim3(nblk(n)), dim3(256), 0, s, n, q, c6, acor, S.d_zn,
-                       save, last, projected);
+                       save, last, projected, y);
     if (save) saved_tq5 = tq[5];
   }
   void set_eta() {
@@ -1035,10 +1049,103 @@ extern "C" int camp_solver_set_mode(void* solver_data, int mode) {
 }
 
 extern "C" int camp_solver_set_cells_per_block(void* solver_data, int cpb) {
+  using namespace camp;
   Solver* s = SD(solver_data);
   if (cpb == 0) cpb = 1;
-  if (cpb == 1 || cpb == 2 || cpb == 4) s->cells_per_block = cpb;
+  if (cpb == 1 || cpb == 2 || cpb == 4 || cpb == 8 || cpb == 16 || cpb == 32) {
+    size_t lds = (size_t)cpb * cell_bdf_lds_doubles(s->P) * sizeof(double);
+    if (!s->m.initialized || !s->stream || cell_bdf_blocks_per_cu(kWave * cpb, lds) > 0)
+      s->cells_per_block = cpb;
+  }
   return s->cells_per_block;
+}
+
+// Autotuner for the persistent per-cell kernel: every cells-per-workgroup
+// candidate in {1, 2, 4, 8, 16, 32} that fits a compute unit solves the same
+// sample of cells (every (nc / n_sample)-th cell, copied with its rate data
+// and J history, so the solver's own cells are untouched) over [t0, t1];
+// the fastest by HIP events on the solver's stream is kept.
+extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
+                                    const double* d_env, double t0, double t1, int n_sample,
+                                    double* ms_out) {
+  using namespace camp;
+  Solver* s = SD(solver_data);
+  if (!s->m.initialized) return 0;
+  s->ensure_device();
+  s->upload_renv();
+  const Model& m = s->m;
+  const DevProgram& P = s->P;
+  long nc = m.nc;
+  long ns = n_sample > 0 ? std::min<long>(n_sample, nc) : std::min<long>(nc, 8192);
+  long stride = std::max<long>(1, nc / ns);
+  std::vector<void*> tmp;
+  auto take = [&](long n) {
+    double* p = nullptr;
+    HIP_OK(hipMalloc(&p, std::max<long>(n, 1) * sizeof(double)));
+    tmp.push_back(p);
+    return p;
+  };
+  double* st = take(ns * P.ns);
+  double* env = take(ns * 2);
+  double* renv = take(ns * P.n_env);
+  double* js = take(ns * P.nnz);
+  double* jy = take(ns * P.nv);
+  double* jf = take(ns * P.nv);
+  double* stats = take(ns * kCellStats);
+  auto gather = [&](const double* src, int len, double* dst) {
+    long n = ns * len;
+    hipLaunchKernelGGL(k_gather_rows, dim3(nblk(n)), dim3(256), 0, s->stream, ns, len, stride,
+                       src, dst);
+  };
+  const int cand[6] = {1, 2, 4, 8, 16, 32};
+  int best = s->cells_per_block;
+  double best_ms = -1.0;
+  for (int k = 0; k < 6; ++k) {
+    int cpb = cand[k];
+    size_t lds = (size_t)cpb * cell_bdf_lds_doubles(P) * sizeof(double);
+    if (ms_out) ms_out[k] = -1.0;
+    if (cell_bdf_blocks_per_cu(kWave * cpb, lds) <= 0) continue;
+    gather(d_state, P.ns, st);
+    gather(d_env, 2, env);
+    gather(s->d_renv, P.n_env, renv);
+    gather(s->d_Js, P.nnz, js);
+    gather(s->d_Jy, P.nv, jy);
+    gather(s->d_Jf, P.nv, jf);
+    CellBdfArgs a = s->cell_args();
+    a.nc = (int)ns;
+    a.state = st;
+    a.env = env;
+    a.rxn_env = renv;
+    a.Js = js;
+    a.Jy = jy;
+    a.Jf = jf;
+    a.t0 = t0;
+    a.t1 = t1;
+    a.counter = s->d_counter;
+    a.stats = stats;
+    int keep = s->cells_per_block;
+    s->cells_per_block = cpb;
+    int grid = 0, block = 0;
+    s->cell_launch_shape(ns, &grid, &block, &lds);
+    s->cells_per_block = keep;
+    a.wave_scratch = s->wave_scratch((size_t)grid * (block / kWave));
+    HIP_OK(hipMemsetAsync(s->d_counter, 0, sizeof(int), s->stream));
+    HIP_OK(hipEventRecord(s->ev0, s->stream));
+    launch_cell_bdf(a, grid, block, lds, s->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    if (ms_out) ms_out[k] = ms;
+    if (best_ms < 0.0 || ms < best_ms) {
+      best_ms = ms;
+      best = cpb;
+    }
+  }
+  for (void* p : tmp) HIP_OK(hipFree(p));
+  s->cells_per_block = best;
+  return best;
 }
 
 extern "C" void camp_solver_set_guess_helper(void* solver_data, int on) { SD(solver_data)->guess_on = on != 0; }

@@ -116,9 +116,22 @@ void sub_model_add_condensed_data(int sub_model_type, int n_int_param,
 /* Select the integration mode; returns the previous mode. */
 int camp_solver_set_mode(void *solver_data, int mode);
 
-/* Cells per workgroup of the persistent kernel (1, 2, 4; one wavefront per
- * cell); 0 selects the built-in default.  Returns the value in effect. */
+/* Cells per workgroup of the persistent kernel (1, 2, 4, 8, 16, 32; one
+ * wavefront per cell; a value whose workgroup does not fit a compute unit
+ * is ignored); 0 selects the built-in default.  Returns the value in
+ * effect. */
 int camp_solver_set_cells_per_block(void *solver_data, int cells_per_block);
+
+/* Autotuner for the per-cell kernel: solves a sample of n_sample cells
+ * (every (n_cells / n_sample)-th; 0 = up to 8192) over [t_initial, t_final]
+ * from the device-resident state/env once per cells-per-workgroup candidate
+ * {1, 2, 4, 8, 16, 32} that fits a compute unit, on copies (the solver's own
+ * cells, rate data and J history are untouched), and keeps the fastest.
+ * ms_out[6] receives each candidate's kernel time (-1: does not fit).
+ * Returns the chosen cells per workgroup. */
+int camp_solver_autotune(void *solver_data, const double *d_state,
+                         const double *d_env, double t_initial, double t_final,
+                         int n_sample, double *ms_out);
 
 /* Enable (1, default) or disable (0) the negative-prediction guess helper
  * (camp_solver.c:1321-1408). */

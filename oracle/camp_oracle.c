@@ -2035,6 +2035,35 @@ static int cv_lsolve(Cv *cv, double *b) {
   return 0;
 }
 
+/* Non-negativity safeguard (not in CVODE).  CAMP's f() rejects
+ * y < -SMALL (camp_solver.c:816-822) as a recoverable failure, and CVODE
+ * answers that only by cutting the step, so a species driven to ~0 by a
+ * fast loss stalls the integration (a converged iterate just below -SMALL
+ * even makes every later prediction fail).  Wherever f() would see the
+ * iterate y = zn0 + corr (the prediction, each Newton iterate, and the
+ * converged value), components below -SMALL by at most their tolerance
+ * (-y_i * ewt_i <= 1) are projected to TINY, the floor solver_run puts
+ * under every concentration (camp_solver.c:554-567), through corr.
+ * Exact zeros are avoided because the aerosol Jacobians divide by
+ * concentrations.  Returns -1 (nothing below -SMALL), 0 (projected) or 1
+ * (a larger excursion: the recoverable f() failure it implies). */
+static int cv_project(Cv *cv, double *corr) {
+  double worst = -1.0;
+  for (int i = 0; i < cv->n; ++i)
+    if (cv->y[i] < -SMALL) {
+      double r = -cv->y[i] * cv->ewt[i];
+      worst = r > worst ? r : worst;
+    }
+  if (worst > 1.0) return 1;
+  if (worst < 0.0) return -1;
+  for (int i = 0; i < cv->n; ++i)
+    if (cv->y[i] < -SMALL) {
+      corr[i] = TINY - cv->zn[0][i];
+      cv->y[i] = TINY;
+    }
+  return 0;
+}
+
 static int cv_newton_iteration(Cv *cv) {
   int m = 0;
   double del = 0.0, delp = 0.0, dcon;
@@ -2052,29 +2081,11 @@ static int cv_newton_iteration(Cv *cv) {
     if (m > 0) cv->crate = fmax(CRDOWN * cv->crate, del / delp);
     dcon = del * fmin(1.0, cv->crate) / cv->tq[4];
     if (dcon <= 1.0) {
-      /* Non-negativity safeguard (not in CVODE): CAMP's f() rejects
-       * y < -SMALL (camp_solver.c:816-822), so a converged iterate below
-       * that would make every later f() at the prediction fail.  Components
-       * below -SMALL by at most their error weight's tolerance
-       * (-y_i * ewt_i <= 1) are projected to 0 through the correction;
-       * a larger excursion is the recoverable f() failure it implies. */
       int projected = 0;
       if (cv->o->neg_mode == 1) {
-        double worst = -1.0;
-        for (int i = 0; i < cv->n; ++i)
-          if (cv->y[i] < -SMALL) {
-            double r = -cv->y[i] * cv->ewt[i];
-            worst = r > worst ? r : worst;
-          }
-        if (worst > 1.0) return cv->jcur ? RHSFUNC_RECVR : TRY_AGAIN;
-        if (worst >= 0.0) {
-          for (int i = 0; i < cv->n; ++i)
-            if (cv->y[i] < -SMALL) {
-              cv->acor[i] = -cv->zn[0][i];
-              cv->y[i] = 0.0;
-            }
-          projected = 1;
-        }
+        int pr = cv_project(cv, cv->acor);
+        if (pr > 0) return cv->jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        projected = pr == 0;
       }
       cv->projected = projected;
       cv->acnrm = (m == 0 && !projected) ? del : wrms(cv, cv->acor);
@@ -2085,6 +2096,8 @@ static int cv_newton_iteration(Cv *cv) {
     if ((m == MAXCOR) || ((m >= 2) && (del > RDIV * delp)))
       return cv->jcur ? CONV_FAIL : TRY_AGAIN;
     delp = del;
+    if (cv->o->neg_mode == 1 && cv_project(cv, cv->acor) > 0)
+      return cv->jcur ? RHSFUNC_RECVR : TRY_AGAIN;
     retval = cv_f(cv, cv->y, cv->ftemp);
     if (retval > 0) return cv->jcur ? RHSFUNC_RECVR : TRY_AGAIN;
   }
@@ -2115,6 +2128,17 @@ static int cv_nls(Cv *cv, int nflag) {
       for (int i = 0; i < cv->n; ++i) cv->y[i] = cv->zn[0][i] + cv->acor_init[i];
     } else {
       memcpy(cv->y, cv->zn[0], cv->n * sizeof(double));
+    }
+    if (cv->o->neg_mode == 1) {
+      double mn = cv->y[0];
+      for (int i = 1; i < cv->n; ++i) mn = cv->y[i] < mn ? cv->y[i] : mn;
+      if (mn < -SMALL) {
+        if (!cv->use_acor_init) {
+          for (int i = 0; i < cv->n; ++i) cv->acor_init[i] = 0.0;
+          cv->use_acor_init = 1;
+        }
+        cv_project(cv, cv->acor_init);
+      }
     }
     int retval = cv_f(cv, cv->y, cv->ftemp);
     if (retval > 0) return RHSFUNC_RECVR;
@@ -2214,6 +2238,15 @@ static void cv_complete_step(Cv *cv) {
     cv->saved_tq5 = cv->tq[5];
     cv->indx_acor = Q_MAX;
   }
+  /* a component projected to TINY restarts flat at TINY: its history is
+   * cleared so that the next prediction is TINY rather than a negative
+   * extrapolation */
+  if (cv->projected)
+    for (int i = 0; i < cv->n; ++i)
+      if (cv->y[i] == TINY) {
+        cv->zn[0][i] = TINY;
+        for (int j = 1; j <= cv->q; ++j) cv->zn[j][i] = 0.0;
+      }
   memcpy(cv->last_yn, cv->zn[0], cv->n * sizeof(double));
 }
 
@@ -2292,12 +2325,6 @@ static int cv_step(Cv *cv) {
     break;
   }
   cv_complete_step(cv);
-  /* a component projected to 0 restarts flat: its history is cleared so
-   * that the next prediction is 0 rather than a negative extrapolation */
-  if (cv->projected)
-    for (int i = 0; i < cv->n; ++i)
-      if (cv->zn[0][i] == 0.0)
-        for (int j = 1; j <= cv->q; ++j) cv->zn[j][i] = 0.0;
   cv_prepare_next_step(cv, dsm);
   cv->etamax = (cv->nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
   for (int i = 0; i < cv->n; ++i) cv->acor[i] *= cv->tq[2];

@@ -197,3 +197,29 @@ def test_cell_alone_equals_cell_in_mix():
         s1 = cells.state[c:c + 1].copy()
         assert one.solve(s1, cells.env[c:c + 1], 0.0, 180.0) == 0
         assert np.array_equal(s1[0], a[c])
+
+
+def test_autotune_picks_a_fitting_candidate_and_leaves_cells_untouched():
+    """The autotuner times every cells-per-workgroup candidate that fits on
+    copies of a sample; afterwards the solver's cells solve exactly as on a
+    solver set to the chosen value, and as with one cell per workgroup."""
+    import torch
+    mech, cells = cb05_inputs(2048, night_fraction=0.5, stressed_fraction=0.1)
+    dev = torch.device("cuda:0")
+    env = torch.from_numpy(cells.env).to(dev)
+    tuned = CampSolver(mech, 2048)
+    tuned.set_rates(cells.rates)
+    st = torch.from_numpy(cells.state).to(dev)
+    chosen, ms = tuned.autotune(st.data_ptr(), env.data_ptr(), 0.0, 180.0, n_sample=1024)
+    print("autotune", chosen, ms)
+    assert chosen in (1, 2, 4, 8, 16, 32) and ms[chosen] is not None
+    assert ms[1] is not None and ms[32] is None  # 32 wavefronts exceed the kernel's block bound
+    assert torch.equal(st, torch.from_numpy(cells.state).to(dev))
+    assert tuned.solve_device(st.data_ptr(), env.data_ptr(), 0.0, 180.0) == 0
+    for cpb in (chosen, 1):
+        ref = CampSolver(mech, 2048)
+        ref.set_rates(cells.rates)
+        assert ref.set_cells_per_block(cpb) == cpb
+        s2 = torch.from_numpy(cells.state).to(dev)
+        assert ref.solve_device(s2.data_ptr(), env.data_ptr(), 0.0, 180.0) == 0
+        assert torch.equal(st, s2)
