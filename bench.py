@@ -163,6 +163,42 @@ def plain_path(mech, n_cells, steps, dt, persistent_value):
             "what": "block mode: one BDF over all cells, one launch per f/J/LU/solve"}
 
 
+def multiphase_leg(n_cells, steps, dt):
+    """The multi-phase workload: the CB05-scale gas mechanism plus 8 aerosol
+    bins x 30 species (20 SIMPOL partitioning species, 10 aqueous
+    equilibria, ZSR water), n_cells cells, one untimed 180 s step from the
+    seeded state, then `steps` timed steps, per-cell kernel."""
+    import torch
+    from camp_amd.core import CampSolver
+    from camp_amd.multiphase import multiphase_cells, multiphase_mechanism
+    mpm = multiphase_mechanism(1)
+    cells = multiphase_cells(mpm, n_cells, seed=2)
+    dev = torch.device("cuda:0")
+    st = torch.from_numpy(cells.state).to(dev)
+    env = torch.from_numpy(cells.env).to(dev)
+    s = CampSolver(mpm, n_cells)
+    s.set_rates(cells.rates)
+    progress("multi-phase: %d cells" % n_cells)
+    failed = int(s.solve_device(st.data_ptr(), env.data_ptr(), 0.0, dt) != 0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    kern = 0.0
+    tot = np.zeros(7)
+    for _ in range(steps):
+        failed += int(s.solve_device(st.data_ptr(), env.data_ptr(), 0.0, dt) != 0)
+        kern += s.last_kernel_ms()
+        tot += s.cell_stats()[:, :7].sum(axis=0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    progress("multi-phase done (%.0f ms/step)" % (1e3 * el / steps))
+    return {"value": n_cells * steps / el, "unit": "cell-timesteps/s", "n_cells": n_cells,
+            "steps": steps, "ms_per_step": 1e3 * el / steps, "kernel_ms_per_step": kern / steps,
+            "failed_steps": failed, "n_state": mpm.n_state, "n_dep": s.nv, "nnz": s.nnz,
+            "bdf_steps_per_cell_step": float(tot[0] / n_cells / steps),
+            "what": "cb05scale + 8 bins x 30 aerosol species (SIMPOL x20, aqueous "
+                    "equilibrium x10, ZSR water), per-cell BDF, rtol 1e-4"}
+
+
 def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -181,6 +217,9 @@ def main():
     ap.add_argument("--plain-cells", type=int, default=10000,
                     help="cells of the plain one-launch-per-RHS leg (0: skip)")
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--mp-cells", type=int, default=20000,
+                    help="cells of the multi-phase leg (0: skip)")
+    ap.add_argument("--mp-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=-1,
                     help="requests in the serving leg (-1: sized to about 40 s, 0: skip)")
     args = ap.parse_args()
@@ -277,6 +316,10 @@ def main():
     if args.plain_cells > 0:
         plain = plain_path(mech, args.plain_cells, args.plain_steps, args.dt, value)
 
+    mpleg = None
+    if args.mp_cells > 0:
+        mpleg = multiphase_leg(args.mp_cells, args.mp_steps, args.dt)
+
     serve = None
     if args.serve_requests != 0:
         n_req = args.serve_requests
@@ -319,6 +362,7 @@ def main():
                                        "newton_iters": float(nni / args.cells / args.steps),
                                        "jac_evals": float(nje / args.cells / args.steps)},
         "plain_path": plain,
+        "multiphase": mpleg,
         "autotune": tune,
         "roofline": roofline,
         "roofline_fj": roofline_fj,

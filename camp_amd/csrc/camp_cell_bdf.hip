@@ -45,7 +45,7 @@ __device__ __forceinline__ double rpoweri(double b, int e) {
 struct CellCv {
   DevProgram P;
   int lane, nv;
-  double *s_state, *s_k, *znb, *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2,
+  double *s_state, *s_k, *s_rate, *znb, *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2,
       *w, *lu;
   double *Js, *Jy, *Jf;
   MpLds mpl;   // multi-phase extras (LDS); unused for gas-phase programs
@@ -86,7 +86,9 @@ struct CellCv {
     int bad = cell_set_state(P, lane, yv, s_state);
     if (!bad) {
       if (P.mp) mp_f_prologue(P, lane, s_state, s_k, mpl);
-      cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, true, ts, out, mpl.slot);
+      cell_rates(P, lane, s_state, s_k, s_rate);
+      wsync();
+      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, true, ts, out, mpl.slot);
     }
     wsync();
     if (prof) c_f += clock64() - t0;
@@ -100,13 +102,17 @@ struct CellCv {
     if (cell_set_state(P, lane, yv, s_state)) return 1;
     if (P.mp) {
       mp_f_prologue(P, lane, s_state, s_k, mpl);
-      cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy, mpl.slot);
+      cell_rates(P, lane, s_state, s_k, s_rate);
+      wsync();
+      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, false, ts, fy, mpl.slot);
       wsync();
       mp_refresh(P, lane, s_state, mpl);
       mp_j_prologue(P, lane, s_state, s_k, mpl);
       cell_jac_mp(P, lane, s_state, s_k, ts, mpl, Jr, Js);
     } else {
-      cell_deriv(P, lane, s_state, s_k, yv, Js, Jy, Jf, false, ts, fy);
+      cell_rates(P, lane, s_state, s_k, s_rate);
+      wsync();
+      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, false, ts, fy);
       cell_jac(P, lane, s_state, s_k, ts, Js);
     }
     for (int i = lane; i < nv; i += kWave) {
@@ -667,7 +673,8 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   cv.s_state = base;
   cv.s_k = cv.s_state + P.ns;
   cv.mpl = mp_lds(P, cv.s_k + P.n_env);
-  double* v = cv.s_k + P.n_env + mp_lds_doubles(P);
+  cv.s_rate = cv.s_k + P.n_env + mp_lds_doubles(P);
+  double* v = cv.s_rate + 2 * (size_t)P.n_terms;
   cv.znb = v;
   v += 6 * (size_t)nv;
   cv.ewt = v; v += nv;

@@ -30,12 +30,13 @@ struct DevProgram {
   const int *colptr, *rowval, *r_ptr, *r_elem, *r_col;
   const int *perm, *m_to_lu, *diag_pos, *e_diag;
   const int *l_ptr, *l_pos, *l_row, *u_ptr, *u_pos, *u_row, *up_ptr;
-  const uint16_t *up_dst, *up_l, *up_u;
+  const uint64_t* up_rec;  // LU update records: dst | l << 16 | u << 32
   const int *lu_rowptr, *lu_col;
   const uint64_t* row_mask;  // 4 per LU row; nullptr when nv > 128
   const int4 *fe_a, *je_a;  // packed gather records (see Program in camp_model.hpp)
   const int *fe_b, *je_b;
   const double *fe_c, *je_c;
+  const int2* fe_tb;  // {term or slot index, guard | flags << 24} per f record
   // rate-constant program (rxn_update_env_state)
   const int *rx_type, *rx_fidx, *rx_eidx, *rx_nreact;
   const double* rx_float;
@@ -166,18 +167,32 @@ __device__ __forceinline__ void cell_rate_constants(const DevProgram& P, int lan
 // est != nullptr selects the Jacobian-extrapolated blend; est is then
 // computed from the cell's J history (J_deriv + J_solver (y - J_state)).
 // ---------------------------------------------------------------------
-__device__ __forceinline__ dd term_rate(const DevProgram& P, int t, const double* s_state,
-                                        const double* s_k) {
-  double k = s_k[P.t_kidx[t]];
-  int nr = P.t_nreact[t];
-  if (P.t_flags[t] & 2) return {k * s_state[P.t_react[3 * t]], 0.0};
-  dd rate = {k, 0.0};
-  for (int j = 0; j < nr; ++j) rate = dd_mul_d(rate, s_state[P.t_react[3 * t + j]]);
-  return rate;
+// Phase 1 of f(y): the rate of every term (reaction) once, into s_rate
+// (n_terms double-doubles): k * prod(reactants) as the per-type
+// calc_deriv_contrib computes it (rxn_arrhenius.c:143-147 and alike);
+// first-order loss rounds k * y to double (rxn_first_order_loss.c:150).
+__host__ __device__ __forceinline__ void cell_rates(const DevProgram& P, int lane, const double* s_state,
+                                           const double* s_k, double* s_rate) {
+  for (int t = lane; t < P.n_terms; t += kWave) {
+    double k = s_k[P.t_kidx[t]];
+    int fl = P.t_flags[t];
+    dd rate;
+    if (fl & 2) {
+      rate = {k * s_state[P.t_react[3 * t]], 0.0};
+    } else {
+      rate = {k, 0.0};
+      for (int j = 0; j < P.t_nreact[t]; ++j) rate = dd_mul_d(rate, s_state[P.t_react[3 * t + j]]);
+    }
+    s_rate[2 * t] = rate.hi;
+    s_rate[2 * t + 1] = rate.lo;
+  }
 }
 
+// Phase 2 (after cell_rates and a wave sync): lane i gathers species i's
+// records -- the term's rate times the yield (or -1), subject to the
+// negative-yield guard, or a multi-phase slot -- into D and S.
 __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const double* s_state,
-                                           const double* s_k, const double* y,
+                                           const double* s_rate, const double* y,
                                            const double* Js, const double* Jy,
                                            const double* Jf, bool use_est, double h,
                                            double* out, const double* s_slot = nullptr) {
@@ -185,27 +200,17 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
     dd D = {0.0, 0.0};
     double S = 0.0;
     for (int q = P.f_ptr[i]; q < P.f_ptr[i + 1]; ++q) {
-      int4 a = P.fe_a[q];
-      int b = P.fe_b[q];
+      int2 tb = P.fe_tb[q];
       double c = P.fe_c[q];
-      int flags = b >> 24;
-      int g = (b << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
+      int flags = tb.y >> 24;
+      int g = (tb.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
       if (flags & 4) {  // multi-phase contribution slot (double-double)
-        dd v = {s_slot[2 * a.x], s_slot[2 * a.x + 1]};
+        dd v = {s_slot[2 * tb.x], s_slot[2 * tb.x + 1]};
         D = dd_add(D, v);
         S += fabs(v.hi);
         continue;
       }
-      double k = s_k[a.x];
-      dd rate;
-      if (flags & 2) {
-        rate = {k * s_state[a.y], 0.0};
-      } else {
-        rate = {k, 0.0};
-        if (a.y >= 0) rate = dd_mul_d(rate, s_state[a.y]);
-        if (a.z >= 0) rate = dd_mul_d(rate, s_state[a.z]);
-        if (a.w >= 0) rate = dd_mul_d(rate, s_state[a.w]);
-      }
+      dd rate = {s_rate[2 * tb.x], s_rate[2 * tb.x + 1]};
       if ((flags & 1) && rate.hi == 0.0) continue;
       if (g >= 0 && !(-rate.hi * c * h <= s_state[g])) continue;
       dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
@@ -312,31 +317,32 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     lu[P.m_to_lu[e]] = v;
   }
   wsync();
-  // The index lists of pivot k+1 (first 64 entries of each) are fetched
-  // from global memory while pivot k is being eliminated, so the chain of
-  // 2*nv dependent LDS phases does not also wait on global-memory latency.
+  // The first 64 division positions and update records of pivot k+1 are
+  // fetched from global memory while pivot k is being eliminated, and a
+  // pivot's remaining update records are all loaded before any of them is
+  // applied, so the chain of 2*nv dependent LDS phases waits on global
+  // memory at most once per pivot.  Within one pivot every update has a
+  // distinct destination, so lanes never conflict.
   int lq0 = P.l_ptr[0], lq1 = P.l_ptr[1], uq0 = P.up_ptr[0], uq1 = P.up_ptr[1];
   int l_p = (lq0 + lane < lq1) ? P.l_pos[lq0 + lane] : -1;
-  int u_d = -1, u_l = 0, u_u = 0;
-  if (uq0 + lane < uq1) {
-    u_d = P.up_dst[uq0 + lane];
-    u_l = P.up_l[uq0 + lane];
-    u_u = P.up_u[uq0 + lane];
-  }
+  uint64_t u_r = (uq0 + lane < uq1) ? P.up_rec[uq0 + lane] : ~0ull;
   int dpos = P.diag_pos[0];
   for (int k = 0; k < P.nv; ++k) {
-    int nlq0 = lq1, nlq1 = 0, nuq0 = uq1, nuq1 = 0, n_lp = -1, n_ud = -1, n_ul = 0, n_uu = 0,
-        ndpos = 0;
+    int nlq0 = lq1, nlq1 = 0, nuq0 = uq1, nuq1 = 0, n_lp = -1, ndpos = 0;
+    uint64_t n_ur = ~0ull;
     if (k + 1 < P.nv) {
       nlq1 = P.l_ptr[k + 2];
       nuq1 = P.up_ptr[k + 2];
       ndpos = P.diag_pos[k + 1];
       if (nlq0 + lane < nlq1) n_lp = P.l_pos[nlq0 + lane];
-      if (nuq0 + lane < nuq1) {
-        n_ud = P.up_dst[nuq0 + lane];
-        n_ul = P.up_l[nuq0 + lane];
-        n_uu = P.up_u[nuq0 + lane];
-      }
+      if (nuq0 + lane < nuq1) n_ur = P.up_rec[nuq0 + lane];
+    }
+    // records beyond the first 64 of this pivot, up to 4 per lane in flight
+    uint64_t more[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int q = uq0 + (r + 1) * kWave + lane;
+      more[r] = q < uq1 ? P.up_rec[q] : ~0ull;
     }
     double piv = lu[dpos];
     if (piv == 0.0 || !isfinite(piv)) return 1;
@@ -346,14 +352,25 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
       lu[p] = lu[p] / piv;
     }
     wsync();
-    if (u_d >= 0) lu[u_d] = lu[u_d] - lu[u_l] * lu[u_u];
-    for (int q = uq0 + kWave + lane; q < uq1; q += kWave) {
-      int d = P.up_dst[q];
-      lu[d] = lu[d] - lu[P.up_l[q]] * lu[P.up_u[q]];
+    if (u_r != ~0ull) {
+      int d = (int)(u_r & 0xFFFF), a = (int)((u_r >> 16) & 0xFFFF), b = (int)((u_r >> 32) & 0xFFFF);
+      lu[d] = lu[d] - lu[a] * lu[b];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (more[r] != ~0ull) {
+        uint64_t x = more[r];
+        int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
+        lu[d] = lu[d] - lu[a] * lu[b];
+      }
+    for (int q = uq0 + 5 * kWave + lane; q < uq1; q += kWave) {
+      uint64_t x = P.up_rec[q];
+      int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
+      lu[d] = lu[d] - lu[a] * lu[b];
     }
     wsync();
     lq0 = nlq0; lq1 = nlq1; uq0 = nuq0; uq1 = nuq1;
-    l_p = n_lp; u_d = n_ud; u_l = n_ul; u_u = n_uu; dpos = ndpos;
+    l_p = n_lp; u_r = n_ur; dpos = ndpos;
   }
   return 0;
 }

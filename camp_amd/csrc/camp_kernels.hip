@@ -50,9 +50,10 @@ __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const doubl
   extern __shared__ double smem[];
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
   int wpb = blockDim.x / kWave;
-  double* s_state = smem + (size_t)wv * (P.ns + P.n_env + mp_lds_doubles(P));
+  double* s_state = smem + (size_t)wv * fj_lds_doubles(P);
   double* s_k = s_state + P.ns;
   MpLds L = mp_lds(P, s_k + P.n_env);
+  double* s_rate = s_k + P.n_env + mp_lds_doubles(P);
   for (int c = blockIdx.x * wpb + wv; c < nc; c += gridDim.x * wpb) {
     stage_cell(P, lane, state + (size_t)c * P.ns, rxn_env + (size_t)c * P.n_env, s_state, s_k);
     const double* yc = y + (size_t)c * P.nv;
@@ -64,7 +65,9 @@ __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const doubl
       mp_cell_env(P, lane, env[2 * c], env[2 * c + 1], L);
       mp_f_prologue(P, lane, s_state, s_k, L);
     }
-    cell_deriv(P, lane, s_state, s_k, yc, Js + (size_t)c * P.nnz, Jy + (size_t)c * P.nv,
+    cell_rates(P, lane, s_state, s_k, s_rate);
+    wsync();
+    cell_deriv(P, lane, s_state, s_rate, yc, Js + (size_t)c * P.nnz, Jy + (size_t)c * P.nv,
                Jf + (size_t)c * P.nv, use_est != 0, h, fout + (size_t)c * P.nv, L.slot);
     wsync();
   }
@@ -80,9 +83,10 @@ __global__ void __launch_bounds__(256) k_jac(DevProgram P, int nc, const double*
   extern __shared__ double smem[];
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
   int wpb = blockDim.x / kWave;
-  double* s_state = smem + (size_t)wv * (P.ns + P.n_env + mp_lds_doubles(P));
+  double* s_state = smem + (size_t)wv * fj_lds_doubles(P);
   double* s_k = s_state + P.ns;
   MpLds L = mp_lds(P, s_k + P.n_env);
+  double* s_rate = s_k + P.n_env + mp_lds_doubles(P);
   for (int c = blockIdx.x * wpb + wv; c < nc; c += gridDim.x * wpb) {
     stage_cell(P, lane, state + (size_t)c * P.ns, rxn_env + (size_t)c * P.n_env, s_state, s_k);
     if (cell_set_state(P, lane, y + (size_t)c * P.nv, s_state)) continue;

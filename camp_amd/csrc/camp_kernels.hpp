@@ -17,7 +17,7 @@ constexpr int kCellProf = 5;    // shader cycles: f, Jac, LU factor, LU solve, w
 // 10 work vectors, the LU factors (in global scratch for multi-phase
 // programs) and 24 slots for the BDF coefficient arrays l, tq and tau.
 __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P) {
-  return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 16 * (size_t)P.nv +
+  return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms + 16 * (size_t)P.nv +
          (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
 }
 // Global scratch doubles per wavefront of multi-phase programs: LU factors
@@ -61,8 +61,8 @@ __global__ void k_jac(DevProgram P, int nc, const double* y, const double* state
                       const double* env, const double* rxn_env, double* J, double h, double* Jr);
 // LDS doubles per wavefront of k_deriv and k_jac, and wavefronts per block
 // (one for multi-phase programs, whose per-cell extras are large)
-inline size_t fj_lds_doubles(const DevProgram& P) {
-  return (size_t)P.ns + P.n_env + mp_lds_doubles(P);
+__host__ __device__ inline size_t fj_lds_doubles(const DevProgram& P) {
+  return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms;
 }
 inline int fj_waves_per_block(const DevProgram& P) { return P.mp ? 1 : 4; }
 __global__ void k_lu_factor(DevProgram P, int nc, const double* J, double gamma, double* LU,

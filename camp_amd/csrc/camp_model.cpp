@@ -596,8 +596,12 @@ std::string Model::initialize() {
       for (int j = 0; j < 3; ++j) P.fe_a.push_back(-1);
       P.fe_b.push_back(0xFFFFFF | (kTermSlot << 24));
       P.fe_c.push_back(1.0);
+      P.fe_tb.push_back(-1 - t);
+      P.fe_tb.push_back(P.fe_b.back());
       continue;
     }
+    P.fe_tb.push_back(t);
+    P.fe_tb.push_back((P.f_guard[q] & 0xFFFFFF) | (P.t_flags[t] << 24));
     P.fe_a.push_back(P.t_kidx[t]);
     for (int j = 0; j < 3; ++j) P.fe_a.push_back(j < P.t_nreact[t] ? P.t_react[3 * t + j] : -1);
     P.fe_b.push_back((P.f_guard[q] & 0xFFFFFF) | (P.t_flags[t] << 24));
@@ -617,6 +621,7 @@ std::string Model::initialize() {
   }
   if (P.fe_c.empty()) {
     P.fe_a.assign(4, -1); P.fe_b.push_back(-1); P.fe_c.push_back(0.0);
+    P.fe_tb.assign(2, -1);
   }
   if (P.je_c.empty()) {
     P.je_a.assign(4, -1); P.je_b.push_back(-1); P.je_c.push_back(0.0);
@@ -719,6 +724,9 @@ std::string Model::initialize() {
   if (P.l_pos.empty()) { P.l_pos.push_back(0); P.l_row.push_back(0); }
   if (P.u_pos.empty()) { P.u_pos.push_back(0); P.u_row.push_back(0); }
   if (P.up_dst.empty()) { P.up_dst.push_back(0); P.up_l.push_back(0); P.up_u.push_back(0); }
+  P.up_rec.resize(P.up_dst.size());
+  for (size_t q = 0; q < P.up_dst.size(); ++q)
+    P.up_rec[q] = (uint64_t)P.up_dst[q] | ((uint64_t)P.up_l[q] << 16) | ((uint64_t)P.up_u[q] << 32);
   initialized = true;
   return std::string();
 }

@@ -66,6 +66,10 @@ struct Program {
   //           je_c = coefficient
   std::vector<int> fe_a, fe_b, je_a, je_b;
   std::vector<double> fe_c, je_c;
+  // f(y) is evaluated in two phases: the rate of every term once (into
+  // LDS), then per species a gather of fe_tb = {term index, or slot index
+  // for slot records; fe_b} and fe_c
+  std::vector<int> fe_tb;
   std::vector<int> colptr, rowval;                  // solver pattern (dep ids)
   std::vector<int> r_ptr, r_elem, r_col;            // row view of the pattern
   // LU (positions in the permuted LU value array)
@@ -75,6 +79,7 @@ struct Program {
   std::vector<int> u_ptr, u_pos, u_row;             // per pivot k: U(:,k) above k
   std::vector<int> up_ptr;                          // per pivot k: update range
   std::vector<uint16_t> up_dst, up_l, up_u;         // LU[dst] -= LU[l] * LU[u]
+  std::vector<uint64_t> up_rec;                     // dst | l << 16 | u << 32
   // Row-major LU storage: row i occupies [lu_rowptr[i], lu_rowptr[i+1]) in
   // column order.  For nv <= 128, row_mask[4i..4i+3] holds the columns of
   // row i's strictly lower (L) part as a 128-bit mask (low, high word), then

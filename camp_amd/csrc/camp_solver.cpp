@@ -72,12 +72,11 @@ void Solver::setup_device() {
   P.u_pos = dev_upload(G.u_pos, owned);
   P.u_row = dev_upload(G.u_row, owned);
   P.up_ptr = dev_upload(G.up_ptr, owned);
-  P.up_dst = dev_upload(G.up_dst, owned);
-  P.up_l = dev_upload(G.up_l, owned);
-  P.up_u = dev_upload(G.up_u, owned);
+  P.up_rec = dev_upload(G.up_rec, owned);
   P.fe_a = reinterpret_cast<const int4*>(dev_upload(G.fe_a, owned));
   P.fe_b = dev_upload(G.fe_b, owned);
   P.fe_c = dev_upload(G.fe_c, owned);
+  P.fe_tb = reinterpret_cast<const int2*>(dev_upload(G.fe_tb, owned));
   P.je_a = reinterpret_cast<const int4*>(dev_upload(G.je_a, owned));
   P.je_b = dev_upload(G.je_b, owned);
   P.je_c = dev_upload(G.je_c, owned);
