@@ -132,34 +132,42 @@ def serving(mech, solver, pool_state, env, rates, n_requests, seed=3):
                            "mechanism only (the multi-phase mechanism is not built)"}
 
 
-def plain_path(mech, n_cells, steps, dt, persistent_value):
-    """The plain GPU path: one BDF over all cells (shared step size and
-    order, block-diagonal Newton) driven from the host with one launch per
-    f(y), J(y), LU, solve and vector operation; the first n_cells cells of
-    the same workload, one untimed step from the seeded state, then `steps`
-    timed steps."""
+def plain_path(mech, n_cells, steps, dt):
+    """The plain GPU path on the 10,000-cell gas workload (T 220..310 K,
+    P 2e4..1.013e5 Pa, no night or stressed cells): one BDF over all cells
+    (shared step size and order, block-diagonal Newton) driven from the host
+    with one launch per f(y), J(y), LU, solve and vector operation; one
+    untimed step from the seeded state, then `steps` timed steps.  The
+    persistent per-cell kernel runs the same cells and steps for the
+    ratio."""
     import torch
-    from camp_amd.core import CampSolver, MODE_BLOCK
-    _, cells = make_inputs(n_cells)
+    from camp_amd.core import CampSolver, MODE_BLOCK, MODE_PER_CELL
+    from camp_amd.mechanisms import synthetic_cells
+    cells = synthetic_cells(mech, n_cells, seed=2)
     dev = torch.device("cuda:0")
-    st = torch.from_numpy(cells.state).to(dev)
     env = torch.from_numpy(cells.env).to(dev)
-    s = CampSolver(mech, n_cells, mode=MODE_BLOCK)
-    s.set_rates(cells.rates)
-    progress("plain path: %d cells" % n_cells)
-    s.solve_device(st.data_ptr(), env.data_ptr(), 0.0, dt)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    failed = 0
-    for _ in range(steps):
-        failed += int(s.solve_device(st.data_ptr(), env.data_ptr(), 0.0, dt) != 0)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t
-    v = n_cells * steps / el
-    progress("plain path done (%.0f ms/step)" % (1e3 * el / steps))
-    return {"value": v, "unit": "cell-timesteps/s", "n_cells": n_cells, "steps": steps,
-            "ms_per_step": 1e3 * el / steps, "failed_steps": failed,
-            "persistent_over_plain": persistent_value / v,
+    out = {}
+    for name, mode in (("plain", MODE_BLOCK), ("persistent", MODE_PER_CELL)):
+        st = torch.from_numpy(cells.state).to(dev)
+        s = CampSolver(mech, n_cells, mode=mode)
+        s.set_rates(cells.rates)
+        progress("%s path: %d cells" % (name, n_cells))
+        s.solve_device(st.data_ptr(), env.data_ptr(), 0.0, dt)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        failed = 0
+        for _ in range(steps):
+            failed += int(s.solve_device(st.data_ptr(), env.data_ptr(), 0.0, dt) != 0)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        progress("%s path done (%.0f ms/step)" % (name, 1e3 * el / steps))
+        out[name] = {"value": n_cells * steps / el, "ms_per_step": 1e3 * el / steps,
+                     "failed_steps": failed}
+    return {"value": out["plain"]["value"], "unit": "cell-timesteps/s", "n_cells": n_cells,
+            "steps": steps, "ms_per_step": out["plain"]["ms_per_step"],
+            "failed_steps": out["plain"]["failed_steps"],
+            "persistent_same_cells": out["persistent"],
+            "persistent_over_plain": out["persistent"]["value"] / out["plain"]["value"],
             "what": "block mode: one BDF over all cells, one launch per f/J/LU/solve"}
 
 
@@ -220,6 +228,10 @@ def main():
     ap.add_argument("--mp-cells", type=int, default=20000,
                     help="cells of the multi-phase leg (0: skip)")
     ap.add_argument("--mp-steps", type=int, default=2)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many MiB device-to-device three times (the "
+                         "known-size kernel that calibrates rocprofv3's FETCH_SIZE and "
+                         "WRITE_SIZE counters) and exit")
     ap.add_argument("--serve-requests", type=int, default=-1,
                     help="requests in the serving leg (-1: sized to about 40 s, 0: skip)")
     args = ap.parse_args()
@@ -230,6 +242,16 @@ def main():
 
     import torch
     from camp_amd.core import CampSolver, MODE_PER_CELL
+
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 8 * n, "copies": 3}))
+        return
 
     mech, cells = make_inputs(args.cells)
     dev = torch.device("cuda:0")
@@ -314,7 +336,7 @@ def main():
 
     plain = None
     if args.plain_cells > 0:
-        plain = plain_path(mech, args.plain_cells, args.plain_steps, args.dt, value)
+        plain = plain_path(mech, args.plain_cells, args.plain_steps, args.dt)
 
     mpleg = None
     if args.mp_cells > 0:

@@ -717,88 +717,91 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
     if (q >= a.nc) break;
     int cell = a.cell_list ? a.cell_list[q] : q;
     double t1 = a.cell_t1 ? a.cell_t1[cell] : a.t1;
-    double* state_c = a.state + (size_t)cell * P.ns;
-    double* renv_c = a.rxn_env + (size_t)cell * P.n_env;
-    double* g_js = a.Js + (size_t)cell * P.nnz;
-    double* g_jy = a.Jy + (size_t)cell * nv;
-    double* g_jf = a.Jf + (size_t)cell * nv;
-    if (P.mp) {
-      cv.Js = g_js;
-      cv.Jy = g_jy;
-      cv.Jf = g_jf;
-    } else {
-      for (int e = lane; e < P.nnz; e += kWave) h_js[e] = g_js[e];
-      for (int i = lane; i < nv; i += kWave) {
-        h_jy[i] = g_jy[i];
-        h_jf[i] = g_jf[i];
-      }
-      cv.Js = h_js;
-      cv.Jy = h_jy;
-      cv.Jf = h_jf;
-    }
-    // stage state (constants clamped in place) and rate data
-    for (int i = lane; i < P.ns; i += kWave) {
-      double s = state_c[i];
-      cv.s_state[i] = s;
-      if (a.var_type[i] != 1) {
-        s = s > 1.0e-60 ? s : 1.0e-60;
-        cv.s_state[i] = s;
-        state_c[i] = s;
-      }
-    }
-    for (int i = lane; i < P.n_env; i += kWave) cv.s_k[i] = renv_c[i];
-    wsync();
-    // rate constants for the cell's environment (rxn_update_env_state)
-    double T = a.env[2 * cell], Pa = a.env[2 * cell + 1];
-    cell_rate_constants(P, lane, T, Pa, cv.s_k);
-    wsync();
-    if (P.mp) mp_cell_env(P, lane, T, Pa, cv.mpl);
-    for (int i = lane; i < P.n_env; i += kWave) renv_c[i] = cv.s_k[i];
-    for (int i = lane; i < nv; i += kWave) {
-      double s = cv.s_state[P.dep_state[i]];
-      cv.zn(0)[i] = s > 1.0e-60 ? s : 1.0e-60;
-    }
-    wsync();
-    cv.c_f = cv.c_j = cv.c_lu = cv.c_sol = 0;
-    long long c_start = cv.tick();
-    int flag = cv.solve(a.t0, t1);
-    if (flag >= 0) {
-      for (int i = lane; i < nv; i += kWave) {
-        double yv = cv.y[i];
-        state_c[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
-        if (P.mp) cv.s_state[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
-      }
+    int nsteps = a.cell_nsteps ? a.cell_nsteps[cell] : 1;
+    for (int step = 0; step < nsteps; ++step) {
+      double* state_c = a.state + (size_t)cell * P.ns;
+      double* renv_c = a.rxn_env + (size_t)cell * P.n_env;
+      double* g_js = a.Js + (size_t)cell * P.nnz;
+      double* g_jy = a.Jy + (size_t)cell * nv;
+      double* g_jf = a.Jf + (size_t)cell * nv;
       if (P.mp) {
-        // the post-solve sub_model_calculate (camp_solver.c:685-687) on the
-        // final state; the sub models' outputs go back to the state array
-        wsync();
-        for (int z = lane; z < P.n_zi; z += kWave) {
-          mp_zsr_water(P, z, cv.mpl.sub_env, cv.s_state);
-          state_c[P.zi_water[z]] = cv.s_state[P.zi_water[z]];
+        cv.Js = g_js;
+        cv.Jy = g_jy;
+        cv.Jf = g_jf;
+      } else {
+        for (int e = lane; e < P.nnz; e += kWave) h_js[e] = g_js[e];
+        for (int i = lane; i < nv; i += kWave) {
+          h_jy[i] = g_jy[i];
+          h_jf[i] = g_jf[i];
+        }
+        cv.Js = h_js;
+        cv.Jy = h_jy;
+        cv.Jf = h_jf;
+      }
+      // stage state (constants clamped in place) and rate data
+      for (int i = lane; i < P.ns; i += kWave) {
+        double s = state_c[i];
+        cv.s_state[i] = s;
+        if (a.var_type[i] != 1) {
+          s = s > 1.0e-60 ? s : 1.0e-60;
+          cv.s_state[i] = s;
+          state_c[i] = s;
         }
       }
-    }
-    if (!P.mp) {
+      for (int i = lane; i < P.n_env; i += kWave) cv.s_k[i] = renv_c[i];
       wsync();
-      for (int e = lane; e < P.nnz; e += kWave) g_js[e] = h_js[e];
+      // rate constants for the cell's environment (rxn_update_env_state)
+      double T = a.env[2 * cell], Pa = a.env[2 * cell + 1];
+      cell_rate_constants(P, lane, T, Pa, cv.s_k);
+      wsync();
+      if (P.mp) mp_cell_env(P, lane, T, Pa, cv.mpl);
+      for (int i = lane; i < P.n_env; i += kWave) renv_c[i] = cv.s_k[i];
       for (int i = lane; i < nv; i += kWave) {
-        g_jy[i] = h_jy[i];
-        g_jf[i] = h_jf[i];
+        double s = cv.s_state[P.dep_state[i]];
+        cv.zn(0)[i] = s > 1.0e-60 ? s : 1.0e-60;
       }
-    }
-    if (lane == 0) {
-      double* st = a.stats + (size_t)cell * kCellStats;
-      st[0] = cv.nst; st[1] = cv.nfe; st[2] = cv.nsetups; st[3] = cv.netf;
-      st[4] = cv.nni; st[5] = cv.ncfn; st[6] = cv.nje; st[7] = flag;
-      st[8] = cv.hu; st[9] = cv.next_h;
-      if (a.fail_count && flag < 0) a.fail_count[cell] += 1;
-      if (cv.prof) {
-        double* pr = a.prof + (size_t)cell * kCellProf;
-        pr[0] = (double)cv.c_f; pr[1] = (double)cv.c_j; pr[2] = (double)cv.c_lu;
-        pr[3] = (double)cv.c_sol; pr[4] = (double)(clock64() - c_start);
+      wsync();
+      cv.c_f = cv.c_j = cv.c_lu = cv.c_sol = 0;
+      long long c_start = cv.tick();
+      int flag = cv.solve(a.t0, t1);
+      if (flag >= 0) {
+        for (int i = lane; i < nv; i += kWave) {
+          double yv = cv.y[i];
+          state_c[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
+          if (P.mp) cv.s_state[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
+        }
+        if (P.mp) {
+          // the post-solve sub_model_calculate (camp_solver.c:685-687) on the
+          // final state; the sub models' outputs go back to the state array
+          wsync();
+          for (int z = lane; z < P.n_zi; z += kWave) {
+            mp_zsr_water(P, z, cv.mpl.sub_env, cv.s_state);
+            state_c[P.zi_water[z]] = cv.s_state[P.zi_water[z]];
+          }
+        }
       }
+      if (!P.mp) {
+        wsync();
+        for (int e = lane; e < P.nnz; e += kWave) g_js[e] = h_js[e];
+        for (int i = lane; i < nv; i += kWave) {
+          g_jy[i] = h_jy[i];
+          g_jf[i] = h_jf[i];
+        }
+      }
+      if (lane == 0) {
+        double* st = a.stats + (size_t)cell * kCellStats;
+        st[0] = cv.nst; st[1] = cv.nfe; st[2] = cv.nsetups; st[3] = cv.netf;
+        st[4] = cv.nni; st[5] = cv.ncfn; st[6] = cv.nje; st[7] = flag;
+        st[8] = cv.hu; st[9] = cv.next_h;
+        if (a.fail_count && flag < 0) a.fail_count[cell] += 1;
+        if (cv.prof) {
+          double* pr = a.prof + (size_t)cell * kCellProf;
+          pr[0] = (double)cv.c_f; pr[1] = (double)cv.c_j; pr[2] = (double)cv.c_lu;
+          pr[3] = (double)cv.c_sol; pr[4] = (double)(clock64() - c_start);
+        }
+      }
+      wsync();
     }
-    wsync();
   }
 }
 

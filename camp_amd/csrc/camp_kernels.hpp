@@ -46,6 +46,10 @@ struct CellBdfArgs {
   double* prof = nullptr;  // [cells][kCellProf] when phase timing is on
   int* fail_count = nullptr;  // per cell: incremented on each failed solve
   double* wave_scratch = nullptr;  // [grid waves][cell_bdf_wave_scratch] (multi-phase)
+  // Optional: cell solves cell_nsteps[cell] consecutive timesteps over
+  // [t0, t1] in this launch, each exactly as a separate launch would
+  // (state written back and restaged between them); nullptr: one.
+  const int* cell_nsteps = nullptr;
 };
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);

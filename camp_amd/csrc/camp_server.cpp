@@ -34,7 +34,7 @@ enum ReqState { kQueued, kReady, kRunning, kDone };
 
 struct Request {
   long id = 0;
-  int mech = 0, n = 0, steps_left = 0;
+  int mech = 0, n = 0, n_steps = 0, steps_left = 0;
   double dt = 0.0;
   const double *in_state = nullptr, *in_env = nullptr, *in_rates = nullptr;
   double *out_state = nullptr, *out_stats = nullptr;
@@ -51,7 +51,7 @@ struct Arena {
   int ns = 0, n_env = 0, nnz = 0, nv = 0, n_upd = 0;
   double *state = nullptr, *env = nullptr, *renv = nullptr, *Js = nullptr, *Jy = nullptr,
          *Jf = nullptr, *stats = nullptr, *t1 = nullptr, *rates = nullptr;
-  int *fail = nullptr, *upd_off = nullptr;
+  int *fail = nullptr, *upd_off = nullptr, *nsteps = nullptr;
   int* h_fail = nullptr;  // pinned [cap]
   std::vector<void*> owned;
   std::map<long, long> free_ranges;  // start -> length
@@ -146,6 +146,7 @@ void Server::upload(Arena& A, Request* r) {
   }
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, up_st, n, r->dt,
                      A.t1 + s);
+  HIP_OK(hipMemsetD32Async((hipDeviceptr_t)(A.nsteps + s), r->n_steps, n, up_st));
   // a fresh request starts with an empty Jacobian history, as a new solver
   HIP_OK(hipMemsetAsync(A.Js + s * A.nnz, 0, n * A.nnz * d, up_st));
   HIP_OK(hipMemsetAsync(A.Jy + s * A.nv, 0, n * A.nv * d, up_st));
@@ -236,6 +237,7 @@ bool Server::launch() {
     a.stats = A.stats;
     a.cell_list = L.d_list;
     a.cell_t1 = A.t1;
+    a.cell_nsteps = A.nsteps;
     a.fail_count = A.fail;
     int grid = 0, block = 0;
     size_t lds = 0;
@@ -269,7 +271,8 @@ bool Server::launch() {
     }
     HIP_OK(hipEventRecord(L.ev, L.st));
     L.busy = true;
-    L.cells = total;
+    L.cells = 0;
+    for (Request* r : L.batch) L.cells += (long)r->n * r->n_steps;
     any = true;
     std::lock_guard<std::mutex> g(mu);
     batches += 1;
@@ -424,6 +427,7 @@ extern "C" int camp_server_add_mechanism(void* server, void* solver_data, long m
   A->Jf = dev_alloc<double>((size_t)n * A->nv, A->owned);
   A->stats = dev_alloc<double>((size_t)n * kCellStats, A->owned);
   A->t1 = dev_alloc<double>((size_t)n, A->owned);
+  A->nsteps = dev_alloc<int>((size_t)n, A->owned);
   A->rates = dev_alloc<double>((size_t)n * std::max(A->n_upd, 1), A->owned);
   A->fail = dev_alloc<int>((size_t)n, A->owned);
   A->upd_off = dev_upload(off, A->owned);
@@ -455,7 +459,10 @@ extern "C" long camp_server_submit(void* server, int mech, int n_cells, const do
   r->mech = mech;
   r->n = n_cells;
   r->dt = dt;
-  r->steps_left = n_steps;
+  // every timestep of a request runs in one launch (the kernel restages the
+  // cell between them, as separate launches would)
+  r->n_steps = n_steps;
+  r->steps_left = 1;
   r->in_state = state;
   r->in_env = env;
   r->in_rates = base_rates;

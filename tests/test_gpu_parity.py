@@ -168,9 +168,11 @@ def test_cb05_rtol_1e4_error_no_worse_than_oracle(oracle_lib, mode):
     """At the workload tolerance both paths are compared with a converged
     solution (oracle at rtol 1e-10).  Integration error at rtol 1e-4 moves
     by tens of bar units between runs whose round-off differs (step-size
-    control amplifies it), so the comparison is over the cells' worst
-    errors, in units of the 1e-4 / 1e-8 bar: the HIP path's median and 90th
-    percentile may not exceed twice and three times the oracle's own."""
+    control amplifies it), so the HIP path is held to the oracle's own
+    spread: the oracle is also run with T perturbed by 1e-15 and 1e-14
+    (relative), and the HIP path's median and 90th percentile of the cells'
+    worst errors, in units of the 1e-4 / 1e-8 bar, may not exceed twice the
+    largest the oracle shows."""
     n = 64 if mode == MODE_PER_CELL else 8
     mech, cells, a, b = _cb05_pair(oracle_lib, n, mode, 1.0e-4, 1.0e-8)
     atol = np.full(mech.n_state, 1.0e-14)
@@ -183,13 +185,27 @@ def test_cb05_rtol_1e4_error_no_worse_than_oracle(oracle_lib, mode):
     def err(x):
         return np.max(np.abs(x - t) / (SOLVE_RTOL * np.abs(t) + SOLVE_ATOL), axis=1)
 
-    e_gpu, e_orc = err(a), err(b)
+    o50, o90 = [], []
+    for eps in (0.0, 1e-15, -1e-15, 1e-14):
+        if eps == 0.0:
+            x = b
+        else:
+            env = cells.env.copy()
+            env[:, 0] *= 1.0 + eps
+            orc = oracle_lib.Oracle(mech, n, rel_tol=1.0e-4, abs_tol=np.full(mech.n_state, 1e-8))
+            orc.set_rates(cells.rates)
+            x = cells.state.copy()
+            for _ in range(2):
+                assert orc.run(x, env, 0.0, 180.0, mode=0 if mode == MODE_BLOCK else 1) == 0
+        e = err(x)
+        o50.append(np.percentile(e, 50))
+        o90.append(np.percentile(e, 90))
+    e_gpu = err(a)
     g50, g90 = np.percentile(e_gpu, 50), np.percentile(e_gpu, 90)
-    o50, o90 = np.percentile(e_orc, 50), np.percentile(e_orc, 90)
-    print("mode", mode, "error in bar units (median, p90, max): hip %.3g %.3g %.3g oracle "
-          "%.3g %.3g %.3g" % (g50, g90, e_gpu.max(), o50, o90, e_orc.max()))
-    assert g50 <= 2.0 * max(o50, 1.0)
-    assert g90 <= 3.0 * max(o90, 1.0)
+    print("mode", mode, "error in bar units (median, p90): hip %.3g %.3g, oracle under "
+          "perturbation %s %s" % (g50, g90, np.round(o50, 3), np.round(o90, 3)))
+    assert g50 <= 2.0 * max(max(o50), 1.0)
+    assert g90 <= 2.0 * max(max(o90), 1.0)
 
 
 def test_cell_alone_equals_cell_in_mix():
