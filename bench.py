@@ -120,16 +120,31 @@ def serving(mech, solver, pool_state, env, rates, n_requests, seed=3):
     lat, span, failed = srv.replay(t_arr, np.zeros(n_requests, np.int32), sizes, off, dts, steps)
     c = srv.counters()
     srv.close()
+    work = sizes.astype(np.float64) * steps
+    by_size = {}
+    for lo, hi in ((1, 100), (100, 3000), (3000, 50001)):
+        m = (sizes >= lo) & (sizes < hi)
+        if np.count_nonzero(m) >= 5:
+            l50, l99 = np.percentile(lat[m], 50), np.percentile(lat[m], 99)
+            by_size["%d..%d cells" % (lo, hi - 1)] = {
+                "requests": int(np.count_nonzero(m)), "p50_ms": float(l50 * 1e3),
+                "p99_ms": float(l99 * 1e3), "p99_over_p50": float(l99 / l50)}
+    # slowdown: latency over the request's own share of saturated throughput
+    slow = lat / np.maximum(work / capacity, 1e-6)
     return {"requests": int(n_requests), "load": 0.7,
             "capacity_cell_timesteps_per_s": capacity,
             "throughput_cell_timesteps_per_s": float(np.sum(sizes * steps)) / span,
             "p50_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+            "p90_ms": float(np.percentile(lat, 90) * 1e3),
+            "by_request_size": by_size,
+            "slowdown_p50": float(np.percentile(slow, 50)),
+            "slowdown_p99": float(np.percentile(slow, 99)),
             "failed_requests": int(failed + failed_cap),
             "mean_batch_cells": c["mean_batch_cells"], "batches": c["batches"],
             "request_mix": "1..50000 cells log-uniform, dt U(60,600) s, 1..4 steps; gas "
-                           "mechanism only (the multi-phase mechanism is not built)"}
+                           "mechanism"}
 
 
 def plain_path(mech, n_cells, steps, dt):
@@ -205,6 +220,19 @@ def multiphase_leg(n_cells, steps, dt):
             "bdf_steps_per_cell_step": float(tot[0] / n_cells / steps),
             "what": "cb05scale + 8 bins x 30 aerosol species (SIMPOL x20, aqueous "
                     "equilibrium x10, ZSR water), per-cell BDF, rtol 1e-4"}
+
+
+def recorded_traffic(n_cells):
+    """HBM bytes per launch of k_cell_bdf, k_deriv and k_jac from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes on this workload
+    (calibrated on a device copy), or None when none was recorded for it."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic_cb05scale_100k.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    if rec["workload"]["n_cells"] != n_cells:
+        return None
+    return {k: v["bytes_per_launch"] for k, v in rec["kernels"].items()}
 
 
 def progress(msg):
@@ -314,8 +342,10 @@ def main():
     # per cell: state in+out, env in, rate data in+out, J history in+out
     hbm_bytes_per_launch = 8.0 * args.cells * (2 * mech.n_state + 2 + 2 * n_env
                                                + 2 * (solver.nnz + 2 * solver.nv))
+    traffic = recorded_traffic(args.cells)
     roofline = {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS,
+                "traffic": traffic["k_cell_bdf"] if traffic else None,
                 "kernel": "k_cell_bdf", "avg_launch_ms": kernel_s * 1e3 / args.steps,
                 "flops_per_launch": total_flops / args.steps,
                 "algorithmic_hbm_bytes_per_launch": hbm_bytes_per_launch,
@@ -329,10 +359,12 @@ def main():
     roofline_fj = {
         "deriv": {"bound": "hbm", "achieved": gbs_f, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": gbs_f / HBM_PEAK_GBS, "avg_launch_ms": ms_f,
-                  "bytes_per_cell": bytes_["f"]},
+                  "bytes_per_cell": bytes_["f"],
+                  "traffic": traffic["k_deriv"] if traffic else None},
         "jac": {"bound": "hbm", "achieved": gbs_j, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": gbs_j / HBM_PEAK_GBS, "avg_launch_ms": ms_j,
-                "bytes_per_cell": bytes_["jac"]}}
+                "bytes_per_cell": bytes_["jac"],
+                "traffic": traffic["k_jac"] if traffic else None}}
 
     plain = None
     if args.plain_cells > 0:

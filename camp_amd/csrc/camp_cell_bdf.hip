@@ -672,7 +672,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   cv.nv = nv;
   cv.s_state = base;
   cv.s_k = cv.s_state + P.ns;
-  cv.mpl = mp_lds(P, cv.s_k + P.n_env);
+  cv.mpl = mp_lds(P, cv.s_k + P.n_env, scratch ? scratch + P.lu_nnz + P.rnnz : nullptr);
   cv.s_rate = cv.s_k + P.n_env + mp_lds_doubles(P);
   double* v = cv.s_rate + 2 * (size_t)P.n_terms;
   cv.znb = v;

@@ -37,6 +37,8 @@ struct DevProgram {
   const int *fe_b, *je_b;
   const double *fe_c, *je_c;
   const int2* fe_tb;  // {term or slot index, guard | flags << 24} per f record
+  const int4* fe_rec;  // {fe_tb, coefficient bits} per f record (one load)
+  const int2* re_rec;  // {element, column} per entry of the row view
   // rate-constant program (rxn_update_env_state)
   const int *rx_type, *rx_fidx, *rx_eidx, *rx_nreact;
   const double* rx_float;
@@ -199,9 +201,11 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
   for (int i = lane; i < P.nv; i += kWave) {
     dd D = {0.0, 0.0};
     double S = 0.0;
+#pragma unroll 4
     for (int q = P.f_ptr[i]; q < P.f_ptr[i + 1]; ++q) {
-      int2 tb = P.fe_tb[q];
-      double c = P.fe_c[q];
+      int4 rec = P.fe_rec[q];
+      int2 tb = {rec.x, rec.y};
+      double c = __hiloint2double(rec.w, rec.z);
       int flags = tb.y >> 24;
       int g = (tb.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
       if (flags & 4) {  // multi-phase contribution slot (double-double)
@@ -221,9 +225,10 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
     if (S != 0.0) {
       if (use_est) {
         double acc = 0.0;
+#pragma unroll 4
         for (int q = P.r_ptr[i]; q < P.r_ptr[i + 1]; ++q) {
-          int j = P.r_col[q];
-          acc += Js[P.r_elem[q]] * (y[j] - Jy[j]);
+          int2 ec = P.re_rec[q];
+          acc += Js[ec.x] * (y[ec.y] - Jy[ec.y]);
         }
         double est = Jf[i] + acc;
         double inv = 1.0 / S;
@@ -317,32 +322,36 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     lu[P.m_to_lu[e]] = v;
   }
   wsync();
-  // The first 64 division positions and update records of pivot k+1 are
-  // fetched from global memory while pivot k is being eliminated, and a
-  // pivot's remaining update records are all loaded before any of them is
-  // applied, so the chain of 2*nv dependent LDS phases waits on global
-  // memory at most once per pivot.  Within one pivot every update has a
-  // distinct destination, so lanes never conflict.
+  // Pivot k+1's division positions (first 64) and update records (first
+  // 5 x 64) are fetched from global memory while pivot k is being
+  // eliminated, so the chain of 2*nv dependent LDS phases does not wait on
+  // global memory.  Within one pivot every update has a distinct
+  // destination, so lanes never conflict.
+  constexpr int R = 5;
   int lq0 = P.l_ptr[0], lq1 = P.l_ptr[1], uq0 = P.up_ptr[0], uq1 = P.up_ptr[1];
   int l_p = (lq0 + lane < lq1) ? P.l_pos[lq0 + lane] : -1;
-  uint64_t u_r = (uq0 + lane < uq1) ? P.up_rec[uq0 + lane] : ~0ull;
+  uint64_t cur[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int q = uq0 + r * kWave + lane;
+    cur[r] = q < uq1 ? P.up_rec[q] : ~0ull;
+  }
   int dpos = P.diag_pos[0];
   for (int k = 0; k < P.nv; ++k) {
     int nlq0 = lq1, nlq1 = 0, nuq0 = uq1, nuq1 = 0, n_lp = -1, ndpos = 0;
-    uint64_t n_ur = ~0ull;
+    uint64_t nxt[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) nxt[r] = ~0ull;
     if (k + 1 < P.nv) {
       nlq1 = P.l_ptr[k + 2];
       nuq1 = P.up_ptr[k + 2];
       ndpos = P.diag_pos[k + 1];
       if (nlq0 + lane < nlq1) n_lp = P.l_pos[nlq0 + lane];
-      if (nuq0 + lane < nuq1) n_ur = P.up_rec[nuq0 + lane];
-    }
-    // records beyond the first 64 of this pivot, up to 4 per lane in flight
-    uint64_t more[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int q = uq0 + (r + 1) * kWave + lane;
-      more[r] = q < uq1 ? P.up_rec[q] : ~0ull;
+      for (int r = 0; r < R; ++r) {
+        int q = nuq0 + r * kWave + lane;
+        if (q < nuq1) nxt[r] = P.up_rec[q];
+      }
     }
     double piv = lu[dpos];
     if (piv == 0.0 || !isfinite(piv)) return 1;
@@ -352,25 +361,23 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
       lu[p] = lu[p] / piv;
     }
     wsync();
-    if (u_r != ~0ull) {
-      int d = (int)(u_r & 0xFFFF), a = (int)((u_r >> 16) & 0xFFFF), b = (int)((u_r >> 32) & 0xFFFF);
-      lu[d] = lu[d] - lu[a] * lu[b];
-    }
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (more[r] != ~0ull) {
-        uint64_t x = more[r];
+    for (int r = 0; r < R; ++r)
+      if (cur[r] != ~0ull) {
+        uint64_t x = cur[r];
         int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
         lu[d] = lu[d] - lu[a] * lu[b];
       }
-    for (int q = uq0 + 5 * kWave + lane; q < uq1; q += kWave) {
+    for (int q = uq0 + R * kWave + lane; q < uq1; q += kWave) {
       uint64_t x = P.up_rec[q];
       int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
       lu[d] = lu[d] - lu[a] * lu[b];
     }
     wsync();
     lq0 = nlq0; lq1 = nlq1; uq0 = nuq0; uq1 = nuq1;
-    l_p = n_lp; u_r = n_ur; dpos = ndpos;
+    l_p = n_lp; dpos = ndpos;
+#pragma unroll
+    for (int r = 0; r < R; ++r) cur[r] = nxt[r];
   }
   return 0;
 }

@@ -52,8 +52,8 @@ __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const doubl
   int wpb = blockDim.x / kWave;
   double* s_state = smem + (size_t)wv * fj_lds_doubles(P);
   double* s_k = s_state + P.ns;
-  MpLds L = mp_lds(P, s_k + P.n_env);
   double* s_rate = s_k + P.n_env + mp_lds_doubles(P);
+  MpLds L = mp_lds(P, s_k + P.n_env, s_rate + 2 * (size_t)P.n_terms);
   for (int c = blockIdx.x * wpb + wv; c < nc; c += gridDim.x * wpb) {
     stage_cell(P, lane, state + (size_t)c * P.ns, rxn_env + (size_t)c * P.n_env, s_state, s_k);
     const double* yc = y + (size_t)c * P.nv;
@@ -85,8 +85,8 @@ __global__ void __launch_bounds__(256) k_jac(DevProgram P, int nc, const double*
   int wpb = blockDim.x / kWave;
   double* s_state = smem + (size_t)wv * fj_lds_doubles(P);
   double* s_k = s_state + P.ns;
-  MpLds L = mp_lds(P, s_k + P.n_env);
   double* s_rate = s_k + P.n_env + mp_lds_doubles(P);
+  MpLds L = mp_lds(P, s_k + P.n_env, s_rate + 2 * (size_t)P.n_terms);
   for (int c = blockIdx.x * wpb + wv; c < nc; c += gridDim.x * wpb) {
     stage_cell(P, lane, state + (size_t)c * P.ns, rxn_env + (size_t)c * P.n_env, s_state, s_k);
     if (cell_set_state(P, lane, y + (size_t)c * P.nv, s_state)) continue;

@@ -632,6 +632,16 @@ std::string Model::initialize() {
     P.j_guard.push_back(-1); P.j_rind.push_back(0);
   }
 
+  P.fe_rec.resize(2 * P.fe_tb.size());
+  for (size_t q = 0; q < P.fe_c.size(); ++q) {
+    uint64_t bits;
+    std::memcpy(&bits, &P.fe_c[q], sizeof(bits));
+    P.fe_rec[4 * q] = P.fe_tb[2 * q];
+    P.fe_rec[4 * q + 1] = P.fe_tb[2 * q + 1];
+    P.fe_rec[4 * q + 2] = (int)(uint32_t)(bits & 0xFFFFFFFFu);
+    P.fe_rec[4 * q + 3] = (int)(uint32_t)(bits >> 32);
+  }
+
   // row view of the solver pattern (entries of each row in column order)
   P.r_ptr.assign(nv + 1, 0);
   for (int e = 0; e < P.nnz; ++e) P.r_ptr[P.rowval[e] + 1]++;
@@ -646,6 +656,11 @@ std::string Model::initialize() {
         P.r_elem[fill[i]] = e;
         P.r_col[fill[i]++] = j;
       }
+  }
+  P.re_rec.resize(2 * (size_t)std::max(P.nnz, 1), 0);
+  for (int q = 0; q < P.nnz; ++q) {
+    P.re_rec[2 * q] = P.r_elem[q];
+    P.re_rec[2 * q + 1] = P.r_col[q];
   }
 
   // --- LU: Markowitz ordering, symbolic fill, per-pivot programs ----------

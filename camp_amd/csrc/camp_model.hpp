@@ -70,6 +70,10 @@ struct Program {
   // LDS), then per species a gather of fe_tb = {term index, or slot index
   // for slot records; fe_b} and fe_c
   std::vector<int> fe_tb;
+  // the same f records as one 16-byte load each: {term or slot, guard |
+  // flags << 24, coefficient (low, high word)}, and the row view of the
+  // solver pattern as {element, column} pairs
+  std::vector<int> fe_rec, re_rec;
   std::vector<int> colptr, rowval;                  // solver pattern (dep ids)
   std::vector<int> r_ptr, r_elem, r_col;            // row view of the pattern
   // LU (positions in the permuted LU value array)

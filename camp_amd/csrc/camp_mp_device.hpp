@@ -388,19 +388,26 @@ struct MpLds {
   double *sub_env, *aero, *slot, *jsc, *jp;
 };
 
+// Per-cell extras kept in LDS (environment, aerosol state, f slots, the
+// parameter Jacobian); the per-instance Jacobian scalars (used only while J
+// is assembled) get their own mp_jsc_doubles(P) region, in LDS or in
+// per-wavefront global scratch as the caller places it.
 __host__ __device__ inline size_t mp_lds_doubles(const DevProgram& P) {
   if (!P.mp) return 0;
-  return (size_t)P.n_sub_env + kAeroSlots * (size_t)P.n_pi + 2 * (size_t)P.n_slots +
-         kSpScalars * (size_t)P.n_sp + kAqScalars * (size_t)P.n_aq + P.pnnz;
+  return (size_t)P.n_sub_env + kAeroSlots * (size_t)P.n_pi + 2 * (size_t)P.n_slots + P.pnnz;
+}
+__host__ __device__ inline size_t mp_jsc_doubles(const DevProgram& P) {
+  if (!P.mp) return 0;
+  return kSpScalars * (size_t)P.n_sp + kAqScalars * (size_t)P.n_aq;
 }
 
-__device__ __forceinline__ MpLds mp_lds(const DevProgram& P, double* base) {
+__device__ __forceinline__ MpLds mp_lds(const DevProgram& P, double* base, double* jsc) {
   MpLds L;
   L.sub_env = base;
   L.aero = L.sub_env + P.n_sub_env;
   L.slot = L.aero + kAeroSlots * P.n_pi;
-  L.jsc = L.slot + 2 * P.n_slots;
-  L.jp = L.jsc + kSpScalars * P.n_sp + kAqScalars * P.n_aq;
+  L.jp = L.slot + 2 * P.n_slots;
+  L.jsc = jsc;
   return L;
 }
 

@@ -77,6 +77,8 @@ void Solver::setup_device() {
   P.fe_b = dev_upload(G.fe_b, owned);
   P.fe_c = dev_upload(G.fe_c, owned);
   P.fe_tb = reinterpret_cast<const int2*>(dev_upload(G.fe_tb, owned));
+  P.fe_rec = reinterpret_cast<const int4*>(dev_upload(G.fe_rec, owned));
+  P.re_rec = reinterpret_cast<const int2*>(dev_upload(G.re_rec, owned));
   P.je_a = reinterpret_cast<const int4*>(dev_upload(G.je_a, owned));
   P.je_b = dev_upload(G.je_b, owned);
   P.je_c = dev_upload(G.je_c, owned);
