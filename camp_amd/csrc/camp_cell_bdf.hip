@@ -42,6 +42,9 @@ __device__ __forceinline__ double rpoweri(double b, int e) {
   return e < 0 ? 1.0 / p : p;
 }
 
+// MP selects the multi-phase instantiation: gas-phase programs run a kernel
+// without the aerosol code (smaller code and register footprint).
+template <bool MP>
 struct CellCv {
   DevProgram P;
   int lane, nv;
@@ -85,7 +88,7 @@ struct CellCv {
     double ts = next_h > 0.0 ? next_h : init_h;
     int bad = cell_set_state(P, lane, yv, s_state);
     if (!bad) {
-      if (P.mp) mp_f_prologue(P, lane, s_state, s_k, mpl);
+      if (MP) mp_f_prologue(P, lane, s_state, s_k, mpl);
       cell_rates(P, lane, s_state, s_k, s_rate);
       wsync();
       cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, true, ts, out, mpl.slot);
@@ -100,7 +103,7 @@ struct CellCv {
     long long t0 = tick();
     double ts = next_h > 0.0 ? next_h : init_h;
     if (cell_set_state(P, lane, yv, s_state)) return 1;
-    if (P.mp) {
+    if (MP) {
       mp_f_prologue(P, lane, s_state, s_k, mpl);
       cell_rates(P, lane, s_state, s_k, s_rate);
       wsync();
@@ -655,6 +658,7 @@ struct CellCv {
 
 }  // namespace
 
+template <bool MP>
 __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   extern __shared__ double smem[];
   const DevProgram& P = a.P;
@@ -666,7 +670,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
                         ? a.wave_scratch + ((size_t)blockIdx.x * (blockDim.x / kWave) + wv) *
                                                cell_bdf_wave_scratch(P)
                         : nullptr;
-  CellCv cv;
+  CellCv<MP> cv;
   cv.P = P;
   cv.lane = lane;
   cv.nv = nv;
@@ -690,7 +694,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   // Gas-phase cells keep their J history (J_solver, J_state, J_deriv) in
   // LDS for the whole solve; multi-phase cells use it in place in HBM.
   double *h_js = nullptr, *h_jy = nullptr, *h_jf = nullptr;
-  if (P.mp) {
+  if (MP) {
     cv.lu = scratch;
     cv.Jr = scratch + P.lu_nnz;
   } else {
@@ -724,7 +728,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
       double* g_js = a.Js + (size_t)cell * P.nnz;
       double* g_jy = a.Jy + (size_t)cell * nv;
       double* g_jf = a.Jf + (size_t)cell * nv;
-      if (P.mp) {
+      if (MP) {
         cv.Js = g_js;
         cv.Jy = g_jy;
         cv.Jf = g_jf;
@@ -754,7 +758,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
       double T = a.env[2 * cell], Pa = a.env[2 * cell + 1];
       cell_rate_constants(P, lane, T, Pa, cv.s_k);
       wsync();
-      if (P.mp) mp_cell_env(P, lane, T, Pa, cv.mpl);
+      if (MP) mp_cell_env(P, lane, T, Pa, cv.mpl);
       for (int i = lane; i < P.n_env; i += kWave) renv_c[i] = cv.s_k[i];
       for (int i = lane; i < nv; i += kWave) {
         double s = cv.s_state[P.dep_state[i]];
@@ -768,9 +772,9 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
         for (int i = lane; i < nv; i += kWave) {
           double yv = cv.y[i];
           state_c[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
-          if (P.mp) cv.s_state[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
+          if (MP) cv.s_state[P.dep_state[i]] = yv > 0.0 ? yv : 0.0;
         }
-        if (P.mp) {
+        if (MP) {
           // the post-solve sub_model_calculate (camp_solver.c:685-687) on the
           // final state; the sub models' outputs go back to the state array
           wsync();
@@ -780,7 +784,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
           }
         }
       }
-      if (!P.mp) {
+      if (!MP) {
         wsync();
         for (int e = lane; e < P.nnz; e += kWave) g_js[e] = h_js[e];
         for (int i = lane; i < nv; i += kWave) {
@@ -807,22 +811,29 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
 
 // Cells of multi-phase programs need more than 64 KiB of LDS per block;
 // gfx950 provides 160 KiB per CU, opted into per kernel.
-static void allow_lds(size_t lds) {
+static const void* cell_bdf_fn(bool mp) {
+  return mp ? (const void*)k_cell_bdf<true> : (const void*)k_cell_bdf<false>;
+}
+static void allow_lds(bool mp, size_t lds) {
   if (lds > 65536)
-    (void)hipFuncSetAttribute((const void*)k_cell_bdf,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute(cell_bdf_fn(mp), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
 }
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s) {
-  allow_lds(lds);
-  hipLaunchKernelGGL(k_cell_bdf, dim3(grid), dim3(block), lds, s, a);
+  allow_lds(a.P.mp != 0, lds);
+  if (a.P.mp)
+    hipLaunchKernelGGL(k_cell_bdf<true>, dim3(grid), dim3(block), lds, s, a);
+  else
+    hipLaunchKernelGGL(k_cell_bdf<false>, dim3(grid), dim3(block), lds, s, a);
 }
 
-int cell_bdf_blocks_per_cu(int block, size_t lds) {
+int cell_bdf_blocks_per_cu(bool mp, int block, size_t lds) {
   if (block > 256 || lds > 160 * 1024) return 0;
-  allow_lds(lds);
+  allow_lds(mp, lds);
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cell_bdf, block, lds) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, cell_bdf_fn(mp), block, lds) != hipSuccess)
+    return 0;
   return n;
 }
 

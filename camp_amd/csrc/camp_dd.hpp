@@ -102,7 +102,7 @@ __host__ __device__ __forceinline__ dd inv_factorial(int n) {
 // x = k ln2 + r, |r| <= ln2/2; e^(r/512) - 1 by its Taylor series to
 // degree 10 (truncation < 2^-110), then nine squarings of (1 + u)
 // carried as u <- 2u + u^2.
-__host__ __device__ inline dd dd_expm1_reduced(dd x, int* k_out) {
+__host__ __device__ inline __noinline__ dd dd_expm1_reduced(dd x, int* k_out) {
   double k = rint(x.hi / kLn2Hi);
   dd r = dd_sub(x, dd_mul_d(dd{kLn2Hi, kLn2Lo}, k));
   r = dd_ldexp(r, -9);
@@ -114,14 +114,14 @@ __host__ __device__ inline dd dd_expm1_reduced(dd x, int* k_out) {
   return u;
 }
 
-__host__ __device__ inline dd dd_exp(dd x) {
+__host__ __device__ inline __noinline__ dd dd_exp(dd x) {
   int k;
   dd u = dd_expm1_reduced(x, &k);
   return dd_ldexp(dd_add(dd_from(1.0), u), k);
 }
 
 // correctly rounded e^x
-__host__ __device__ inline double cr_exp(double x) {
+__host__ __device__ inline __noinline__ double cr_exp(double x) {
   if (!(x == x)) return x;
   if (x > 709.8) return HUGE_VAL;
   if (x < -745.2) return 0.0;
@@ -134,7 +134,7 @@ __host__ __device__ inline double cr_exp(double x) {
 // natural log of a positive finite double, to ~2^-100: x = m 2^e, then
 // one Newton step y = y0 + m e^(-y0) - 1 (minus half its square) on the
 // library's log(m), whose error is squared away
-__host__ __device__ inline dd dd_log(double x) {
+__host__ __device__ inline __noinline__ dd dd_log(double x) {
   int e = 0;
   double m = x;
   if (!(x > 0.5 && x < 2.0)) m = frexp(x, &e);
@@ -147,21 +147,36 @@ __host__ __device__ inline dd dd_log(double x) {
   return y;
 }
 
-__host__ __device__ inline double cr_log(double x) {
+__host__ __device__ inline __noinline__ double cr_log(double x) {
   if (!(x > 0.0) || x == HUGE_VAL) return log(x);
   if (x == 1.0) return 0.0;
   return dd_round(dd_log(x));
 }
 
-__host__ __device__ inline double cr_log10(double x) {
+__host__ __device__ inline __noinline__ double cr_log10(double x) {
   if (!(x > 0.0) || x == HUGE_VAL) return log10(x);
   if (x == 1.0) return 0.0;
   return dd_round(dd_div(dd_log(x), dd{kLn10Hi, kLn10Lo}));
 }
 
+// correctly rounded x^n for a small non-negative integer n (the integer
+// branch of cr_pow, inline for the aerosol code's per-evaluation powers)
+__host__ __device__ inline double cr_pow(double x, double y);
+__host__ __device__ __forceinline__ double cr_powi(double x, int n) {
+  if (n < 0) return cr_pow(x, (double)n);
+  if (n == 0) return 1.0;
+  if (n == 1) return x;
+  if (n == 2) return x * x;
+  double ax = fabs(x);
+  dd p = dd_from(ax);
+  for (int i = 1; i < n; ++i) p = dd_mul_d(p, ax);
+  double r = dd_round(p);
+  return (x < 0.0 && (n & 1)) ? -r : r;
+}
+
 // correctly rounded x^y for the cases the restated formulas meet: any y
 // for x > 0, integer y for x < 0, and the exact special cases
-__host__ __device__ inline double cr_pow(double x, double y) {
+__host__ __device__ inline __noinline__ double cr_pow(double x, double y) {
   if (y == 0.0 || x == 1.0) return 1.0;
   if (y == 1.0) return x;
   if (y == 2.0) return x * x;

@@ -31,6 +31,7 @@ struct DevProgram {
   const int *perm, *m_to_lu, *diag_pos, *e_diag;
   const int *l_ptr, *l_pos, *l_row, *u_ptr, *u_pos, *u_row, *up_ptr;
   const uint64_t* up_rec;  // LU update records: dst | l << 16 | u << 32
+  const int4* lu_hdr;       // per pivot: {l range, update range}, {diag position, -}
   const int *lu_rowptr, *lu_col;
   const uint64_t* row_mask;  // 4 per LU row; nullptr when nv > 128
   const int4 *fe_a, *je_a;  // packed gather records (see Program in camp_model.hpp)
@@ -322,60 +323,58 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     lu[P.m_to_lu[e]] = v;
   }
   wsync();
-  // Pivot k+1's division positions (first 64) and update records (first
-  // 5 x 64) are fetched from global memory while pivot k is being
-  // eliminated, so the chain of 2*nv dependent LDS phases does not wait on
-  // global memory.  Within one pivot every update has a distinct
-  // destination, so lanes never conflict.
+  // Software pipeline over pivots: pivot k+2's header, pivot k+1's first
+  // 64 division positions and first 5 x 64 update records are fetched from
+  // global memory while pivot k is eliminated, so no pivot waits on a
+  // global load.  Within one pivot every update has a distinct destination
+  // (neither the pivot's L column nor its U row), so lanes never conflict
+  // and each lane loads all its operands before storing.
   constexpr int R = 5;
-  int lq0 = P.l_ptr[0], lq1 = P.l_ptr[1], uq0 = P.up_ptr[0], uq1 = P.up_ptr[1];
-  int l_p = (lq0 + lane < lq1) ? P.l_pos[lq0 + lane] : -1;
+  int4 h0 = P.lu_hdr[0], h1 = P.lu_hdr[2];
+  int d0 = P.lu_hdr[1].x, d1 = P.lu_hdr[3].x;
+  int l_p = (h0.x + lane < h0.y) ? P.l_pos[h0.x + lane] : -1;
   uint64_t cur[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    int q = uq0 + r * kWave + lane;
-    cur[r] = q < uq1 ? P.up_rec[q] : ~0ull;
+    int q = h0.z + r * kWave + lane;
+    cur[r] = q < h0.w ? P.up_rec[q] : ~0ull;
   }
-  int dpos = P.diag_pos[0];
   for (int k = 0; k < P.nv; ++k) {
-    int nlq0 = lq1, nlq1 = 0, nuq0 = uq1, nuq1 = 0, n_lp = -1, ndpos = 0;
+    int4 h2 = P.lu_hdr[2 * (k + 2)];
+    int d2 = P.lu_hdr[2 * (k + 2) + 1].x;
+    int n_lp = (k + 1 < P.nv && h1.x + lane < h1.y) ? P.l_pos[h1.x + lane] : -1;
     uint64_t nxt[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) nxt[r] = ~0ull;
-    if (k + 1 < P.nv) {
-      nlq1 = P.l_ptr[k + 2];
-      nuq1 = P.up_ptr[k + 2];
-      ndpos = P.diag_pos[k + 1];
-      if (nlq0 + lane < nlq1) n_lp = P.l_pos[nlq0 + lane];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        int q = nuq0 + r * kWave + lane;
-        if (q < nuq1) nxt[r] = P.up_rec[q];
-      }
+    for (int r = 0; r < R; ++r) {
+      int q = h1.z + r * kWave + lane;
+      nxt[r] = (k + 1 < P.nv && q < h1.w) ? P.up_rec[q] : ~0ull;
     }
-    double piv = lu[dpos];
+    double piv = lu[d0];
     if (piv == 0.0 || !isfinite(piv)) return 1;
     if (l_p >= 0) lu[l_p] = lu[l_p] / piv;
-    for (int q = lq0 + kWave + lane; q < lq1; q += kWave) {
+    for (int q = h0.x + kWave + lane; q < h0.y; q += kWave) {
       int p = P.l_pos[q];
       lu[p] = lu[p] / piv;
     }
     wsync();
+    double od[R], oa[R], ob[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      uint64_t x = cur[r] != ~0ull ? cur[r] : 0ull;
+      od[r] = lu[(int)(x & 0xFFFF)];
+      oa[r] = lu[(int)((x >> 16) & 0xFFFF)];
+      ob[r] = lu[(int)((x >> 32) & 0xFFFF)];
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (cur[r] != ~0ull) {
-        uint64_t x = cur[r];
-        int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
-        lu[d] = lu[d] - lu[a] * lu[b];
-      }
-    for (int q = uq0 + R * kWave + lane; q < uq1; q += kWave) {
+      if (cur[r] != ~0ull) lu[(int)(cur[r] & 0xFFFF)] = od[r] - oa[r] * ob[r];
+    for (int q = h0.z + R * kWave + lane; q < h0.w; q += kWave) {
       uint64_t x = P.up_rec[q];
       int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
       lu[d] = lu[d] - lu[a] * lu[b];
     }
     wsync();
-    lq0 = nlq0; lq1 = nlq1; uq0 = nuq0; uq1 = nuq1;
-    l_p = n_lp; dpos = ndpos;
+    h0 = h1; h1 = h2; d0 = d1; d1 = d2; l_p = n_lp;
 #pragma unroll
     for (int r = 0; r < R; ++r) cur[r] = nxt[r];
   }
@@ -426,35 +425,62 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
   const double* row0 = lu + base0;
   const double* row1 = lu + base1;
   int o0 = 0, o1 = 0;
-#pragma unroll 8
-  for (int k = 0; k < nv - 1; ++k) {
-    bool a0 = bit128(l0a, l0b, k), a1 = bit128(l1a, l1b, k);
-    double v0 = row0[o0], v1 = row1[o1];
-    double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
-    x0 = a0 ? x0 - v0 * xk : x0;
-    x1 = a1 ? x1 - v1 * xk : x1;
-    o0 += a0;
-    o1 += a1;
+  // Blocks of 8 pivots: the block's entry flags, positions and LDS loads
+  // come first (they depend only on the masks), then the 8-step chain.
+  constexpr int B = 8;
+  for (int kb = 0; kb < nv - 1; kb += B) {
+    bool a0[B], a1[B];
+    double v0[B], v1[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      int k = kb + u;
+      a0[u] = k < nv - 1 && bit128(l0a, l0b, k);
+      a1[u] = k < nv - 1 && bit128(l1a, l1b, k);
+      v0[u] = row0[o0];
+      v1[u] = row1[o1];
+      o0 += a0[u];
+      o1 += a1[u];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      int k = kb + u;
+      if (k >= nv - 1) break;
+      double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
+      x0 = a0[u] ? x0 - v0[u] * xk : x0;
+      x1 = a1[u] ? x1 - v1[u] * xk : x1;
+    }
   }
   // o0, o1 now index the diagonal of each row
   double d0 = has0 ? 1.0 / row0[o0] : 0.0;
   double d1 = has1 ? 1.0 / row1[o1] : 0.0;
   o0 = len0 - 1;
   o1 = len1 - 1;
-#pragma unroll 8
-  for (int k = nv - 1; k >= 0; --k) {
-    if (k < kWave) {
-      if (lane == k) x0 *= d0;
-    } else if (lane == k - kWave) {
-      x1 *= d1;
+  for (int kb = nv - 1; kb >= 0; kb -= B) {
+    bool a0[B], a1[B];
+    double v0[B], v1[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      int k = kb - u;
+      a0[u] = k >= 0 && bit128(u0a, u0b, k);
+      a1[u] = k >= 0 && bit128(u1a, u1b, k);
+      v0[u] = row0[o0];
+      v1[u] = row1[o1];
+      o0 -= a0[u];
+      o1 -= a1[u];
     }
-    bool a0 = bit128(u0a, u0b, k), a1 = bit128(u1a, u1b, k);
-    double v0 = row0[o0], v1 = row1[o1];
-    double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
-    x0 = a0 ? x0 - v0 * xk : x0;
-    x1 = a1 ? x1 - v1 * xk : x1;
-    o0 -= a0;
-    o1 -= a1;
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      int k = kb - u;
+      if (k < 0) break;
+      if (k < kWave) {
+        if (lane == k) x0 *= d0;
+      } else if (lane == k - kWave) {
+        x1 *= d1;
+      }
+      double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
+      x0 = a0[u] ? x0 - v0[u] * xk : x0;
+      x1 = a1[u] ? x1 - v1[u] * xk : x1;
+    }
   }
   wsync();
   if (has0) b[P.perm[r0]] = x0;

@@ -53,7 +53,7 @@ struct CellBdfArgs {
 };
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);
-int cell_bdf_blocks_per_cu(int block, size_t lds);
+int cell_bdf_blocks_per_cu(bool mp, int block, size_t lds);
 
 __global__ void k_rates(DevProgram P, int nc, const double* env, double* rxn_env);
 __global__ void k_deriv(DevProgram P, int nc, const double* y, const double* state,

@@ -739,6 +739,12 @@ std::string Model::initialize() {
   if (P.l_pos.empty()) { P.l_pos.push_back(0); P.l_row.push_back(0); }
   if (P.u_pos.empty()) { P.u_pos.push_back(0); P.u_row.push_back(0); }
   if (P.up_dst.empty()) { P.up_dst.push_back(0); P.up_l.push_back(0); P.up_u.push_back(0); }
+  P.lu_hdr.assign((size_t)8 * (nv + 2), 0);
+  for (int k = 0; k < nv; ++k) {
+    int* h = &P.lu_hdr[(size_t)8 * k];
+    h[0] = P.l_ptr[k]; h[1] = P.l_ptr[k + 1]; h[2] = P.up_ptr[k]; h[3] = P.up_ptr[k + 1];
+    h[4] = P.diag_pos[k];
+  }
   P.up_rec.resize(P.up_dst.size());
   for (size_t q = 0; q < P.up_dst.size(); ++q)
     P.up_rec[q] = (uint64_t)P.up_dst[q] | ((uint64_t)P.up_l[q] << 16) | ((uint64_t)P.up_u[q] << 32);

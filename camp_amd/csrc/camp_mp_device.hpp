@@ -35,7 +35,7 @@ __host__ __device__ __forceinline__ double mp_cond_rate(double D, double mfp, do
 __host__ __device__ __forceinline__ double mp_dcond_rate_dr(double D, double mfp, double r, double alpha) {
   double kn = mfp / r;
   double dcorr = (0.75 * alpha * mfp * (kn * kn + 2.0 * kn + 1.0 + (0.283 - 0.75) * alpha)) /
-                 cr_pow(r * (kn * kn + (1.0 + 0.283 * alpha) * kn + 0.75 * alpha), 2.0);
+                 cr_powi(r * (kn * kn + (1.0 + 0.283 * alpha) * kn + 0.75 * alpha), 2);
   return 4.0 * kPi * D * (mp_transition_corr(mfp, r, alpha) + r * dcorr);
 }
 
@@ -70,7 +70,7 @@ __host__ __device__ __forceinline__ void mp_phase_update(const DevProgram& P, in
   a[0] = mass_d;
   a[1] = dd_round(dd_div(dd_from(mass_d), moles));
   double r = P.pi_dp[i] / 2.0;
-  a[2] = vol * 3.0 / (4.0 * kPi) / cr_pow(r, 3.0);
+  a[2] = vol * 3.0 / (4.0 * kPi) / cr_powi(r, 3);
 }
 
 // sums of the state after the sub models ran (the getters' partials see it)
@@ -96,7 +96,7 @@ __host__ __device__ __forceinline__ void mp_zsr_water(const DevProgram& P, int z
     dd jaw = dd_gt(aw, low) ? aw : dd_from(low);
     double jaw_d = dd_round(jaw);
     dd mol = dd_from(0.0);
-    for (int k = 0; k < P.zp_ny[x]; ++k) mol = dd_add(mol, dd_from(P.zp_y[8 * x + k] * cr_pow(jaw_d, (double)k)));
+    for (int k = 0; k < P.zp_ny[x]; ++k) mol = dd_add(mol, dd_from(P.zp_y[8 * x + k] * cr_powi(jaw_d, k)));
     mol = dd_mul(mol, mol);
     // ion concentrations are double expressions (state / int / MW / 1000)
     dd cat = dd_from(s_state[P.zp_cat[x]] / (double)P.zp_ncat[x] / P.zp_catmw[x] / 1000.0);
@@ -123,8 +123,8 @@ __host__ __device__ __forceinline__ void mp_zsr_jac(const DevProgram& P, int z, 
     const double* Y = P.zp_y + 8 * x;
     dd mol = dd_from(Y[0]), dmol = dd_from(0.0);
     for (int k = 1; k < P.zp_ny[x]; ++k) {
-      mol = dd_add(mol, dd_from(Y[k] * cr_pow(jaw_d, (double)k)));
-      dmol = dd_add(dmol, dd_from(Y[k] * k * cr_pow(jaw_d, (double)(k - 1))));
+      mol = dd_add(mol, dd_from(Y[k] * cr_powi(jaw_d, k)));
+      dmol = dd_add(dmol, dd_from(Y[k] * k * cr_powi(jaw_d, k - 1)));
     }
     dmol = dd_mul(dmol, djaw);
     dd cat = dd_from(s_state[P.zp_cat[x]] / (double)P.zp_ncat[x] / P.zp_catmw[x] / 1000.0);
@@ -141,7 +141,7 @@ __host__ __device__ __forceinline__ void mp_zsr_jac(const DevProgram& P, int z, 
     dd dcdc = dd_div(dd_add(dd_mul(e_ac, e_ac), dd_mul(dd_mul(e_ac, e_aa), kc)), den);
     dd dcda = dd_div(dd_add(dd_mul(e_aa, e_aa), dd_mul(dd_mul(e_ac, e_aa), ka)), den);
     double mol_d = dd_round(mol);
-    double p3 = cr_pow(mol_d, 3.0), p2 = cr_pow(mol_d, 2.0);
+    double p3 = cr_powi(mol_d, 3), p2 = cr_powi(mol_d, 2);
     // J[...] += value: each a double += (long double) rounded to double
     // -2.0 * conc / pow(m, 3) * 1000.0 * dm and 1.0 / pow(m, 2) * 1000.0 * dc * dion,
     // evaluated left to right as the reference does
@@ -326,7 +326,7 @@ __host__ __device__ __forceinline__ dd mp_jac_records(const DevProgram& P, int e
           int pi = P.sp_pi[a.y];
           const double* ag = s_aero + kAeroSlots * pi;
           double dens = P.pm_dens[a.z], mw = P.pm_mw[a.z];
-          double factor = 3.0 / (4.0 * kPi) / cr_pow(P.pi_dp[pi] / 2.0, 3.0);
+          double factor = 3.0 / (4.0 * kPi) / cr_powi(P.pi_dp[pi] / 2.0, 3);
           double numj = 1.0 / dens * factor;
           double massj = 1.0;
           dd lm = {ag[3], ag[4]}, mo = {ag[5], ag[6]};

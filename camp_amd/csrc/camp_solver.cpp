@@ -72,6 +72,7 @@ void Solver::setup_device() {
   P.u_pos = dev_upload(G.u_pos, owned);
   P.u_row = dev_upload(G.u_row, owned);
   P.up_ptr = dev_upload(G.up_ptr, owned);
+  P.lu_hdr = reinterpret_cast<const int4*>(dev_upload(G.lu_hdr, owned));
   P.up_rec = dev_upload(G.up_rec, owned);
   P.fe_a = reinterpret_cast<const int4*>(dev_upload(G.fe_a, owned));
   P.fe_b = dev_upload(G.fe_b, owned);
@@ -213,7 +214,7 @@ void Solver::cell_launch_shape(long nq, int* grid, int* block, size_t* lds) cons
   int dev = 0, ncu = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  int per_cu = cell_bdf_blocks_per_cu(*block, *lds);
+  int per_cu = cell_bdf_blocks_per_cu(P.mp != 0, *block, *lds);
   if (per_cu <= 0) die("per-cell kernel does not fit on a compute unit (LDS)");
   *grid = (int)std::max<long>(1, std::min<long>((long)per_cu * ncu, (nq + cpb - 1) / cpb));
 }
@@ -1055,7 +1056,7 @@ extern "C" int camp_solver_set_cells_per_block(void* solver_data, int cpb) {
   if (cpb == 0) cpb = 1;
   if (cpb == 1 || cpb == 2 || cpb == 4 || cpb == 8 || cpb == 16 || cpb == 32) {
     size_t lds = (size_t)cpb * cell_bdf_lds_doubles(s->P) * sizeof(double);
-    if (!s->m.initialized || !s->stream || cell_bdf_blocks_per_cu(kWave * cpb, lds) > 0)
+    if (!s->m.initialized || !s->stream || cell_bdf_blocks_per_cu(s->P.mp != 0, kWave * cpb, lds) > 0)
       s->cells_per_block = cpb;
   }
   return s->cells_per_block;
@@ -1105,7 +1106,7 @@ extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
     int cpb = cand[k];
     size_t lds = (size_t)cpb * cell_bdf_lds_doubles(P) * sizeof(double);
     if (ms_out) ms_out[k] = -1.0;
-    if (cell_bdf_blocks_per_cu(kWave * cpb, lds) <= 0) continue;
+    if (cell_bdf_blocks_per_cu(P.mp != 0, kWave * cpb, lds) <= 0) continue;
     gather(d_state, P.ns, st);
     gather(d_env, 2, env);
     gather(s->d_renv, P.n_env, renv);
