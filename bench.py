@@ -10,11 +10,17 @@ Newton control, on state already resident in HBM.  W warmup steps from the
 seeded initial state (the first carries the stiff start-up transient) run
 untimed on the measured solver, then K consecutive steps are timed.
 
-Serving leg: the same mechanism behind the request server.  Requests of
-1..50,000 cells (log-uniform), dt uniform 60..600 s and 1..4 timesteps draw
-their cells from the spun-up 100k-cell pool.  Capacity is measured with all
-requests arriving at once; then Poisson arrivals at 70% of that capacity
-give p50/p99 request latency (submission to results on the host).
+Serving leg: the request server with the gas mechanism and the
+multi-phase mechanism registered.  Requests are gas (1..50,000 cells drawn
+from the spun-up 100k-cell pool) or multi-phase (1..2,000 cells of a seeded
+multi-phase pool) with equal odds, log-uniform sizes, dt uniform 60..600 s
+and 1..4 timesteps.  Capacity is measured with half the requests arriving at
+once; then Poisson arrivals at 70% of it give p50/p99 request latency
+(submission to results on the host).
+
+Further legs: the plain one-launch-per-RHS path against the persistent
+kernel on the 10,000-cell gas workload, the multi-phase workload (20,000
+cells), the autotuner's choice of cells per workgroup.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (k_cell_bdf,
 the persistent per-cell kernel): FP64 algorithmic flops (per-evaluation
@@ -90,61 +96,80 @@ def cpu_baseline(n_cells, warm, steps, dt, cores, per_core):
                           cores * per_core, cores, per_core, n_cells, steps, dt, warm)}
 
 
-def serving(mech, solver, pool_state, env, rates, n_requests, seed=3):
-    """Capacity at saturation, then Poisson arrivals at 70% of it."""
+def serving(mech, solver, pool_state, env, rates, n_requests, mp_max_cells, seed=3):
+    """Request server with both mechanisms: capacity at saturation, then
+    Poisson arrivals at 70% of it.  Each request is a gas request (1..50,000
+    cells of the spun-up 100k-cell pool) or a multi-phase request (1..
+    mp_max_cells cells of a seeded multi-phase pool) with equal probability,
+    dt U(60, 600) s and 1..4 timesteps."""
     from camp_amd.core import CampSolver
+    from camp_amd.multiphase import multiphase_cells, multiphase_mechanism
     from camp_amd.server import CampServer, base_rate_matrix
     n_pool = pool_state.shape[0]
     rng = np.random.default_rng(seed)
     srv = CampServer(n_streams=4, max_batch_cells=1 << 17)
-    mid = srv.add_mechanism(CampSolver(mech, 1))
-    srv.set_pool(mid, pool_state, env, base_rate_matrix(solver, rates, n_pool))
+    gas = srv.add_mechanism(CampSolver(mech, 1))
+    srv.set_pool(gas, pool_state, env, base_rate_matrix(solver, rates, n_pool))
+    mp_pool = 0
+    if mp_max_cells > 0:
+        mpm = multiphase_mechanism(1)
+        mp_pool = max(mp_max_cells, 2000)
+        mcells = multiphase_cells(mpm, mp_pool, seed=2)
+        msol = CampSolver(mpm, 1)
+        mpid = srv.add_mechanism(msol)
+        srv.set_pool(mpid, mcells.state, mcells.env,
+                     base_rate_matrix(msol, mcells.rates, mp_pool))
 
     def draw(k):
-        sizes = np.minimum(np.exp(rng.uniform(0.0, np.log(50000.0), size=k)), n_pool)
-        sizes = np.maximum(sizes.astype(np.int32), 1)
-        off = (rng.random(k) * (n_pool - sizes + 1)).astype(np.int64)
+        is_mp = (rng.random(k) < 0.5) if mp_pool else np.zeros(k, bool)
+        top = np.where(is_mp, float(mp_max_cells), 50000.0)
+        sizes = np.exp(rng.uniform(0.0, 1.0, size=k) * np.log(top))
+        sizes = np.maximum(np.minimum(sizes, np.where(is_mp, mp_pool, n_pool)).astype(np.int32), 1)
+        pool = np.where(is_mp, mp_pool, n_pool)
+        off = (rng.random(k) * (pool - sizes + 1)).astype(np.int64)
         dts = rng.uniform(60.0, 600.0, size=k)
         steps = rng.integers(1, 5, size=k).astype(np.int32)
-        return sizes, off, dts, steps
+        return is_mp.astype(np.int32), sizes, off, dts, steps
 
-    k_cap = max(8, n_requests // 4)
-    sizes, off, dts, steps = draw(k_cap)
-    lat, span, failed_cap = srv.replay(np.zeros(k_cap), np.zeros(k_cap, np.int32), sizes, off,
-                                       dts, steps)
+    k_cap = max(16, n_requests // 2)
+    m, sizes, off, dts, steps = draw(k_cap)
+    lat, span, failed_cap = srv.replay(np.zeros(k_cap), m, sizes, off, dts, steps)
+    cap_rps = k_cap / span
     capacity = float(np.sum(sizes * steps)) / span
-    sizes, off, dts, steps = draw(n_requests)
-    mean_work = (50000.0 - 1.0) / np.log(50000.0) * 2.5
-    rate = 0.7 * capacity / mean_work
-    t_arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
-    lat, span, failed = srv.replay(t_arr, np.zeros(n_requests, np.int32), sizes, off, dts, steps)
+    m, sizes, off, dts, steps = draw(n_requests)
+    t_arr = np.cumsum(rng.exponential(1.0 / (0.7 * cap_rps), size=n_requests))
+    lat, span, failed = srv.replay(t_arr, m, sizes, off, dts, steps)
     c = srv.counters()
     srv.close()
     work = sizes.astype(np.float64) * steps
     by_size = {}
     for lo, hi in ((1, 100), (100, 3000), (3000, 50001)):
-        m = (sizes >= lo) & (sizes < hi)
-        if np.count_nonzero(m) >= 5:
-            l50, l99 = np.percentile(lat[m], 50), np.percentile(lat[m], 99)
+        sel = (sizes >= lo) & (sizes < hi)
+        if np.count_nonzero(sel) >= 5:
+            l50, l99 = np.percentile(lat[sel], 50), np.percentile(lat[sel], 99)
             by_size["%d..%d cells" % (lo, hi - 1)] = {
-                "requests": int(np.count_nonzero(m)), "p50_ms": float(l50 * 1e3),
+                "requests": int(np.count_nonzero(sel)), "p50_ms": float(l50 * 1e3),
                 "p99_ms": float(l99 * 1e3), "p99_over_p50": float(l99 / l50)}
-    # slowdown: latency over the request's own share of saturated throughput
-    slow = lat / np.maximum(work / capacity, 1e-6)
+    by_mech = {}
+    for name, k in (("gas", 0), ("multiphase", 1)):
+        sel = m == k
+        if np.count_nonzero(sel) >= 5:
+            by_mech[name] = {"requests": int(np.count_nonzero(sel)),
+                             "p50_ms": float(np.percentile(lat[sel], 50) * 1e3),
+                             "p99_ms": float(np.percentile(lat[sel], 99) * 1e3)}
     return {"requests": int(n_requests), "load": 0.7,
+            "capacity_requests_per_s": cap_rps,
             "capacity_cell_timesteps_per_s": capacity,
-            "throughput_cell_timesteps_per_s": float(np.sum(sizes * steps)) / span,
+            "throughput_cell_timesteps_per_s": float(np.sum(work)) / span,
             "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p90_ms": float(np.percentile(lat, 90) * 1e3),
             "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
-            "p90_ms": float(np.percentile(lat, 90) * 1e3),
-            "by_request_size": by_size,
-            "slowdown_p50": float(np.percentile(slow, 50)),
-            "slowdown_p99": float(np.percentile(slow, 99)),
+            "by_request_size": by_size, "by_mechanism": by_mech,
             "failed_requests": int(failed + failed_cap),
             "mean_batch_cells": c["mean_batch_cells"], "batches": c["batches"],
-            "request_mix": "1..50000 cells log-uniform, dt U(60,600) s, 1..4 steps; gas "
-                           "mechanism"}
+            "request_mix": "gas 1..50000 cells or multi-phase 1..%d cells (equal odds), "
+                           "log-uniform, dt U(60,600) s, 1..4 steps" % mp_max_cells}
 
 
 def plain_path(mech, n_cells, steps, dt):
@@ -260,8 +285,10 @@ def main():
                     help="only copy this many MiB device-to-device three times (the "
                          "known-size kernel that calibrates rocprofv3's FETCH_SIZE and "
                          "WRITE_SIZE counters) and exit")
-    ap.add_argument("--serve-requests", type=int, default=-1,
-                    help="requests in the serving leg (-1: sized to about 40 s, 0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests in the serving leg (0: skip)")
+    ap.add_argument("--serve-mp-cells", type=int, default=2000,
+                    help="largest multi-phase request (0: gas requests only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (cells shard "
@@ -375,14 +402,10 @@ def main():
         mpleg = multiphase_leg(args.mp_cells, args.mp_steps, args.dt)
 
     serve = None
-    if args.serve_requests != 0:
-        n_req = args.serve_requests
-        if n_req < 0:
-            # about 40 s of work at the measured throughput, within 50..10000
-            n_req = int(np.clip(40.0 * value / ((50000.0 - 1.0) / np.log(50000.0) * 2.5),
-                                50, 10000))
-        progress("serving leg: %d requests" % n_req)
-        serve = serving(mech, solver, st.cpu().numpy(), cells.env, cells.rates, n_req)
+    if args.serve_requests > 0:
+        progress("serving leg: %d requests" % args.serve_requests)
+        serve = serving(mech, solver, st.cpu().numpy(), cells.env, cells.rates,
+                        args.serve_requests, args.serve_mp_cells)
         progress("serving leg done")
 
     cpu = None

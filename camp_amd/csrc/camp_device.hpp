@@ -40,6 +40,10 @@ struct DevProgram {
   const int2* fe_tb;  // {term or slot index, guard | flags << 24} per f record
   const int4* fe_rec;  // {fe_tb, coefficient bits} per f record (one load)
   const int2* re_rec;  // {element, column} per entry of the row view
+  const int* fs_species;             // lane slot -> species of the f gather
+  const int4* fl_hdr;                // per pass: fe_ell offset, width, re_ell offset, width
+  const int4* fe_ell;                // f records in slot-major (ELL) order
+  const int2* re_ell;                // row-view entries in slot-major order
   // rate-constant program (rxn_update_env_state)
   const int *rx_type, *rx_fidx, *rx_eidx, *rx_nreact;
   const double* rx_float;
@@ -191,24 +195,30 @@ __host__ __device__ __forceinline__ void cell_rates(const DevProgram& P, int lan
   }
 }
 
-// Phase 2 (after cell_rates and a wave sync): lane i gathers species i's
-// records -- the term's rate times the yield (or -1), subject to the
-// negative-yield guard, or a multi-phase slot -- into D and S.
+// Phase 2 (after cell_rates and a wave sync): each lane gathers one
+// species' records -- the term's rate times the yield (or -1), subject to
+// the negative-yield guard, or a multi-phase slot -- into D and S, in the
+// species' record order.  Records are read in ELL order (record j of every
+// lane contiguous), species heaviest first so the second pass is short.
 __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const double* s_state,
                                            const double* s_rate, const double* y,
                                            const double* Js, const double* Jy,
                                            const double* Jf, bool use_est, double h,
                                            double* out, const double* s_slot = nullptr) {
-  for (int i = lane; i < P.nv; i += kWave) {
+  for (int pass = 0; pass * kWave < P.nv; ++pass) {
+    int slot = pass * kWave + lane;
+    int i = slot < P.nv ? P.fs_species[slot] : -1;
+    int4 hd = P.fl_hdr[pass];
     dd D = {0.0, 0.0};
     double S = 0.0;
 #pragma unroll 4
-    for (int q = P.f_ptr[i]; q < P.f_ptr[i + 1]; ++q) {
-      int4 rec = P.fe_rec[q];
+    for (int j = 0; j < hd.y; ++j) {
+      int4 rec = P.fe_ell[hd.x + j * kWave + lane];
       int2 tb = {rec.x, rec.y};
       double c = __hiloint2double(rec.w, rec.z);
       int flags = tb.y >> 24;
       int g = (tb.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
+      if (flags & 8) continue;   // padding
       if (flags & 4) {  // multi-phase contribution slot (double-double)
         dd v = {s_slot[2 * tb.x], s_slot[2 * tb.x + 1]};
         D = dd_add(D, v);
@@ -227,8 +237,9 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
       if (use_est) {
         double acc = 0.0;
 #pragma unroll 4
-        for (int q = P.r_ptr[i]; q < P.r_ptr[i + 1]; ++q) {
-          int2 ec = P.re_rec[q];
+        for (int j = 0; j < hd.w; ++j) {
+          int2 ec = P.re_ell[hd.z + j * kWave + lane];
+          if (ec.x < 0) continue;
           acc += Js[ec.x] * (y[ec.y] - Jy[ec.y]);
         }
         double est = Jf[i] + acc;
@@ -239,7 +250,7 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
         v = D.hi;
       }
     }
-    out[i] = v;
+    if (i >= 0) out[i] = v;
   }
 }
 
@@ -391,15 +402,50 @@ __device__ __forceinline__ bool bit128(uint64_t lo, uint64_t hi, int k) {
   return ((k < 64 ? lo : hi) >> (k & 63)) & 1ull;
 }
 
+// One block of up to 8 pivots of a triangular sweep whose pivot values sit
+// in register X of their owner lanes (lane k - k_base): the block's entry
+// weights come first (LDS loads that depend only on the masks), then the
+// chain, one readlane pair and one FMA per row and pivot.  DIR is +1
+// (forward, L) or -1 (backward, U, weights pre-scaled by the row's inverse
+// diagonal); o0/o1 walk the rows' stored entries.
+template <int DIR, bool X1>
+__device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t m0a, uint64_t m0b,
+                                          uint64_t m1a, uint64_t m1b, const double* row0,
+                                          const double* row1, int& o0, int& o1, double s0,
+                                          double s1, double& x0, double& x1) {
+  constexpr int B = 8;
+  double w0[B], w1[B];
+#pragma unroll
+  for (int u = 0; u < B; ++u) {
+    int k = kb + DIR * u;
+    bool in = DIR > 0 ? k < kend : k >= kend;
+    bool a0 = in && bit128(m0a, m0b, k), a1 = in && bit128(m1a, m1b, k);
+    double v0 = row0[o0], v1 = row1[o1];
+    w0[u] = a0 ? v0 * s0 : 0.0;
+    w1[u] = a1 ? v1 * s1 : 0.0;
+    o0 += DIR * (int)a0;
+    o1 += DIR * (int)a1;
+  }
+#pragma unroll
+  for (int u = 0; u < B; ++u) {
+    int k = kb + DIR * u;
+    if (DIR > 0 ? k >= kend : k < kend) break;
+    double xk = readlane_d(X1 ? x1 : x0, k - k_base);
+    x0 = fma(-w0[u], xk, x0);
+    x1 = fma(-w1[u], xk, x1);
+  }
+}
+
 // Triangular solves with the solution vector in registers (nv <= 128):
 // lane t owns permuted rows t and t+64.  Pivot k's value is broadcast with
 // v_readlane (no LDS round trip, no barrier), so the dependency chain per
-// pivot is one readlane and one FMA.  Each lane walks its rows' stored
+// pivot is one readlane pair and one FMA.  Each lane walks its rows' stored
 // entries in order: whether row r has an entry in column k is a bit of the
 // row's 128-bit L or U mask, and the entry's position is a running count,
-// so the LU value each lane needs is an LDS load that depends on neither
-// the solution nor a table lookup and is issued ahead of the chain.  The
-// diagonal is applied as a reciprocal by the pivot's own lane.
+// so the LU values each lane needs are LDS loads that depend on neither the
+// solution nor a table lookup and are issued ahead of the chain.  The
+// backward sweep solves D^-1 U x = D^-1 y, so no division sits on the chain.
+// Sweeps are split at pivot 64 so each loop reads one register.
 __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane, const double* lu,
                                                   double* b) {
   const int nv = P.nv;
@@ -424,64 +470,25 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
   }
   const double* row0 = lu + base0;
   const double* row1 = lu + base1;
-  int o0 = 0, o1 = 0;
-  // Blocks of 8 pivots: the block's entry flags, positions and LDS loads
-  // come first (they depend only on the masks), then the 8-step chain.
   constexpr int B = 8;
-  for (int kb = 0; kb < nv - 1; kb += B) {
-    bool a0[B], a1[B];
-    double v0[B], v1[B];
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-      int k = kb + u;
-      a0[u] = k < nv - 1 && bit128(l0a, l0b, k);
-      a1[u] = k < nv - 1 && bit128(l1a, l1b, k);
-      v0[u] = row0[o0];
-      v1[u] = row1[o1];
-      o0 += a0[u];
-      o1 += a1[u];
-    }
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-      int k = kb + u;
-      if (k >= nv - 1) break;
-      double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
-      x0 = a0[u] ? x0 - v0[u] * xk : x0;
-      x1 = a1[u] ? x1 - v1[u] * xk : x1;
-    }
-  }
+  int o0 = 0, o1 = 0;
+  const int kf = nv - 1;  // forward pivots 0 .. nv-2
+  const int split = kf < kWave ? kf : kWave;
+  for (int kb = 0; kb < split; kb += B)
+    tri_block<1, false>(kb, split, 0, l0a, l0b, l1a, l1b, row0, row1, o0, o1, 1.0, 1.0, x0, x1);
+  for (int kb = split; kb < kf; kb += B)
+    tri_block<1, true>(kb, kf, kWave, l0a, l0b, l1a, l1b, row0, row1, o0, o1, 1.0, 1.0, x0, x1);
   // o0, o1 now index the diagonal of each row
   double d0 = has0 ? 1.0 / row0[o0] : 0.0;
   double d1 = has1 ? 1.0 / row1[o1] : 0.0;
+  x0 *= d0;
+  x1 *= d1;
   o0 = len0 - 1;
   o1 = len1 - 1;
-  for (int kb = nv - 1; kb >= 0; kb -= B) {
-    bool a0[B], a1[B];
-    double v0[B], v1[B];
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-      int k = kb - u;
-      a0[u] = k >= 0 && bit128(u0a, u0b, k);
-      a1[u] = k >= 0 && bit128(u1a, u1b, k);
-      v0[u] = row0[o0];
-      v1[u] = row1[o1];
-      o0 -= a0[u];
-      o1 -= a1[u];
-    }
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-      int k = kb - u;
-      if (k < 0) break;
-      if (k < kWave) {
-        if (lane == k) x0 *= d0;
-      } else if (lane == k - kWave) {
-        x1 *= d1;
-      }
-      double xk = k < kWave ? readlane_d(x0, k) : readlane_d(x1, k - kWave);
-      x0 = a0[u] ? x0 - v0[u] * xk : x0;
-      x1 = a1[u] ? x1 - v1[u] * xk : x1;
-    }
-  }
+  for (int kb = nv - 1; kb >= kWave; kb -= B)
+    tri_block<-1, true>(kb, kWave, kWave, u0a, u0b, u1a, u1b, row0, row1, o0, o1, d0, d1, x0, x1);
+  for (int kb = (nv - 1 < kWave - 1 ? nv - 1 : kWave - 1); kb >= 0; kb -= B)
+    tri_block<-1, false>(kb, 0, 0, u0a, u0b, u1a, u1b, row0, row1, o0, o1, d0, d1, x0, x1);
   wsync();
   if (has0) b[P.perm[r0]] = x0;
   if (has1) b[P.perm[r1]] = x1;

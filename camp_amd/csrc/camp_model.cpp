@@ -662,6 +662,59 @@ std::string Model::initialize() {
     P.re_rec[2 * q] = P.r_elem[q];
     P.re_rec[2 * q + 1] = P.r_col[q];
   }
+  {
+    const int W = 64;
+    std::vector<int> order(nv);
+    for (int i = 0; i < nv; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return P.f_ptr[a + 1] - P.f_ptr[a] > P.f_ptr[b + 1] - P.f_ptr[b];
+    });
+    P.fs_species = order;
+    P.fl_hdr.clear();
+    P.fe_ell.clear();
+    P.re_ell.clear();
+    int npass = (nv + W - 1) / W;
+    for (int pass = 0; pass < npass; ++pass) {
+      int cf = 0, cr = 0;
+      for (int l = 0; l < W && pass * W + l < nv; ++l) {
+        int i = order[pass * W + l];
+        cf = std::max(cf, P.f_ptr[i + 1] - P.f_ptr[i]);
+        cr = std::max(cr, P.r_ptr[i + 1] - P.r_ptr[i]);
+      }
+      P.fl_hdr.push_back((int)(P.fe_ell.size() / 4));
+      P.fl_hdr.push_back(cf);
+      P.fl_hdr.push_back((int)(P.re_ell.size() / 2));
+      P.fl_hdr.push_back(cr);
+      for (int j = 0; j < cf; ++j)
+        for (int l = 0; l < W; ++l) {
+          int slot = pass * W + l, q = -1;
+          if (slot < nv) {
+            int i = order[slot];
+            if (j < P.f_ptr[i + 1] - P.f_ptr[i]) q = P.f_ptr[i] + j;
+          }
+          if (q >= 0) {
+            for (int k = 0; k < 4; ++k) P.fe_ell.push_back(P.fe_rec[4 * (size_t)q + k]);
+          } else {
+            P.fe_ell.push_back(0);
+            P.fe_ell.push_back((8 << 24) | 0xFFFFFF);
+            P.fe_ell.push_back(0);
+            P.fe_ell.push_back(0);
+          }
+        }
+      for (int j = 0; j < cr; ++j)
+        for (int l = 0; l < W; ++l) {
+          int slot = pass * W + l, q = -1;
+          if (slot < nv) {
+            int i = order[slot];
+            if (j < P.r_ptr[i + 1] - P.r_ptr[i]) q = P.r_ptr[i] + j;
+          }
+          P.re_ell.push_back(q >= 0 ? P.r_elem[q] : -1);
+          P.re_ell.push_back(q >= 0 ? P.r_col[q] : -1);
+        }
+    }
+    if (P.fe_ell.empty()) P.fe_ell.assign(4, 0);
+    if (P.re_ell.empty()) P.re_ell.assign(2, -1);
+  }
 
   // --- LU: Markowitz ordering, symbolic fill, per-pivot programs ----------
   P.perm = markowitz_order(nv, P.colptr, P.rowval);

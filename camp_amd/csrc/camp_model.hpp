@@ -74,6 +74,14 @@ struct Program {
   // flags << 24, coefficient (low, high word)}, and the row view of the
   // solver pattern as {element, column} pairs
   std::vector<int> fe_rec, re_rec;
+  // The gathers of f(y) in lane-slot (ELL) order, so that a wavefront's
+  // record loads are contiguous: species are assigned to slots heaviest
+  // first (slot s = pass * 64 + lane; the light tail goes to the second
+  // pass), fs_species[slot] is the species, and per pass fl_hdr = {offset
+  // and width of fe_ell (int4 records, padding flagged 8), offset and width
+  // of re_ell ({element, column}, padding -1)}; entry j of lane l is at
+  // offset + j * 64 + l.
+  std::vector<int> fs_species, fl_hdr, fe_ell, re_ell;
   std::vector<int> colptr, rowval;                  // solver pattern (dep ids)
   std::vector<int> r_ptr, r_elem, r_col;            // row view of the pattern
   // LU (positions in the permuted LU value array)

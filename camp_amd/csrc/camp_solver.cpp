@@ -80,6 +80,10 @@ void Solver::setup_device() {
   P.fe_tb = reinterpret_cast<const int2*>(dev_upload(G.fe_tb, owned));
   P.fe_rec = reinterpret_cast<const int4*>(dev_upload(G.fe_rec, owned));
   P.re_rec = reinterpret_cast<const int2*>(dev_upload(G.re_rec, owned));
+  P.fs_species = dev_upload(G.fs_species, owned);
+  P.fl_hdr = reinterpret_cast<const int4*>(dev_upload(G.fl_hdr, owned));
+  P.fe_ell = reinterpret_cast<const int4*>(dev_upload(G.fe_ell, owned));
+  P.re_ell = reinterpret_cast<const int2*>(dev_upload(G.re_ell, owned));
   P.je_a = reinterpret_cast<const int4*>(dev_upload(G.je_a, owned));
   P.je_b = dev_upload(G.je_b, owned);
   P.je_c = dev_upload(G.je_c, owned);
