@@ -46,7 +46,10 @@ __device__ __forceinline__ double rpoweri(double b, int e) {
 // without the aerosol code (smaller code and register footprint).
 template <bool MP>
 struct CellCv {
-  DevProgram P;
+  // the kernel argument's program (kernarg memory: uniform, read-only, so
+  // its fields are reloaded with scalar loads rather than kept in registers)
+  const DevProgram& P;
+  __device__ explicit CellCv(const DevProgram& prog) : P(prog) {}
   int lane, nv;
   double *s_state, *s_k, *s_rate, *znb, *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2,
       *w, *lu;
@@ -670,8 +673,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
                         ? a.wave_scratch + ((size_t)blockIdx.x * (blockDim.x / kWave) + wv) *
                                                cell_bdf_wave_scratch(P)
                         : nullptr;
-  CellCv<MP> cv;
-  cv.P = P;
+  CellCv<MP> cv(P);
   cv.lane = lane;
   cv.nv = nv;
   cv.s_state = base;

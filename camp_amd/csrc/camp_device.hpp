@@ -41,6 +41,7 @@ struct DevProgram {
   const int4* fe_rec;  // {fe_tb, coefficient bits} per f record (one load)
   const int2* re_rec;  // {element, column} per entry of the row view
   const int* fs_species;             // lane slot -> species of the f gather
+  const int4* t_rec;                 // per term: {k index | flags << 28, reactants x3}
   const int4* fl_hdr;                // per pass: fe_ell offset, width, re_ell offset, width
   const int4* fe_ell;                // f records in slot-major (ELL) order
   const int2* re_ell;                // row-view entries in slot-major order
@@ -181,14 +182,18 @@ __device__ __forceinline__ void cell_rate_constants(const DevProgram& P, int lan
 __host__ __device__ __forceinline__ void cell_rates(const DevProgram& P, int lane, const double* s_state,
                                            const double* s_k, double* s_rate) {
   for (int t = lane; t < P.n_terms; t += kWave) {
-    double k = s_k[P.t_kidx[t]];
-    int fl = P.t_flags[t];
-    dd rate;
+    int4 tr = P.t_rec[t];
+    int fl = tr.x >> 28;
+    double k = s_k[tr.x & 0x0FFFFFFF];
+    double y0 = s_state[tr.y >= 0 ? tr.y : 0], y1 = s_state[tr.z >= 0 ? tr.z : 0],
+           y2 = s_state[tr.w >= 0 ? tr.w : 0];
+    dd rate = {k, 0.0};
     if (fl & 2) {
-      rate = {k * s_state[P.t_react[3 * t]], 0.0};
+      rate = {k * y0, 0.0};
     } else {
-      rate = {k, 0.0};
-      for (int j = 0; j < P.t_nreact[t]; ++j) rate = dd_mul_d(rate, s_state[P.t_react[3 * t + j]]);
+      rate = tr.y >= 0 ? dd_mul_d(rate, y0) : rate;
+      rate = tr.z >= 0 ? dd_mul_d(rate, y1) : rate;
+      rate = tr.w >= 0 ? dd_mul_d(rate, y2) : rate;
     }
     s_rate[2 * t] = rate.hi;
     s_rate[2 * t + 1] = rate.lo;
@@ -211,26 +216,41 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
     int4 hd = P.fl_hdr[pass];
     dd D = {0.0, 0.0};
     double S = 0.0;
-#pragma unroll 4
-    for (int j = 0; j < hd.y; ++j) {
-      int4 rec = P.fe_ell[hd.x + j * kWave + lane];
-      int2 tb = {rec.x, rec.y};
-      double c = __hiloint2double(rec.w, rec.z);
-      int flags = tb.y >> 24;
-      int g = (tb.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
-      if (flags & 8) continue;   // padding
-      if (flags & 4) {  // multi-phase contribution slot (double-double)
-        dd v = {s_slot[2 * tb.x], s_slot[2 * tb.x + 1]};
-        D = dd_add(D, v);
-        S += fabs(v.hi);
-        continue;
+    // blocks of 4 records: all loads first, then branch-free accumulation
+    // (skipped records leave D and S untouched)
+    for (int j0 = 0; j0 < hd.y; j0 += 4) {
+      int4 rec[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        rec[u] = j0 + u < hd.y ? P.fe_ell[hd.x + (j0 + u) * kWave + lane]
+                               : int4{0, (8 << 24) | 0xFFFFFF, 0, 0};
+      dd val[4];
+      double sg[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int flags = rec[u].y >> 24;
+        int g = (rec[u].y << 8) >> 8;
+        const double* src = (flags & 4) ? s_slot : s_rate;
+        int idx = (flags & 8) ? 0 : rec[u].x;
+        val[u] = dd{src ? src[2 * idx] : 0.0, src ? src[2 * idx + 1] : 0.0};
+        sg[u] = s_state[g >= 0 ? g : 0];
       }
-      dd rate = {s_rate[2 * tb.x], s_rate[2 * tb.x + 1]};
-      if ((flags & 1) && rate.hi == 0.0) continue;
-      if (g >= 0 && !(-rate.hi * c * h <= s_state[g])) continue;
-      dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
-      D = dd_add(D, contrib);
-      S += fabs(contrib.hi);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int flags = rec[u].y >> 24;
+        int g = (rec[u].y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
+        double c = __hiloint2double(rec[u].w, rec[u].z);
+        dd rate = val[u];
+        bool slot = (flags & 4) != 0;
+        bool skip = (flags & 8) != 0 ||
+                    (!slot && (((flags & 1) && rate.hi == 0.0) ||
+                               (g >= 0 && !(-rate.hi * c * h <= sg[u]))));
+        dd contrib = slot ? rate : ((c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c));
+        dd nD = dd_add(D, contrib);
+        double nS = S + fabs(contrib.hi);
+        D = skip ? D : nD;
+        S = skip ? S : nS;
+      }
     }
     double v = 0.0;
     if (S != 0.0) {
@@ -239,8 +259,10 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
 #pragma unroll 4
         for (int j = 0; j < hd.w; ++j) {
           int2 ec = P.re_ell[hd.z + j * kWave + lane];
-          if (ec.x < 0) continue;
-          acc += Js[ec.x] * (y[ec.y] - Jy[ec.y]);
+          bool ok = ec.x >= 0;
+          int e = ok ? ec.x : 0, col = ok ? ec.y : 0;
+          double nacc = acc + Js[e] * (y[col] - Jy[col]);
+          acc = ok ? nacc : acc;
         }
         double est = Jf[i] + acc;
         double inv = 1.0 / S;

@@ -632,6 +632,11 @@ std::string Model::initialize() {
     P.j_guard.push_back(-1); P.j_rind.push_back(0);
   }
 
+  P.t_rec.assign(4 * (size_t)std::max(P.n_terms, 1), -1);
+  for (int t = 0; t < P.n_terms; ++t) {
+    P.t_rec[4 * t] = P.t_kidx[t] | (P.t_flags[t] << 28);
+    for (int j = 0; j < 3; ++j) P.t_rec[4 * t + 1 + j] = j < P.t_nreact[t] ? P.t_react[3 * t + j] : -1;
+  }
   P.fe_rec.resize(2 * P.fe_tb.size());
   for (size_t q = 0; q < P.fe_c.size(); ++q) {
     uint64_t bits;

@@ -82,6 +82,9 @@ struct Program {
   // of re_ell ({element, column}, padding -1)}; entry j of lane l is at
   // offset + j * 64 + l.
   std::vector<int> fs_species, fl_hdr, fe_ell, re_ell;
+  // rate terms as one 16-byte record: {k index | flags << 28, reactant state
+  // ids x3 (-1 = none)}
+  std::vector<int> t_rec;
   std::vector<int> colptr, rowval;                  // solver pattern (dep ids)
   std::vector<int> r_ptr, r_elem, r_col;            // row view of the pattern
   // LU (positions in the permuted LU value array)

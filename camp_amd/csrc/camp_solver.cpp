@@ -81,6 +81,7 @@ void Solver::setup_device() {
   P.fe_rec = reinterpret_cast<const int4*>(dev_upload(G.fe_rec, owned));
   P.re_rec = reinterpret_cast<const int2*>(dev_upload(G.re_rec, owned));
   P.fs_species = dev_upload(G.fs_species, owned);
+  P.t_rec = reinterpret_cast<const int4*>(dev_upload(G.t_rec, owned));
   P.fl_hdr = reinterpret_cast<const int4*>(dev_upload(G.fl_hdr, owned));
   P.fe_ell = reinterpret_cast<const int4*>(dev_upload(G.fe_ell, owned));
   P.re_ell = reinterpret_cast<const int2*>(dev_upload(G.re_ell, owned));
