@@ -195,8 +195,9 @@ class CampSolver:
         lib().camp_solver_set_profiling(self.h, int(bool(on)))
 
     def cell_profile(self):
-        """Shader cycles per cell: [f, Jac, LU factor, LU solve, whole solve]."""
-        out = np.zeros((self.n_cells, 5))
+        """Shader cycles per cell: [f, Jac, LU factor, LU solve, whole solve,
+        f's state update + rates, f's gather + estimate blend]."""
+        out = np.zeros((self.n_cells, 7))
         if lib().camp_solver_get_cell_profile(self.h, _dp(out)) != 0:
             raise RuntimeError("no profiled per-cell run yet")
         return out

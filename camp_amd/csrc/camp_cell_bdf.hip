@@ -63,7 +63,7 @@ struct CellCv {
   int nst, nfe, nsetups, nni, ncfn, netf, nje, nstlp, nstlj;
   // optional shader-clock counters (cycles in f, Jac, LU factor, LU solve)
   bool prof;
-  long long c_f, c_j, c_lu, c_sol;
+  long long c_f, c_j, c_lu, c_sol, c_fr, c_fg;
   __device__ __forceinline__ long long tick() const { return prof ? clock64() : 0; }
 
   __device__ __forceinline__ double* zn(int j) const { return znb + (size_t)j * nv; }
@@ -94,7 +94,11 @@ struct CellCv {
       if (MP) mp_f_prologue(P, lane, s_state, s_k, mpl);
       cell_rates(P, lane, s_state, s_k, s_rate);
       wsync();
+      long long t1 = tick();
+      if (prof) c_fr += t1 - t0;
       cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, true, ts, out, mpl.slot);
+      wsync();
+      if (prof) c_fg += clock64() - t1;
     }
     wsync();
     if (prof) c_f += clock64() - t0;
@@ -767,7 +771,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
         cv.zn(0)[i] = s > 1.0e-60 ? s : 1.0e-60;
       }
       wsync();
-      cv.c_f = cv.c_j = cv.c_lu = cv.c_sol = 0;
+      cv.c_f = cv.c_j = cv.c_lu = cv.c_sol = cv.c_fr = cv.c_fg = 0;
       long long c_start = cv.tick();
       int flag = cv.solve(a.t0, t1);
       if (flag >= 0) {
@@ -804,6 +808,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
           double* pr = a.prof + (size_t)cell * kCellProf;
           pr[0] = (double)cv.c_f; pr[1] = (double)cv.c_j; pr[2] = (double)cv.c_lu;
           pr[3] = (double)cv.c_sol; pr[4] = (double)(clock64() - c_start);
+          pr[5] = (double)cv.c_fr; pr[6] = (double)cv.c_fg;
         }
       }
       wsync();

@@ -216,41 +216,26 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
     int4 hd = P.fl_hdr[pass];
     dd D = {0.0, 0.0};
     double S = 0.0;
-    // blocks of 4 records: all loads first, then branch-free accumulation
-    // (skipped records leave D and S untouched)
-    for (int j0 = 0; j0 < hd.y; j0 += 4) {
-      int4 rec[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        rec[u] = j0 + u < hd.y ? P.fe_ell[hd.x + (j0 + u) * kWave + lane]
-                               : int4{0, (8 << 24) | 0xFFFFFF, 0, 0};
-      dd val[4];
-      double sg[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        int flags = rec[u].y >> 24;
-        int g = (rec[u].y << 8) >> 8;
-        const double* src = (flags & 4) ? s_slot : s_rate;
-        int idx = (flags & 8) ? 0 : rec[u].x;
-        val[u] = dd{src ? src[2 * idx] : 0.0, src ? src[2 * idx + 1] : 0.0};
-        sg[u] = s_state[g >= 0 ? g : 0];
+#pragma unroll 4
+    for (int j = 0; j < hd.y; ++j) {
+      int4 rec = P.fe_ell[hd.x + j * kWave + lane];
+      int2 tb = {rec.x, rec.y};
+      double c = __hiloint2double(rec.w, rec.z);
+      int flags = tb.y >> 24;
+      int g = (tb.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
+      if (flags & 8) continue;   // padding
+      if (flags & 4) {  // multi-phase contribution slot (double-double)
+        dd v = {s_slot[2 * tb.x], s_slot[2 * tb.x + 1]};
+        D = dd_add(D, v);
+        S += fabs(v.hi);
+        continue;
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        int flags = rec[u].y >> 24;
-        int g = (rec[u].y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
-        double c = __hiloint2double(rec[u].w, rec[u].z);
-        dd rate = val[u];
-        bool slot = (flags & 4) != 0;
-        bool skip = (flags & 8) != 0 ||
-                    (!slot && (((flags & 1) && rate.hi == 0.0) ||
-                               (g >= 0 && !(-rate.hi * c * h <= sg[u]))));
-        dd contrib = slot ? rate : ((c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c));
-        dd nD = dd_add(D, contrib);
-        double nS = S + fabs(contrib.hi);
-        D = skip ? D : nD;
-        S = skip ? S : nS;
-      }
+      dd rate = {s_rate[2 * tb.x], s_rate[2 * tb.x + 1]};
+      if ((flags & 1) && rate.hi == 0.0) continue;
+      if (g >= 0 && !(-rate.hi * c * h <= s_state[g])) continue;
+      dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
+      D = dd_add(D, contrib);
+      S += fabs(contrib.hi);
     }
     double v = 0.0;
     if (S != 0.0) {

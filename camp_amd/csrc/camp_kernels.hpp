@@ -10,7 +10,9 @@
 namespace camp {
 
 constexpr int kCellStats = 10;  // nst nfe nsetups netf nni ncfn nje flag hu next_h
-constexpr int kCellProf = 5;    // shader cycles: f, Jac, LU factor, LU solve, whole solve
+// shader cycles per cell: f, Jac, LU factor, LU solve, whole solve, and the
+// two parts of f (state update + rates, species gather + estimate blend)
+constexpr int kCellProf = 7;
 
 // LDS doubles one wavefront needs for a cell in the persistent kernel:
 // state, rate data, the multi-phase extras, Nordsieck history (6 vectors),

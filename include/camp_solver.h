@@ -191,8 +191,10 @@ double camp_solver_last_kernel_ms(void *solver_data);
  * when on, each solver_run records per cell the shader-clock cycles spent in
  * f(y), J(y), LU factorisation, triangular solves and the whole integration,
  * and solver_get_statistics reports the f and J sums as RHS_time__s and
- * Jac_time__s.  get_cell_profile copies [n_cells][5] cycles; returns 1 if no
- * profiled run has happened. */
+ * Jac_time__s.  get_cell_profile copies [n_cells][7] cycles: f, J, LU,
+ * solves, whole integration, then f's two parts (state update + rates,
+ * species gather + estimate blend); returns 1 if no profiled run has
+ * happened. */
 void camp_solver_set_profiling(void *solver_data, int on);
 int camp_solver_get_cell_profile(void *solver_data, double *out);
 
