@@ -200,55 +200,81 @@ __host__ __device__ __forceinline__ void cell_rates(const DevProgram& P, int lan
   }
 }
 
+// One gather record of f: the term's rate times the yield (or -1), subject
+// to the negative-yield guard, or a multi-phase slot; padding is skipped.
+__host__ __device__ __forceinline__ void f_record(int4 rec, const double* s_state, const double* s_rate,
+                                         const double* s_slot, double h, dd& D, double& S) {
+  int flags = rec.y >> 24;
+  int g = (rec.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
+  if (flags & 8) return;      // padding
+  if (flags & 4) {            // multi-phase contribution slot (double-double)
+    dd v = {s_slot[2 * rec.x], s_slot[2 * rec.x + 1]};
+    D = dd_add(D, v);
+    S += fabs(v.hi);
+    return;
+  }
+  double c = __hiloint2double(rec.w, rec.z);
+  dd rate = {s_rate[2 * rec.x], s_rate[2 * rec.x + 1]};
+  if ((flags & 1) && rate.hi == 0.0) return;
+  if (g >= 0 && !(-rate.hi * c * h <= s_state[g])) return;
+  dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
+  D = dd_add(D, contrib);
+  S += fabs(contrib.hi);
+}
+
 // Phase 2 (after cell_rates and a wave sync): each lane gathers one
-// species' records -- the term's rate times the yield (or -1), subject to
-// the negative-yield guard, or a multi-phase slot -- into D and S, in the
+// species' records into D (= production - loss) and S (magnitudes), in the
 // species' record order.  Records are read in ELL order (record j of every
-// lane contiguous), species heaviest first so the second pass is short.
+// lane contiguous), species heaviest first so the second pass is short, and
+// each lane keeps the next four records (and row entries of the estimate)
+// in flight while it processes the current four, so the loop does not wait
+// on the memory latency of every record.
 __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const double* s_state,
                                            const double* s_rate, const double* y,
                                            const double* Js, const double* Jy,
                                            const double* Jf, bool use_est, double h,
                                            double* out, const double* s_slot = nullptr) {
+  const int4 pad4 = {0, (8 << 24) | 0xFFFFFF, 0, 0};
+  const int2 pad2 = {-1, -1};
   for (int pass = 0; pass * kWave < P.nv; ++pass) {
     int slot = pass * kWave + lane;
     int i = slot < P.nv ? P.fs_species[slot] : -1;
     int4 hd = P.fl_hdr[pass];
+    const int4* fr = P.fe_ell + hd.x + lane;
+    const int2* er = P.re_ell + hd.z + lane;
+    int4 a0 = 0 < hd.y ? fr[0] : pad4, a1 = 1 < hd.y ? fr[kWave] : pad4,
+         a2 = 2 < hd.y ? fr[2 * kWave] : pad4, a3 = 3 < hd.y ? fr[3 * kWave] : pad4;
     dd D = {0.0, 0.0};
     double S = 0.0;
-#pragma unroll 4
-    for (int j = 0; j < hd.y; ++j) {
-      int4 rec = P.fe_ell[hd.x + j * kWave + lane];
-      int2 tb = {rec.x, rec.y};
-      double c = __hiloint2double(rec.w, rec.z);
-      int flags = tb.y >> 24;
-      int g = (tb.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
-      if (flags & 8) continue;   // padding
-      if (flags & 4) {  // multi-phase contribution slot (double-double)
-        dd v = {s_slot[2 * tb.x], s_slot[2 * tb.x + 1]};
-        D = dd_add(D, v);
-        S += fabs(v.hi);
-        continue;
+    for (int j = 0; j < hd.y; j += 4) {
+      int4 c0 = a0, c1 = a1, c2 = a2, c3 = a3;
+      a0 = j + 4 < hd.y ? fr[(j + 4) * kWave] : pad4;
+      a1 = j + 5 < hd.y ? fr[(j + 5) * kWave] : pad4;
+      a2 = j + 6 < hd.y ? fr[(j + 6) * kWave] : pad4;
+      a3 = j + 7 < hd.y ? fr[(j + 7) * kWave] : pad4;
+      f_record(c0, s_state, s_rate, s_slot, h, D, S);
+      f_record(c1, s_state, s_rate, s_slot, h, D, S);
+      f_record(c2, s_state, s_rate, s_slot, h, D, S);
+      f_record(c3, s_state, s_rate, s_slot, h, D, S);
+    }
+    double acc = 0.0;
+    if (use_est) {
+      int2 b0 = 0 < hd.w ? er[0] : pad2, b1 = 1 < hd.w ? er[kWave] : pad2,
+           b2 = 2 < hd.w ? er[2 * kWave] : pad2, b3 = 3 < hd.w ? er[3 * kWave] : pad2;
+      for (int j = 0; j < hd.w; j += 4) {
+        int2 c[4] = {b0, b1, b2, b3};
+        b0 = j + 4 < hd.w ? er[(j + 4) * kWave] : pad2;
+        b1 = j + 5 < hd.w ? er[(j + 5) * kWave] : pad2;
+        b2 = j + 6 < hd.w ? er[(j + 6) * kWave] : pad2;
+        b3 = j + 7 < hd.w ? er[(j + 7) * kWave] : pad2;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (c[u].x >= 0) acc += Js[c[u].x] * (y[c[u].y] - Jy[c[u].y]);
       }
-      dd rate = {s_rate[2 * tb.x], s_rate[2 * tb.x + 1]};
-      if ((flags & 1) && rate.hi == 0.0) continue;
-      if (g >= 0 && !(-rate.hi * c * h <= s_state[g])) continue;
-      dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
-      D = dd_add(D, contrib);
-      S += fabs(contrib.hi);
     }
     double v = 0.0;
     if (S != 0.0) {
       if (use_est) {
-        double acc = 0.0;
-#pragma unroll 4
-        for (int j = 0; j < hd.w; ++j) {
-          int2 ec = P.re_ell[hd.z + j * kWave + lane];
-          bool ok = ec.x >= 0;
-          int e = ok ? ec.x : 0, col = ok ? ec.y : 0;
-          double nacc = acc + Js[e] * (y[col] - Jy[col]);
-          acc = ok ? nacc : acc;
-        }
         double est = Jf[i] + acc;
         double inv = 1.0 / S;
         double sf = inv / (inv + 1.0e-14 / fabs(D.hi));
