@@ -202,21 +202,38 @@ __host__ __device__ __forceinline__ void cell_rates(const DevProgram& P, int lan
 
 // One gather record of f: the term's rate times the yield (or -1), subject
 // to the negative-yield guard, or a multi-phase slot; padding is skipped.
-__host__ __device__ __forceinline__ void f_record(int4 rec, const double* s_state, const double* s_rate,
-                                         const double* s_slot, double h, dd& D, double& S) {
+// The record's LDS operands (the rate or slot value, the guarded product's
+// concentration) are read by f_operands first, for a whole batch of
+// records, so the batch waits on LDS once.
+struct FOps {
+  double hi, lo, sg;
+};
+__host__ __device__ __forceinline__ FOps f_operands(int4 rec, const double* s_state,
+                                           const double* s_rate, const double* s_slot) {
+  int flags = rec.y >> 24;
+  int g = (rec.y << 8) >> 8;
+  const double* src = (flags & 4) ? s_slot : s_rate;
+  int idx = (flags & 8) ? 0 : rec.x;
+  FOps o;
+  o.hi = src[2 * idx];
+  o.lo = src[2 * idx + 1];
+  o.sg = s_state[g >= 0 ? g : 0];
+  return o;
+}
+__host__ __device__ __forceinline__ void f_record(int4 rec, FOps o, double h, dd& D, double& S) {
   int flags = rec.y >> 24;
   int g = (rec.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
   if (flags & 8) return;      // padding
   if (flags & 4) {            // multi-phase contribution slot (double-double)
-    dd v = {s_slot[2 * rec.x], s_slot[2 * rec.x + 1]};
+    dd v = {o.hi, o.lo};
     D = dd_add(D, v);
     S += fabs(v.hi);
     return;
   }
   double c = __hiloint2double(rec.w, rec.z);
-  dd rate = {s_rate[2 * rec.x], s_rate[2 * rec.x + 1]};
+  dd rate = {o.hi, o.lo};
   if ((flags & 1) && rate.hi == 0.0) return;
-  if (g >= 0 && !(-rate.hi * c * h <= s_state[g])) return;
+  if (g >= 0 && !(-rate.hi * c * h <= o.sg)) return;
   dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
   D = dd_add(D, contrib);
   S += fabs(contrib.hi);
@@ -252,10 +269,15 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
       a1 = j + 5 < hd.y ? fr[(j + 5) * kWave] : pad4;
       a2 = j + 6 < hd.y ? fr[(j + 6) * kWave] : pad4;
       a3 = j + 7 < hd.y ? fr[(j + 7) * kWave] : pad4;
-      f_record(c0, s_state, s_rate, s_slot, h, D, S);
-      f_record(c1, s_state, s_rate, s_slot, h, D, S);
-      f_record(c2, s_state, s_rate, s_slot, h, D, S);
-      f_record(c3, s_state, s_rate, s_slot, h, D, S);
+      const double* slot_or_rate = s_slot ? s_slot : s_rate;
+      FOps o0 = f_operands(c0, s_state, s_rate, slot_or_rate),
+           o1 = f_operands(c1, s_state, s_rate, slot_or_rate),
+           o2 = f_operands(c2, s_state, s_rate, slot_or_rate),
+           o3 = f_operands(c3, s_state, s_rate, slot_or_rate);
+      f_record(c0, o0, h, D, S);
+      f_record(c1, o1, h, D, S);
+      f_record(c2, o2, h, D, S);
+      f_record(c3, o3, h, D, S);
     }
     double acc = 0.0;
     if (use_est) {
@@ -267,9 +289,17 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
         b1 = j + 5 < hd.w ? er[(j + 5) * kWave] : pad2;
         b2 = j + 6 < hd.w ? er[(j + 6) * kWave] : pad2;
         b3 = j + 7 < hd.w ? er[(j + 7) * kWave] : pad2;
+        double jv[4], yv[4], jyv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          int e = c[u].x >= 0 ? c[u].x : 0, col = c[u].x >= 0 ? c[u].y : 0;
+          jv[u] = Js[e];
+          yv[u] = y[col];
+          jyv[u] = Jy[col];
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-          if (c[u].x >= 0) acc += Js[c[u].x] * (y[c[u].y] - Jy[c[u].y]);
+          if (c[u].x >= 0) acc += jv[u] * (yv[u] - jyv[u]);
       }
     }
     double v = 0.0;
