@@ -401,8 +401,12 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
   // 64 division positions and first 5 x 64 update records are fetched from
   // global memory while pivot k is eliminated, so no pivot waits on a
   // global load.  Within one pivot every update has a distinct destination
-  // (neither the pivot's L column nor its U row), so lanes never conflict
-  // and each lane loads all its operands before storing.
+  // (neither the pivot's L column nor its U row).  A wavefront's LDS
+  // accesses execute in program order, so the pivot, the L values and every
+  // update operand are read before any of them is written: the updates use
+  // the unscaled L value times the pivot's reciprocal -- the same product
+  // that is stored as the scaled L value -- and the pivot needs a single
+  // wavefront sync.
   constexpr int R = 5;
   int4 h0 = P.lu_hdr[0], h1 = P.lu_hdr[2];
   int d0 = P.lu_hdr[1].x, d1 = P.lu_hdr[3].x;
@@ -424,13 +428,7 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
       nxt[r] = (k + 1 < P.nv && q < h1.w) ? P.up_rec[q] : ~0ull;
     }
     double piv = lu[d0];
-    if (piv == 0.0 || !isfinite(piv)) return 1;
-    if (l_p >= 0) lu[l_p] = lu[l_p] / piv;
-    for (int q = h0.x + kWave + lane; q < h0.y; q += kWave) {
-      int p = P.l_pos[q];
-      lu[p] = lu[p] / piv;
-    }
-    wsync();
+    double lval = lu[l_p >= 0 ? l_p : 0];
     double od[R], oa[R], ob[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -439,13 +437,23 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
       oa[r] = lu[(int)((x >> 16) & 0xFFFF)];
       ob[r] = lu[(int)((x >> 32) & 0xFFFF)];
     }
+    if (piv == 0.0 || !isfinite(piv)) return 1;
+    double rinv = 1.0 / piv;
+    if (l_p >= 0) lu[l_p] = lval * rinv;
+    for (int q = h0.x + kWave + lane; q < h0.y; q += kWave) {
+      int p = P.l_pos[q];
+      lu[p] = lu[p] * rinv;
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (cur[r] != ~0ull) lu[(int)(cur[r] & 0xFFFF)] = od[r] - oa[r] * ob[r];
-    for (int q = h0.z + R * kWave + lane; q < h0.w; q += kWave) {
-      uint64_t x = P.up_rec[q];
-      int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
-      lu[d] = lu[d] - lu[a] * lu[b];
+      if (cur[r] != ~0ull) lu[(int)(cur[r] & 0xFFFF)] = od[r] - (oa[r] * rinv) * ob[r];
+    if (h0.w - h0.z > R * kWave) {
+      wsync();  // the remaining updates read the scaled L values
+      for (int q = h0.z + R * kWave + lane; q < h0.w; q += kWave) {
+        uint64_t x = P.up_rec[q];
+        int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
+        lu[d] = lu[d] - lu[a] * lu[b];
+      }
     }
     wsync();
     h0 = h1; h1 = h2; d0 = d1; d1 = d2; l_p = n_lp;
@@ -565,25 +573,55 @@ __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, con
     cell_lu_solve_reg(P, lane, lu, b);
     return;
   }
+  // Column sweeps with the solution in LDS (nv > 128).  Pivot k's L (or U)
+  // column: the first 64 entries go one per lane, the rest inline.  The
+  // program and LU reads form a software pipeline over pivots -- header of
+  // k+2, positions and rows of k+1, LU values of k+1 -- so a pivot waits on
+  // LDS only (the LU of multi-phase cells lives in global memory).
   for (int k = lane; k < P.nv; k += kWave) w[k] = b[P.perm[k]];
   wsync();
-  for (int k = 0; k < P.nv; ++k) {
-    double xk = w[k];
-    for (int q = P.l_ptr[k] + lane; q < P.l_ptr[k + 1]; q += kWave) {
-      int i = P.l_row[q];
-      w[i] = w[i] - lu[P.l_pos[q]] * xk;
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool fwd = dir == 0;
+    const int* pos_a = fwd ? P.l_pos : P.u_pos;
+    const int* row_a = fwd ? P.l_row : P.u_row;
+    auto range = [&](int k) -> int2 {
+      if (k < 0 || k >= P.nv) return int2{0, 0};
+      int4 h = P.lu_hdr[2 * k];
+      int4 h2 = P.lu_hdr[2 * k + 1];
+      return fwd ? int2{h.x, h.y} : int2{h2.y, h2.z};
+    };
+    int k0 = fwd ? 0 : P.nv - 1, step = fwd ? 1 : -1;
+    int2 r0 = range(k0), r1 = range(k0 + step);
+    int q0 = r0.x + lane, q1 = r1.x + lane;
+    int row0 = q0 < r0.y ? row_a[q0] : -1, pos0 = q0 < r0.y ? pos_a[q0] : 0;
+    int row1 = q1 < r1.y ? row_a[q1] : -1, pos1 = q1 < r1.y ? pos_a[q1] : 0;
+    double val0 = row0 >= 0 ? lu[pos0] : 0.0;
+    double dg0 = fwd ? 1.0 : lu[P.lu_hdr[2 * k0 + 1].x];
+    for (int n = 0; n < P.nv; ++n) {
+      int k = k0 + step * n;
+      // issue: range of k+2, positions/rows of k+1's successor, values of k+1
+      int2 r2 = range(k + 2 * step);
+      int q2 = r2.x + lane;
+      int row2 = q2 < r2.y ? row_a[q2] : -1, pos2 = q2 < r2.y ? pos_a[q2] : 0;
+      double val1 = row1 >= 0 ? lu[pos1] : 0.0;
+      int kn = k + step;
+      double dg1 = (!fwd && kn >= 0) ? lu[P.lu_hdr[2 * kn + 1].x] : 1.0;
+      double xk = w[k];
+      if (!fwd) {
+        xk = xk / dg0;
+        wsync();
+        if (lane == 0) w[k] = xk;
+      }
+      if (row0 >= 0) w[row0] = w[row0] - val0 * xk;
+      for (int q = r0.x + kWave + lane; q < r0.y; q += kWave) {
+        int i = row_a[q];
+        w[i] = w[i] - lu[pos_a[q]] * xk;
+      }
+      wsync();
+      r0 = r1; r1 = r2;
+      row0 = row1; val0 = val1; dg0 = dg1;
+      row1 = row2; pos1 = pos2;
     }
-    wsync();
-  }
-  for (int k = P.nv - 1; k >= 0; --k) {
-    double xk = w[k] / lu[P.diag_pos[k]];
-    wsync();
-    if (lane == 0) w[k] = xk;
-    for (int q = P.u_ptr[k] + lane; q < P.u_ptr[k + 1]; q += kWave) {
-      int i = P.u_row[q];
-      w[i] = w[i] - lu[P.u_pos[q]] * xk;
-    }
-    wsync();
   }
   for (int k = lane; k < P.nv; k += kWave) b[P.perm[k]] = w[k];
   wsync();

@@ -801,7 +801,7 @@ std::string Model::initialize() {
   for (int k = 0; k < nv; ++k) {
     int* h = &P.lu_hdr[(size_t)8 * k];
     h[0] = P.l_ptr[k]; h[1] = P.l_ptr[k + 1]; h[2] = P.up_ptr[k]; h[3] = P.up_ptr[k + 1];
-    h[4] = P.diag_pos[k];
+    h[4] = P.diag_pos[k]; h[5] = P.u_ptr[k]; h[6] = P.u_ptr[k + 1];
   }
   P.up_rec.resize(P.up_dst.size());
   for (size_t q = 0; q < P.up_dst.size(); ++q)

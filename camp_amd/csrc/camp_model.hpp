@@ -94,7 +94,8 @@ struct Program {
   std::vector<int> u_ptr, u_pos, u_row;             // per pivot k: U(:,k) above k
   std::vector<int> up_ptr;                          // per pivot k: update range
   std::vector<int> lu_hdr;  // per pivot k: {l_ptr[k], l_ptr[k+1], up_ptr[k], up_ptr[k+1],
-                            //  diag_pos[k], 0, 0, 0} (two 16-byte loads), +2 zero pivots
+                            //  diag_pos[k], u_ptr[k], u_ptr[k+1], 0} (two 16-byte
+                            //  loads), then 2 empty pivots
   std::vector<uint16_t> up_dst, up_l, up_u;         // LU[dst] -= LU[l] * LU[u]
   std::vector<uint64_t> up_rec;                     // dst | l << 16 | u << 32
   // Row-major LU storage: row i occupies [lu_rowptr[i], lu_rowptr[i+1]) in
