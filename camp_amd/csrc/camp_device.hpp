@@ -584,28 +584,29 @@ __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, con
     const bool fwd = dir == 0;
     const int* pos_a = fwd ? P.l_pos : P.u_pos;
     const int* row_a = fwd ? P.l_row : P.u_row;
-    auto range = [&](int k) -> int2 {
-      if (k < 0 || k >= P.nv) return int2{0, 0};
+    // {column range, diagonal position} of pivot k (empty outside 0..nv-1)
+    auto range = [&](int k) -> int4 {
+      if (k < 0 || k >= P.nv) return int4{0, 0, 0, 0};
       int4 h = P.lu_hdr[2 * k];
       int4 h2 = P.lu_hdr[2 * k + 1];
-      return fwd ? int2{h.x, h.y} : int2{h2.y, h2.z};
+      return fwd ? int4{h.x, h.y, h2.x, 0} : int4{h2.y, h2.z, h2.x, 0};
     };
     int k0 = fwd ? 0 : P.nv - 1, step = fwd ? 1 : -1;
-    int2 r0 = range(k0), r1 = range(k0 + step);
+    int4 r0 = range(k0), r1 = range(k0 + step), r2 = range(k0 + 2 * step);
     int q0 = r0.x + lane, q1 = r1.x + lane;
     int row0 = q0 < r0.y ? row_a[q0] : -1, pos0 = q0 < r0.y ? pos_a[q0] : 0;
     int row1 = q1 < r1.y ? row_a[q1] : -1, pos1 = q1 < r1.y ? pos_a[q1] : 0;
     double val0 = row0 >= 0 ? lu[pos0] : 0.0;
-    double dg0 = fwd ? 1.0 : lu[P.lu_hdr[2 * k0 + 1].x];
+    double dg0 = fwd ? 1.0 : lu[r0.z];
     for (int n = 0; n < P.nv; ++n) {
       int k = k0 + step * n;
-      // issue: range of k+2, positions/rows of k+1's successor, values of k+1
-      int2 r2 = range(k + 2 * step);
+      // stages issued for later pivots, each from data loaded one pivot
+      // earlier: range of k+3, positions/rows of k+2, values of k+1
+      int4 r3 = range(k + 3 * step);
       int q2 = r2.x + lane;
       int row2 = q2 < r2.y ? row_a[q2] : -1, pos2 = q2 < r2.y ? pos_a[q2] : 0;
       double val1 = row1 >= 0 ? lu[pos1] : 0.0;
-      int kn = k + step;
-      double dg1 = (!fwd && kn >= 0) ? lu[P.lu_hdr[2 * kn + 1].x] : 1.0;
+      double dg1 = (!fwd && n + 1 < P.nv) ? lu[r1.z] : 1.0;
       double xk = w[k];
       if (!fwd) {
         xk = xk / dg0;
@@ -618,7 +619,7 @@ __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, con
         w[i] = w[i] - lu[pos_a[q]] * xk;
       }
       wsync();
-      r0 = r1; r1 = r2;
+      r0 = r1; r1 = r2; r2 = r3;
       row0 = row1; val0 = val1; dg0 = dg1;
       row1 = row2; pos1 = pos2;
     }

@@ -5,11 +5,14 @@
  * solve requests of any cell count for any initialised mechanism while
  * others run: a request is admitted when its cells fit in the mechanism's
  * HBM cell arena (sized from free device memory), its state stays in HBM
- * across its timesteps, and each scheduling round packs every ready request
- * of one mechanism -- new ones and ones between timesteps -- into one
- * persistent-kernel launch on a free stream (continuous batching at
- * timestep granularity).  Cells never share integrator control, so a
- * request's results are bit-identical alone and in any mix.
+ * across its timesteps, and whenever a stream is free the scheduler packs
+ * every ready request of one mechanism into one persistent-kernel launch
+ * that runs each request's timesteps back to back per cell (continuous
+ * batching: requests that arrive while others run are admitted as arena
+ * slots free up and start in the next launch).  One launch takes at most
+ * 2 / n_streams of the GPU's resident cells, so later batches start at once
+ * beside it.  Cells never share integrator control, so a request's results
+ * are bit-identical alone and in any mix.
  *
  * Threading: one scheduler thread per server owns every HIP call; submit,
  * wait and the counters may be called from any thread.  Host buffers passed
