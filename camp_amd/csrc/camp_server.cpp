@@ -52,7 +52,6 @@ struct Arena {
   double *state = nullptr, *env = nullptr, *renv = nullptr, *Js = nullptr, *Jy = nullptr,
          *Jf = nullptr, *stats = nullptr, *t1 = nullptr, *rates = nullptr;
   int *fail = nullptr, *upd_off = nullptr, *nsteps = nullptr;
-  int full_grid = 0;  // workgroups of a launch that fills the GPU
   int* h_fail = nullptr;  // pinned [cap]
   std::vector<void*> owned;
   std::map<long, long> free_ranges;  // start -> length
@@ -243,17 +242,6 @@ bool Server::launch() {
     int grid = 0, block = 0;
     size_t lds = 0;
     A.S->cell_launch_shape(total, &grid, &block, &lds);
-    // A persistent launch holds its compute units until its last cell is
-    // done, so one batch may take at most 2 / n_streams of the GPU's
-    // resident cells: batches that arrive later start on the rest at once
-    // instead of queueing behind the whole GPU.
-    if (A.full_grid == 0) {
-      int fg = 0, fb = 0;
-      size_t fl = 0;
-      A.S->cell_launch_shape(1L << 30, &fg, &fb, &fl);
-      A.full_grid = fg;
-    }
-    grid = std::max(1, std::min(grid, A.full_grid * 2 / (int)lanes.size()));
     if (A.S->P.mp) {
       size_t waves = (size_t)grid * (block / kWave);
       if (L.scratch.size() < mechs.size()) {

@@ -9,10 +9,9 @@
  * every ready request of one mechanism into one persistent-kernel launch
  * that runs each request's timesteps back to back per cell (continuous
  * batching: requests that arrive while others run are admitted as arena
- * slots free up and start in the next launch).  One launch takes at most
- * 2 / n_streams of the GPU's resident cells, so later batches start at once
- * beside it.  Cells never share integrator control, so a request's results
- * are bit-identical alone and in any mix.
+ * slots free up and start in the next launch).  Cells never share
+ * integrator control, so a request's results are bit-identical alone and
+ * in any mix.
  *
  * Threading: one scheduler thread per server owns every HIP call; submit,
  * wait and the counters may be called from any thread.  Host buffers passed
