@@ -265,40 +265,36 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
     int4 hd = P.fl_hdr[pass];
     const int4* fr = P.fe_ell + hd.x + lane;
     const int2* er = P.re_ell + hd.z + lane;
-    // eight records in flight (this batch of four and the next), and the
-    // estimate's first four row entries from the start
-    int4 a[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = u < hd.y ? fr[u * kWave] : pad4;
-    int2 b[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) b[u] = (use_est && u < hd.w) ? er[u * kWave] : pad2;
+    int4 a0 = 0 < hd.y ? fr[0] : pad4, a1 = 1 < hd.y ? fr[kWave] : pad4,
+         a2 = 2 < hd.y ? fr[2 * kWave] : pad4, a3 = 3 < hd.y ? fr[3 * kWave] : pad4;
     dd D = {0.0, 0.0};
     double S = 0.0;
-    const double* slot_or_rate = s_slot ? s_slot : s_rate;
     for (int j = 0; j < hd.y; j += 4) {
-      int4 c[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        c[u] = a[u];
-        a[u] = a[u + 4];
-        a[u + 4] = j + 8 + u < hd.y ? fr[(j + 8 + u) * kWave] : pad4;
-      }
-      FOps o[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) o[u] = f_operands(c[u], s_state, s_rate, slot_or_rate);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) f_record(c[u], o[u], h, D, S);
+      int4 c0 = a0, c1 = a1, c2 = a2, c3 = a3;
+      a0 = j + 4 < hd.y ? fr[(j + 4) * kWave] : pad4;
+      a1 = j + 5 < hd.y ? fr[(j + 5) * kWave] : pad4;
+      a2 = j + 6 < hd.y ? fr[(j + 6) * kWave] : pad4;
+      a3 = j + 7 < hd.y ? fr[(j + 7) * kWave] : pad4;
+      const double* slot_or_rate = s_slot ? s_slot : s_rate;
+      FOps o0 = f_operands(c0, s_state, s_rate, slot_or_rate),
+           o1 = f_operands(c1, s_state, s_rate, slot_or_rate),
+           o2 = f_operands(c2, s_state, s_rate, slot_or_rate),
+           o3 = f_operands(c3, s_state, s_rate, slot_or_rate);
+      f_record(c0, o0, h, D, S);
+      f_record(c1, o1, h, D, S);
+      f_record(c2, o2, h, D, S);
+      f_record(c3, o3, h, D, S);
     }
     double acc = 0.0;
     if (use_est) {
+      int2 b0 = 0 < hd.w ? er[0] : pad2, b1 = 1 < hd.w ? er[kWave] : pad2,
+           b2 = 2 < hd.w ? er[2 * kWave] : pad2, b3 = 3 < hd.w ? er[3 * kWave] : pad2;
       for (int j = 0; j < hd.w; j += 4) {
-        int2 c[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          c[u] = b[u];
-          b[u] = j + 4 + u < hd.w ? er[(j + 4 + u) * kWave] : pad2;
-        }
+        int2 c[4] = {b0, b1, b2, b3};
+        b0 = j + 4 < hd.w ? er[(j + 4) * kWave] : pad2;
+        b1 = j + 5 < hd.w ? er[(j + 5) * kWave] : pad2;
+        b2 = j + 6 < hd.w ? er[(j + 6) * kWave] : pad2;
+        b3 = j + 7 < hd.w ? er[(j + 7) * kWave] : pad2;
         double jv[4], yv[4], jyv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
