@@ -403,10 +403,10 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     lu[P.m_to_lu[e]] = v;
   }
   wsync();
-  // Software pipeline over pivots: pivot k+3's header, pivot k+2's first
+  // Software pipeline over pivots: pivot k+2's header, pivot k+1's first
   // 64 division positions and first 5 x 64 update records are fetched from
-  // global memory while pivot k is eliminated, so a load has two pivots'
-  // time to arrive.  Within one pivot every update has a distinct destination
+  // global memory while pivot k is eliminated, so no pivot waits on a
+  // global load.  Within one pivot every update has a distinct destination
   // (neither the pivot's L column nor its U row).  A wavefront's LDS
   // accesses execute in program order, so the pivot, the L values and every
   // update operand are read before any of them is written: the updates use
@@ -414,29 +414,24 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
   // that is stored as the scaled L value -- and the pivot needs a single
   // wavefront sync.
   constexpr int R = 5;
-  int4 h0 = P.lu_hdr[0], h1 = P.lu_hdr[2], h2 = P.lu_hdr[4];
-  int d0 = P.lu_hdr[1].x, d1 = P.lu_hdr[3].x, d2 = P.lu_hdr[5].x;
+  int4 h0 = P.lu_hdr[0], h1 = P.lu_hdr[2];
+  int d0 = P.lu_hdr[1].x, d1 = P.lu_hdr[3].x;
   int l_p = (h0.x + lane < h0.y) ? P.l_pos[h0.x + lane] : -1;
-  int n_lp = (1 < P.nv && h1.x + lane < h1.y) ? P.l_pos[h1.x + lane] : -1;
-  uint64_t cur[R], nxt[R];
+  uint64_t cur[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     int q = h0.z + r * kWave + lane;
     cur[r] = q < h0.w ? P.up_rec[q] : ~0ull;
-    q = h1.z + r * kWave + lane;
-    nxt[r] = (1 < P.nv && q < h1.w) ? P.up_rec[q] : ~0ull;
   }
   for (int k = 0; k < P.nv; ++k) {
-    // two pivots ahead: header of k+3, division positions and update
-    // records of k+2 (from the header loaded one pivot earlier)
-    int4 h3 = P.lu_hdr[2 * min(k + 3, P.nv + 1)];
-    int d3 = P.lu_hdr[2 * min(k + 3, P.nv + 1) + 1].x;
-    int n2_lp = (k + 2 < P.nv && h2.x + lane < h2.y) ? P.l_pos[h2.x + lane] : -1;
-    uint64_t nxt2[R];
+    int4 h2 = P.lu_hdr[2 * (k + 2)];
+    int d2 = P.lu_hdr[2 * (k + 2) + 1].x;
+    int n_lp = (k + 1 < P.nv && h1.x + lane < h1.y) ? P.l_pos[h1.x + lane] : -1;
+    uint64_t nxt[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      int q = h2.z + r * kWave + lane;
-      nxt2[r] = (k + 2 < P.nv && q < h2.w) ? P.up_rec[q] : ~0ull;
+      int q = h1.z + r * kWave + lane;
+      nxt[r] = (k + 1 < P.nv && q < h1.w) ? P.up_rec[q] : ~0ull;
     }
     double piv = lu[d0];
     double lval = lu[l_p >= 0 ? l_p : 0];
@@ -467,12 +462,9 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
       }
     }
     wsync();
-    h0 = h1; h1 = h2; h2 = h3; d0 = d1; d1 = d2; d2 = d3; l_p = n_lp; n_lp = n2_lp;
+    h0 = h1; h1 = h2; d0 = d1; d1 = d2; l_p = n_lp;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      cur[r] = nxt[r];
-      nxt[r] = nxt2[r];
-    }
+    for (int r = 0; r < R; ++r) cur[r] = nxt[r];
   }
   return 0;
 }
