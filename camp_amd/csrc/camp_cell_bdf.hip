@@ -52,7 +52,7 @@ struct CellCv {
   __device__ explicit CellCv(const DevProgram& prog) : P(prog) {}
   int lane, nv;
   double *s_state, *s_k, *s_rate, *znb, *ewt, *acor, *y, *tempv, *ftemp, *last, *acinit, *gh1, *gh2,
-      *w, *lu;
+      *w, *lu, *s_fb;
   double *Js, *Jy, *Jf;
   MpLds mpl;   // multi-phase extras (LDS); unused for gas-phase programs
   double* Jr;  // reaction Jacobian scratch (global) of multi-phase programs
@@ -96,7 +96,7 @@ struct CellCv {
       wsync();
       long long t1 = tick();
       if (prof) c_fr += t1 - t0;
-      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, true, ts, out, mpl.slot);
+      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, true, ts, out, s_fb, mpl.slot);
       wsync();
       if (prof) c_fg += clock64() - t1;
     }
@@ -114,7 +114,7 @@ struct CellCv {
       mp_f_prologue(P, lane, s_state, s_k, mpl);
       cell_rates(P, lane, s_state, s_k, s_rate);
       wsync();
-      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, false, ts, fy, mpl.slot);
+      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, false, ts, fy, s_fb, mpl.slot);
       wsync();
       mp_refresh(P, lane, s_state, mpl);
       mp_j_prologue(P, lane, s_state, s_k, mpl);
@@ -122,7 +122,7 @@ struct CellCv {
     } else {
       cell_rates(P, lane, s_state, s_k, s_rate);
       wsync();
-      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, false, ts, fy);
+      cell_deriv(P, lane, s_state, s_rate, yv, Js, Jy, Jf, false, ts, fy, s_fb);
       cell_jac(P, lane, s_state, s_k, ts, Js);
     }
     for (int i = lane; i < nv; i += kWave) {
@@ -697,6 +697,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   cv.gh1 = v; v += nv;
   cv.gh2 = v; v += nv;
   cv.w = v; v += nv;
+  cv.s_fb = v; v += fb_lds_doubles(P);
   // Gas-phase cells keep their J history (J_solver, J_state, J_deriv) in
   // LDS for the whole solve; multi-phase cells use it in place in HBM.
   double *h_js = nullptr, *h_jy = nullptr, *h_jf = nullptr;

@@ -42,9 +42,11 @@ struct DevProgram {
   const int2* re_rec;  // {element, column} per entry of the row view
   const int* fs_species;             // lane slot -> species of the f gather
   const int4* t_rec;                 // per term: {k index | flags << 28, reactants x3}
-  const int4* fl_hdr;                // per pass: fe_ell offset, width, re_ell offset, width
-  const int4* fe_ell;                // f records in slot-major (ELL) order
+  const int4* fl_hdr;                // per pass: -, -, re_ell offset, width
   const int2* re_ell;                // row-view entries in slot-major order
+  int fb_rows, fb_nparts;            // balanced f gather: rows per lane, partials
+  const int4* fb_rec;                // {index | target << 16, guard | flags << 24, coef}
+  const int2* fb_fin;                // per species: {first partial, count}
   // rate-constant program (rxn_update_env_state)
   const int *rx_type, *rx_fidx, *rx_eidx, *rx_nreact;
   const double* rx_float;
@@ -206,20 +208,29 @@ __host__ __device__ __forceinline__ double hilo_to_double(int hi, int lo) {
   return __builtin_bit_cast(double, b);
 }
 
-// One gather record of f: the term's rate times the yield (or -1), subject
-// to the negative-yield guard, or a multi-phase slot; padding is skipped.
-// The record's LDS operands (the rate or slot value, the guarded product's
-// concentration) are read by f_operands first, for a whole batch of
-// records, so the batch waits on LDS once.
+// LDS scratch of the f gather: per species D.hi and S, and the partials of
+// split species (hi, lo, S).
+__host__ __device__ inline size_t fb_lds_doubles(const DevProgram& P) {
+  return 2 * (size_t)P.nv + 3 * (size_t)P.fb_nparts;
+}
+
+// One record of the f gather, without branches: the term's rate (or a
+// multi-phase slot value, whose coefficient is 1) times the yield (or -1),
+// unless the record is padding (flag 8), a zero-rate term flagged 1, or
+// fails the negative-yield guard -rate * yield * h <= y[guard]
+// (rxn_arrhenius.c:157 and alike); a dropped record adds zero, which leaves a
+// normalised double-double sum unchanged.  Multiplying a normalised
+// double-double by -1 or 1 is exact, so the sum is the signed sum of the
+// reference's records in their order.
 struct FOps {
   double hi, lo, sg;
 };
 __host__ __device__ __forceinline__ FOps f_operands(int4 rec, const double* s_state,
                                            const double* s_rate, const double* s_slot) {
-  int flags = rec.y >> 24;
-  int g = (rec.y << 8) >> 8;
+  int flags = (int)((unsigned)rec.y >> 24);
+  int g = (rec.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
   const double* src = (flags & 4) ? s_slot : s_rate;
-  int idx = (flags & 8) ? 0 : rec.x;
+  int idx = rec.x & 0xFFFF;
   FOps o;
   o.hi = src[2 * idx];
   o.lo = src[2 * idx + 1];
@@ -227,64 +238,89 @@ __host__ __device__ __forceinline__ FOps f_operands(int4 rec, const double* s_st
   return o;
 }
 __host__ __device__ __forceinline__ void f_record(int4 rec, FOps o, double h, dd& D, double& S) {
-  int flags = rec.y >> 24;
-  int g = (rec.y << 8) >> 8;  // sign-extended 24-bit guard (-1 = none)
-  if (flags & 8) return;      // padding
-  if (flags & 4) {            // multi-phase contribution slot (double-double)
-    dd v = {o.hi, o.lo};
-    D = dd_add(D, v);
-    S += fabs(v.hi);
-    return;
-  }
+  int flags = (int)((unsigned)rec.y >> 24);
+  int g = (rec.y << 8) >> 8;
   double c = hilo_to_double(rec.w, rec.z);
-  dd rate = {o.hi, o.lo};
-  if ((flags & 1) && rate.hi == 0.0) return;
-  if (g >= 0 && !(-rate.hi * c * h <= o.sg)) return;
-  dd contrib = (c == -1.0) ? dd{-rate.hi, -rate.lo} : dd_mul_d(rate, c);
-  D = dd_add(D, contrib);
-  S += fabs(contrib.hi);
+  bool keep = !(flags & 8) && !((flags & 1) && o.hi == 0.0) && (g < 0 || -o.hi * c * h <= o.sg);
+  dd t = dd_mul_d(dd{o.hi, o.lo}, c);
+  t.hi = keep ? t.hi : 0.0;
+  t.lo = keep ? t.lo : 0.0;
+  D = dd_add(D, t);
+  S += fabs(t.hi);
+}
+// The end of a piece (flag 16): its sum goes to the species (target < nv:
+// D.hi and S) or to a partial of a split species (hi, lo, S), and the lane
+// starts the next piece from zero.
+__host__ __device__ __forceinline__ void f_flush(int4 rec, int nv, double* s_fb, dd& D, double& S) {
+  if (!((unsigned)rec.y & (16u << 24))) return;
+  int t = (int)((unsigned)rec.x >> 16);
+  if (t < nv) {
+    s_fb[t] = D.hi;
+    s_fb[nv + t] = S;
+  } else {
+    double* q = s_fb + 2 * nv + 3 * (t - nv);
+    q[0] = D.hi;
+    q[1] = D.lo;
+    q[2] = S;
+  }
+  D = dd{0.0, 0.0};
+  S = 0.0;
 }
 
-// Phase 2 (after cell_rates and a wave sync): each lane gathers one
-// species' records into D (= production - loss) and S (magnitudes), in the
-// species' record order.  Records are read in ELL order (record j of every
-// lane contiguous), species heaviest first so the second pass is short, and
-// each lane keeps the next four records (and row entries of the estimate)
-// in flight while it processes the current four, so the loop does not wait
-// on the memory latency of every record.
+// f(y) after cell_rates and a wave sync.  Phase 2: every lane walks its bin
+// of fb_rows records (pieces of species' record lists, packed so that the
+// lanes' bins are balanced) accumulating D (= production - loss, double-
+// double, replacing CAMP's long double production/loss sums) and S (the
+// magnitudes), with the next four records in flight while the current four
+// are processed and each batch's LDS operands read before the batch is
+// accumulated.  Phase 3: lane per species, the pieces of a split species
+// are added in order, the Jacobian-based estimate J_hist (y - y_hist) +
+// f_hist is formed (ELL over the row view), and the precision-loss blend of
+// time_derivative.c:58-64 gives f.  s_fb is LDS scratch of 2 nv + 3
+// fb_nparts doubles.
 __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const double* s_state,
                                            const double* s_rate, const double* y,
                                            const double* Js, const double* Jy,
                                            const double* Jf, bool use_est, double h,
-                                           double* out, const double* s_slot = nullptr) {
+                                           double* out, double* s_fb,
+                                           const double* s_slot = nullptr) {
   const int4 pad4 = {0, (8 << 24) | 0xFFFFFF, 0, 0};
   const int2 pad2 = {-1, -1};
-  for (int pass = 0; pass * kWave < P.nv; ++pass) {
-    int slot = pass * kWave + lane;
-    int i = slot < P.nv ? P.fs_species[slot] : -1;
-    int4 hd = P.fl_hdr[pass];
-    const int4* fr = P.fe_ell + hd.x + lane;
-    const int2* er = P.re_ell + hd.z + lane;
-    int4 a0 = 0 < hd.y ? fr[0] : pad4, a1 = 1 < hd.y ? fr[kWave] : pad4,
-         a2 = 2 < hd.y ? fr[2 * kWave] : pad4, a3 = 3 < hd.y ? fr[3 * kWave] : pad4;
+  const int nv = P.nv, L = P.fb_rows;
+  const double* slot_or_rate = s_slot ? s_slot : s_rate;
+  {
+    const int4* fr = P.fb_rec + lane;
+    int4 a0 = 0 < L ? fr[0] : pad4, a1 = 1 < L ? fr[kWave] : pad4,
+         a2 = 2 < L ? fr[2 * kWave] : pad4, a3 = 3 < L ? fr[3 * kWave] : pad4;
     dd D = {0.0, 0.0};
     double S = 0.0;
-    for (int j = 0; j < hd.y; j += 4) {
+    for (int j = 0; j < L; j += 4) {
       int4 c0 = a0, c1 = a1, c2 = a2, c3 = a3;
-      a0 = j + 4 < hd.y ? fr[(j + 4) * kWave] : pad4;
-      a1 = j + 5 < hd.y ? fr[(j + 5) * kWave] : pad4;
-      a2 = j + 6 < hd.y ? fr[(j + 6) * kWave] : pad4;
-      a3 = j + 7 < hd.y ? fr[(j + 7) * kWave] : pad4;
-      const double* slot_or_rate = s_slot ? s_slot : s_rate;
+      a0 = j + 4 < L ? fr[(j + 4) * kWave] : pad4;
+      a1 = j + 5 < L ? fr[(j + 5) * kWave] : pad4;
+      a2 = j + 6 < L ? fr[(j + 6) * kWave] : pad4;
+      a3 = j + 7 < L ? fr[(j + 7) * kWave] : pad4;
       FOps o0 = f_operands(c0, s_state, s_rate, slot_or_rate),
            o1 = f_operands(c1, s_state, s_rate, slot_or_rate),
            o2 = f_operands(c2, s_state, s_rate, slot_or_rate),
            o3 = f_operands(c3, s_state, s_rate, slot_or_rate);
       f_record(c0, o0, h, D, S);
+      f_flush(c0, nv, s_fb, D, S);
       f_record(c1, o1, h, D, S);
+      f_flush(c1, nv, s_fb, D, S);
       f_record(c2, o2, h, D, S);
+      f_flush(c2, nv, s_fb, D, S);
       f_record(c3, o3, h, D, S);
+      f_flush(c3, nv, s_fb, D, S);
     }
+  }
+  wsync();
+  const double* part = s_fb + 2 * nv;
+  for (int pass = 0; pass * kWave < nv; ++pass) {
+    int slot = pass * kWave + lane;
+    int i = slot < nv ? P.fs_species[slot] : -1;
+    int4 hd = P.fl_hdr[pass];
+    const int2* er = P.re_ell + hd.z + lane;
     double acc = 0.0;
     if (use_est) {
       int2 b0 = 0 < hd.w ? er[0] : pad2, b1 = 1 < hd.w ? er[kWave] : pad2,
@@ -308,18 +344,34 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
           if (c[u].x >= 0) acc += jv[u] * (yv[u] - jyv[u]);
       }
     }
+    if (i < 0) continue;
+    int2 fin = P.fb_fin[i];
+    double Dhi = 0.0, S = 0.0;
+    if (fin.y == 0) {
+      Dhi = s_fb[i];
+      S = s_fb[nv + i];
+    } else if (fin.y > 0) {
+      dd D = {part[3 * fin.x], part[3 * fin.x + 1]};
+      S = part[3 * fin.x + 2];
+      for (int k = 1; k < fin.y; ++k) {
+        const double* q = part + 3 * (fin.x + k);
+        D = dd_add(D, dd{q[0], q[1]});
+        S += q[2];
+      }
+      Dhi = D.hi;
+    }
     double v = 0.0;
     if (S != 0.0) {
       if (use_est) {
         double est = Jf[i] + acc;
         double inv = 1.0 / S;
-        double sf = inv / (inv + 1.0e-14 / fabs(D.hi));
-        v = sf * D.hi + (1.0 - sf) * est;
+        double sf = inv / (inv + 1.0e-14 / fabs(Dhi));
+        v = sf * Dhi + (1.0 - sf) * est;
       } else {
-        v = D.hi;
+        v = Dhi;
       }
     }
-    if (i >= 0) out[i] = v;
+    out[i] = v;
   }
 }
 
@@ -454,11 +506,27 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     for (int r = 0; r < R; ++r)
       if (cur[r] != ~0ull) lu[(int)(cur[r] & 0xFFFF)] = od[r] - (oa[r] * rinv) * ob[r];
     if (h0.w - h0.z > R * kWave) {
-      wsync();  // the remaining updates read the scaled L values
-      for (int q = h0.z + R * kWave + lane; q < h0.w; q += kWave) {
-        uint64_t x = P.up_rec[q];
-        int d = (int)(x & 0xFFFF), a = (int)((x >> 16) & 0xFFFF), b = (int)((x >> 32) & 0xFFFF);
-        lu[d] = lu[d] - lu[a] * lu[b];
+      // the remaining updates (pivots of the dense trailing block) read the
+      // scaled L values; they go in batches of R x 64 records whose loads
+      // are all issued before the first is used
+      wsync();
+      for (int base = h0.z + R * kWave; base < h0.w; base += R * kWave) {
+        uint64_t x[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          int q = base + r * kWave + lane;
+          x[r] = q < h0.w ? P.up_rec[q] : ~0ull;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          uint64_t y = x[r] != ~0ull ? x[r] : 0ull;
+          od[r] = lu[(int)(y & 0xFFFF)];
+          oa[r] = lu[(int)((y >> 16) & 0xFFFF)];
+          ob[r] = lu[(int)((y >> 32) & 0xFFFF)];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (x[r] != ~0ull) lu[(int)(x[r] & 0xFFFF)] = od[r] - oa[r] * ob[r];
       }
     }
     wsync();

@@ -54,6 +54,7 @@ __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const doubl
   double* s_k = s_state + P.ns;
   double* s_rate = s_k + P.n_env + mp_lds_doubles(P);
   MpLds L = mp_lds(P, s_k + P.n_env, s_rate + 2 * (size_t)P.n_terms);
+  double* s_fb = s_rate + 2 * (size_t)P.n_terms + mp_jsc_doubles(P);
   for (int c = blockIdx.x * wpb + wv; c < nc; c += gridDim.x * wpb) {
     stage_cell(P, lane, state + (size_t)c * P.ns, rxn_env + (size_t)c * P.n_env, s_state, s_k);
     const double* yc = y + (size_t)c * P.nv;
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(256) k_deriv(DevProgram P, int nc, const doubl
     cell_rates(P, lane, s_state, s_k, s_rate);
     wsync();
     cell_deriv(P, lane, s_state, s_rate, yc, Js + (size_t)c * P.nnz, Jy + (size_t)c * P.nv,
-               Jf + (size_t)c * P.nv, use_est != 0, h, fout + (size_t)c * P.nv, L.slot);
+               Jf + (size_t)c * P.nv, use_est != 0, h, fout + (size_t)c * P.nv, s_fb, L.slot);
     wsync();
   }
 }

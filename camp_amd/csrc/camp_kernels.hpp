@@ -20,7 +20,7 @@ constexpr int kCellProf = 7;
 // programs) and 24 slots for the BDF coefficient arrays l, tq and tau.
 __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P) {
   return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms + 16 * (size_t)P.nv +
-         (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
+         fb_lds_doubles(P) + (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
 }
 // Global scratch doubles per wavefront of multi-phase programs: LU factors,
 // the reaction Jacobian and the per-instance Jacobian scalars.
@@ -68,7 +68,8 @@ __global__ void k_jac(DevProgram P, int nc, const double* y, const double* state
 // LDS doubles per wavefront of k_deriv and k_jac, and wavefronts per block
 // (one for multi-phase programs, whose per-cell extras are large)
 __host__ __device__ inline size_t fj_lds_doubles(const DevProgram& P) {
-  return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms + mp_jsc_doubles(P);
+  return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms + mp_jsc_doubles(P) +
+         fb_lds_doubles(P);
 }
 inline int fj_waves_per_block(const DevProgram& P) { return P.mp ? 1 : 4; }
 __global__ void k_lu_factor(DevProgram P, int nc, const double* J, double gamma, double* LU,

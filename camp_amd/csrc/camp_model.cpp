@@ -672,40 +672,22 @@ std::string Model::initialize() {
     std::vector<int> order(nv);
     for (int i = 0; i < nv; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-      return P.f_ptr[a + 1] - P.f_ptr[a] > P.f_ptr[b + 1] - P.f_ptr[b];
+      return P.r_ptr[a + 1] - P.r_ptr[a] > P.r_ptr[b + 1] - P.r_ptr[b];
     });
     P.fs_species = order;
     P.fl_hdr.clear();
-    P.fe_ell.clear();
     P.re_ell.clear();
     int npass = (nv + W - 1) / W;
     for (int pass = 0; pass < npass; ++pass) {
-      int cf = 0, cr = 0;
+      int cr = 0;
       for (int l = 0; l < W && pass * W + l < nv; ++l) {
         int i = order[pass * W + l];
-        cf = std::max(cf, P.f_ptr[i + 1] - P.f_ptr[i]);
         cr = std::max(cr, P.r_ptr[i + 1] - P.r_ptr[i]);
       }
-      P.fl_hdr.push_back((int)(P.fe_ell.size() / 4));
-      P.fl_hdr.push_back(cf);
+      P.fl_hdr.push_back(0);
+      P.fl_hdr.push_back(0);
       P.fl_hdr.push_back((int)(P.re_ell.size() / 2));
       P.fl_hdr.push_back(cr);
-      for (int j = 0; j < cf; ++j)
-        for (int l = 0; l < W; ++l) {
-          int slot = pass * W + l, q = -1;
-          if (slot < nv) {
-            int i = order[slot];
-            if (j < P.f_ptr[i + 1] - P.f_ptr[i]) q = P.f_ptr[i] + j;
-          }
-          if (q >= 0) {
-            for (int k = 0; k < 4; ++k) P.fe_ell.push_back(P.fe_rec[4 * (size_t)q + k]);
-          } else {
-            P.fe_ell.push_back(0);
-            P.fe_ell.push_back((8 << 24) | 0xFFFFFF);
-            P.fe_ell.push_back(0);
-            P.fe_ell.push_back(0);
-          }
-        }
       for (int j = 0; j < cr; ++j)
         for (int l = 0; l < W; ++l) {
           int slot = pass * W + l, q = -1;
@@ -717,8 +699,116 @@ std::string Model::initialize() {
           P.re_ell.push_back(q >= 0 ? P.r_col[q] : -1);
         }
     }
-    if (P.fe_ell.empty()) P.fe_ell.assign(4, 0);
     if (P.re_ell.empty()) P.re_ell.assign(2, -1);
+  }
+  {
+    // balanced bins of the f gather (first-fit decreasing over pieces)
+    const int W = 64;
+    int total = P.f_ptr[nv];
+    if (P.n_terms >= 65536 || P.n_slots >= 65536) return "more than 65535 f terms or slots";
+    struct Piece { int species, first, count, part; };
+    // Pieces: species longer than L split into near-equal pieces, placed
+    // longest first into the fullest bin that holds them; a piece that no
+    // bin holds fills the emptiest bin and its rest goes on.  L grows until
+    // the pieces fit with at most kMaxParts partials (LDS of the gather).
+    const int kMaxParts = 64;
+    std::vector<std::vector<Piece>> bins;
+    int L = std::max(1, (total + W - 1) / W);
+    for (;; ++L) {
+      std::vector<Piece> pieces;
+      for (int i = 0; i < nv; ++i) {
+        int cnt = P.f_ptr[i + 1] - P.f_ptr[i];
+        if (cnt == 0) continue;
+        int np = (cnt + L - 1) / L, at = P.f_ptr[i];
+        for (int k = 0; k < np; ++k) {
+          int c = cnt / np + (k < cnt % np ? 1 : 0);
+          pieces.push_back({i, at, c, -1});
+          at += c;
+        }
+      }
+      std::stable_sort(pieces.begin(), pieces.end(),
+                       [](const Piece& a, const Piece& b) { return a.count > b.count; });
+      bins.assign(W, {});
+      std::vector<int> fill(W, 0);
+      bool ok = true;
+      for (Piece pc : pieces) {
+        while (pc.count > 0) {
+          int best = -1, most = -1;
+          for (int b = 0; b < W; ++b) {
+            int room = L - fill[b];
+            if (room >= pc.count && (best < 0 || room < L - fill[best])) best = b;
+            if (room > 0 && (most < 0 || room > L - fill[most])) most = b;
+          }
+          if (best >= 0) {
+            bins[best].push_back(pc);
+            fill[best] += pc.count;
+            break;
+          }
+          if (most < 0) {
+            ok = false;
+            break;
+          }
+          int r = L - fill[most];
+          bins[most].push_back({pc.species, pc.first, r, -1});
+          fill[most] += r;
+          pc.first += r;
+          pc.count -= r;
+        }
+        if (!ok) break;
+      }
+      if (!ok) continue;
+      // partials: the pieces of a species in record order get consecutive ids
+      std::vector<std::vector<Piece*>> of(nv);
+      for (auto& b : bins)
+        for (Piece& pc : b) of[pc.species].push_back(&pc);
+      int np_total = 0;
+      for (int i = 0; i < nv; ++i) {
+        if (of[i].size() < 2) continue;
+        std::sort(of[i].begin(), of[i].end(),
+                  [](const Piece* a, const Piece* b) { return a->first < b->first; });
+        for (Piece* pc : of[i]) pc->part = np_total++;
+      }
+      if (np_total <= kMaxParts || L >= P.max_f_len) {
+        P.fb_nparts = np_total;
+        break;
+      }
+    }
+    if (nv + P.fb_nparts >= 65536) return "more than 65535 f gather targets";
+    P.fb_rows = L;
+    P.fb_rec.assign((size_t)4 * W * L, 0);
+    for (int l = 0; l < W; ++l) {
+      int j = 0;
+      for (const Piece& pc : bins[l])
+        for (int r = 0; r < pc.count; ++r, ++j) {
+          const int* src = &P.fe_rec[4 * (size_t)(pc.first + r)];
+          int* dst = &P.fb_rec[4 * ((size_t)j * W + l)];
+          int target = pc.part >= 0 ? nv + pc.part : pc.species;
+          dst[0] = (src[0] & 0xFFFF) | (target << 16);
+          dst[1] = src[1] | (r + 1 == pc.count ? (16 << 24) : 0);
+          dst[2] = src[2];
+          dst[3] = src[3];
+        }
+      for (; j < L; ++j) {
+        int* dst = &P.fb_rec[4 * ((size_t)j * W + l)];
+        dst[0] = 0;
+        dst[1] = (8 << 24) | 0xFFFFFF;
+        dst[2] = dst[3] = 0;
+      }
+    }
+    P.fb_fin.assign(2 * (size_t)std::max(nv, 1), 0);
+    for (int i = 0; i < nv; ++i) {
+      int cnt = P.f_ptr[i + 1] - P.f_ptr[i];
+      P.fb_fin[2 * i] = 0;
+      P.fb_fin[2 * i + 1] = cnt == 0 ? -1 : 0;
+    }
+    for (const auto& b : bins)
+      for (const Piece& pc : b)
+        if (pc.part >= 0) {
+          int* f = &P.fb_fin[2 * pc.species];
+          if (f[1] == 0) f[0] = pc.part;
+          f[0] = std::min(f[0], pc.part);
+          f[1] += 1;
+        }
   }
 
   // --- LU: Markowitz ordering, symbolic fill, per-pivot programs ----------

@@ -74,14 +74,24 @@ struct Program {
   // flags << 24, coefficient (low, high word)}, and the row view of the
   // solver pattern as {element, column} pairs
   std::vector<int> fe_rec, re_rec;
-  // The gathers of f(y) in lane-slot (ELL) order, so that a wavefront's
-  // record loads are contiguous: species are assigned to slots heaviest
-  // first (slot s = pass * 64 + lane; the light tail goes to the second
-  // pass), fs_species[slot] is the species, and per pass fl_hdr = {offset
-  // and width of fe_ell (int4 records, padding flagged 8), offset and width
-  // of re_ell ({element, column}, padding -1)}; entry j of lane l is at
-  // offset + j * 64 + l.
-  std::vector<int> fs_species, fl_hdr, fe_ell, re_ell;
+  // The per-species pass of f(y) (estimate and blend) in lane-slot order:
+  // species are assigned to slots by row length, longest first (slot s =
+  // pass * 64 + lane), fs_species[slot] is the species, and per pass fl_hdr
+  // = {0, 0, offset and width of re_ell ({element, column}, padding -1)};
+  // entry j of lane l is at offset + j * 64 + l, so a wavefront's loads are
+  // contiguous.
+  std::vector<int> fs_species, fl_hdr, re_ell;
+  // The f gather in balanced bins: the species' record lists are cut into
+  // pieces of at most fb_rows records (a species longer than that is split
+  // into near-equal pieces) and the pieces packed first-fit-decreasing into
+  // 64 lane bins of fb_rows records, so every lane walks fb_rows records.
+  // fb_rec holds row j of lane l at j * 64 + l as {term or slot index |
+  // target << 16, guard | flags << 24, coefficient (low, high word)}; flag
+  // 16 ends a piece, whose sum goes to target (a species < nv, or partial
+  // nv + p of a split species).  fb_fin[i] = {first partial, partial count}
+  // per species (count 0: one piece, -1: no records).
+  int fb_rows = 0, fb_nparts = 0;
+  std::vector<int> fb_rec, fb_fin;
   // rate terms as one 16-byte record: {k index | flags << 28, reactant state
   // ids x3 (-1 = none)}
   std::vector<int> t_rec;

@@ -83,8 +83,11 @@ void Solver::setup_device() {
   P.fs_species = dev_upload(G.fs_species, owned);
   P.t_rec = reinterpret_cast<const int4*>(dev_upload(G.t_rec, owned));
   P.fl_hdr = reinterpret_cast<const int4*>(dev_upload(G.fl_hdr, owned));
-  P.fe_ell = reinterpret_cast<const int4*>(dev_upload(G.fe_ell, owned));
   P.re_ell = reinterpret_cast<const int2*>(dev_upload(G.re_ell, owned));
+  P.fb_rows = G.fb_rows;
+  P.fb_nparts = G.fb_nparts;
+  P.fb_rec = reinterpret_cast<const int4*>(dev_upload(G.fb_rec, owned));
+  P.fb_fin = reinterpret_cast<const int2*>(dev_upload(G.fb_fin, owned));
   P.je_a = reinterpret_cast<const int4*>(dev_upload(G.je_a, owned));
   P.je_b = dev_upload(G.je_b, owned);
   P.je_c = dev_upload(G.je_c, owned);
