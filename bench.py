@@ -96,28 +96,42 @@ def cpu_baseline(n_cells, warm, steps, dt, cores, per_core):
                           cores * per_core, cores, per_core, n_cells, steps, dt, warm)}
 
 
-def serving(mech, solver, pool_state, env, rates, n_requests, mp_max_cells, seed=3):
+def serving(mech, solver, pool_state, env, rates, n_requests, mp_max_cells, mp_spun=None,
+            dt=180.0, max_batch=1 << 17, seed=3):
     """Request server with both mechanisms: capacity at saturation, then
     Poisson arrivals at 70% of it.  Each request is a gas request (1..50,000
     cells of the spun-up 100k-cell pool) or a multi-phase request (1..
-    mp_max_cells cells of a seeded multi-phase pool) with equal probability,
-    dt U(60, 600) s and 1..4 timesteps."""
+    mp_max_cells cells of a multi-phase pool, spun up by one dt step like
+    the gas pool: the multi-phase leg's cells when it ran) with equal
+    probability, dt U(60, 600) s and 1..4 timesteps."""
+    import torch
     from camp_amd.core import CampSolver
     from camp_amd.multiphase import multiphase_cells, multiphase_mechanism
     from camp_amd.server import CampServer, base_rate_matrix
     n_pool = pool_state.shape[0]
     rng = np.random.default_rng(seed)
-    srv = CampServer(n_streams=4, max_batch_cells=1 << 17)
+    srv = CampServer(n_streams=4, max_batch_cells=max_batch)
     gas = srv.add_mechanism(CampSolver(mech, 1))
     srv.set_pool(gas, pool_state, env, base_rate_matrix(solver, rates, n_pool))
     mp_pool = 0
     if mp_max_cells > 0:
         mpm = multiphase_mechanism(1)
         mp_pool = max(mp_max_cells, 2000)
-        mcells = multiphase_cells(mpm, mp_pool, seed=2)
+        if mp_spun is not None and mp_spun[1].shape[0] >= mp_pool:
+            mcells, mstate = mp_spun
+        else:
+            mcells = multiphase_cells(mpm, mp_pool, seed=2)
+            sp = CampSolver(mpm, mp_pool)
+            sp.set_rates(mcells.rates)
+            dev = torch.device("cuda:0")
+            st_d = torch.from_numpy(mcells.state).to(dev)
+            env_d = torch.from_numpy(mcells.env).to(dev)
+            sp.solve_device(st_d.data_ptr(), env_d.data_ptr(), 0.0, dt)
+            torch.cuda.synchronize()
+            mstate = st_d.cpu().numpy()
         msol = CampSolver(mpm, 1)
         mpid = srv.add_mechanism(msol)
-        srv.set_pool(mpid, mcells.state, mcells.env,
+        srv.set_pool(mpid, np.ascontiguousarray(mstate[:mp_pool]), mcells.env[:mp_pool],
                      base_rate_matrix(msol, mcells.rates, mp_pool))
 
     def draw(k):
@@ -239,7 +253,8 @@ def multiphase_leg(n_cells, steps, dt):
     torch.cuda.synchronize()
     el = time.perf_counter() - t
     progress("multi-phase done (%.0f ms/step)" % (1e3 * el / steps))
-    return {"value": n_cells * steps / el, "unit": "cell-timesteps/s", "n_cells": n_cells,
+    spun = (cells, st.cpu().numpy())
+    return spun, {"value": n_cells * steps / el, "unit": "cell-timesteps/s", "n_cells": n_cells,
             "steps": steps, "ms_per_step": 1e3 * el / steps, "kernel_ms_per_step": kern / steps,
             "failed_steps": failed, "n_state": mpm.n_state, "n_dep": s.nv, "nnz": s.nnz,
             "bdf_steps_per_cell_step": float(tot[0] / n_cells / steps),
@@ -287,6 +302,9 @@ def main():
                          "WRITE_SIZE counters) and exit")
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests in the serving leg (0: skip)")
+    ap.add_argument("--serve-batch-cells", type=int, default=1 << 17,
+                    help="cap on the cells of one server launch (the server's own chunk, "
+                         "twice the cells the GPU holds at once, applies below it)")
     ap.add_argument("--serve-mp-cells", type=int, default=2000,
                     help="largest multi-phase request (0: gas requests only)")
     args = ap.parse_args()
@@ -397,15 +415,16 @@ def main():
     if args.plain_cells > 0:
         plain = plain_path(mech, args.plain_cells, args.plain_steps, args.dt)
 
-    mpleg = None
+    mpleg = mp_spun = None
     if args.mp_cells > 0:
-        mpleg = multiphase_leg(args.mp_cells, args.mp_steps, args.dt)
+        mp_spun, mpleg = multiphase_leg(args.mp_cells, args.mp_steps, args.dt)
 
     serve = None
     if args.serve_requests > 0:
         progress("serving leg: %d requests" % args.serve_requests)
         serve = serving(mech, solver, st.cpu().numpy(), cells.env, cells.rates,
-                        args.serve_requests, args.serve_mp_cells)
+                        args.serve_requests, args.serve_mp_cells, mp_spun, args.dt,
+                        args.serve_batch_cells)
         progress("serving leg done")
 
     cpu = None

@@ -1,15 +1,21 @@
 // Request server: continuous batching of CAMP chemistry solves on one GPU.
 //
 // Per mechanism an HBM arena holds cell slots (state, env, rate data,
-// Jacobian history, statistics, end time, failure counter).  A request is
-// admitted when a contiguous range of slots is free; its inputs are copied
-// in on the upload stream.  Every scheduling round, each idle batch stream
-// takes the ready requests of one mechanism (oldest first, up to
-// max_batch_cells), builds the slot list and launches the persistent
-// per-cell kernel over it; requests on their last step copy their results
-// out on the same stream.  When a batch's event completes its requests
-// either become ready for their next timestep (state stays in the arena)
-// or complete and release their slots.  One thread makes every HIP call.
+// Jacobian history, statistics, end time, step count, failure counter).  A
+// request is admitted when a contiguous range of slots is free.  Admitted
+// requests form one first-come queue of cells: whenever a batch stream is
+// free, the scheduler takes the next cells of the oldest request's
+// mechanism -- from that request and the ones after it, a request being
+// split across launches when it is larger than what is left -- up to a
+// chunk of twice the cells the GPU holds at once, and launches the
+// persistent per-cell kernel over them (each cell runs all of its
+// request's timesteps).  A request's inputs are copied in on the stream of
+// its first chunk (later chunks on other streams wait for that copy's
+// event).  When the last chunk of a request completes its results are
+// copied out on that stream and the request completes with the copy's
+// event.  Chunks keep every launch short, so the streams' launches overlap
+// and a request never waits for an unrelated large one to finish.  One
+// thread makes every HIP call.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -34,15 +40,22 @@ enum ReqState { kQueued, kReady, kRunning, kDone };
 
 struct Request {
   long id = 0;
-  int mech = 0, n = 0, n_steps = 0, steps_left = 0;
+  int mech = 0, n = 0, n_steps = 0;
   double dt = 0.0;
   const double *in_state = nullptr, *in_env = nullptr, *in_rates = nullptr;
   double *out_state = nullptr, *out_stats = nullptr;
   Clock::time_point t_submit, t_done;
   long slot0 = -1;
-  hipEvent_t up_ev = nullptr;
+  long launched = 0, finished = 0;  // cells handed to launches / completed
+  hipEvent_t up_ev = nullptr, out_ev = nullptr;
   int state = kQueued;
   bool failed = false;
+};
+
+// A run of consecutive cells of one request in a launch.
+struct Piece {
+  Request* r;
+  long first, count;
 };
 
 struct Arena {
@@ -103,15 +116,15 @@ struct Lane {
   long list_cap = 0;
   bool busy = false;
   long cells = 0;
-  std::vector<Request*> batch;
+  std::vector<Piece> batch;
 };
 
 struct Server {
   int dev = 0;
   long max_batch = 0;
   std::vector<std::unique_ptr<Arena>> mechs;
+  std::vector<long> chunk;  // per mechanism: cells per launch
   std::vector<Lane> lanes;
-  hipStream_t up_st = nullptr;
   std::thread thr;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -119,42 +132,45 @@ struct Server {
   long next_id = 0;
   std::deque<Request*> inbox;                        // guarded by mu
   std::unordered_map<long, std::unique_ptr<Request>> reqs;  // guarded by mu
-  // scheduler-thread state
+  // scheduler-thread state: admitted requests with cells not yet launched
+  // (arrival order), and requests whose results are being copied out
   std::deque<Request*> pending, ready;
+  std::vector<Request*> copying;
   // counters (guarded by mu)
   double n_done = 0, n_failed = 0, cell_steps = 0, batches = 0, batch_cells = 0, peak = 0;
 
   void loop();
   bool poll_lanes();
+  bool poll_copies();
   bool admit();
   bool launch();
-  void upload(Arena& A, Request* r);
+  void upload(Arena& A, Request* r, hipStream_t st);
 };
 
-void Server::upload(Arena& A, Request* r) {
+void Server::upload(Arena& A, Request* r, hipStream_t st) {
   long s = r->slot0, n = r->n;
   size_t d = sizeof(double);
-  HIP_OK(hipMemcpyAsync(A.state + s * A.ns, r->in_state, n * A.ns * d, hipMemcpyHostToDevice, up_st));
-  HIP_OK(hipMemcpyAsync(A.env + s * 2, r->in_env, n * 2 * d, hipMemcpyHostToDevice, up_st));
+  HIP_OK(hipMemcpyAsync(A.state + s * A.ns, r->in_state, n * A.ns * d, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(A.env + s * 2, r->in_env, n * 2 * d, hipMemcpyHostToDevice, st));
   if (A.n_upd > 0) {
     HIP_OK(hipMemcpyAsync(A.rates + s * A.n_upd, r->in_rates, n * A.n_upd * d,
-                          hipMemcpyHostToDevice, up_st));
+                          hipMemcpyHostToDevice, st));
     long m = n * A.n_upd;
-    hipLaunchKernelGGL(k_scatter_rates, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, up_st,
+    hipLaunchKernelGGL(k_scatter_rates, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
                        (int)n, A.n_upd, A.upd_off, A.rates + s * A.n_upd, A.n_env,
                        A.renv + s * A.n_env);
   }
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, up_st, n, r->dt,
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, r->dt,
                      A.t1 + s);
-  HIP_OK(hipMemsetD32Async((hipDeviceptr_t)(A.nsteps + s), r->n_steps, n, up_st));
+  HIP_OK(hipMemsetD32Async((hipDeviceptr_t)(A.nsteps + s), r->n_steps, n, st));
   // a fresh request starts with an empty Jacobian history, as a new solver
-  HIP_OK(hipMemsetAsync(A.Js + s * A.nnz, 0, n * A.nnz * d, up_st));
-  HIP_OK(hipMemsetAsync(A.Jy + s * A.nv, 0, n * A.nv * d, up_st));
-  HIP_OK(hipMemsetAsync(A.Jf + s * A.nv, 0, n * A.nv * d, up_st));
-  HIP_OK(hipMemsetAsync(A.fail + s, 0, n * sizeof(int), up_st));
+  HIP_OK(hipMemsetAsync(A.Js + s * A.nnz, 0, n * A.nnz * d, st));
+  HIP_OK(hipMemsetAsync(A.Jy + s * A.nv, 0, n * A.nv * d, st));
+  HIP_OK(hipMemsetAsync(A.Jf + s * A.nv, 0, n * A.nv * d, st));
+  HIP_OK(hipMemsetAsync(A.fail + s, 0, n * sizeof(int), st));
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventCreateWithFlags(&r->up_ev, hipEventDisableTiming));
-  HIP_OK(hipEventRecord(r->up_ev, up_st));
+  HIP_OK(hipEventRecord(r->up_ev, st));
 }
 
 bool Server::admit() {
@@ -176,7 +192,6 @@ bool Server::admit() {
       continue;
     }
     r->slot0 = s;
-    upload(A, r);
     r->state = kReady;
     ready.push_back(r);
     it = pending.erase(it);
@@ -193,18 +208,21 @@ bool Server::launch() {
     if (L.busy || ready.empty()) continue;
     int mech = ready.front()->mech;
     Arena& A = *mechs[mech];
+    long cap = std::min(max_batch, chunk[mech]);
     L.batch.clear();
     long total = 0;
-    for (auto it = ready.begin(); it != ready.end();) {
+    for (auto it = ready.begin(); it != ready.end() && total < cap;) {
       Request* r = *it;
-      if (r->mech != mech || (total > 0 && total + r->n > max_batch)) {
+      if (r->mech != mech) {
         ++it;
         continue;
       }
-      L.batch.push_back(r);
-      total += r->n;
-      it = ready.erase(it);
-      if (total >= max_batch) break;
+      long take = std::min(r->n - r->launched, cap - total);
+      L.batch.push_back({r, r->launched, take});
+      r->launched += take;
+      total += take;
+      if (r->launched == r->n) it = ready.erase(it);
+      else ++it;
     }
     if (total > L.list_cap) {
       if (L.d_list) (void)hipFree(L.d_list);
@@ -214,11 +232,11 @@ bool Server::launch() {
       HIP_OK(hipHostMalloc((void**)&L.h_list, L.list_cap * sizeof(int), hipHostMallocDefault));
     }
     long q = 0;
-    for (Request* r : L.batch) {
-      for (long c = 0; c < r->n; ++c) L.h_list[q++] = (int)(r->slot0 + c);
-      if (r->up_ev) {
-        HIP_OK(hipStreamWaitEvent(L.st, r->up_ev, 0));
-      }
+    for (const Piece& pc : L.batch) {
+      Request* r = pc.r;
+      for (long c = 0; c < pc.count; ++c) L.h_list[q++] = (int)(r->slot0 + pc.first + c);
+      if (!r->up_ev) upload(A, r, L.st);
+      else HIP_OK(hipStreamWaitEvent(L.st, r->up_ev, 0));
       r->state = kRunning;
     }
     HIP_OK(hipMemcpyAsync(L.d_list, L.h_list, total * sizeof(int), hipMemcpyHostToDevice, L.st));
@@ -257,22 +275,10 @@ bool Server::launch() {
     }
     launch_cell_bdf(a, grid, block, lds, L.st);
     HIP_OK(hipGetLastError());
-    for (Request* r : L.batch) {
-      if (r->steps_left != 1) continue;
-      size_t d = sizeof(double);
-      HIP_OK(hipMemcpyAsync(r->out_state, A.state + r->slot0 * A.ns, r->n * A.ns * d,
-                            hipMemcpyDeviceToHost, L.st));
-      if (r->out_stats) {
-        HIP_OK(hipMemcpyAsync(r->out_stats, A.stats + r->slot0 * kCellStats,
-                              r->n * kCellStats * d, hipMemcpyDeviceToHost, L.st));
-      }
-      HIP_OK(hipMemcpyAsync(A.h_fail + r->slot0, A.fail + r->slot0, r->n * sizeof(int),
-                            hipMemcpyDeviceToHost, L.st));
-    }
     HIP_OK(hipEventRecord(L.ev, L.st));
     L.busy = true;
     L.cells = 0;
-    for (Request* r : L.batch) L.cells += (long)r->n * r->n_steps;
+    for (const Piece& pc : L.batch) L.cells += pc.count * pc.r->n_steps;
     any = true;
     std::lock_guard<std::mutex> g(mu);
     batches += 1;
@@ -281,6 +287,8 @@ bool Server::launch() {
   return any;
 }
 
+// A launch has completed: a request whose every cell is now done gets its
+// results copied out on the same stream, completing with that copy's event.
 bool Server::poll_lanes() {
   bool any = false;
   for (Lane& L : lanes) {
@@ -290,37 +298,56 @@ bool Server::poll_lanes() {
     HIP_OK(e);
     L.busy = false;
     any = true;
-    auto now = Clock::now();
-    std::vector<Request*> again;
-    {
-      std::lock_guard<std::mutex> g(mu);
-      cell_steps += (double)L.cells;
-      for (Request* r : L.batch) {
-        if (r->up_ev) {
-          (void)hipEventDestroy(r->up_ev);
-          r->up_ev = nullptr;
-        }
-        if (--r->steps_left > 0) {
-          r->state = kReady;
-          again.push_back(r);
-          continue;
-        }
-        Arena& A = *mechs[r->mech];
-        for (long c = 0; c < r->n; ++c) r->failed |= A.h_fail[r->slot0 + c] != 0;
-        A.release(r->slot0, r->n);
-        r->t_done = now;
-        r->state = kDone;
-        n_done += 1;
-        n_failed += r->failed ? 1 : 0;
+    for (const Piece& pc : L.batch) {
+      Request* r = pc.r;
+      r->finished += pc.count;
+      if (r->finished < r->n) continue;
+      Arena& A = *mechs[r->mech];
+      size_t d = sizeof(double);
+      HIP_OK(hipMemcpyAsync(r->out_state, A.state + r->slot0 * A.ns, r->n * A.ns * d,
+                            hipMemcpyDeviceToHost, L.st));
+      if (r->out_stats) {
+        HIP_OK(hipMemcpyAsync(r->out_stats, A.stats + r->slot0 * kCellStats,
+                              r->n * kCellStats * d, hipMemcpyDeviceToHost, L.st));
       }
+      HIP_OK(hipMemcpyAsync(A.h_fail + r->slot0, A.fail + r->slot0, r->n * sizeof(int),
+                            hipMemcpyDeviceToHost, L.st));
+      HIP_OK(hipEventCreateWithFlags(&r->out_ev, hipEventDisableTiming));
+      HIP_OK(hipEventRecord(r->out_ev, L.st));
+      copying.push_back(r);
     }
-    // requests between timesteps rejoin the ready queue in arrival order
-    for (Request* r : again) {
-      auto pos = ready.begin();
-      while (pos != ready.end() && (*pos)->id < r->id) ++pos;
-      ready.insert(pos, r);
-    }
+    std::lock_guard<std::mutex> g(mu);
+    cell_steps += (double)L.cells;
     L.batch.clear();
+  }
+  return any;
+}
+
+bool Server::poll_copies() {
+  bool any = false;
+  for (size_t i = 0; i < copying.size();) {
+    Request* r = copying[i];
+    hipError_t e = hipEventQuery(r->out_ev);
+    if (e == hipErrorNotReady) {
+      ++i;
+      continue;
+    }
+    HIP_OK(e);
+    auto now = Clock::now();
+    (void)hipEventDestroy(r->out_ev);
+    (void)hipEventDestroy(r->up_ev);
+    r->out_ev = r->up_ev = nullptr;
+    copying[i] = copying.back();
+    copying.pop_back();
+    any = true;
+    std::lock_guard<std::mutex> g(mu);
+    Arena& A = *mechs[r->mech];
+    for (long c = 0; c < r->n; ++c) r->failed |= A.h_fail[r->slot0 + c] != 0;
+    A.release(r->slot0, r->n);
+    r->t_done = now;
+    r->state = kDone;
+    n_done += 1;
+    n_failed += r->failed ? 1 : 0;
     cv_done.notify_all();
   }
   return any;
@@ -335,7 +362,7 @@ void Server::loop() {
         pending.push_back(inbox.front());
         inbox.pop_front();
       }
-      bool idle = pending.empty() && ready.empty();
+      bool idle = pending.empty() && ready.empty() && copying.empty();
       for (Lane& L : lanes) idle = idle && !L.busy;
       if (idle) {
         if (stop) break;
@@ -344,6 +371,7 @@ void Server::loop() {
       }
     }
     bool progress = poll_lanes();
+    progress |= poll_copies();
     progress |= admit();
     progress |= launch();
     if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
@@ -363,7 +391,6 @@ extern "C" void* camp_server_new(int n_streams, long max_batch_cells) {
   Server* s = new Server();
   HIP_OK(hipGetDevice(&s->dev));
   s->max_batch = max_batch_cells;
-  HIP_OK(hipStreamCreateWithFlags(&s->up_st, hipStreamNonBlocking));
   s->lanes.resize(n_streams);
   for (Lane& L : s->lanes) {
     HIP_OK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
@@ -433,8 +460,14 @@ extern "C" int camp_server_add_mechanism(void* server, void* solver_data, long m
   A->upd_off = dev_upload(off, A->owned);
   HIP_OK(hipHostMalloc((void**)&A->h_fail, (size_t)n * sizeof(int), hipHostMallocDefault));
   A->free_ranges[0] = n;
+  // a launch takes at most twice the cells the GPU holds at once
+  int grid = 0, block = 0;
+  size_t lds = 0;
+  S->cell_launch_shape(1L << 30, &grid, &block, &lds);
+  long per_launch = 2L * grid * (block / kWave);
   std::lock_guard<std::mutex> g(s->mu);
   s->mechs.push_back(std::move(A));
+  s->chunk.push_back(per_launch);
   return (int)s->mechs.size() - 1;
 }
 
@@ -459,10 +492,9 @@ extern "C" long camp_server_submit(void* server, int mech, int n_cells, const do
   r->mech = mech;
   r->n = n_cells;
   r->dt = dt;
-  // every timestep of a request runs in one launch (the kernel restages the
+  // every timestep of a cell runs in one launch (the kernel restages the
   // cell between them, as separate launches would)
   r->n_steps = n_steps;
-  r->steps_left = 1;
   r->in_state = state;
   r->in_env = env;
   r->in_rates = base_rates;
@@ -579,8 +611,6 @@ extern "C" void camp_server_free(void* server) {
     for (double* p : L.scratch)
       if (p) (void)hipFree(p);
   }
-  (void)hipStreamSynchronize(s->up_st);
-  (void)hipStreamDestroy(s->up_st);
   for (auto& A : s->mechs) {
     for (void* p : A->owned) (void)hipFree(p);
     (void)hipHostFree(A->h_fail);

@@ -5,13 +5,16 @@
  * solve requests of any cell count for any initialised mechanism while
  * others run: a request is admitted when its cells fit in the mechanism's
  * HBM cell arena (sized from free device memory), its state stays in HBM
- * across its timesteps, and whenever a stream is free the scheduler packs
- * every ready request of one mechanism into one persistent-kernel launch
- * that runs each request's timesteps back to back per cell (continuous
- * batching: requests that arrive while others run are admitted as arena
- * slots free up and start in the next launch).  Cells never share
- * integrator control, so a request's results are bit-identical alone and
- * in any mix.
+ * across its timesteps, and admitted requests form one first-come queue of
+ * cells.  Whenever a stream is free the scheduler launches the persistent
+ * kernel over the next cells of the oldest request's mechanism (from that
+ * request and later ones of the same mechanism, up to twice the cells the
+ * GPU holds at once; a large request spans several launches), each cell
+ * running its request's timesteps back to back (continuous batching:
+ * requests that arrive while others run join the queue and start as soon
+ * as a stream frees up; a request completes when its last cells do).
+ * Cells never share integrator control, so a request's results are
+ * bit-identical alone and in any mix.
  *
  * Threading: one scheduler thread per server owns every HIP call; submit,
  * wait and the counters may be called from any thread.  Host buffers passed
@@ -24,7 +27,9 @@
 extern "C" {
 #endif
 
-/* n_streams concurrent batches (1..8); max_batch_cells caps one launch. */
+/* n_streams concurrent launches (1..8; the GPU runs at most as many
+ * kernels at once as the process has hardware queues, 4 by default);
+ * max_batch_cells caps one launch below the default chunk. */
 void *camp_server_new(int n_streams, long max_batch_cells);
 
 /* Registers an initialised per-cell solver (solver_new + solver_initialize)
