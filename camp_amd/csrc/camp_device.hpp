@@ -543,28 +543,37 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ bool bit128(uint64_t lo, uint64_t hi, int k) {
-  return ((k < 64 ? lo : hi) >> (k & 63)) & 1ull;
-}
-
 // One block of up to 8 pivots of a triangular sweep whose pivot values sit
-// in register X of their owner lanes (lane k - k_base): the block's entry
-// weights come first (LDS loads that depend only on the masks), then the
-// chain, one readlane pair and one FMA per row and pivot.  DIR is +1
-// (forward, L) or -1 (backward, U, weights pre-scaled by the row's inverse
-// diagonal); o0/o1 walk the rows' stored entries.
+// in register X of their owner lanes (lane k - k_base, k_base = 64 when X1):
+// the block's entry weights come first (LDS loads that depend only on the
+// masks), then the chain, one readlane pair and one FMA per row and pivot.
+// DIR is +1 (forward, L) or -1 (backward, U, weights pre-scaled by the
+// row's inverse diagonal); mr0/mr1 are the rows' masks over the 64 columns
+// k_base .. k_base + 63, of which the block's 8 bits are taken once, with
+// the pivots outside [kend, kb] (or [kb, kend)) cleared; o0/o1 walk the
+// rows' stored entries.
 template <int DIR, bool X1>
-__device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t m0a, uint64_t m0b,
-                                          uint64_t m1a, uint64_t m1b, const double* row0,
-                                          const double* row1, int& o0, int& o1, double s0,
-                                          double s1, double& x0, double& x1) {
+__device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t mr0,
+                                          uint64_t mr1, const double* row0, const double* row1,
+                                          int& o0, int& o1, double s0, double s1, double& x0,
+                                          double& x1) {
   constexpr int B = 8;
+  // bit u of the block's masks is pivot kb + DIR * u
+  int lo = DIR > 0 ? kb - k_base : kb - k_base - (B - 1);
+  int n_in = DIR > 0 ? min(B, kend - kb) : min(B, kb - kend + 1);
+  uint32_t valid = (1u << n_in) - 1u;
+  uint32_t b0 = (uint32_t)(lo >= 0 ? (mr0 >> lo) : (mr0 << -lo)) & 0xFFu;
+  uint32_t b1 = (uint32_t)(lo >= 0 ? (mr1 >> lo) : (mr1 << -lo)) & 0xFFu;
+  if (DIR < 0) {  // reverse, so that bit u is pivot kb - u
+    b0 = __builtin_bitreverse32(b0) >> 24;
+    b1 = __builtin_bitreverse32(b1) >> 24;
+  }
+  b0 &= valid;
+  b1 &= valid;
   double w0[B], w1[B];
 #pragma unroll
   for (int u = 0; u < B; ++u) {
-    int k = kb + DIR * u;
-    bool in = DIR > 0 ? k < kend : k >= kend;
-    bool a0 = in && bit128(m0a, m0b, k), a1 = in && bit128(m1a, m1b, k);
+    bool a0 = (b0 >> u) & 1u, a1 = (b1 >> u) & 1u;
     double v0 = row0[o0], v1 = row1[o1];
     w0[u] = a0 ? v0 * s0 : 0.0;
     w1[u] = a1 ? v1 * s1 : 0.0;
@@ -573,8 +582,8 @@ __device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t
   }
 #pragma unroll
   for (int u = 0; u < B; ++u) {
+    if (u >= n_in) break;
     int k = kb + DIR * u;
-    if (DIR > 0 ? k >= kend : k < kend) break;
     double xk = readlane_d(X1 ? x1 : x0, k - k_base);
     x0 = fma(-w0[u], xk, x0);
     x1 = fma(-w1[u], xk, x1);
@@ -620,9 +629,9 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
   const int kf = nv - 1;  // forward pivots 0 .. nv-2
   const int split = kf < kWave ? kf : kWave;
   for (int kb = 0; kb < split; kb += B)
-    tri_block<1, false>(kb, split, 0, l0a, l0b, l1a, l1b, row0, row1, o0, o1, 1.0, 1.0, x0, x1);
+    tri_block<1, false>(kb, split, 0, l0a, l1a, row0, row1, o0, o1, 1.0, 1.0, x0, x1);
   for (int kb = split; kb < kf; kb += B)
-    tri_block<1, true>(kb, kf, kWave, l0a, l0b, l1a, l1b, row0, row1, o0, o1, 1.0, 1.0, x0, x1);
+    tri_block<1, true>(kb, kf, kWave, l0b, l1b, row0, row1, o0, o1, 1.0, 1.0, x0, x1);
   // o0, o1 now index the diagonal of each row
   double d0 = has0 ? 1.0 / row0[o0] : 0.0;
   double d1 = has1 ? 1.0 / row1[o1] : 0.0;
@@ -631,9 +640,9 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
   o0 = len0 - 1;
   o1 = len1 - 1;
   for (int kb = nv - 1; kb >= kWave; kb -= B)
-    tri_block<-1, true>(kb, kWave, kWave, u0a, u0b, u1a, u1b, row0, row1, o0, o1, d0, d1, x0, x1);
+    tri_block<-1, true>(kb, kWave, kWave, u0b, u1b, row0, row1, o0, o1, d0, d1, x0, x1);
   for (int kb = (nv - 1 < kWave - 1 ? nv - 1 : kWave - 1); kb >= 0; kb -= B)
-    tri_block<-1, false>(kb, 0, 0, u0a, u0b, u1a, u1b, row0, row1, o0, o1, d0, d1, x0, x1);
+    tri_block<-1, false>(kb, 0, 0, u0a, u1a, row0, row1, o0, o1, d0, d1, x0, x1);
   wsync();
   if (has0) b[P.perm[r0]] = x0;
   if (has1) b[P.perm[r1]] = x1;
