@@ -208,20 +208,25 @@ __host__ __device__ __forceinline__ double hilo_to_double(int hi, int lo) {
   return __builtin_bit_cast(double, b);
 }
 
-// LDS scratch of the f gather: per species D.hi and S, and the partials of
-// split species (hi, lo, S).
+// LDS scratch of the f gather: per species value and magnitude sum, the
+// partials of split species (hi, lo, magnitudes) and a spare partial that
+// padding records write to.
 __host__ __device__ inline size_t fb_lds_doubles(const DevProgram& P) {
-  return 2 * (size_t)P.nv + 3 * (size_t)P.fb_nparts;
+  return 2 * (size_t)P.nv + 3 * ((size_t)P.fb_nparts + 1);
 }
 
 // One record of the f gather, without branches: the term's rate (or a
 // multi-phase slot value, whose coefficient is 1) times the yield (or -1),
-// unless the record is padding (flag 8), a zero-rate term flagged 1, or
-// fails the negative-yield guard -rate * yield * h <= y[guard]
-// (rxn_arrhenius.c:157 and alike); a dropped record adds zero, which leaves a
-// normalised double-double sum unchanged.  Multiplying a normalised
-// double-double by -1 or 1 is exact, so the sum is the signed sum of the
-// reference's records in their order.
+// unless the record fails the negative-yield guard -rate * yield * h <=
+// y[guard] (rxn_arrhenius.c:157 and alike), in which case its coefficient
+// is zero.
+// The product of the double-double rate and the coefficient is p + e
+// (exact up to the rounding of lo * c); p joins the running sum Dh with an
+// error-free TwoSum whose error, with e, goes to Dl, so Dh + Dl is the
+// record sum to about 2^-100 of the magnitudes S (the reference's long
+// double production/loss sums keep 2^-64).  The running sums are stored at
+// the piece's target after every record -- the last store of a piece is its
+// value -- and a piece's last record (flag 16) restarts them from zero.
 struct FOps {
   double hi, lo, sg;
 };
@@ -237,81 +242,77 @@ __host__ __device__ __forceinline__ FOps f_operands(int4 rec, const double* s_st
   o.sg = s_state[g >= 0 ? g : 0];
   return o;
 }
-__host__ __device__ __forceinline__ void f_record(int4 rec, FOps o, double h, dd& D, double& S) {
+__host__ __device__ __forceinline__ void f_record(int4 rec, FOps o, double h, int nv, double* s_fb,
+                                         double& Dh, double& Dl, double& S) {
   int flags = (int)((unsigned)rec.y >> 24);
   int g = (rec.y << 8) >> 8;
   double c = hilo_to_double(rec.w, rec.z);
-  bool keep = !(flags & 8) && !((flags & 1) && o.hi == 0.0) && (g < 0 || -o.hi * c * h <= o.sg);
-  dd t = dd_mul_d(dd{o.hi, o.lo}, c);
-  t.hi = keep ? t.hi : 0.0;
-  t.lo = keep ? t.lo : 0.0;
-  D = dd_add(D, t);
-  S += fabs(t.hi);
-}
-// The end of a piece (flag 16): its sum goes to the species (target < nv:
-// D.hi and S) or to a partial of a split species (hi, lo, S), and the lane
-// starts the next piece from zero.
-__host__ __device__ __forceinline__ void f_flush(int4 rec, int nv, double* s_fb, dd& D, double& S) {
-  if (!((unsigned)rec.y & (16u << 24))) return;
+  // (padding has coefficient 0 and no guard; a zero-rate term flagged 1
+  // adds zero, as the reference's skip does)
+  bool keep = g < 0 || -o.hi * c * h <= o.sg;
+  c = keep ? c : 0.0;
+  double p = o.hi * c;
+  double e = fma(o.hi, c, -p);
+  e = fma(o.lo, c, e);
+  double s = Dh + p;
+  double bb = s - Dh;
+  double err = (Dh - (s - bb)) + (p - bb);
+  Dh = s;
+  Dl = Dl + (err + e);
+  S += fabs(p);
+  // whole species: value Dh + Dl at [t], S at [nv + t]; partial: (Dh, Dl,
+  // S) at [t], [t + 1], [t + 2] (the lo store first, so that for a whole
+  // species the value overwrites it)
   int t = (int)((unsigned)rec.x >> 16);
-  if (t < nv) {
-    s_fb[t] = D.hi;
-    s_fb[nv + t] = S;
-  } else {
-    double* q = s_fb + 2 * nv + 3 * (t - nv);
-    q[0] = D.hi;
-    q[1] = D.lo;
-    q[2] = S;
+  bool part = flags & 32;
+  s_fb[part ? t + 1 : t] = Dl;
+  s_fb[t] = part ? Dh : Dh + Dl;
+  s_fb[part ? t + 2 : nv + t] = S;
+  if (flags & 16) {
+    Dh = 0.0;
+    Dl = 0.0;
+    S = 0.0;
   }
-  D = dd{0.0, 0.0};
-  S = 0.0;
 }
 
 // f(y) after cell_rates and a wave sync.  Phase 2: every lane walks its bin
 // of fb_rows records (pieces of species' record lists, packed so that the
-// lanes' bins are balanced) accumulating D (= production - loss, double-
-// double, replacing CAMP's long double production/loss sums) and S (the
+// lanes' bins are balanced) accumulating Dh + Dl (= production - loss,
+// replacing CAMP's long double production/loss sums) and S (the
 // magnitudes), with the next four records in flight while the current four
 // are processed and each batch's LDS operands read before the batch is
 // accumulated.  Phase 3: lane per species, the pieces of a split species
 // are added in order, the Jacobian-based estimate J_hist (y - y_hist) +
 // f_hist is formed (ELL over the row view), and the precision-loss blend of
-// time_derivative.c:58-64 gives f.  s_fb is LDS scratch of 2 nv + 3
-// fb_nparts doubles.
+// time_derivative.c:58-64 gives f.  s_fb is LDS scratch of fb_lds_doubles.
 __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lane, const double* s_state,
                                            const double* s_rate, const double* y,
                                            const double* Js, const double* Jy,
                                            const double* Jf, bool use_est, double h,
                                            double* out, double* s_fb,
                                            const double* s_slot = nullptr) {
-  const int4 pad4 = {0, (8 << 24) | 0xFFFFFF, 0, 0};
-  const int2 pad2 = {-1, -1};
   const int nv = P.nv, L = P.fb_rows;
+  const int2 pad2 = {-1, -1};
   const double* slot_or_rate = s_slot ? s_slot : s_rate;
   {
     const int4* fr = P.fb_rec + lane;
-    int4 a0 = 0 < L ? fr[0] : pad4, a1 = 1 < L ? fr[kWave] : pad4,
-         a2 = 2 < L ? fr[2 * kWave] : pad4, a3 = 3 < L ? fr[3 * kWave] : pad4;
-    dd D = {0.0, 0.0};
-    double S = 0.0;
+    // fb_rec has padding rows up to a multiple of 4 plus 4 beyond fb_rows
+    int4 a0 = fr[0], a1 = fr[kWave], a2 = fr[2 * kWave], a3 = fr[3 * kWave];
+    double Dh = 0.0, Dl = 0.0, S = 0.0;
     for (int j = 0; j < L; j += 4) {
       int4 c0 = a0, c1 = a1, c2 = a2, c3 = a3;
-      a0 = j + 4 < L ? fr[(j + 4) * kWave] : pad4;
-      a1 = j + 5 < L ? fr[(j + 5) * kWave] : pad4;
-      a2 = j + 6 < L ? fr[(j + 6) * kWave] : pad4;
-      a3 = j + 7 < L ? fr[(j + 7) * kWave] : pad4;
+      a0 = fr[(j + 4) * kWave];
+      a1 = fr[(j + 5) * kWave];
+      a2 = fr[(j + 6) * kWave];
+      a3 = fr[(j + 7) * kWave];
       FOps o0 = f_operands(c0, s_state, s_rate, slot_or_rate),
            o1 = f_operands(c1, s_state, s_rate, slot_or_rate),
            o2 = f_operands(c2, s_state, s_rate, slot_or_rate),
            o3 = f_operands(c3, s_state, s_rate, slot_or_rate);
-      f_record(c0, o0, h, D, S);
-      f_flush(c0, nv, s_fb, D, S);
-      f_record(c1, o1, h, D, S);
-      f_flush(c1, nv, s_fb, D, S);
-      f_record(c2, o2, h, D, S);
-      f_flush(c2, nv, s_fb, D, S);
-      f_record(c3, o3, h, D, S);
-      f_flush(c3, nv, s_fb, D, S);
+      f_record(c0, o0, h, nv, s_fb, Dh, Dl, S);
+      f_record(c1, o1, h, nv, s_fb, Dh, Dl, S);
+      f_record(c2, o2, h, nv, s_fb, Dh, Dl, S);
+      f_record(c3, o3, h, nv, s_fb, Dh, Dl, S);
     }
   }
   wsync();
@@ -351,11 +352,12 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
       Dhi = s_fb[i];
       S = s_fb[nv + i];
     } else if (fin.y > 0) {
-      dd D = {part[3 * fin.x], part[3 * fin.x + 1]};
+      // the partials' (hi, lo) pairs are not normalised: TwoSum each first
+      dd D = two_sum(part[3 * fin.x], part[3 * fin.x + 1]);
       S = part[3 * fin.x + 2];
       for (int k = 1; k < fin.y; ++k) {
         const double* q = part + 3 * (fin.x + k);
-        D = dd_add(D, dd{q[0], q[1]});
+        D = dd_add(D, two_sum(q[0], q[1]));
         S += q[2];
       }
       Dhi = D.hi;

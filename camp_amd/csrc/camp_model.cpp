@@ -773,25 +773,32 @@ std::string Model::initialize() {
         break;
       }
     }
-    if (nv + P.fb_nparts >= 65536) return "more than 65535 f gather targets";
+    if (2 * nv + 3 * P.fb_nparts + 3 >= 65536) return "more than 65535 f gather targets";
     P.fb_rows = L;
-    P.fb_rec.assign((size_t)4 * W * L, 0);
+    // rows padded to a multiple of 4 plus 4 more, so that the gather's
+    // batches of 4 and its loads 4 rows ahead need no bounds checks
+    const int L_alloc = (L + 3) / 4 * 4 + 4;
+    P.fb_rec.assign((size_t)4 * W * L_alloc, 0);
     for (int l = 0; l < W; ++l) {
       int j = 0;
       for (const Piece& pc : bins[l])
         for (int r = 0; r < pc.count; ++r, ++j) {
           const int* src = &P.fe_rec[4 * (size_t)(pc.first + r)];
           int* dst = &P.fb_rec[4 * ((size_t)j * W + l)];
-          int target = pc.part >= 0 ? nv + pc.part : pc.species;
+          // the piece's sum goes to s_fb[target]: a whole species' value at
+          // [species] (its magnitude sum at [nv + species]), a partial's
+          // (hi, lo, magnitudes) at [2 nv + 3 part ..] (flag 32)
+          int target = pc.part >= 0 ? 2 * nv + 3 * pc.part : pc.species;
           dst[0] = (src[0] & 0xFFFF) | (target << 16);
-          dst[1] = src[1] | (r + 1 == pc.count ? (16 << 24) : 0);
+          dst[1] = src[1] | (r + 1 == pc.count ? (16 << 24) : 0) | (pc.part >= 0 ? (32 << 24) : 0);
           dst[2] = src[2];
           dst[3] = src[3];
         }
-      for (; j < L; ++j) {
+      // padding: no contribution, running sums stored to a spare partial
+      for (; j < L_alloc; ++j) {
         int* dst = &P.fb_rec[4 * ((size_t)j * W + l)];
-        dst[0] = 0;
-        dst[1] = (8 << 24) | 0xFFFFFF;
+        dst[0] = (2 * nv + 3 * P.fb_nparts) << 16;
+        dst[1] = ((8 | 32) << 24) | 0xFFFFFF;
         dst[2] = dst[3] = 0;
       }
     }

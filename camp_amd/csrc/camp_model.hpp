@@ -83,13 +83,18 @@ struct Program {
   std::vector<int> fs_species, fl_hdr, re_ell;
   // The f gather in balanced bins: the species' record lists are cut into
   // pieces of at most fb_rows records (a species longer than that is split
-  // into near-equal pieces) and the pieces packed first-fit-decreasing into
-  // 64 lane bins of fb_rows records, so every lane walks fb_rows records.
-  // fb_rec holds row j of lane l at j * 64 + l as {term or slot index |
+  // into near-equal pieces) and the pieces packed longest first into 64
+  // lane bins of fb_rows records (a piece that no bin holds is split across
+  // bins), so every lane walks fb_rows records.
+  // fb_rec holds row j of lane l at j * 64 + l (rows beyond fb_rows, up to
+  // a multiple of 4 plus 4, are padding) as {term or slot index |
   // target << 16, guard | flags << 24, coefficient (low, high word)}; flag
-  // 16 ends a piece, whose sum goes to target (a species < nv, or partial
-  // nv + p of a split species).  fb_fin[i] = {first partial, partial count}
-  // per species (count 0: one piece, -1: no records).
+  // 16 ends a piece, flag 32 marks a piece of a split species.  A piece's
+  // running sum is kept at its target in the gather's LDS scratch: a whole
+  // species' value at [species] and magnitude sum at [nv + species], the
+  // p-th partial's (hi, lo, magnitudes) at [2 nv + 3 p].  fb_fin[i] =
+  // {first partial, partial count} per species (count 0: one piece, -1: no
+  // records).
   int fb_rows = 0, fb_nparts = 0;
   std::vector<int> fb_rec, fb_fin;
   // rate terms as one 16-byte record: {k index | flags << 28, reactant state
