@@ -682,7 +682,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   cv.nv = nv;
   cv.s_state = base;
   cv.s_k = cv.s_state + P.ns;
-  cv.mpl = mp_lds(P, cv.s_k + P.n_env, scratch ? scratch + P.lu_nnz + P.rnnz : nullptr);
+  cv.mpl = mp_lds(P, cv.s_k + P.n_env, scratch ? scratch + P.lu_nnz + kWave + P.rnnz : nullptr);
   cv.s_rate = cv.s_k + P.n_env + mp_lds_doubles(P);
   double* v = cv.s_rate + 2 * (size_t)P.n_terms;
   cv.znb = v;
@@ -697,17 +697,19 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   cv.gh1 = v; v += nv;
   cv.gh2 = v; v += nv;
   cv.w = v; v += nv;
-  cv.s_fb = v; v += fb_lds_doubles(P);
   // Gas-phase cells keep their J history (J_solver, J_state, J_deriv) in
   // LDS for the whole solve; multi-phase cells use it in place in HBM.
   double *h_js = nullptr, *h_jy = nullptr, *h_jf = nullptr;
+  // the f gather's scratch follows the LU values (gas: also the LU's sinks)
   if (MP) {
     cv.lu = scratch;
-    cv.Jr = scratch + P.lu_nnz;
+    cv.Jr = scratch + P.lu_nnz + kWave;
+    cv.s_fb = v; v += fb_sink_doubles(P);
   } else {
     cv.lu = v;
     cv.Jr = nullptr;
     v += P.lu_nnz;
+    cv.s_fb = v; v += fb_sink_doubles(P);
     h_js = v; v += P.nnz;
     h_jy = v; v += nv;
     h_jf = v; v += nv;

@@ -30,8 +30,11 @@ struct DevProgram {
   const int *colptr, *rowval, *r_ptr, *r_elem, *r_col;
   const int *perm, *m_to_lu, *diag_pos, *e_diag;
   const int *l_ptr, *l_pos, *l_row, *u_ptr, *u_pos, *u_row, *up_ptr;
-  const uint64_t* up_rec;  // LU update records: dst | l << 16 | u << 32
-  const int4* lu_hdr;       // per pivot: {l range, update range}, {diag position, -}
+  const uint64_t* up_rec;  // LU update records: dst | l << 16 | u << 32, rows of 64
+  const int4* lu_hdr;       // per pivot: {l range, update range}, {diag position, u range}
+  const int* lf_pos;        // L positions the factorisation scales, rows of 64
+  const int4* lf_hdr;       // per pivot: {lf_pos row start, up_rec row start,
+                            //  L rows | update rows << 16, diag position}
   const int *lu_rowptr, *lu_col;
   const uint64_t* row_mask;  // 4 per LU row; nullptr when nv > 128
   const int4 *fe_a, *je_a;  // packed gather records (see Program in camp_model.hpp)
@@ -213,6 +216,12 @@ __host__ __device__ __forceinline__ double hilo_to_double(int hi, int lo) {
 // padding records write to.
 __host__ __device__ inline size_t fb_lds_doubles(const DevProgram& P) {
   return 2 * (size_t)P.nv + 3 * ((size_t)P.fb_nparts + 1);
+}
+// The per-cell kernel places the f gather's scratch right after the LU
+// values, where it also serves as the factorisation's 64 sink slots.
+__host__ __device__ inline size_t fb_sink_doubles(const DevProgram& P) {
+  size_t n = fb_lds_doubles(P);
+  return n > (size_t)kWave ? n : (size_t)kWave;
 }
 
 // One record of the f gather, without branches: the term's rate (or a
@@ -445,7 +454,8 @@ __device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const do
 // ---------------------------------------------------------------------
 // Sparse LU of M = I - gamma*J (SUNMatScaleAddI) with the static pivot
 // order of the program; right-looking, one pivot at a time.  Returns 1 on
-// a zero or non-finite pivot (a recoverable linear-setup failure).
+// a zero or non-finite pivot (a recoverable linear-setup failure).  lu
+// holds lu_nnz values and 64 sink slots after them (scratch).
 // ---------------------------------------------------------------------
 __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, const double* J,
                                               double gamma, double* lu) {
@@ -458,81 +468,70 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
   }
   wsync();
   // Software pipeline over pivots: pivot k+2's header, pivot k+1's first
-  // 64 division positions and first 5 x 64 update records are fetched from
-  // global memory while pivot k is eliminated, so no pivot waits on a
-  // global load.  Within one pivot every update has a distinct destination
-  // (neither the pivot's L column nor its U row).  A wavefront's LDS
-  // accesses execute in program order, so the pivot, the L values and every
-  // update operand are read before any of them is written: the updates use
-  // the unscaled L value times the pivot's reciprocal -- the same product
-  // that is stored as the scaled L value -- and the pivot needs a single
-  // wavefront sync.
+  // row of L positions and first 5 rows of update records are fetched from
+  // global memory while pivot k is eliminated.  The program is padded to
+  // rows of 64 with entries that write the lane's own sink slot lu[lu_nnz +
+  // lane], so every load and store is unconditional.  Within one pivot
+  // every update has a distinct destination (neither the pivot's L column
+  // nor its U row).  A wavefront's LDS accesses execute in program order,
+  // so the pivot, the L values and every update operand are read before any
+  // of them is written: the updates use the unscaled L value times the
+  // pivot's reciprocal -- the same product that is stored as the scaled L
+  // value -- and the pivot needs a single wavefront sync.
   constexpr int R = 5;
-  int4 h0 = P.lu_hdr[0], h1 = P.lu_hdr[2];
-  int d0 = P.lu_hdr[1].x, d1 = P.lu_hdr[3].x;
-  int l_p = (h0.x + lane < h0.y) ? P.l_pos[h0.x + lane] : -1;
+  const uint64_t sink = (uint64_t)(P.lu_nnz + lane);  // {dst = sink, l = 0, u = 0}
+  int4 h0 = P.lf_hdr[0], h1 = P.lf_hdr[1];
+  int l_p = P.lf_pos[h0.x + lane];
+  int nb0 = h0.z >> 16;
   uint64_t cur[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    int q = h0.z + r * kWave + lane;
-    cur[r] = q < h0.w ? P.up_rec[q] : ~0ull;
-  }
+  for (int r = 0; r < R; ++r) cur[r] = r < nb0 ? P.up_rec[h0.y + r * kWave + lane] : sink;
   for (int k = 0; k < P.nv; ++k) {
-    int4 h2 = P.lu_hdr[2 * (k + 2)];
-    int d2 = P.lu_hdr[2 * (k + 2) + 1].x;
-    int n_lp = (k + 1 < P.nv && h1.x + lane < h1.y) ? P.l_pos[h1.x + lane] : -1;
+    int4 h2 = P.lf_hdr[k + 2];
+    int n_lp = P.lf_pos[h1.x + lane];
+    int nb1 = h1.z >> 16;
     uint64_t nxt[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int q = h1.z + r * kWave + lane;
-      nxt[r] = (k + 1 < P.nv && q < h1.w) ? P.up_rec[q] : ~0ull;
-    }
-    double piv = lu[d0];
-    double lval = lu[l_p >= 0 ? l_p : 0];
+    for (int r = 0; r < R; ++r) nxt[r] = r < nb1 ? P.up_rec[h1.y + r * kWave + lane] : sink;
+    double piv = lu[h0.w];
+    double lval = lu[l_p];
     double od[R], oa[R], ob[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      uint64_t x = cur[r] != ~0ull ? cur[r] : 0ull;
-      od[r] = lu[(int)(x & 0xFFFF)];
-      oa[r] = lu[(int)((x >> 16) & 0xFFFF)];
-      ob[r] = lu[(int)((x >> 32) & 0xFFFF)];
+      od[r] = lu[(int)(cur[r] & 0xFFFF)];
+      oa[r] = lu[(int)((cur[r] >> 16) & 0xFFFF)];
+      ob[r] = lu[(int)((cur[r] >> 32) & 0xFFFF)];
     }
     if (piv == 0.0 || !isfinite(piv)) return 1;
     double rinv = 1.0 / piv;
-    if (l_p >= 0) lu[l_p] = lval * rinv;
-    for (int q = h0.x + kWave + lane; q < h0.y; q += kWave) {
-      int p = P.l_pos[q];
+    lu[l_p] = lval * rinv;
+    for (int r = 1; r < (h0.z & 0xFFFF); ++r) {
+      int p = P.lf_pos[h0.x + r * kWave + lane];
       lu[p] = lu[p] * rinv;
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (cur[r] != ~0ull) lu[(int)(cur[r] & 0xFFFF)] = od[r] - (oa[r] * rinv) * ob[r];
-    if (h0.w - h0.z > R * kWave) {
-      // the remaining updates (pivots of the dense trailing block) read the
-      // scaled L values; they go in batches of R x 64 records whose loads
-      // are all issued before the first is used
+    for (int r = 0; r < R; ++r) lu[(int)(cur[r] & 0xFFFF)] = od[r] - (oa[r] * rinv) * ob[r];
+    if (nb0 > R) {
+      // the remaining rows (pivots of the dense trailing block) read the
+      // scaled L values; they go in batches of R rows whose loads are all
+      // issued before the first is used
       wsync();
-      for (int base = h0.z + R * kWave; base < h0.w; base += R * kWave) {
+      for (int b = R; b < nb0; b += R) {
         uint64_t x[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          int q = base + r * kWave + lane;
-          x[r] = q < h0.w ? P.up_rec[q] : ~0ull;
-        }
+        for (int r = 0; r < R; ++r) x[r] = b + r < nb0 ? P.up_rec[h0.y + (b + r) * kWave + lane] : sink;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          uint64_t y = x[r] != ~0ull ? x[r] : 0ull;
-          od[r] = lu[(int)(y & 0xFFFF)];
-          oa[r] = lu[(int)((y >> 16) & 0xFFFF)];
-          ob[r] = lu[(int)((y >> 32) & 0xFFFF)];
+          od[r] = lu[(int)(x[r] & 0xFFFF)];
+          oa[r] = lu[(int)((x[r] >> 16) & 0xFFFF)];
+          ob[r] = lu[(int)((x[r] >> 32) & 0xFFFF)];
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-          if (x[r] != ~0ull) lu[(int)(x[r] & 0xFFFF)] = od[r] - oa[r] * ob[r];
+        for (int r = 0; r < R; ++r) lu[(int)(x[r] & 0xFFFF)] = od[r] - oa[r] * ob[r];
       }
     }
     wsync();
-    h0 = h1; h1 = h2; d0 = d1; d1 = d2; l_p = n_lp;
+    h0 = h1; h1 = h2; l_p = n_lp; nb0 = nb1;
 #pragma unroll
     for (int r = 0; r < R; ++r) cur[r] = nxt[r];
   }

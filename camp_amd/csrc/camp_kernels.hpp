@@ -20,12 +20,13 @@ constexpr int kCellProf = 7;
 // programs) and 24 slots for the BDF coefficient arrays l, tq and tau.
 __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P) {
   return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms + 16 * (size_t)P.nv +
-         fb_lds_doubles(P) + (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
+         fb_sink_doubles(P) + (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
 }
-// Global scratch doubles per wavefront of multi-phase programs: LU factors,
-// the reaction Jacobian and the per-instance Jacobian scalars.
+// Global scratch doubles per wavefront of multi-phase programs: LU factors
+// (with the factorisation's 64 sink slots), the reaction Jacobian and the
+// per-instance Jacobian scalars.
 __host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P) {
-  return P.mp ? (size_t)P.lu_nnz + P.rnnz + mp_jsc_doubles(P) : 0;
+  return P.mp ? (size_t)P.lu_nnz + kWave + P.rnnz + mp_jsc_doubles(P) : 0;
 }
 
 struct CellBdfArgs {

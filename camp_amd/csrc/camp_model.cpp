@@ -900,9 +900,37 @@ std::string Model::initialize() {
     h[0] = P.l_ptr[k]; h[1] = P.l_ptr[k + 1]; h[2] = P.up_ptr[k]; h[3] = P.up_ptr[k + 1];
     h[4] = P.diag_pos[k]; h[5] = P.u_ptr[k]; h[6] = P.u_ptr[k + 1];
   }
-  P.up_rec.resize(P.up_dst.size());
-  for (size_t q = 0; q < P.up_dst.size(); ++q)
-    P.up_rec[q] = (uint64_t)P.up_dst[q] | ((uint64_t)P.up_l[q] << 16) | ((uint64_t)P.up_u[q] << 32);
+  // The factorisation's program: per pivot, the L positions to scale and the
+  // update records {dst, l, u}, each padded to whole rows of 64 (at least
+  // one row of L positions) with entries that write lane l's sink slot
+  // lu_nnz + l, so the device loops need no per-lane bounds checks;
+  // lf_hdr[k] = {first L position, first update record, L rows | update
+  // rows << 16, diagonal position} (two empty pivots past the end).
+  if (P.lu_nnz + 64 > 65535) return "LU fill exceeds 65471 entries per cell";
+  P.lf_pos.clear();
+  P.up_rec.clear();
+  P.lf_hdr.assign((size_t)4 * (nv + 2), 0);
+  for (int k = 0; k < nv; ++k) {
+    int* h = &P.lf_hdr[(size_t)4 * k];
+    h[0] = (int)P.lf_pos.size();
+    h[1] = (int)P.up_rec.size();
+    int nl = P.l_ptr[k + 1] - P.l_ptr[k], nu = P.up_ptr[k + 1] - P.up_ptr[k];
+    int rl = std::max(1, (nl + 63) / 64), ru = (nu + 63) / 64;
+    for (int q = 0; q < 64 * rl; ++q)
+      P.lf_pos.push_back(q < nl ? P.l_pos[P.l_ptr[k] + q] : P.lu_nnz + q % 64);
+    for (int q = 0; q < 64 * ru; ++q) {
+      if (q < nu) {
+        size_t e = (size_t)P.up_ptr[k] + q;
+        P.up_rec.push_back((uint64_t)P.up_dst[e] | ((uint64_t)P.up_l[e] << 16) |
+                           ((uint64_t)P.up_u[e] << 32));
+      } else {
+        P.up_rec.push_back((uint64_t)(P.lu_nnz + q % 64));
+      }
+    }
+    h[2] = rl | (ru << 16);
+    h[3] = P.diag_pos[k];
+  }
+  if (P.up_rec.empty()) P.up_rec.push_back((uint64_t)P.lu_nnz);
   initialized = true;
   return std::string();
 }

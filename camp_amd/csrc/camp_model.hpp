@@ -112,7 +112,12 @@ struct Program {
                             //  diag_pos[k], u_ptr[k], u_ptr[k+1], 0} (two 16-byte
                             //  loads), then 2 empty pivots
   std::vector<uint16_t> up_dst, up_l, up_u;         // LU[dst] -= LU[l] * LU[u]
-  std::vector<uint64_t> up_rec;                     // dst | l << 16 | u << 32
+  // the factorisation's program (built in Model::initialize): per pivot L
+  // positions (lf_pos) and update records dst | l << 16 | u << 32 (up_rec),
+  // each padded to rows of 64 with writes to the lanes' sink slots past the
+  // LU values, and lf_hdr = 4 ints per pivot
+  std::vector<int> lf_pos, lf_hdr;
+  std::vector<uint64_t> up_rec;
   // Row-major LU storage: row i occupies [lu_rowptr[i], lu_rowptr[i+1]) in
   // column order.  For nv <= 128, row_mask[4i..4i+3] holds the columns of
   // row i's strictly lower (L) part as a 128-bit mask (low, high word), then

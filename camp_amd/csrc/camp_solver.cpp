@@ -74,6 +74,8 @@ void Solver::setup_device() {
   P.up_ptr = dev_upload(G.up_ptr, owned);
   P.lu_hdr = reinterpret_cast<const int4*>(dev_upload(G.lu_hdr, owned));
   P.up_rec = dev_upload(G.up_rec, owned);
+  P.lf_pos = dev_upload(G.lf_pos, owned);
+  P.lf_hdr = reinterpret_cast<const int4*>(dev_upload(G.lf_hdr, owned));
   P.fe_a = reinterpret_cast<const int4*>(dev_upload(G.fe_a, owned));
   P.fe_b = dev_upload(G.fe_b, owned);
   P.fe_c = dev_upload(G.fe_c, owned);
@@ -569,7 +571,7 @@ struct BlockCv {
     }
     HIP_OK(hipMemsetAsync(S.d_fail, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_lu_factor, dim3(std::min(nc, 16384)), dim3(64),
-                       (size_t)S.P.lu_nnz * sizeof(double), s, S.P, nc, S.d_Js, gamma, S.d_LU,
+                       ((size_t)S.P.lu_nnz + kWave) * sizeof(double), s, S.P, nc, S.d_Js, gamma, S.d_LU,
                        S.d_fail);
     return check_fail();
   }
@@ -1291,7 +1293,8 @@ extern "C" int camp_eval_lu_solve(void* solver_data, const double* jac, double g
   HIP_OK(hipMemcpyAsync(d_b, b, (size_t)nc * P.nv * sizeof(double), hipMemcpyHostToDevice, s->stream));
   HIP_OK(hipMemsetAsync(s->d_fail, 0, sizeof(int), s->stream));
   hipLaunchKernelGGL(camp::k_lu_factor, dim3(std::min(nc, 16384)), dim3(64),
-                     (size_t)P.lu_nnz * sizeof(double), s->stream, P, nc, d_J, gamma, d_LU, s->d_fail);
+                     ((size_t)P.lu_nnz + camp::kWave) * sizeof(double), s->stream, P, nc, d_J,
+                     gamma, d_LU, s->d_fail);
   hipLaunchKernelGGL(camp::k_lu_solve, dim3(std::min(nc, 16384)), dim3(64),
                      ((size_t)P.lu_nnz + 2 * (size_t)P.nv) * sizeof(double), s->stream, P, nc, d_LU,
                      d_b, 1.0);

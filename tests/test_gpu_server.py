@@ -36,6 +36,30 @@ def test_request_equals_solver_run():
     srv.close()
 
 
+def test_request_larger_than_a_launch_chunk():
+    # a launch takes at most twice the cells the GPU holds at once (2 048
+    # gas cells on MI355X): this request spans launches on several streams,
+    # and a small request submitted after it interleaves with its chunks
+    n = 2600
+    mech, cells = _cb05(n)
+    ref = CampSolver(mech, n)
+    ref.set_rates(cells.rates)
+    a = cells.state.copy()
+    assert ref.solve(a, cells.env, 0.0, 180.0) == 0
+    srv = CampServer(n_streams=4)
+    m = srv.add_mechanism(CampSolver(mech, 1), max_cells=8192)
+    br = base_rate_matrix(ref, cells.rates, n)
+    big = srv.submit(m, cells.state, cells.env, br, 180.0, n_steps=1, want_stats=True)
+    small = srv.submit(m, cells.state[:5], cells.env[:5], br[:5], 180.0, n_steps=1)
+    assert big.wait() and small.wait()
+    assert np.array_equal(big.state, a)
+    assert np.array_equal(big.stats, ref.cell_stats())
+    assert np.array_equal(small.state, a[:5])
+    c = srv.counters()
+    assert c["batches"] >= 2 and c["done"] == 2 and c["failed"] == 0
+    srv.close()
+
+
 def test_alone_equals_mix_across_requests_and_mechanisms():
     n = 32
     mech, cells = _cb05(n)
