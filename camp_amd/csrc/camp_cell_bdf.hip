@@ -682,7 +682,8 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   cv.nv = nv;
   cv.s_state = base;
   cv.s_k = cv.s_state + P.ns;
-  cv.mpl = mp_lds(P, cv.s_k + P.n_env, scratch ? scratch + P.lu_nnz + kWave + P.rnnz : nullptr);
+  cv.mpl = mp_lds(P, cv.s_k + P.n_env,
+                  scratch ? scratch + P.lu_nnz + kWave + 1 + P.rnnz : nullptr);
   cv.s_rate = cv.s_k + P.n_env + mp_lds_doubles(P);
   double* v = cv.s_rate + 2 * (size_t)P.n_terms;
   cv.znb = v;
@@ -703,7 +704,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   // the f gather's scratch follows the LU values (gas: also the LU's sinks)
   if (MP) {
     cv.lu = scratch;
-    cv.Jr = scratch + P.lu_nnz + kWave;
+    cv.Jr = scratch + P.lu_nnz + kWave + 1;
     cv.s_fb = v; v += fb_sink_doubles(P);
   } else {
     cv.lu = v;

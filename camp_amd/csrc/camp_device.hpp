@@ -218,10 +218,11 @@ __host__ __device__ inline size_t fb_lds_doubles(const DevProgram& P) {
   return 2 * (size_t)P.nv + 3 * ((size_t)P.fb_nparts + 1);
 }
 // The per-cell kernel places the f gather's scratch right after the LU
-// values, where it also serves as the factorisation's 64 sink slots.
+// values, where it also serves as the factorisation's 64 sink slots and
+// the triangular solve's zero slot (the 65th).
 __host__ __device__ inline size_t fb_sink_doubles(const DevProgram& P) {
   size_t n = fb_lds_doubles(P);
-  return n > (size_t)kWave ? n : (size_t)kWave;
+  return n > (size_t)kWave + 1 ? n : (size_t)kWave + 1;
 }
 
 // One record of the f gather, without branches: the term's rate (or a
@@ -544,20 +545,22 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
-// One block of up to 8 pivots of a triangular sweep whose pivot values sit
-// in register X of their owner lanes (lane k - k_base, k_base = 64 when X1):
-// the block's entry weights come first (LDS loads that depend only on the
-// masks), then the chain, one readlane pair and one FMA per row and pivot.
+// One block of 8 pivots of a triangular sweep whose pivot values sit in
+// register X of their owner lanes (lane k - k_base, k_base = 64 when X1):
+// the block's 16 entry weights are LDS loads issued first -- an entry a row
+// does not have is read from a slot holding zero, so no select follows the
+// load -- then the chain, one readlane pair and one FMA per row and pivot.
 // DIR is +1 (forward, L) or -1 (backward, U, weights pre-scaled by the
 // row's inverse diagonal); mr0/mr1 are the rows' masks over the 64 columns
 // k_base .. k_base + 63, of which the block's 8 bits are taken once, with
-// the pivots outside [kend, kb] (or [kb, kend)) cleared; o0/o1 walk the
-// rows' stored entries.
+// the pivots outside [kend, kb] (or [kb, kend)) cleared: such a pivot adds
+// zero times some lane's value, so every block runs all 8 steps.  o0/o1
+// walk the rows' stored entries.
 template <int DIR, bool X1>
 __device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t mr0,
                                           uint64_t mr1, const double* row0, const double* row1,
-                                          int& o0, int& o1, double s0, double s1, double& x0,
-                                          double& x1) {
+                                          const double* zero, int& o0, int& o1, double s0,
+                                          double s1, double& x0, double& x1) {
   constexpr int B = 8;
   // bit u of the block's masks is pivot kb + DIR * u
   int lo = DIR > 0 ? kb - k_base : kb - k_base - (B - 1);
@@ -574,18 +577,16 @@ __device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t
   double w0[B], w1[B];
 #pragma unroll
   for (int u = 0; u < B; ++u) {
-    bool a0 = (b0 >> u) & 1u, a1 = (b1 >> u) & 1u;
-    double v0 = row0[o0], v1 = row1[o1];
-    w0[u] = a0 ? v0 * s0 : 0.0;
-    w1[u] = a1 ? v1 * s1 : 0.0;
-    o0 += DIR * (int)a0;
-    o1 += DIR * (int)a1;
+    int a0 = (b0 >> u) & 1u, a1 = (b1 >> u) & 1u;
+    w0[u] = *(a0 ? row0 + o0 : zero) * s0;
+    w1[u] = *(a1 ? row1 + o1 : zero) * s1;
+    o0 += DIR * a0;
+    o1 += DIR * a1;
   }
 #pragma unroll
   for (int u = 0; u < B; ++u) {
-    if (u >= n_in) break;
     int k = kb + DIR * u;
-    double xk = readlane_d(X1 ? x1 : x0, k - k_base);
+    double xk = readlane_d(X1 ? x1 : x0, (k - k_base) & (kWave - 1));
     x0 = fma(-w0[u], xk, x0);
     x1 = fma(-w1[u], xk, x1);
   }
@@ -601,7 +602,7 @@ __device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t
 // solution nor a table lookup and are issued ahead of the chain.  The
 // backward sweep solves D^-1 U x = D^-1 y, so no division sits on the chain.
 // Sweeps are split at pivot 64 so each loop reads one register.
-__device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane, const double* lu,
+__device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane, double* lu,
                                                   double* b) {
   const int nv = P.nv;
   const int r0 = lane, r1 = lane + kWave;
@@ -625,14 +626,18 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
   }
   const double* row0 = lu + base0;
   const double* row1 = lu + base1;
+  // a slot holding zero, after the LU values and the factorisation's sinks
+  double* zero = lu + P.lu_nnz + kWave;
+  if (lane == 0) *zero = 0.0;
+  wsync();
   constexpr int B = 8;
   int o0 = 0, o1 = 0;
   const int kf = nv - 1;  // forward pivots 0 .. nv-2
   const int split = kf < kWave ? kf : kWave;
   for (int kb = 0; kb < split; kb += B)
-    tri_block<1, false>(kb, split, 0, l0a, l1a, row0, row1, o0, o1, 1.0, 1.0, x0, x1);
+    tri_block<1, false>(kb, split, 0, l0a, l1a, row0, row1, zero, o0, o1, 1.0, 1.0, x0, x1);
   for (int kb = split; kb < kf; kb += B)
-    tri_block<1, true>(kb, kf, kWave, l0b, l1b, row0, row1, o0, o1, 1.0, 1.0, x0, x1);
+    tri_block<1, true>(kb, kf, kWave, l0b, l1b, row0, row1, zero, o0, o1, 1.0, 1.0, x0, x1);
   // o0, o1 now index the diagonal of each row
   double d0 = has0 ? 1.0 / row0[o0] : 0.0;
   double d1 = has1 ? 1.0 / row1[o1] : 0.0;
@@ -641,9 +646,9 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
   o0 = len0 - 1;
   o1 = len1 - 1;
   for (int kb = nv - 1; kb >= kWave; kb -= B)
-    tri_block<-1, true>(kb, kWave, kWave, u0b, u1b, row0, row1, o0, o1, d0, d1, x0, x1);
+    tri_block<-1, true>(kb, kWave, kWave, u0b, u1b, row0, row1, zero, o0, o1, d0, d1, x0, x1);
   for (int kb = (nv - 1 < kWave - 1 ? nv - 1 : kWave - 1); kb >= 0; kb -= B)
-    tri_block<-1, false>(kb, 0, 0, u0a, u1a, row0, row1, o0, o1, d0, d1, x0, x1);
+    tri_block<-1, false>(kb, 0, 0, u0a, u1a, row0, row1, zero, o0, o1, d0, d1, x0, x1);
   wsync();
   if (has0) b[P.perm[r0]] = x0;
   if (has1) b[P.perm[r1]] = x1;
@@ -651,7 +656,7 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
 }
 
 // Solve (LU) x = b in place; b in dep order, w a scratch vector (nv).
-__device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, const double* lu,
+__device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, double* lu,
                                               double* b, double* w) {
   if (P.row_mask) {
     cell_lu_solve_reg(P, lane, lu, b);

@@ -128,8 +128,8 @@ __global__ void __launch_bounds__(64) k_lu_solve(DevProgram P, int nc, const dou
                                                  double* __restrict__ b, double s) {
   extern __shared__ double smem[];
   int lane = threadIdx.x;
-  double* s_lu = smem;
-  double* s_b = s_lu + P.lu_nnz;
+  double* s_lu = smem;  // LU values, then the factorisation's sinks and the solve's zero slot
+  double* s_b = s_lu + P.lu_nnz + kWave + 1;
   double* s_w = s_b + P.nv;
   for (int c = blockIdx.x; c < nc; c += gridDim.x) {
     const double* in = LU + (size_t)c * P.lu_nnz;

@@ -23,10 +23,10 @@ __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P) {
          fb_sink_doubles(P) + (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
 }
 // Global scratch doubles per wavefront of multi-phase programs: LU factors
-// (with the factorisation's 64 sink slots), the reaction Jacobian and the
-// per-instance Jacobian scalars.
+// (with the factorisation's 64 sink slots and the solve's zero slot), the
+// reaction Jacobian and the per-instance Jacobian scalars.
 __host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P) {
-  return P.mp ? (size_t)P.lu_nnz + kWave + P.rnnz + mp_jsc_doubles(P) : 0;
+  return P.mp ? (size_t)P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) : 0;
 }
 
 struct CellBdfArgs {

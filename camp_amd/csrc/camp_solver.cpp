@@ -578,7 +578,8 @@ struct BlockCv {
   void lsolve(double* b) {
     double sc = gamrat != 1.0 ? 2.0 / (1.0 + gamrat) : 1.0;
     hipLaunchKernelGGL(k_lu_solve, dim3(std::min(nc, 16384)), dim3(64),
-                       ((size_t)S.P.lu_nnz + 2 * (size_t)S.P.nv) * sizeof(double), s, S.P, nc,
+                       ((size_t)S.P.lu_nnz + kWave + 1 + 2 * (size_t)S.P.nv) * sizeof(double), s,
+                       S.P, nc,
                        S.d_LU, b, sc);
   }
   // Non-negativity safeguard, the per-cell kernel's rule: components of y
@@ -1296,7 +1297,8 @@ extern "C" int camp_eval_lu_solve(void* solver_data, const double* jac, double g
                      ((size_t)P.lu_nnz + camp::kWave) * sizeof(double), s->stream, P, nc, d_J,
                      gamma, d_LU, s->d_fail);
   hipLaunchKernelGGL(camp::k_lu_solve, dim3(std::min(nc, 16384)), dim3(64),
-                     ((size_t)P.lu_nnz + 2 * (size_t)P.nv) * sizeof(double), s->stream, P, nc, d_LU,
+                     ((size_t)P.lu_nnz + camp::kWave + 1 + 2 * (size_t)P.nv) * sizeof(double),
+                     s->stream, P, nc, d_LU,
                      d_b, 1.0);
   int fail = 0;
   HIP_OK(hipMemcpyAsync(&fail, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, s->stream));
