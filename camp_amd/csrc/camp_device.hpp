@@ -86,17 +86,53 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+// Wavefront reductions with DPP lane moves (no LDS round trip): quad
+// butterflies, half-row and row mirrors give every lane its row's value,
+// then row broadcasts of lanes 15 and 31 carry the rows into lane 63, whose
+// value is read into a scalar register.  Rows a broadcast does not write
+// keep the identity.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int v, int id) {
+  return __builtin_amdgcn_update_dpp(id, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double v, double id) {
+  int lo = dpp_i<CTRL, ROWS>(__double2loint(v), __double2loint(id));
+  int hi = dpp_i<CTRL, ROWS>(__double2hiint(v), __double2hiint(id));
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane63_d(double v) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), kWave - 1);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), kWave - 1);
+  return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return __shfl(v, 0, kWave);
+  v += dpp_d<0xB1, 0xF>(v, 0.0);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E, 0xF>(v, 0.0);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141, 0xF>(v, 0.0);  // row_half_mirror
+  v += dpp_d<0x140, 0xF>(v, 0.0);  // row_mirror
+  v += dpp_d<0x142, 0xA>(v, 0.0);  // row_bcast:15 into rows 1, 3
+  v += dpp_d<0x143, 0xC>(v, 0.0);  // row_bcast:31 into rows 2, 3
+  return lane63_d(v);
 }
 __device__ __forceinline__ double wave_min(double v) {
-  for (int off = kWave / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
-  return __shfl(v, 0, kWave);
+  const double inf = __builtin_huge_val();
+  v = fmin(v, dpp_d<0xB1, 0xF>(v, inf));
+  v = fmin(v, dpp_d<0x4E, 0xF>(v, inf));
+  v = fmin(v, dpp_d<0x141, 0xF>(v, inf));
+  v = fmin(v, dpp_d<0x140, 0xF>(v, inf));
+  v = fmin(v, dpp_d<0x142, 0xA>(v, inf));
+  v = fmin(v, dpp_d<0x143, 0xC>(v, inf));
+  return lane63_d(v);
 }
 __device__ __forceinline__ int wave_or(int v) {
-  for (int off = kWave / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
-  return __shfl(v, 0, kWave);
+  v |= dpp_i<0xB1, 0xF>(v, 0);
+  v |= dpp_i<0x4E, 0xF>(v, 0);
+  v |= dpp_i<0x141, 0xF>(v, 0);
+  v |= dpp_i<0x140, 0xF>(v, 0);
+  v |= dpp_i<0x142, 0xA>(v, 0);
+  v |= dpp_i<0x143, 0xC>(v, 0);
+  return __builtin_amdgcn_readlane(v, kWave - 1);
 }
 
 constexpr double kPi = 3.14159265358979323846;
