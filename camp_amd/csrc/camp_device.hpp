@@ -338,7 +338,6 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
                                            double* out, double* s_fb,
                                            const double* s_slot = nullptr) {
   const int nv = P.nv, L = P.fb_rows;
-  const int2 pad2 = {-1, -1};
   const double* slot_or_rate = s_slot ? s_slot : s_rate;
   {
     const int4* fr = P.fb_rec + lane;
@@ -370,14 +369,14 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
     const int2* er = P.re_ell + hd.z + lane;
     double acc = 0.0;
     if (use_est) {
-      int2 b0 = 0 < hd.w ? er[0] : pad2, b1 = 1 < hd.w ? er[kWave] : pad2,
-           b2 = 2 < hd.w ? er[2 * kWave] : pad2, b3 = 3 < hd.w ? er[3 * kWave] : pad2;
+      // re_ell has padding rows up to a multiple of 4 plus 4 beyond hd.w
+      int2 b0 = er[0], b1 = er[kWave], b2 = er[2 * kWave], b3 = er[3 * kWave];
       for (int j = 0; j < hd.w; j += 4) {
         int2 c[4] = {b0, b1, b2, b3};
-        b0 = j + 4 < hd.w ? er[(j + 4) * kWave] : pad2;
-        b1 = j + 5 < hd.w ? er[(j + 5) * kWave] : pad2;
-        b2 = j + 6 < hd.w ? er[(j + 6) * kWave] : pad2;
-        b3 = j + 7 < hd.w ? er[(j + 7) * kWave] : pad2;
+        b0 = er[(j + 4) * kWave];
+        b1 = er[(j + 5) * kWave];
+        b2 = er[(j + 6) * kWave];
+        b3 = er[(j + 7) * kWave];
         double jv[4], yv[4], jyv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {

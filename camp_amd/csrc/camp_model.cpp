@@ -688,7 +688,10 @@ std::string Model::initialize() {
       P.fl_hdr.push_back(0);
       P.fl_hdr.push_back((int)(P.re_ell.size() / 2));
       P.fl_hdr.push_back(cr);
-      for (int j = 0; j < cr; ++j)
+      // rows padded (with -1 entries) to a multiple of 4 plus 4, so that the
+      // estimate's batches of 4 and its loads 4 rows ahead need no checks
+      int cr_alloc = (cr + 3) / 4 * 4 + 4;
+      for (int j = 0; j < cr_alloc; ++j)
         for (int l = 0; l < W; ++l) {
           int slot = pass * W + l, q = -1;
           if (slot < nv) {
@@ -702,7 +705,7 @@ std::string Model::initialize() {
     if (P.re_ell.empty()) P.re_ell.assign(2, -1);
   }
   {
-    // balanced bins of the f gather (first-fit decreasing over pieces)
+    // balanced bins of the f gather
     const int W = 64;
     int total = P.f_ptr[nv];
     if (P.n_terms >= 65536 || P.n_slots >= 65536) return "more than 65535 f terms or slots";

@@ -77,9 +77,9 @@ struct Program {
   // The per-species pass of f(y) (estimate and blend) in lane-slot order:
   // species are assigned to slots by row length, longest first (slot s =
   // pass * 64 + lane), fs_species[slot] is the species, and per pass fl_hdr
-  // = {0, 0, offset and width of re_ell ({element, column}, padding -1)};
-  // entry j of lane l is at offset + j * 64 + l, so a wavefront's loads are
-  // contiguous.
+  // = {0, 0, offset and width of re_ell ({element, column}, padding -1,
+  // rows padded to a multiple of 4 plus 4)}; entry j of lane l is at
+  // offset + j * 64 + l, so a wavefront's loads are contiguous.
   std::vector<int> fs_species, fl_hdr, re_ell;
   // The f gather in balanced bins: the species' record lists are cut into
   // pieces of at most fb_rows records (a species longer than that is split
