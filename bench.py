@@ -300,8 +300,9 @@ def main():
                     help="only copy this many MiB device-to-device three times (the "
                          "known-size kernel that calibrates rocprofv3's FETCH_SIZE and "
                          "WRITE_SIZE counters) and exit")
-    ap.add_argument("--serve-requests", type=int, default=200,
-                    help="requests in the serving leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=1000,
+                    help="requests in the serving leg at 70%% load (0: skip); the "
+                         "capacity pass before it submits half as many at once")
     ap.add_argument("--serve-batch-cells", type=int, default=1 << 17,
                     help="cap on the cells of one server launch (the server's own chunk, "
                          "twice the cells the GPU holds at once, applies below it)")

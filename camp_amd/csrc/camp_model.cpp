@@ -819,6 +819,30 @@ std::string Model::initialize() {
           f[0] = std::min(f[0], pc.part);
           f[1] += 1;
         }
+    // the packing must cover every species' records exactly once, in pieces
+    // of consecutive records, with consecutive partial ids in record order
+    std::vector<std::vector<const Piece*>> by(nv);
+    for (const auto& b : bins) {
+      int fill = 0;
+      for (const Piece& pc : b) {
+        by[pc.species].push_back(&pc);
+        fill += pc.count;
+      }
+      if (fill > L) return "f gather packing: a bin exceeds its rows";
+    }
+    for (int i = 0; i < nv; ++i) {
+      std::sort(by[i].begin(), by[i].end(),
+                [](const Piece* a, const Piece* b) { return a->first < b->first; });
+      int at = P.f_ptr[i];
+      for (size_t k = 0; k < by[i].size(); ++k) {
+        const Piece* pc = by[i][k];
+        bool ok = pc->first == at && pc->count > 0 &&
+                  (by[i].size() == 1 ? pc->part < 0 : pc->part == P.fb_fin[2 * i] + (int)k);
+        if (!ok) return "f gather packing: species records not covered exactly once";
+        at += pc->count;
+      }
+      if (at != P.f_ptr[i + 1]) return "f gather packing: species records not covered";
+    }
   }
 
   // --- LU: Markowitz ordering, symbolic fill, per-pivot programs ----------
