@@ -36,7 +36,8 @@ struct DevProgram {
   const int4* lf_hdr;       // per pivot: {lf_pos row start, up_rec row start,
                             //  L rows | update rows << 16, diag position}
   const int *lu_rowptr, *lu_col;
-  const uint64_t* row_mask;  // 4 per LU row; nullptr when nv > 128
+  const int4* ts_tbl;        // register solve: per block of 8 pivots and lane, 2 int4
+                             //  of pos(row t) | pos(row t + 64) << 16; nullptr when nv > 128
   const int4 *fe_a, *je_a;  // packed gather records (see Program in camp_model.hpp)
   const int *fe_b, *je_b;
   const double *fe_c, *je_c;
@@ -581,42 +582,23 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 }
 
 // One block of 8 pivots of a triangular sweep whose pivot values sit in
-// register X of their owner lanes (lane k - k_base, k_base = 64 when X1):
-// the block's 16 entry weights are LDS loads issued first -- an entry a row
-// does not have is read from a slot holding zero, so no select follows the
-// load -- then the chain, one readlane pair and one FMA per row and pivot.
-// DIR is +1 (forward, L) or -1 (backward, U, weights pre-scaled by the
-// row's inverse diagonal); mr0/mr1 are the rows' masks over the 64 columns
-// k_base .. k_base + 63, of which the block's 8 bits are taken once, with
-// the pivots outside [kend, kb] (or [kb, kend)) cleared: such a pivot adds
-// zero times some lane's value, so every block runs all 8 steps.  o0/o1
-// walk the rows' stored entries.
+// register X of their owner lanes (lane k - k_base, k_base = 64 when X1).
+// The lane's two program words per pivot give the LU positions of its rows'
+// entries in the pivot's column (the zero slot where a row has none or the
+// pivot is outside the sweep), so the block's 16 weights are LDS loads that
+// depend on nothing but the program; then the chain, one readlane pair and
+// one FMA per row and pivot.  DIR is +1 (forward, L) or -1 (backward, U,
+// weights pre-scaled by the row's inverse diagonal).
 template <int DIR, bool X1>
-__device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t mr0,
-                                          uint64_t mr1, const double* row0, const double* row1,
-                                          const double* zero, int& o0, int& o1, double s0,
-                                          double s1, double& x0, double& x1) {
+__device__ __forceinline__ void tri_block(int kb, int k_base, int4 t0, int4 t1, const double* lu,
+                                          double s0, double s1, double& x0, double& x1) {
   constexpr int B = 8;
-  // bit u of the block's masks is pivot kb + DIR * u
-  int lo = DIR > 0 ? kb - k_base : kb - k_base - (B - 1);
-  int n_in = DIR > 0 ? min(B, kend - kb) : min(B, kb - kend + 1);
-  uint32_t valid = (1u << n_in) - 1u;
-  uint32_t b0 = (uint32_t)(lo >= 0 ? (mr0 >> lo) : (mr0 << -lo)) & 0xFFu;
-  uint32_t b1 = (uint32_t)(lo >= 0 ? (mr1 >> lo) : (mr1 << -lo)) & 0xFFu;
-  if (DIR < 0) {  // reverse, so that bit u is pivot kb - u
-    b0 = __builtin_bitreverse32(b0) >> 24;
-    b1 = __builtin_bitreverse32(b1) >> 24;
-  }
-  b0 &= valid;
-  b1 &= valid;
+  const int tw[B] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
   double w0[B], w1[B];
 #pragma unroll
   for (int u = 0; u < B; ++u) {
-    int a0 = (b0 >> u) & 1u, a1 = (b1 >> u) & 1u;
-    w0[u] = *(a0 ? row0 + o0 : zero) * s0;
-    w1[u] = *(a1 ? row1 + o1 : zero) * s1;
-    o0 += DIR * a0;
-    o1 += DIR * a1;
+    w0[u] = lu[tw[u] & 0xFFFF] * s0;
+    w1[u] = lu[(unsigned)tw[u] >> 16] * s1;
   }
 #pragma unroll
   for (int u = 0; u < B; ++u) {
@@ -630,60 +612,55 @@ __device__ __forceinline__ void tri_block(int kb, int kend, int k_base, uint64_t
 // Triangular solves with the solution vector in registers (nv <= 128):
 // lane t owns permuted rows t and t+64.  Pivot k's value is broadcast with
 // v_readlane (no LDS round trip, no barrier), so the dependency chain per
-// pivot is one readlane pair and one FMA.  Each lane walks its rows' stored
-// entries in order: whether row r has an entry in column k is a bit of the
-// row's 128-bit L or U mask, and the entry's position is a running count,
-// so the LU values each lane needs are LDS loads that depend on neither the
-// solution nor a table lookup and are issued ahead of the chain.  The
-// backward sweep solves D^-1 U x = D^-1 y, so no division sits on the chain.
-// Sweeps are split at pivot 64 so each loop reads one register.
+// pivot is one readlane pair and one FMA.  The LU values each lane needs
+// come from the program's position table (the next block's words are
+// fetched while a block runs), so they are LDS loads issued ahead of the
+// chain; every block runs its 8 steps, pivots outside the sweep having
+// zero weights.  The sweeps are split at pivot 64 so each loop reads one
+// register, and the backward sweep solves D^-1 U x = D^-1 y, so no division
+// sits on the chain.
 __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane, double* lu,
                                                   double* b) {
   const int nv = P.nv;
   const int r0 = lane, r1 = lane + kWave;
   const bool has0 = r0 < nv, has1 = r1 < nv;
-  uint64_t l0a = 0, l0b = 0, u0a = 0, u0b = 0, l1a = 0, l1b = 0, u1a = 0, u1b = 0;
-  int base0 = 0, base1 = 0, len0 = 1, len1 = 1;
-  double x0 = 0.0, x1 = 0.0;
-  if (has0) {
-    const uint64_t* m = P.row_mask + 4 * r0;
-    l0a = m[0]; l0b = m[1]; u0a = m[2]; u0b = m[3];
-    base0 = P.lu_rowptr[r0];
-    len0 = P.lu_rowptr[r0 + 1] - base0;
-    x0 = b[P.perm[r0]];
-  }
-  if (has1) {
-    const uint64_t* m = P.row_mask + 4 * r1;
-    l1a = m[0]; l1b = m[1]; u1a = m[2]; u1b = m[3];
-    base1 = P.lu_rowptr[r1];
-    len1 = P.lu_rowptr[r1 + 1] - base1;
-    x1 = b[P.perm[r1]];
-  }
-  const double* row0 = lu + base0;
-  const double* row1 = lu + base1;
-  // a slot holding zero, after the LU values and the factorisation's sinks
-  double* zero = lu + P.lu_nnz + kWave;
-  if (lane == 0) *zero = 0.0;
+  double x0 = has0 ? b[P.perm[r0]] : 0.0;
+  double x1 = has1 ? b[P.perm[r1]] : 0.0;
+  // the zero slot, after the LU values and the factorisation's sinks
+  if (lane == 0) lu[P.lu_nnz + kWave] = 0.0;
   wsync();
   constexpr int B = 8;
-  int o0 = 0, o1 = 0;
+  const int4* tb = P.ts_tbl + 2 * lane;
+  int blk = 0;
+  int4 t0 = tb[0], t1 = tb[1];
   const int kf = nv - 1;  // forward pivots 0 .. nv-2
   const int split = kf < kWave ? kf : kWave;
-  for (int kb = 0; kb < split; kb += B)
-    tri_block<1, false>(kb, split, 0, l0a, l1a, row0, row1, zero, o0, o1, 1.0, 1.0, x0, x1);
-  for (int kb = split; kb < kf; kb += B)
-    tri_block<1, true>(kb, kf, kWave, l0b, l1b, row0, row1, zero, o0, o1, 1.0, 1.0, x0, x1);
-  // o0, o1 now index the diagonal of each row
-  double d0 = has0 ? 1.0 / row0[o0] : 0.0;
-  double d1 = has1 ? 1.0 / row1[o1] : 0.0;
+  // the table has one block of padding past the last, so the look-ahead
+  // load of the next block never leaves it
+  for (int kb = 0; kb < split; kb += B) {
+    int4 n0 = tb[2 * kWave * (blk + 1)], n1 = tb[2 * kWave * (blk + 1) + 1];
+    tri_block<1, false>(kb, 0, t0, t1, lu, 1.0, 1.0, x0, x1);
+    t0 = n0; t1 = n1; ++blk;
+  }
+  for (int kb = split; kb < kf; kb += B) {
+    int4 n0 = tb[2 * kWave * (blk + 1)], n1 = tb[2 * kWave * (blk + 1) + 1];
+    tri_block<1, true>(kb, kWave, t0, t1, lu, 1.0, 1.0, x0, x1);
+    t0 = n0; t1 = n1; ++blk;
+  }
+  double d0 = has0 ? 1.0 / lu[P.diag_pos[r0]] : 0.0;
+  double d1 = has1 ? 1.0 / lu[P.diag_pos[r1]] : 0.0;
   x0 *= d0;
   x1 *= d1;
-  o0 = len0 - 1;
-  o1 = len1 - 1;
-  for (int kb = nv - 1; kb >= kWave; kb -= B)
-    tri_block<-1, true>(kb, kWave, kWave, u0b, u1b, row0, row1, zero, o0, o1, d0, d1, x0, x1);
-  for (int kb = (nv - 1 < kWave - 1 ? nv - 1 : kWave - 1); kb >= 0; kb -= B)
-    tri_block<-1, false>(kb, 0, 0, u0a, u1a, row0, row1, zero, o0, o1, d0, d1, x0, x1);
+  for (int kb = nv - 1; kb >= kWave; kb -= B) {
+    int4 n0 = tb[2 * kWave * (blk + 1)], n1 = tb[2 * kWave * (blk + 1) + 1];
+    tri_block<-1, true>(kb, kWave, t0, t1, lu, d0, d1, x0, x1);
+    t0 = n0; t1 = n1; ++blk;
+  }
+  for (int kb = (nv - 1 < kWave - 1 ? nv - 1 : kWave - 1); kb >= 0; kb -= B) {
+    int4 n0 = tb[2 * kWave * (blk + 1)], n1 = tb[2 * kWave * (blk + 1) + 1];
+    tri_block<-1, false>(kb, 0, t0, t1, lu, d0, d1, x0, x1);
+    t0 = n0; t1 = n1; ++blk;
+  }
   wsync();
   if (has0) b[P.perm[r0]] = x0;
   if (has1) b[P.perm[r1]] = x1;
@@ -693,7 +670,7 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
 // Solve (LU) x = b in place; b in dep order, w a scratch vector (nv).
 __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, double* lu,
                                               double* b, double* w) {
-  if (P.row_mask) {
+  if (P.ts_tbl) {
     cell_lu_solve_reg(P, lane, lu, b);
     return;
   }

@@ -876,8 +876,6 @@ std::string Model::initialize() {
   for (int k = 0; k < nv; ++k) P.diag_pos[k] = pos[(size_t)k * nv + k];
   P.lu_rowptr.assign(nv + 1, 0);
   P.lu_col.assign(P.lu_nnz, 0);
-  P.row_mask.clear();
-  if (nv <= 128) P.row_mask.assign((size_t)4 * nv, 0);
   for (int i = 0; i < nv; ++i) {
     int first = -1;
     for (int j = 0; j < nv; ++j) {
@@ -885,8 +883,6 @@ std::string Model::initialize() {
       if (q < 0) continue;
       if (first < 0) first = q;
       P.lu_col[q] = j;
-      if (nv <= 128 && j != i)
-        P.row_mask[(size_t)4 * i + (j < i ? 0 : 2) + (j >> 6)] |= 1ull << (j & 63);
     }
     P.lu_rowptr[i] = first;
   }
@@ -926,6 +922,37 @@ std::string Model::initialize() {
     int* h = &P.lu_hdr[(size_t)8 * k];
     h[0] = P.l_ptr[k]; h[1] = P.l_ptr[k + 1]; h[2] = P.up_ptr[k]; h[3] = P.up_ptr[k + 1];
     h[4] = P.diag_pos[k]; h[5] = P.u_ptr[k]; h[6] = P.u_ptr[k + 1];
+  }
+  // The register triangular solve's program (nv <= 128): per block of 8
+  // pivots, in the order the device sweeps visit them, and per lane t, the
+  // LU positions of the entries (row t, pivot) and (row t + 64, pivot) as
+  // pos0 | pos1 << 16 -- the zero slot lu_nnz + 64 where the row has no
+  // entry or the pivot is outside the sweep -- two int4 per lane and block.
+  P.ts_tbl.clear();
+  P.ts_fwd_blocks = P.ts_bwd_blocks = 0;
+  if (nv <= 128) {
+    if (P.lu_nnz + 65 > 65535) return "LU fill too large for the register solve";
+    const int W = 64, zero = P.lu_nnz + 64;
+    auto block = [&](int kb, int dir, int lo, int hi) {  // pivots kb + dir*u in [lo, hi)
+      for (int l = 0; l < W; ++l)
+        for (int u = 0; u < 8; ++u) {
+          int k = kb + dir * u, w = 0;
+          for (int h = 0; h < 2; ++h) {
+            int r = l + h * W, p = zero;
+            bool in_sweep = k >= lo && k < hi && r < nv;
+            if (in_sweep && (dir > 0 ? r > k : r < k) && F[(size_t)r * nv + k])
+              p = pos[(size_t)r * nv + k];
+            w |= p << (16 * h);
+          }
+          P.ts_tbl.push_back(w);
+        }
+    };
+    const int kf = nv - 1, split = std::min(kf, W);
+    for (int kb = 0; kb < split; kb += 8, ++P.ts_fwd_blocks) block(kb, 1, 0, split);
+    for (int kb = split; kb < kf; kb += 8, ++P.ts_fwd_blocks) block(kb, 1, split, kf);
+    for (int kb = nv - 1; kb >= W; kb -= 8, ++P.ts_bwd_blocks) block(kb, -1, W, nv);
+    for (int kb = std::min(nv - 1, W - 1); kb >= 0; kb -= 8, ++P.ts_bwd_blocks) block(kb, -1, 0, W);
+    P.ts_tbl.insert(P.ts_tbl.end(), (size_t)8 * W, zero | (zero << 16));  // look-ahead padding
   }
   // The factorisation's program: per pivot, the L positions to scale and the
   // update records {dst, l, u}, each padded to whole rows of 64 (at least

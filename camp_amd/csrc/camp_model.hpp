@@ -117,13 +117,14 @@ struct Program {
   // each padded to rows of 64 with writes to the lanes' sink slots past the
   // LU values, and lf_hdr = 4 ints per pivot
   std::vector<int> lf_pos, lf_hdr;
+  // the register triangular solve's program (built in Model::initialize):
+  // per block of 8 pivots and lane, 8 words pos(row t) | pos(row t+64) << 16
+  std::vector<int> ts_tbl;
+  int ts_fwd_blocks = 0, ts_bwd_blocks = 0;
   std::vector<uint64_t> up_rec;
   // Row-major LU storage: row i occupies [lu_rowptr[i], lu_rowptr[i+1]) in
-  // column order.  For nv <= 128, row_mask[4i..4i+3] holds the columns of
-  // row i's strictly lower (L) part as a 128-bit mask (low, high word), then
-  // those of its strictly upper (U) part.
+  // column order.
   std::vector<int> lu_rowptr, lu_col;
-  std::vector<uint64_t> row_mask;
   long n_updates() const { return (long)up_dst.size(); }
 
   // ---- multi-phase tables (empty for gas-phase models) -------------------

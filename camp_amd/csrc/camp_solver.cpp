@@ -95,7 +95,8 @@ void Solver::setup_device() {
   P.je_c = dev_upload(G.je_c, owned);
   P.lu_rowptr = dev_upload(G.lu_rowptr, owned);
   P.lu_col = dev_upload(G.lu_col, owned);
-  P.row_mask = (G.nv <= 2 * kWave && !G.row_mask.empty()) ? dev_upload(G.row_mask, owned) : nullptr;
+  P.ts_tbl = G.ts_tbl.empty() ? nullptr
+                              : reinterpret_cast<const int4*>(dev_upload(G.ts_tbl, owned));
   std::vector<int> rx_type(m.n_rxn), rx_fidx(m.n_rxn), rx_eidx(m.n_rxn), rx_nreact(m.n_rxn);
   for (int r = 0; r < m.n_rxn; ++r) {
     int* id = m.rint(r);
