@@ -22,28 +22,20 @@ constexpr int kWave = 64;
 
 struct DevProgram {
   int ns, nv, nnz, n_env, n_terms, lu_nnz, n_rxn;
-  const int *dep_state, *t_kidx, *t_react, *t_nreact, *t_flags;
-  const int *f_ptr, *f_term, *f_guard;
-  const double* f_coef;
-  const int *j_ptr, *j_term, *j_skip, *j_guard, *j_rind;
-  const double* j_coef;
-  const int *colptr, *rowval, *r_ptr, *r_elem, *r_col;
+  const int* dep_state;
+  const int* j_ptr;
   const int *perm, *m_to_lu, *diag_pos, *e_diag;
-  const int *l_ptr, *l_pos, *l_row, *u_ptr, *u_pos, *u_row, *up_ptr;
+  const int *l_pos, *l_row, *u_pos, *u_row;
   const uint64_t* up_rec;  // LU update records: dst | l << 16 | u << 32, rows of 64
   const int4* lu_hdr;       // per pivot: {l range, update range}, {diag position, u range}
   const int* lf_pos;        // L positions the factorisation scales, rows of 64
   const int4* lf_hdr;       // per pivot: {lf_pos row start, up_rec row start,
                             //  L rows | update rows << 16, diag position}
-  const int *lu_rowptr, *lu_col;
   const int4* ts_tbl;        // register solve: per block of 8 pivots and lane, 2 int4
                              //  of pos(row t) | pos(row t + 64) << 16; nullptr when nv > 128
-  const int4 *fe_a, *je_a;  // packed gather records (see Program in camp_model.hpp)
-  const int *fe_b, *je_b;
-  const double *fe_c, *je_c;
-  const int2* fe_tb;  // {term or slot index, guard | flags << 24} per f record
-  const int4* fe_rec;  // {fe_tb, coefficient bits} per f record (one load)
-  const int2* re_rec;  // {element, column} per entry of the row view
+  const int4* je_a;  // packed J records (see Program in camp_model.hpp)
+  const int* je_b;
+  const double* je_c;
   const int* fs_species;             // lane slot -> species of the f gather
   const int4* t_rec;                 // per term: {k index | flags << 28, reactants x3}
   const int4* fl_hdr;                // per pass: -, -, re_ell offset, width

@@ -592,19 +592,13 @@ std::string Model::initialize() {
   for (size_t q = 0; q < P.f_term.size(); ++q) {
     int t = P.f_term[q];
     if (t < 0) {  // slot record
-      P.fe_a.push_back(-1 - t);
-      for (int j = 0; j < 3; ++j) P.fe_a.push_back(-1);
-      P.fe_b.push_back(0xFFFFFF | (kTermSlot << 24));
       P.fe_c.push_back(1.0);
       P.fe_tb.push_back(-1 - t);
-      P.fe_tb.push_back(P.fe_b.back());
+      P.fe_tb.push_back(0xFFFFFF | (kTermSlot << 24));
       continue;
     }
     P.fe_tb.push_back(t);
     P.fe_tb.push_back((P.f_guard[q] & 0xFFFFFF) | (P.t_flags[t] << 24));
-    P.fe_a.push_back(P.t_kidx[t]);
-    for (int j = 0; j < 3; ++j) P.fe_a.push_back(j < P.t_nreact[t] ? P.t_react[3 * t + j] : -1);
-    P.fe_b.push_back((P.f_guard[q] & 0xFFFFFF) | (P.t_flags[t] << 24));
     P.fe_c.push_back(P.f_coef[q]);
   }
   for (size_t q = 0; q < P.j_term.size(); ++q) {
@@ -620,7 +614,7 @@ std::string Model::initialize() {
     P.je_c.push_back(P.j_coef[q]);
   }
   if (P.fe_c.empty()) {
-    P.fe_a.assign(4, -1); P.fe_b.push_back(-1); P.fe_c.push_back(0.0);
+    P.fe_c.push_back(0.0);
     P.fe_tb.assign(2, -1);
   }
   if (P.je_c.empty()) {
@@ -661,11 +655,6 @@ std::string Model::initialize() {
         P.r_elem[fill[i]] = e;
         P.r_col[fill[i]++] = j;
       }
-  }
-  P.re_rec.resize(2 * (size_t)std::max(P.nnz, 1), 0);
-  for (int q = 0; q < P.nnz; ++q) {
-    P.re_rec[2 * q] = P.r_elem[q];
-    P.re_rec[2 * q + 1] = P.r_col[q];
   }
   {
     const int W = 64;
@@ -874,19 +863,6 @@ std::string Model::initialize() {
     }
   P.diag_pos.resize(nv);
   for (int k = 0; k < nv; ++k) P.diag_pos[k] = pos[(size_t)k * nv + k];
-  P.lu_rowptr.assign(nv + 1, 0);
-  P.lu_col.assign(P.lu_nnz, 0);
-  for (int i = 0; i < nv; ++i) {
-    int first = -1;
-    for (int j = 0; j < nv; ++j) {
-      int q = pos[(size_t)i * nv + j];
-      if (q < 0) continue;
-      if (first < 0) first = q;
-      P.lu_col[q] = j;
-    }
-    P.lu_rowptr[i] = first;
-  }
-  P.lu_rowptr[nv] = P.lu_nnz;
   P.l_ptr.assign(1, 0);
   P.u_ptr.assign(1, 0);
   P.up_ptr.assign(1, 0);

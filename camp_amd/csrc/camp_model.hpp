@@ -57,23 +57,19 @@ struct Program {
   std::vector<double> f_coef;
   std::vector<int> j_ptr, j_term, j_skip, j_guard, j_rind;  // per element
   std::vector<double> j_coef;
-  // The same gather lists as self-contained records, one vector load each:
-  //  f entry: fe_a = {k index, reactant state ids (-1 = none) x3},
-  //           fe_b = product state id for the negative-yield guard (-1 =
-  //           none) | flags << 24, fe_c = coefficient
-  //  J entry: je_a = {k index, the other reactants (-1 = none) x2, the
-  //           differentiated reactant}, je_b = guard product state id,
-  //           je_c = coefficient
-  std::vector<int> fe_a, fe_b, je_a, je_b;
-  std::vector<double> fe_c, je_c;
-  // f(y) is evaluated in two phases: the rate of every term once (into
-  // LDS), then per species a gather of fe_tb = {term index, or slot index
-  // for slot records; fe_b} and fe_c
+  // The J gather lists as self-contained records: je_a = {k index, the
+  // other reactants (-1 = none) x2, the differentiated reactant}, je_b =
+  // guard product state id, je_c = coefficient
+  std::vector<int> je_a, je_b;
+  std::vector<double> je_c;
+  // The f records per species (f_ptr order): fe_tb = {term index, or slot
+  // index for slot records; product state id of the negative-yield guard
+  // (-1 = none) | flags << 24} and fe_c = coefficient, combined as one
+  // 16-byte record each in fe_rec {term or slot, guard | flags << 24,
+  // coefficient (low, high word)}, from which the balanced gather is built
   std::vector<int> fe_tb;
-  // the same f records as one 16-byte load each: {term or slot, guard |
-  // flags << 24, coefficient (low, high word)}, and the row view of the
-  // solver pattern as {element, column} pairs
-  std::vector<int> fe_rec, re_rec;
+  std::vector<double> fe_c;
+  std::vector<int> fe_rec;
   // The per-species pass of f(y) (estimate and blend) in lane-slot order:
   // species are assigned to slots by row length, longest first (slot s =
   // pass * 64 + lane), fs_species[slot] is the species, and per pass fl_hdr
@@ -122,9 +118,6 @@ struct Program {
   std::vector<int> ts_tbl;
   int ts_fwd_blocks = 0, ts_bwd_blocks = 0;
   std::vector<uint64_t> up_rec;
-  // Row-major LU storage: row i occupies [lu_rowptr[i], lu_rowptr[i+1]) in
-  // column order.
-  std::vector<int> lu_rowptr, lu_col;
   long n_updates() const { return (long)up_dst.size(); }
 
   // ---- multi-phase tables (empty for gas-phase models) -------------------

@@ -42,46 +42,19 @@ void Solver::setup_device() {
   P.lu_nnz = G.lu_nnz;
   P.n_rxn = m.n_rxn;
   P.dep_state = dev_upload(G.dep_state, owned);
-  P.t_kidx = dev_upload(G.t_kidx, owned);
-  P.t_react = dev_upload(G.t_react, owned);
-  P.t_nreact = dev_upload(G.t_nreact, owned);
-  P.t_flags = dev_upload(G.t_flags, owned);
-  P.f_ptr = dev_upload(G.f_ptr, owned);
-  P.f_term = dev_upload(G.f_term, owned);
-  P.f_guard = dev_upload(G.f_guard, owned);
-  P.f_coef = dev_upload(G.f_coef, owned);
   P.j_ptr = dev_upload(G.j_ptr, owned);
-  P.j_term = dev_upload(G.j_term, owned);
-  P.j_skip = dev_upload(G.j_skip, owned);
-  P.j_guard = dev_upload(G.j_guard, owned);
-  P.j_rind = dev_upload(G.j_rind, owned);
-  P.j_coef = dev_upload(G.j_coef, owned);
-  P.colptr = dev_upload(G.colptr, owned);
-  P.rowval = dev_upload(G.rowval, owned);
-  P.r_ptr = dev_upload(G.r_ptr, owned);
-  P.r_elem = dev_upload(G.r_elem, owned);
-  P.r_col = dev_upload(G.r_col, owned);
   P.perm = dev_upload(G.perm, owned);
   P.m_to_lu = dev_upload(G.m_to_lu, owned);
   P.diag_pos = dev_upload(G.diag_pos, owned);
   P.e_diag = dev_upload(G.e_diag, owned);
-  P.l_ptr = dev_upload(G.l_ptr, owned);
   P.l_pos = dev_upload(G.l_pos, owned);
   P.l_row = dev_upload(G.l_row, owned);
-  P.u_ptr = dev_upload(G.u_ptr, owned);
   P.u_pos = dev_upload(G.u_pos, owned);
   P.u_row = dev_upload(G.u_row, owned);
-  P.up_ptr = dev_upload(G.up_ptr, owned);
   P.lu_hdr = reinterpret_cast<const int4*>(dev_upload(G.lu_hdr, owned));
   P.up_rec = dev_upload(G.up_rec, owned);
   P.lf_pos = dev_upload(G.lf_pos, owned);
   P.lf_hdr = reinterpret_cast<const int4*>(dev_upload(G.lf_hdr, owned));
-  P.fe_a = reinterpret_cast<const int4*>(dev_upload(G.fe_a, owned));
-  P.fe_b = dev_upload(G.fe_b, owned);
-  P.fe_c = dev_upload(G.fe_c, owned);
-  P.fe_tb = reinterpret_cast<const int2*>(dev_upload(G.fe_tb, owned));
-  P.fe_rec = reinterpret_cast<const int4*>(dev_upload(G.fe_rec, owned));
-  P.re_rec = reinterpret_cast<const int2*>(dev_upload(G.re_rec, owned));
   P.fs_species = dev_upload(G.fs_species, owned);
   P.t_rec = reinterpret_cast<const int4*>(dev_upload(G.t_rec, owned));
   P.fl_hdr = reinterpret_cast<const int4*>(dev_upload(G.fl_hdr, owned));
@@ -93,8 +66,6 @@ void Solver::setup_device() {
   P.je_a = reinterpret_cast<const int4*>(dev_upload(G.je_a, owned));
   P.je_b = dev_upload(G.je_b, owned);
   P.je_c = dev_upload(G.je_c, owned);
-  P.lu_rowptr = dev_upload(G.lu_rowptr, owned);
-  P.lu_col = dev_upload(G.lu_col, owned);
   P.ts_tbl = G.ts_tbl.empty() ? nullptr
                               : reinterpret_cast<const int4*>(dev_upload(G.ts_tbl, owned));
   std::vector<int> rx_type(m.n_rxn), rx_fidx(m.n_rxn), rx_eidx(m.n_rxn), rx_nreact(m.n_rxn);
