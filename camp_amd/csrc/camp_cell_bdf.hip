@@ -686,15 +686,20 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
                   scratch ? scratch + P.lu_nnz + kWave + 1 + P.rnnz : nullptr);
   cv.s_rate = cv.s_k + P.n_env + mp_lds_doubles(P);
   double* v = cv.s_rate + 2 * (size_t)P.n_terms;
-  cv.znb = v;
-  v += 6 * (size_t)nv;
+  // Multi-phase cells keep the Nordsieck history and the last/guess vectors
+  // in their global scratch (after the LU, J_rxn and Jacobian scalars), so
+  // that three cells fit a CU's LDS; gas-phase cells keep them in LDS.
+  double* g = MP ? scratch + P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) : nullptr;
+  double*& vh = MP ? g : v;
+  cv.znb = vh;
+  vh += 6 * (size_t)nv;
+  cv.last = vh; vh += nv;
+  cv.acinit = vh; vh += nv;
   cv.ewt = v; v += nv;
   cv.acor = v; v += nv;
   cv.y = v; v += nv;
   cv.tempv = v; v += nv;
   cv.ftemp = v; v += nv;
-  cv.last = v; v += nv;
-  cv.acinit = v; v += nv;
   cv.gh1 = v; v += nv;
   cv.gh2 = v; v += nv;
   cv.w = v; v += nv;
