@@ -513,15 +513,24 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
   int l_p = P.lf_pos[h0.x + lane];
   int nb0 = h0.z >> 16;
   uint64_t cur[R];
+  // a pivot's first R rows are loaded unconditionally (the program has R
+  // rows of sink records past its end) and those past its own replaced by
+  // sinks, so the loads need no branches
 #pragma unroll
-  for (int r = 0; r < R; ++r) cur[r] = r < nb0 ? P.up_rec[h0.y + r * kWave + lane] : sink;
+  for (int r = 0; r < R; ++r) {
+    uint64_t v = P.up_rec[h0.y + r * kWave + lane];
+    cur[r] = r < nb0 ? v : sink;
+  }
   for (int k = 0; k < P.nv; ++k) {
     int4 h2 = P.lf_hdr[k + 2];
     int n_lp = P.lf_pos[h1.x + lane];
     int nb1 = h1.z >> 16;
     uint64_t nxt[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) nxt[r] = r < nb1 ? P.up_rec[h1.y + r * kWave + lane] : sink;
+    for (int r = 0; r < R; ++r) {
+      uint64_t v = P.up_rec[h1.y + r * kWave + lane];
+      nxt[r] = r < nb1 ? v : sink;
+    }
     double piv = lu[h0.w];
     double lval = lu[l_p];
     double od[R], oa[R], ob[R];
@@ -548,7 +557,10 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
       for (int b = R; b < nb0; b += R) {
         uint64_t x[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[r] = b + r < nb0 ? P.up_rec[h0.y + (b + r) * kWave + lane] : sink;
+        for (int r = 0; r < R; ++r) {
+          uint64_t v = P.up_rec[h0.y + (b + r) * kWave + lane];
+          x[r] = b + r < nb0 ? v : sink;
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           od[r] = lu[(int)(x[r] & 0xFFFF)];
