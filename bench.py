@@ -310,8 +310,8 @@ def main():
                     help="largest multi-phase request (0: gas requests only)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (cells shard "
-              "across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (cells would shard "
+              "across GPUs with no exchange step). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
