@@ -214,7 +214,6 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   a.t1 = t1;
   a.counter = d_counter;
   a.stats = d_stats;
-  if (have_order) a.cell_list = d_order;
   if (profiling) {
     if (!d_prof) d_prof = dev_alloc<double>((size_t)m.nc * kCellProf, owned);
     a.prof = d_prof;
@@ -251,27 +250,6 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   }
   bstats = s;
   stats_from_cells = true;
-  // next call's queue order: counting sort of the cells by this call's BDF
-  // step count on a log scale (quarter octaves), largest first; the cells'
-  // results do not depend on the order
-  {
-    constexpr int kKeys = 128;
-    auto key = [&](int c) {
-      double n = stats[(size_t)c * kCellStats];
-      int k = n > 0.0 ? (int)(4.0 * std::log2(1.0 + n)) : 0;
-      return std::min(kKeys - 1, std::max(0, k));
-    };
-    std::vector<int> start(kKeys + 1, 0);
-    for (int c = 0; c < m.nc; ++c) start[kKeys - 1 - key(c) + 1]++;
-    for (int k = 0; k < kKeys; ++k) start[k + 1] += start[k];
-    h_order.resize(m.nc);
-    for (int c = 0; c < m.nc; ++c) h_order[start[kKeys - 1 - key(c)]++] = c;
-    if (!d_order) d_order = dev_alloc<int>((size_t)m.nc, owned);
-    HIP_OK(hipMemcpyAsync(d_order, h_order.data(), (size_t)m.nc * sizeof(int),
-                          hipMemcpyHostToDevice, stream));
-    HIP_OK(hipStreamSynchronize(stream));
-    have_order = true;
-  }
   if (profiling) {
     std::vector<double> pr((size_t)m.nc * kCellProf);
     HIP_OK(hipMemcpy(pr.data(), d_prof, pr.size() * sizeof(double), hipMemcpyDeviceToHost));

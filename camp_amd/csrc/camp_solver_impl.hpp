@@ -71,12 +71,6 @@ struct Solver {
   double *d_state = nullptr, *d_env = nullptr, *d_renv = nullptr;
   double *d_Js = nullptr, *d_Jy = nullptr, *d_Jf = nullptr, *d_stats = nullptr;
   int *d_counter = nullptr, *d_fail = nullptr, *d_var_type = nullptr;
-  // Queue order of the per-cell kernel: cells by descending BDF step count
-  // of the previous call (longest first, so that the launch's tail holds
-  // cheap cells); empty before the first call.
-  int* d_order = nullptr;
-  std::vector<int> h_order;
-  bool have_order = false;
   // block-path buffers
   double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_y = nullptr,
          *d_tempv = nullptr, *d_ftemp = nullptr, *d_last = nullptr, *d_acinit = nullptr,
