@@ -686,22 +686,23 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
                   scratch ? scratch + P.lu_nnz + kWave + 1 + P.rnnz : nullptr);
   cv.s_rate = cv.s_k + P.n_env + mp_lds_doubles(P);
   double* v = cv.s_rate + 2 * (size_t)P.n_terms;
-  // Multi-phase cells keep the Nordsieck history and the last/guess vectors
-  // in their global scratch (after the LU, J_rxn and Jacobian scalars), so
-  // that three cells fit a CU's LDS; gas-phase cells keep them in LDS.
+  // Multi-phase cells keep the Nordsieck history and the work vectors
+  // other than y, acor and w in their global scratch (after the LU, J_rxn
+  // and Jacobian scalars), so that four cells fit a CU's LDS; gas-phase
+  // cells keep them all in LDS.
   double* g = MP ? scratch + P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) : nullptr;
   double*& vh = MP ? g : v;
   cv.znb = vh;
   vh += 6 * (size_t)nv;
   cv.last = vh; vh += nv;
   cv.acinit = vh; vh += nv;
-  cv.ewt = v; v += nv;
+  cv.ewt = vh; vh += nv;
+  cv.tempv = vh; vh += nv;
+  cv.ftemp = vh; vh += nv;
+  cv.gh1 = vh; vh += nv;
+  cv.gh2 = vh; vh += nv;
   cv.acor = v; v += nv;
   cv.y = v; v += nv;
-  cv.tempv = v; v += nv;
-  cv.ftemp = v; v += nv;
-  cv.gh1 = v; v += nv;
-  cv.gh2 = v; v += nv;
   cv.w = v; v += nv;
   // Gas-phase cells keep their J history (J_solver, J_state, J_deriv) in
   // LDS for the whole solve; multi-phase cells use it in place in HBM.

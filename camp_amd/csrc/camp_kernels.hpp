@@ -17,20 +17,20 @@ constexpr int kCellProf = 7;
 // LDS doubles one wavefront needs for a cell in the persistent kernel:
 // state, rate data, the multi-phase extras, Nordsieck history (6 vectors),
 // 10 work vectors, the LU factors (for multi-phase programs the LU, the
-// history and 2 of the work vectors live in global scratch instead) and 24
+// history and 7 of the work vectors live in global scratch instead) and 24
 // slots for the BDF coefficient arrays l, tq and tau.
 __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P) {
   return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms +
-         (P.mp ? 8 : 16) * (size_t)P.nv + fb_sink_doubles(P) +
+         (P.mp ? 3 : 16) * (size_t)P.nv + fb_sink_doubles(P) +
          (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
 }
 // Global scratch doubles per wavefront of multi-phase programs: LU factors
 // (with the factorisation's 64 sink slots and the solve's zero slot), the
 // reaction Jacobian, the per-instance Jacobian scalars, and the Nordsieck
-// history and the last/guess vectors (8 vectors), so that three cells fit
+// history and 7 of the 10 work vectors (13 vectors), so that four cells fit
 // a CU's LDS.
 __host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P) {
-  return P.mp ? (size_t)P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) + 8 * (size_t)P.nv : 0;
+  return P.mp ? (size_t)P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) + 13 * (size_t)P.nv : 0;
 }
 
 struct CellBdfArgs {
