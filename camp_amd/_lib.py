@@ -36,9 +36,23 @@ SIGNATURES = {
     "aero_phase_add_condensed_data": (None, [_i, _i, _ip, _dp, _p]),
     "aero_rep_add_condensed_data": (None, [_i, _i, _i, _i, _ip, _dp, _p]),
     "sub_model_add_condensed_data": (None, [_i, _i, _i, _i, _ip, _dp, _p]),
+    "sub_model_update_data": (None, [_i, _ip, _i, _p, _p]),
+    "sub_model_print_data": (None, [_p]),
+    "aero_phase_print_data": (None, [_p]),
+    "aero_rep_update_data": (None, [_i, _ip, _i, _p, _p]),
+    "aero_rep_print_data": (None, [_p]),
+    "aero_rep_modal_binned_mass_create_gmd_update_data": (_p, []),
+    "aero_rep_modal_binned_mass_set_gmd_update_data": (None, [_p, _i, _i, _d]),
+    "aero_rep_modal_binned_mass_create_gsd_update_data": (_p, []),
+    "aero_rep_modal_binned_mass_set_gsd_update_data": (None, [_p, _i, _i, _d]),
+    "aero_rep_free_update_data": (None, [_p]),
+    "solver_set_debug_out": (_i, [_p, C.c_bool]),
+    "solver_set_eval_jac": (_i, [_p, C.c_bool]),
     "camp_solver_set_mode": (_i, [_p, _i]),
     "camp_solver_set_cells_per_block": (_i, [_p, _i]),
     "camp_solver_set_guess_helper": (None, [_p, _i]),
+    "camp_solver_set_cell_order": (_i, [_p, _ip, _i]),
+    "camp_solver_set_poison": (None, [_p, _i]),
     "camp_solver_run_device": (_i, [_p, _p, _p, _d, _d]),
     "camp_solver_set_base_rates": (_i, [_p, _i, _i, _dp]),
     "camp_solver_dims": (_i, [_p, _ip, _ip, _ip, _ip]),

@@ -129,6 +129,22 @@ class CampSolver:
     def set_guess_helper(self, on):
         lib().camp_solver_set_guess_helper(self.h, int(bool(on)))
 
+    def set_cell_order(self, order=None):
+        """Queue order of the per-cell kernel (a permutation of the cells;
+        None: index order).  Results do not depend on it."""
+        if order is None:
+            rc = lib().camp_solver_set_cell_order(self.h, None, 0)
+        else:
+            o = np.ascontiguousarray(order, dtype=np.int32)
+            rc = lib().camp_solver_set_cell_order(self.h, _ip(o), len(o))
+        if rc:
+            raise ValueError("cell order must be a permutation of the %d cells" % self.n_cells)
+
+    def set_poison(self, on=True):
+        """Fill each wavefront's working memory with NaNs before every cell
+        (a check that no cell reads what an earlier one left)."""
+        lib().camp_solver_set_poison(self.h, int(bool(on)))
+
     def update_rxn_data(self, cell, rxn_unique_id, rate):
         """One rxn_update_data call with a freshly packed payload."""
         L = lib()

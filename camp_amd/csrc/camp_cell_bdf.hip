@@ -739,6 +739,14 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
     double t1 = a.cell_t1 ? a.cell_t1[cell] : a.t1;
     int nsteps = a.cell_nsteps ? a.cell_nsteps[cell] : 1;
     for (int step = 0; step < nsteps; ++step) {
+      if (a.poison) {
+        const double nan = __builtin_nan("");
+        for (size_t i = lane; i < per_cell; i += kWave) base[i] = nan;
+        if (scratch)
+          for (size_t i = lane; i < cell_bdf_wave_scratch(P); i += kWave) scratch[i] = nan;
+        __threadfence_block();
+        wsync();
+      }
       double* state_c = a.state + (size_t)cell * P.ns;
       double* renv_c = a.rxn_env + (size_t)cell * P.n_env;
       double* g_js = a.Js + (size_t)cell * P.nnz;

@@ -57,6 +57,11 @@ struct CellBdfArgs {
   // [t0, t1] in this launch, each exactly as a separate launch would
   // (state written back and restaged between them); nullptr: one.
   const int* cell_nsteps = nullptr;
+  // Check of per-cell independence: when non-zero, every wavefront fills its
+  // whole working memory (the cell's LDS region and, for multi-phase
+  // programs, its global scratch) with quiet NaNs before it stages each
+  // cell, so that any value a cell reads before writing shows in its result.
+  int poison = 0;
 };
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);

@@ -214,6 +214,8 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   a.t1 = t1;
   a.counter = d_counter;
   a.stats = d_stats;
+  if (!h_order.empty()) a.cell_list = d_order;
+  a.poison = poison;
   if (profiling) {
     if (!d_prof) d_prof = dev_alloc<double>((size_t)m.nc * kCellProf, owned);
     a.prof = d_prof;
@@ -1023,6 +1025,127 @@ extern "C" void sub_model_add_condensed_data(int sub_model_type, int n_int_param
                              float_param);
 }
 
+// sub_model_solver.c:466-504: sub models of the requested type are scanned
+// from *sub_model_id on, advancing it; no supported sub-model type takes
+// update data (ZSR_AEROSOL_WATER has none), so the scan only advances the id.
+extern "C" void sub_model_update_data(int cell_id, int* sub_model_id, int update_sub_model_type,
+                                      void* update_data, void* solver_data) {
+  (void)cell_id;
+  (void)update_sub_model_type;
+  (void)update_data;
+  camp::Model& m = SD(solver_data)->m;
+  for (; *sub_model_id < m.n_added_sub; ++(*sub_model_id)) {
+  }
+}
+
+// sub_model_solver.c:509-541
+extern "C" void sub_model_print_data(void* solver_data) {
+  camp::Model& m = SD(solver_data)->m;
+  for (int i = 0; i < m.n_added_sub; ++i) {
+    const int* d = &m.sub_int[m.sub_int_idx[i]];
+    std::printf("\n\nSub model %d: type %d\n", i, d[0]);
+  }
+  std::fflush(stdout);
+}
+
+// aero_phase_solver.c:199-218
+extern "C" void aero_phase_print_data(void* solver_data) {
+  camp::Model& m = SD(solver_data)->m;
+  for (int i = 0; i < m.n_added_phase; ++i) std::printf("\n\nAerosol Phase %d\n\n", i);
+  std::fflush(stdout);
+}
+
+// aero_rep_solver.c:492-541 with aero_rep_modal_binned_mass.c:594-642: the
+// representations of the requested type are scanned from *aero_rep_id on
+// (advancing it); a MODAL_BINNED_MASS one whose unique id (int slot 3 after
+// the type) matches a non-zero payload id takes the update.  The payload is
+// {int aero_rep_id, int update_type (0 GMD, 1 GSD), int section_id, double
+// value} as the *_create/set_*_update_data helpers below pack it.  GMD and
+// GSD describe MODAL sections; this solver accepts BINNED sections only, so
+// a matching update is the reference's error for a non-modal section.
+extern "C" void aero_rep_update_data(int cell_id, int* aero_rep_id, int update_aero_rep_type,
+                                     void* update_data, void* solver_data) {
+  (void)cell_id;
+  camp::Model& m = SD(solver_data)->m;
+  int uid = 0, utype = 0, section = 0;
+  std::memcpy(&uid, update_data, sizeof(int));
+  std::memcpy(&utype, static_cast<char*>(update_data) + sizeof(int), sizeof(int));
+  std::memcpy(&section, static_cast<char*>(update_data) + 2 * sizeof(int), sizeof(int));
+  for (; *aero_rep_id < m.n_added_rep; ++(*aero_rep_id)) {
+    const int* d = &m.rep_int[m.rep_int_idx[*aero_rep_id]];
+    if (d[0] != update_aero_rep_type || d[0] != camp::kAeroRepModalBinnedMass) continue;
+    const int* id = d + 1;
+    if (uid != id[3] || id[3] == 0) continue;
+    if (utype == 0) {
+      std::printf("\n\nERROR Trying to set geometric mean diameter for non-modal"
+                  " aerosol section.");
+    } else {
+      std::printf("\n\nERROR Trying to set geometric standard deviation for non-modal"
+                  " aerosol section.");
+    }
+    std::fflush(stdout);
+    std::exit(1);
+  }
+}
+
+// aero_rep_solver.c:546-582
+extern "C" void aero_rep_print_data(void* solver_data) {
+  camp::Model& m = SD(solver_data)->m;
+  std::printf("\n\nAerosol representation data\n\nnumber of aerosol representations: %d\n\n",
+              m.n_added_rep);
+  for (int i = 0; i < m.n_added_rep; ++i) {
+    const int* d = &m.rep_int[m.rep_int_idx[i]];
+    if (d[0] == camp::kAeroRepModalBinnedMass)
+      std::printf("\n\nModal/binned mass-only aerosol representation\n");
+  }
+  std::fflush(stdout);
+}
+
+// aero_rep_solver.c:589
+extern "C" void aero_rep_free_update_data(void* update_data) { std::free(update_data); }
+
+// aero_rep_modal_binned_mass.c:677-738: payload {int aero_rep_id, int
+// update_type, int section_id, double value} (3 ints then a double, packed)
+static void* new_section_payload() {
+  void* p = std::malloc(3 * sizeof(int) + sizeof(double));
+  if (!p) camp::die("allocating space for aerosol representation update data");
+  return p;
+}
+static void set_section_payload(void* p, int rep_id, int type, int section, double v) {
+  std::memcpy(p, &rep_id, sizeof(int));
+  std::memcpy(static_cast<char*>(p) + sizeof(int), &type, sizeof(int));
+  std::memcpy(static_cast<char*>(p) + 2 * sizeof(int), &section, sizeof(int));
+  std::memcpy(static_cast<char*>(p) + 3 * sizeof(int), &v, sizeof(double));
+}
+extern "C" void* aero_rep_modal_binned_mass_create_gmd_update_data(void) {
+  return new_section_payload();
+}
+extern "C" void aero_rep_modal_binned_mass_set_gmd_update_data(void* update_data, int aero_rep_id,
+                                                               int section_id, double gmd) {
+  set_section_payload(update_data, aero_rep_id, 0, section_id, gmd);
+}
+extern "C" void* aero_rep_modal_binned_mass_create_gsd_update_data(void) {
+  return new_section_payload();
+}
+extern "C" void aero_rep_modal_binned_mass_set_gsd_update_data(void* update_data, int aero_rep_id,
+                                                               int section_id, double gsd) {
+  set_section_payload(update_data, aero_rep_id, 1, section_id, gsd);
+}
+
+// camp_solver.c:502-533 (CAMP_DEBUG builds): the debug-output flag is
+// accepted and ignored (this solver prints no per-step trace); the
+// Jacobian-check flag turns on the finite-difference check of every
+// Jacobian the solver evaluates (Jac_eval_fails counts cells that fail).
+extern "C" int solver_set_debug_out(void* solver_data, bool do_output) {
+  (void)solver_data;
+  (void)do_output;
+  return CAMP_SOLVER_SUCCESS;
+}
+extern "C" int solver_set_eval_jac(void* solver_data, bool eval_jac) {
+  SD(solver_data)->eval_jac = eval_jac;
+  return CAMP_SOLVER_SUCCESS;
+}
+
 // ---------------------------------------------------------------------------
 // Extensions
 // ---------------------------------------------------------------------------
@@ -1134,6 +1257,28 @@ extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
 }
 
 extern "C" void camp_solver_set_guess_helper(void* solver_data, int on) { SD(solver_data)->guess_on = on != 0; }
+
+extern "C" int camp_solver_set_cell_order(void* solver_data, const int* order, int n) {
+  using namespace camp;
+  Solver* s = SD(solver_data);
+  if (!order || n == 0) {
+    s->h_order.clear();
+    return 0;
+  }
+  if (n != s->m.nc) return 1;
+  std::vector<char> seen((size_t)n, 0);
+  for (int i = 0; i < n; ++i) {
+    if (order[i] < 0 || order[i] >= n || seen[order[i]]) return 1;
+    seen[order[i]] = 1;
+  }
+  s->ensure_device();
+  if (!s->d_order) s->d_order = dev_alloc<int>((size_t)n, s->owned);
+  s->h_order.assign(order, order + n);
+  HIP_OK(hipMemcpy(s->d_order, s->h_order.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" void camp_solver_set_poison(void* solver_data, int on) { SD(solver_data)->poison = on != 0; }
 
 extern "C" int camp_solver_run_device(void* solver_data, double* d_state, double* d_env,
                                       double t_initial, double t_final) {

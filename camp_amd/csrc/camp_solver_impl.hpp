@@ -71,6 +71,14 @@ struct Solver {
   double *d_state = nullptr, *d_env = nullptr, *d_renv = nullptr;
   double *d_Js = nullptr, *d_Jy = nullptr, *d_Jf = nullptr, *d_stats = nullptr;
   int *d_counter = nullptr, *d_fail = nullptr, *d_var_type = nullptr;
+  // Queue order of the per-cell kernel (a permutation of the cells; empty:
+  // index order).  Results do not depend on it; it only decides which cells
+  // the launch starts first.
+  std::vector<int> h_order;
+  int* d_order = nullptr;
+  int poison = 0;  // CellBdfArgs::poison for per-cell runs
+  // solver_set_eval_jac: finite-difference check of every evaluated Jacobian
+  bool eval_jac = false;
   // block-path buffers
   double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_y = nullptr,
          *d_tempv = nullptr, *d_ftemp = nullptr, *d_last = nullptr, *d_acinit = nullptr,

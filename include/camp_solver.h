@@ -15,6 +15,10 @@
 #ifndef CAMP_MI355X_SOLVER_H
 #define CAMP_MI355X_SOLVER_H
 
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -108,6 +112,55 @@ void sub_model_add_condensed_data(int sub_model_type, int n_int_param,
                                   int *int_param, double *float_param,
                                   void *solver_data);
 
+/* replaces sub_model_update_data, src/sub_model_solver.c:466-504 (bound at
+ * camp_solver_data.F90:267-280): scans the sub models from *sub_model_id,
+ * advancing it; no supported sub-model type takes update data */
+void sub_model_update_data(int cell_id, int *sub_model_id,
+                           int update_sub_model_type, void *update_data,
+                           void *solver_data);
+
+/* replaces sub_model_print_data, src/sub_model_solver.c:509-541
+ * (camp_solver_data.F90:283) */
+void sub_model_print_data(void *solver_data);
+
+/* replaces aero_phase_print_data, src/aero_phase_solver.c:199-218
+ * (camp_solver_data.F90:306) */
+void aero_phase_print_data(void *solver_data);
+
+/* replaces aero_rep_update_data, src/aero_rep_solver.c:492-541 with
+ * aero_rep_modal_binned_mass.c:594-642 (camp_solver_data.F90:334-347):
+ * GMD/GSD updates address MODAL sections; a matching update of this
+ * solver's BINNED-only representations is the reference's error exit */
+void aero_rep_update_data(int cell_id, int *aero_rep_id,
+                          int update_aero_rep_type, void *update_data,
+                          void *solver_data);
+
+/* replaces aero_rep_print_data, src/aero_rep_solver.c:546-582
+ * (camp_solver_data.F90:350) */
+void aero_rep_print_data(void *solver_data);
+
+/* replace the bind(c) helpers of aero_rep_modal_binned_mass.F90:258-304,
+ * whose C bodies are aero_rep_modal_binned_mass.c:677-738 and
+ * aero_rep_solver.c:589: payload {int aero_rep_id, int update_type, int
+ * section_id, double value} */
+void *aero_rep_modal_binned_mass_create_gmd_update_data(void);
+void aero_rep_modal_binned_mass_set_gmd_update_data(void *update_data,
+                                                    int aero_rep_id,
+                                                    int section_id, double gmd);
+void *aero_rep_modal_binned_mass_create_gsd_update_data(void);
+void aero_rep_modal_binned_mass_set_gsd_update_data(void *update_data,
+                                                    int aero_rep_id,
+                                                    int section_id, double gsd);
+void aero_rep_free_update_data(void *update_data);
+
+/* replace solver_set_debug_out and solver_set_eval_jac,
+ * src/camp_solver.c:502-533 (CAMP_DEBUG builds; camp_solver_data.F90:
+ * 100-118): debug output is accepted and ignored; eval_jac turns on the
+ * finite-difference check of every Jacobian evaluated by the solver, whose
+ * failures solver_get_statistics reports as Jac_eval_fails */
+int solver_set_debug_out(void *solver_data, bool do_output);
+int solver_set_eval_jac(void *solver_data, bool eval_jac);
+
 /* ---- extensions (no counterpart in the reference) ------------------------ */
 
 #define CAMP_MODE_PER_CELL 0 /* each cell its own BDF control (default) */
@@ -136,6 +189,20 @@ int camp_solver_autotune(void *solver_data, const double *d_state,
 /* Enable (1, default) or disable (0) the negative-prediction guess helper
  * (camp_solver.c:1321-1408). */
 void camp_solver_set_guess_helper(void *solver_data, int on);
+
+/* Queue order of the per-cell kernel: order[n] is a permutation of the
+ * cells, which the launch's wavefronts then take first to last (n = 0 or
+ * order = NULL restores index order).  Every cell's result is independent
+ * of the order.  Returns 0, or 1 if order is not a permutation of n_cells
+ * cells. */
+int camp_solver_set_cell_order(void *solver_data, const int *order, int n);
+
+/* Independence check of the per-cell kernel (on = 1): before it stages each
+ * cell, a wavefront fills all of its working memory -- the cell's LDS region
+ * and, for multi-phase mechanisms, its global scratch -- with NaNs, so that a
+ * value read before it is written in the same cell would show in the result.
+ * Results with it on equal those with it off. */
+void camp_solver_set_poison(void *solver_data, int on);
 
 /* solver_run on device-resident arrays (HIP device pointers, same layout
  * as solver_run's host arrays); ordered on the solver's stream and
