@@ -2539,6 +2539,17 @@ int oracle_eval_f(void *om, int cell, const double *state_cell, double *deriv,
   return rc;
 }
 
+/* The Jacobian-extrapolated derivative alone (the `est` argument that
+ * camp_solver.c:877-879 hands to time_derivative_output), from the cell's
+ * current J history. */
+void oracle_eval_est(void *om, int cell, const double *state_cell, double *est) {
+  Oracle *o = om;
+  double *y = xmalloc((size_t)(o->nv ? o->nv : 1) * sizeof(double));
+  for (int d = 0; d < o->nv; ++d) y[d] = state_cell[o->dep_state[d]];
+  cell_est(o, cell, y, est);
+  free(y);
+}
+
 int oracle_eval_jac(void *om, int cell, const double *state_cell, double *jac,
                     double time_step) {
   Oracle *o = om;

@@ -74,6 +74,9 @@ int oracle_eval_f(void *om, int cell, const double *y_dep, double *deriv,
 int oracle_eval_jac(void *om, int cell, const double *y_dep, double *jac,
                     double time_step);
 void oracle_reset_jac_history(void *om);
+/* J_deriv + J_solver (y - J_state) of one cell from its J history (the
+ * estimate the blend of time_derivative.c:58-64 uses). */
+void oracle_eval_est(void *om, int cell, const double *state_cell, double *est);
 
 /* solver_run: mode 0 = reference block semantics (one BDF over all
  * cells), mode 1 = per-cell (each cell its own BDF, n_cells = 1

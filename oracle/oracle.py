@@ -71,6 +71,8 @@ def lib():
         L.oracle_eval_f.restype = _i
         L.oracle_eval_jac.argtypes = [_p, _i, _dp, _dp, _d]
         L.oracle_eval_jac.restype = _i
+        L.oracle_eval_est.argtypes = [_p, _i, _dp, _dp]
+        L.oracle_eval_est.restype = None
         L.oracle_reset_jac_history.argtypes = [_p]
         L.oracle_solver_run.argtypes = [_p, _dp, _dp, _d, _d, _i]
         L.oracle_solver_run.restype = _i
@@ -231,6 +233,13 @@ class Oracle:
         out = np.zeros(self.nnz)
         rc = lib().oracle_eval_jac(self.h, cell, _dp_(s), _dp_(out), time_step)
         return rc, out
+
+    def eval_est(self, cell, state_cell):
+        """J_deriv + J_solver (y - J_state) from the cell's J history."""
+        s = np.ascontiguousarray(state_cell, dtype=np.float64)
+        out = np.zeros(self.nv)
+        lib().oracle_eval_est(self.h, cell, _dp_(s), _dp_(out))
+        return out
 
     def reset_jac_history(self):
         lib().oracle_reset_jac_history(self.h)

@@ -24,3 +24,56 @@ def rel_err(a, b, atol):
 
 def close_mask(a, b, rtol, atol):
     return np.abs(a - b) <= rtol * np.abs(b) + atol
+
+
+def affine_mechanism(seed=5, n_species=11):
+    """A mechanism whose f(y) is affine in y: unimolecular Arrhenius,
+    photolysis with one reactant, first-order loss and emission only.
+    Besides a random network it has an isolated pair A <-> B (species n-4,
+    n-3) with equal rate constants, and an isolated Y -> Z (n-2 -> n-1,
+    k = 2^-12 exactly: A = 2^-12, C = 0) with an emission of Z (id 5) and a
+    first-order loss of Z (id 6, 2^-10).  With [Y] = 2^-8, [Z] = 3 * 2^-11
+    and emission 2^-21, Z's production (2^-21 + 2^-20) and loss (3 * 2^-21)
+    cancel exactly, and Z's Jacobian row couples to Y one way (dZ'/dY =
+    2^-12, dY'/dZ = 0).  Returns (mechanism, rates for one cell)."""
+    from camp_amd.mechanisms import Mechanism, Reaction
+    rng = np.random.default_rng(seed)
+    n = n_species
+    g = n - 4  # general network 0 .. g-1
+    rx = []
+    for i in range(g):
+        k = [j for j in range(g) if j != i]
+        prods = [(int(j), float(rng.uniform(0.1, 1.0))) for j in rng.choice(k, 2, replace=False)]
+        rx.append(Reaction("ARRHENIUS", [i], prods,
+                           {"A": float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-1)))),
+                            "C": float(-rng.uniform(0, 500))}))
+    rx.append(Reaction("ARRHENIUS", [g], [(g + 1, 1.0)], {"A": 0.03}))
+    rx.append(Reaction("ARRHENIUS", [g + 1], [(g, 1.0)], {"A": 0.03}))
+    rx.append(Reaction("ARRHENIUS", [n - 2], [(n - 1, 1.0)], {"A": 2.0 ** -12}))
+    rx.append(Reaction("PHOTOLYSIS", [0], [(3, 0.7), (4, 0.4)], {}, rxn_id=1))
+    rx.append(Reaction("PHOTOLYSIS", [2], [(5, 1.0)], {}, rxn_id=2))
+    rx.append(Reaction("FIRST_ORDER_LOSS", [1], [], {}, rxn_id=3))
+    rx.append(Reaction("EMISSION", [], [(6, 1.0)], {}, rxn_id=4))
+    rx.append(Reaction("EMISSION", [], [(n - 1, 1.0)], {}, rxn_id=5))
+    rx.append(Reaction("FIRST_ORDER_LOSS", [n - 1], [], {}, rxn_id=6))
+    mech = Mechanism(["S%d" % i for i in range(n)], [1] * n, [1e-8] * n, rx, 1e-4, "affine")
+    rates = {1: np.array([2e-3]), 2: np.array([5e-4]), 3: np.array([1e-3]),
+             4: np.array([1e-5]), 5: np.array([3e-6]), 6: np.array([2.0 ** -10])}
+    return mech, rates
+
+
+def affine_cancel_state(y):
+    """y with the A <-> B pair and Z of affine_mechanism at exact balance."""
+    y = y.copy()
+    y[-3] = y[-4]
+    y[-2] = 2.0 ** -8
+    y[-1] = 3 * 2.0 ** -11
+    return y
+
+
+def dense_from_csc(colptr, rowval, vals, n):
+    J = np.zeros((n, n))
+    for j in range(n):
+        for k in range(colptr[j], colptr[j + 1]):
+            J[rowval[k], j] = vals[k]
+    return J

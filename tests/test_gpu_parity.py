@@ -246,3 +246,47 @@ def test_autotune_picks_a_fitting_candidate_and_leaves_cells_untouched():
         s2 = torch.from_numpy(cells.state).to(dev)
         assert ref.solve_device(s2.data_ptr(), env.data_ptr(), 0.0, 180.0) == 0
         assert torch.equal(st, s2)
+
+
+def test_deriv_estimate_and_blend_match_oracle_on_affine_mechanism(oracle_lib):
+    """The HIP f(y) with the J-extrapolated blend where it matters: on the
+    affine mechanism with the A <-> B pair and Z at exact balance, the blend
+    returns the estimate J_deriv + J_solver (y - J_state) for those species
+    (Z's through the one-way Y -> Z coupling, with Z's emission changed
+    since the stored Jacobian); every species is held to the oracle at
+    1e-12 and Z to the closed form."""
+    from tests.helpers import affine_cancel_state, affine_mechanism
+    mech, rates = affine_mechanism()
+    n = 8
+    rates = {k: np.repeat(v, n) for k, v in rates.items()}
+    sol = CampSolver(mech, n)
+    orc = oracle_lib.Oracle(mech, n)
+    _set_rates_both(sol, orc, rates)
+    rng = np.random.default_rng(12)
+    env = np.column_stack([rng.uniform(230, 300, n), rng.uniform(3e4, 1e5, n)])
+    y0 = np.exp(rng.uniform(np.log(1e-3), np.log(10.0), size=(n, mech.n_state)))
+    assert sol.eval_jac(y0, env, 1.0)[0] == 0
+    orc.update_env(env)
+    for c in range(n):
+        assert orc.eval_jac(c, y0[c], 1.0)[0] == 0
+    new = {5: np.full(n, 2.0 ** -21)}
+    _set_rates_both(sol, orc, new)
+    y1 = np.stack([affine_cancel_state(y0[c] * np.exp(rng.uniform(-0.5, 0.5, mech.n_state)))
+                   for c in range(n)])
+    rc, f_gpu = sol.eval_deriv(y1, env, 1.0, True)
+    assert rc == 0
+    orc.update_env(env)
+    f_ref = np.stack([orc.eval_f(c, y1[c], 1.0, True)[1] for c in range(n)])
+    # the pair's estimate is k (y_B - y_A) = 0 up to the rounding of the
+    # expansion (terms of size k * y): held to that, not to a relative bar
+    pair = [-4, -3]
+    noise = 1e-14 * 0.03 * np.max(np.abs(np.concatenate([y0, y1], axis=1)), axis=1)
+    assert np.all(np.abs(f_gpu[:, pair]) <= noise[:, None])
+    assert np.all(np.abs(f_ref[:, pair]) <= noise[:, None])
+    rest = np.ones(mech.n_state, bool)
+    rest[pair] = False
+    assert np.all(np.abs(f_gpu[:, rest] - f_ref[:, rest]) <= F_RTOL * np.abs(f_ref[:, rest]))
+    want = 3e-6 + 2.0 ** -12 * 2.0 ** -8 - 2.0 ** -10 * (3 * 2.0 ** -11)
+    assert np.all(np.abs(f_gpu[:, -1] - want) <= 1e-12 * want)
+    est = np.stack([orc.eval_est(c, y1[c]) for c in range(n)])
+    assert np.all(np.abs(f_gpu[:, -1] - est[:, -1]) <= 1e-12 * np.abs(est[:, -1]))

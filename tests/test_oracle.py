@@ -248,3 +248,68 @@ def test_block_and_per_cell_modes_agree_within_tolerance(O):
         assert o.run(b, cells.env, 0.0, 60.0, mode=1) == 0
         tol_r, tol_a = 1e-8 * (i + 1) * 100, 1e-12 * (i + 1) * 100
         assert np.all(np.abs(a - b) <= tol_r * np.abs(b) + tol_a)
+
+
+def test_deriv_estimate_is_exact_for_an_affine_mechanism(O):
+    """camp_solver.c:877-879 / time_derivative.c:58-64: the estimate is
+    J_deriv + J_solver (y - J_state), the first-order expansion around the
+    state of the last Jac().  For a mechanism of first- and zero-order
+    reactions f is affine, so away from J_state the estimate must equal
+    f(y) to rounding.  A transposed or sign-flipped matvec is far off (the
+    test checks that it would be)."""
+    from tests.helpers import affine_mechanism, dense_from_csc
+    mech, rates = affine_mechanism()
+    o = O.Oracle(mech, 1)
+    o.set_rates(rates)
+    env = np.array([[285.0, 9.0e4]])
+    o.update_env(env)
+    rng = np.random.default_rng(8)
+    y0 = np.exp(rng.uniform(np.log(1e-3), np.log(10.0), size=mech.n_state))
+    rc, jv = o.eval_jac(0, y0)
+    assert rc == 0
+    pat = o.pattern()
+    J = dense_from_csc(pat["colptr"], pat["rowval"], jv, o.nv)
+    f0 = o.eval_f(0, y0)[1]
+    assert np.linalg.norm(J - J.T) > 1e-2 * np.linalg.norm(J)
+    for trial in range(3):
+        y1 = y0 * np.exp(rng.uniform(-1.0, 1.0, size=mech.n_state))
+        est = o.eval_est(0, y1)
+        f1 = o.eval_f(0, y1)[1]
+        scale = np.abs(f0) + np.abs(J) @ np.abs(y1 - y0) + np.abs(f1)
+        assert np.all(np.abs(est - f1) <= 1e-13 * scale), np.max(np.abs(est - f1) / scale)
+        d = y1 - y0
+        for wrong in (f0 + J.T @ d, f0 - J @ d, J @ d):
+            assert np.max(np.abs(wrong - f1) / scale) > 1e-3
+
+
+def test_deriv_blend_returns_the_estimate_where_production_and_loss_cancel(O):
+    """time_derivative.c:58-64: a species whose production and loss cancel
+    exactly gets weight 0 on its (zero) sum and 1 on the estimate, while the
+    others keep weight ~1 on their own sums.  The emission of Z changes
+    between the Jac() that stores the history and the f() call, so Z's
+    estimate (old emission + k_Y [Y] - k_Z [Z], through the one-way Y -> Z
+    coupling) is not zero; the transposed coupling would give another
+    value."""
+    from tests.helpers import affine_cancel_state, affine_mechanism
+    mech, rates = affine_mechanism()
+    o = O.Oracle(mech, 1)
+    o.set_rates(rates)
+    env = np.array([[285.0, 9.0e4]])
+    o.update_env(env)
+    rng = np.random.default_rng(9)
+    y0 = np.exp(rng.uniform(np.log(1e-3), np.log(10.0), size=mech.n_state))
+    o.eval_jac(0, y0)
+    o.set_rates({5: np.array([2.0 ** -21])})
+    o.update_env(env)
+    y1 = affine_cancel_state(y0 * np.exp(rng.uniform(-0.5, 0.5, size=mech.n_state)))
+    pure = o.eval_f(0, y1)[1]
+    blend = o.eval_f(0, y1, use_deriv_est=True)[1]
+    est = o.eval_est(0, y1)
+    assert np.all(pure[[-4, -3, -1]] == 0.0)
+    assert np.array_equal(blend[[-4, -3, -1]], est[[-4, -3, -1]])
+    want = 3e-6 + 2.0 ** -12 * 2.0 ** -8 - 2.0 ** -10 * (3 * 2.0 ** -11)
+    assert abs(est[-1] - want) <= 1e-12 * 3e-6
+    assert abs(est[-1] - (want - 2.0 ** -12 * (y1[-2] - y0[-2]))) > 1e-3 * abs(want)
+    others = np.ones(mech.n_state, bool)
+    others[[-4, -3, -1]] = False
+    assert np.allclose(blend[others], pure[others], rtol=1e-12, atol=0)
