@@ -110,7 +110,7 @@ class MultiPhaseMechanism:
     rep: BinnedRep
     simpol: list
     aqeq: list
-    zsr: ZsrModel
+    zsr: ZsrModel                 # None: no aerosol-water sub model
     species: list = field(default_factory=list)
     var_type: list = field(default_factory=list)
     abs_tol: list = field(default_factory=list)
@@ -323,7 +323,7 @@ def pack_multiphase(mp):
     rxn += [pack_simpol(mp, r) for r in mp.simpol]
     phases = [pack_aero_phase(ph) for ph in mp.rep.phases]
     reps = [pack_binned_rep(mp, list(range(1, len(mp.rep.phases) + 1)))]
-    subs = [pack_zsr(mp)]
+    subs = [pack_zsr(mp)] if mp.zsr is not None else []
     return {"rxn": rxn, "phase": phases, "rep": reps, "sub": subs}
 
 

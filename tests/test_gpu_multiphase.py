@@ -121,3 +121,53 @@ def test_multiphase_alone_equals_mix(case):
     s1 = cells.state[5:6].copy()
     assert one.solve(s1, cells.env[5:6], 0.0, 180.0) == 0
     assert np.array_equal(s1[0], a[5])
+
+
+# --------------------------------------------------------------------------
+# The HIP path against the closed forms of the reference's SIMPOL and
+# aqueous-equilibrium unit tests (tests/mp_closed_forms.py)
+# --------------------------------------------------------------------------
+from tests import mp_closed_forms as CF  # noqa: E402
+
+
+@pytest.mark.parametrize("kind", ["simpol", "aqeq"])
+def test_multiphase_f_matches_reference_closed_forms_on_gpu(kind):
+    mk, stf, ff = ((CF.simpol_mechanism, CF.simpol_state, CF.simpol_f) if kind == "simpol"
+                   else (CF.aqeq_mechanism, CF.aqeq_state, CF.aqeq_f))
+    mp = mk()
+    n = 6
+    rng = np.random.default_rng(3)
+    env = np.array([[CF.T_TEST, CF.P_TEST]] + [[rng.uniform(250, 310), rng.uniform(5e4, 1.05e5)]
+                                               for _ in range(n - 1)])
+    st = np.stack([stf(mp)] + [stf(mp) * np.exp(rng.uniform(-1, 1, mp.n_state))
+                               for _ in range(n - 1)])
+    sol = CampSolver(mp, n)
+    rc, f = sol.eval_deriv(st, env)
+    assert rc == 0
+    dep = [i for i in range(mp.n_state) if mp.var_type[i] == 1]
+    for c in range(n):
+        T, P = env[c]
+        want = (ff(mp, st[c], T, P) if kind == "simpol" else ff(mp, st[c], T))[dep]
+        scale = np.max(np.abs(want))
+        assert np.all(np.abs(f[c] - want) <= 1e-12 * np.maximum(np.abs(want), 1e-3 * scale))
+
+
+@pytest.mark.parametrize("kind", ["simpol", "aqeq"])
+def test_multiphase_box_solve_reaches_closed_form_equilibrium_on_gpu(kind):
+    """solver_run on the GPU (per-cell kernel) reaches the Raoult's-law
+    (SIMPOL) and closed-form aqueous equilibria within 1e-6."""
+    if kind == "simpol":
+        mp, st0, eq, steps = CF.simpol_mechanism(), None, CF.simpol_equilibrium, 10
+        st0 = CF.simpol_state(mp)
+    else:
+        mp, eq, steps = CF.aqeq_mechanism(), CF.aqeq_equilibrium, 20
+        st0 = CF.aqeq_state(mp)
+    sol = CampSolver(mp, 2, rel_tol=1e-10, abs_tol=np.full(mp.n_state, 1e-22),
+                     max_steps=100000)
+    st = np.stack([st0, st0])
+    env = np.array([[CF.T_TEST, CF.P_TEST]] * 2)
+    for _ in range(steps):
+        assert sol.solve(st, env, 0.0, 3600.0) == 0
+    want = eq(mp, st0)
+    for c in range(2):
+        assert np.all(np.abs(st[c] - want) <= 1e-6 * want)

@@ -159,3 +159,77 @@ def test_multiphase_box_solve_conserves_and_converges(oracle_lib, mp, cells):
         out.append(st)
     ok = np.abs(out[0] - out[1]) <= 1e-4 * np.abs(out[1]) + 1e-8
     assert ok.all(), np.max(np.abs(out[0] - out[1]) / (np.abs(out[1]) + 1e-8))
+
+
+# --------------------------------------------------------------------------
+# Closed forms of the reference's SIMPOL and aqueous-equilibrium unit tests
+# (test_rxn_SIMPOL_phase_transfer.F90:330-410, test_rxn_aqueous_
+# equilibrium.F90:300-372) on BINNED mechanisms: see tests/mp_closed_forms.py
+# --------------------------------------------------------------------------
+from tests import mp_closed_forms as CF  # noqa: E402
+
+
+def _dep(mp):
+    return [i for i in range(mp.n_state) if mp.var_type[i] == 1]
+
+
+@pytest.mark.parametrize("kind", ["simpol", "aqeq"])
+def test_multiphase_f_matches_reference_closed_forms(oracle_lib, kind):
+    """f at the initial state and at perturbed states and environments
+    against the closed-form condensation/evaporation (SIMPOL) and forward/
+    reverse molality rates (aqueous equilibrium), to 1e-12 of the largest
+    term of each derivative."""
+    mk, stf, ff = ((CF.simpol_mechanism, CF.simpol_state, CF.simpol_f) if kind == "simpol"
+                   else (CF.aqeq_mechanism, CF.aqeq_state, CF.aqeq_f))
+    mp = mk()
+    rng = np.random.default_rng(3)
+    st0 = stf(mp)
+    for trial in range(6):
+        T = CF.T_TEST if trial == 0 else rng.uniform(250.0, 310.0)
+        P = CF.P_TEST if trial == 0 else rng.uniform(5e4, 1.05e5)
+        st = st0 if trial == 0 else st0 * np.exp(rng.uniform(-1, 1, st0.size))
+        o = oracle_lib.Oracle(mp, 1)
+        o.update_env(np.array([[T, P]]))
+        rc, f = o.eval_f(0, st)
+        assert rc == 0
+        want = (ff(mp, st, T, P) if kind == "simpol" else ff(mp, st, T))[_dep(mp)]
+        scale = np.max(np.abs(want))
+        assert np.all(np.abs(f - want) <= 1e-12 * np.maximum(np.abs(want), 1e-3 * scale)), \
+            np.max(np.abs(f - want) / np.maximum(np.abs(want), 1e-300))
+
+
+def test_simpol_box_solve_reaches_raoult_equilibrium(oracle_lib):
+    """Ten hours of ethanol partitioning into two bins of aqueous aerosol end
+    at Raoult's-law equilibrium in both bins with the total mass conserved
+    (the equilibrium the reference test's fixture value 2.15877e-7 kg m-3
+    iterates to for its MODAL/UNIFAC setup), within 1e-6."""
+    mp = CF.simpol_mechanism()
+    st = CF.simpol_state(mp)[None, :].copy()
+    o = oracle_lib.Oracle(mp, 1, rel_tol=1e-10, abs_tol=np.full(mp.n_state, 1e-22), max_steps=100000)
+    env = np.array([[CF.T_TEST, CF.P_TEST]])
+    for _ in range(10):
+        assert o.run(st, env, 0.0, 3600.0, mode=1) == 0
+    want = CF.simpol_equilibrium(mp, CF.simpol_state(mp))
+    assert np.all(np.abs(st[0] - want) <= 1e-6 * want)
+    # the closed form really moved: gas depleted, both bins grew
+    assert want[0] < 0.9 * CF.simpol_state(mp)[0]
+
+
+def test_aqueous_equilibria_reach_closed_forms(oracle_lib):
+    """2A <-> B + C, D + E <-> 2F and G <-> H in two bins end at their
+    closed-form equilibria (test_rxn_aqueous_equilibrium.F90:327-348)
+    within 1e-6; the linear G <-> H follows its exact exponential transient
+    (:426-429) at t = 1000 s within 1e-8."""
+    mp = CF.aqeq_mechanism()
+    env = np.array([[CF.T_TEST, CF.P_TEST]])
+    st = CF.aqeq_state(mp)[None, :].copy()
+    o = oracle_lib.Oracle(mp, 1, rel_tol=1e-10, abs_tol=np.full(mp.n_state, 1e-22), max_steps=100000)
+    for _ in range(20):
+        assert o.run(st, env, 0.0, 3600.0, mode=1) == 0
+    want = CF.aqeq_equilibrium(mp, CF.aqeq_state(mp))
+    assert np.all(np.abs(st[0] - want) <= 1e-6 * want)
+    st2 = CF.aqeq_state(mp)[None, :].copy()
+    o2 = oracle_lib.Oracle(mp, 1, rel_tol=1e-10, abs_tol=np.full(mp.n_state, 1e-22), max_steps=100000)
+    assert o2.run(st2, env, 0.0, 1000.0, mode=1) == 0
+    for i, v in CF.relax_GH(mp, CF.aqeq_state(mp), 1000.0).items():
+        assert abs(st2[0][i] - v) <= 1e-8 * v
