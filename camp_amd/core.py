@@ -191,7 +191,13 @@ class CampSolver:
                 "NLS_iters", "NLS_convergence_fails", "DLS_Jac_evals", "DLS_RHS_evals")
         d = {k: v.value for k, v in zip(keys, ints)}
         d["last_time_step__s"], d["next_time_step__s"] = dbl[0].value, dbl[1].value
+        for k, v in zip(("Jac_eval_fails", "RHS_evals_total", "Jac_evals_total"), more):
+            d[k] = v.value
+        for k, v in zip(("RHS_time__s", "Jac_time__s", "max_loss_precision"), tail):
+            d[k] = v.value
         return d
+
+    full_stats = stats
 
     def cell_stats(self):
         out = np.zeros((self.n_cells, 10))

@@ -162,8 +162,9 @@ __host__ __device__ __forceinline__ double mp_ppm_to_rh(double T, double Pa) {
 
 // ---------------------------------------------------------------------
 // Rate constants of one cell: rxn_update_env_state (rxn_solver.c:224-313)
-// with rxn_arrhenius.c:103-118, rxn_troe.c:106-130, rxn_photolysis.c:
-// 137-147, rxn_emission.c:109-119, rxn_first_order_loss.c:119-129.
+// with rxn_arrhenius.c:103-118, rxn_troe.c:106-130, rxn_CMAQ_H2O2.c:
+// 106-124, rxn_CMAQ_OH_HNO3.c:110-133, rxn_photolysis.c:137-147,
+// rxn_emission.c:109-119, rxn_first_order_loss.c:119-129.
 // env_cell points at the cell's rxn_env block (k slots written in place).
 // ---------------------------------------------------------------------
 __host__ __device__ __forceinline__ void rate_constant(const DevProgram& P, int r, double T, double Pa,
@@ -184,6 +185,23 @@ __host__ __device__ __forceinline__ void rate_constant(const DevProgram& P, int 
     double lg = cr_log10(kinf) / f[7];
     e[0] = (k0 / (1.0 + kinf)) * cr_pow(f[6], (1.0 / (1.0 + cr_pow(lg, 2.0)))) *
            cr_pow(conv, (double)(nr - 1)) * f[8];
+  } else if (t == 3) {  // CMAQ_H2O2: (k1 + k2 [M]) conv^(nr-1), [M] in k2_A
+    double conv = f[6] * Pa / T;
+    e[0] = (f[0] * (f[2] == 0.0 ? 1.0 : cr_exp(f[2] / T)) *
+                (f[1] == 0.0 ? 1.0 : cr_pow(T / 300.0, f[1])) +
+            f[3] * (f[5] == 0.0 ? 1.0 : cr_exp(f[5] / T)) *
+                (f[4] == 0.0 ? 1.0 : cr_pow(T / 300.0, f[4])) * conv) *
+           cr_pow(conv, (double)(nr - 1));
+  } else if (t == 4) {  // CMAQ_OH_HNO3: (k0 + k3 / (1 + k3 / k2)) conv^(nr-1) SCALING
+    double conv = f[10] * Pa / T;
+    double k2 = f[3] * (f[5] == 0.0 ? 1.0 : cr_exp(f[5] / T)) *
+                (f[4] == 0.0 ? 1.0 : cr_pow(T / 300.0, f[4]));
+    double k3 = f[6] * (f[8] == 0.0 ? 1.0 : cr_exp(f[8] / T)) *
+                (f[7] == 0.0 ? 1.0 : cr_pow(T / 300.0, f[7])) * conv;
+    e[0] = (f[0] * (f[2] == 0.0 ? 1.0 : cr_exp(f[2] / T)) *
+                (f[1] == 0.0 ? 1.0 : cr_pow(T / 300.0, f[1])) +
+            k3 / (1.0 + k3 / k2)) *
+           cr_pow(conv, (double)(nr - 1)) * f[9];
   } else if (t == 10) {  // SIMPOL_PHASE_TRANSFER
     mp_simpol_env(f, e, T, Pa);
   } else if (t == 7) {  // AQUEOUS_EQUILIBRIUM

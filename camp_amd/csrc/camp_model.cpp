@@ -41,17 +41,22 @@ struct PatternBuilder {
 };
 
 bool mass_action(int t) {
-  return t == kRxnArrhenius || t == kRxnTroe || t == kRxnPhotolysis;
+  return t == kRxnArrhenius || t == kRxnTroe || t == kRxnPhotolysis || t == kRxnCmaqH2O2 ||
+         t == kRxnCmaqOhHno3;
 }
 
 // Packed layout of the mass-action types (rxn_arrhenius.c:20-36,
-// rxn_troe.c:20-39, rxn_photolysis.c:20-33); d points after the type slot.
+// rxn_troe.c:20-39, rxn_CMAQ_H2O2.c:19-35, rxn_CMAQ_OH_HNO3.c:19-41,
+// rxn_photolysis.c:20-33); d points after the type slot.
 struct MassActionView {
   int* d;
   double* f;
   int t;
   int npi() const { return t == kRxnPhotolysis ? 3 : 2; }
-  int npf() const { return t == kRxnArrhenius ? 6 : t == kRxnTroe ? 10 : 1; }
+  int npf() const {
+    return t == kRxnArrhenius ? 6 : t == kRxnTroe ? 10 : t == kRxnCmaqH2O2 ? 7
+                                  : t == kRxnCmaqOhHno3 ? 11 : 1;
+  }
   int nr() const { return d[0]; }
   int np() const { return d[1]; }
   int react(int x) const { return d[npi() + x] - 1; }

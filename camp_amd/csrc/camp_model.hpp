@@ -27,6 +27,8 @@ constexpr double kMaxPrecisionLoss = 1.0e-14;
 // src/rxn_solver.c:19-35
 constexpr int kRxnArrhenius = 1;
 constexpr int kRxnTroe = 2;
+constexpr int kRxnCmaqH2O2 = 3;
+constexpr int kRxnCmaqOhHno3 = 4;
 constexpr int kRxnPhotolysis = 5;
 constexpr int kRxnFirstOrderLoss = 12;
 constexpr int kRxnEmission = 13;

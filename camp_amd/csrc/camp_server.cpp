@@ -7,9 +7,12 @@
 // free, the scheduler takes the next cells of the oldest request's
 // mechanism -- from that request and the ones after it, a request being
 // split across launches when it is larger than what is left -- up to a
-// chunk of twice the cells the GPU holds at once, and launches the
-// persistent per-cell kernel over them (each cell runs all of its
-// request's timesteps).  A request's inputs are copied in on the stream of
+// chunk of twice the cells the GPU holds at once, counted in
+// cell-timesteps, and launches the persistent per-cell kernel over them
+// (each cell runs all of its request's timesteps).  One mechanism never
+// holds every stream while another has cells waiting.  Launches allocate
+// nothing: cell lists and multi-phase scratch are per stream and sized when
+// the mechanism is registered.  A request's inputs are copied in on the stream of
 // its first chunk (later chunks on other streams wait for that copy's
 // event).  When the last chunk of a request completes its results are
 // copied out on that stream and the request completes with the copy's
@@ -69,6 +72,13 @@ struct Arena {
   std::vector<void*> owned;
   std::map<long, long> free_ranges;  // start -> length
   long resident = 0;
+  // per stream: the launch's cell list (device + pinned host, chunk
+  // entries) and, for multi-phase programs, the wavefronts' global scratch
+  // for the largest launch -- allocated when the mechanism is registered,
+  // so that launching allocates and frees nothing
+  std::vector<int*> d_list, h_list;
+  std::vector<double*> scratch;
+  long chunk = 0;
   // replay pool (pinned host memory)
   long pool_n = 0;
   double *pool_state = nullptr, *pool_env = nullptr, *pool_rates = nullptr, *pool_out = nullptr;
@@ -107,23 +117,25 @@ struct Arena {
 
 struct Lane {
   hipStream_t st = nullptr;
-  std::vector<double*> scratch;       // per mechanism: multi-phase wave scratch
-  std::vector<size_t> scratch_waves;
   hipEvent_t ev = nullptr;
   int* d_counter = nullptr;
-  int* d_list = nullptr;
-  int* h_list = nullptr;  // pinned
-  long list_cap = 0;
   bool busy = false;
+  int mech = -1;  // mechanism of the launch in flight
   long cells = 0;
   std::vector<Piece> batch;
 };
 
+// Mechanisms a server can hold: the arena table never reallocates, so the
+// scheduler thread reads it without the lock while a caller registers
+// another mechanism.
+constexpr int kMaxMechs = 16;
+
 struct Server {
   int dev = 0;
   long max_batch = 0;
-  std::vector<std::unique_ptr<Arena>> mechs;
-  std::vector<long> chunk;  // per mechanism: cells per launch
+  std::unique_ptr<Arena> mechs_[kMaxMechs];
+  int n_mechs = 0;  // guarded by mu
+  Arena* mech(int m) const { return mechs_[m].get(); }
   std::vector<Lane> lanes;
   std::thread thr;
   std::mutex mu;
@@ -177,10 +189,10 @@ bool Server::admit() {
   bool any = false;
   // strict arrival order per mechanism: a request that does not fit yet
   // holds back later requests of its mechanism (no starvation of large ones)
-  std::vector<char> blocked(mechs.size(), 0);
+  std::vector<char> blocked(kMaxMechs, 0);
   for (auto it = pending.begin(); it != pending.end();) {
     Request* r = *it;
-    Arena& A = *mechs[r->mech];
+    Arena& A = *mech(r->mech);
     if (blocked[r->mech]) {
       ++it;
       continue;
@@ -204,42 +216,60 @@ bool Server::admit() {
 
 bool Server::launch() {
   bool any = false;
-  for (Lane& L : lanes) {
+  for (size_t li = 0; li < lanes.size(); ++li) {
+    Lane& L = lanes[li];
     if (L.busy || ready.empty()) continue;
-    int mech = ready.front()->mech;
-    Arena& A = *mechs[mech];
-    long cap = std::min(max_batch, chunk[mech]);
+    // The oldest ready request's mechanism goes next -- unless every other
+    // stream is busy with that mechanism and another mechanism has ready
+    // cells: then the oldest request of another mechanism goes, so that a
+    // long run of one mechanism's launches never holds every stream.
+    int m = ready.front()->mech;
+    int same = 0, busy = 0;
+    for (const Lane& o : lanes)
+      if (o.busy) {
+        ++busy;
+        same += o.mech == m;
+      }
+    if (lanes.size() > 1 && same == busy && busy == (int)lanes.size() - 1) {
+      for (Request* r : ready)
+        if (r->mech != m) {
+          m = r->mech;
+          break;
+        }
+    }
+    Arena& A = *mech(m);
+    // Cost-aware chunk: each cell runs all of its request's timesteps in the
+    // launch, so a chunk is capped in cell-timesteps (twice the cells the GPU
+    // holds at once), not in cells: launches of multi-step requests are no
+    // longer than others, and the streams free up at an even pace.
+    long cap = std::min(max_batch, A.chunk);
     L.batch.clear();
-    long total = 0;
-    for (auto it = ready.begin(); it != ready.end() && total < cap;) {
+    long total = 0, work = 0;
+    for (auto it = ready.begin(); it != ready.end() && work < cap;) {
       Request* r = *it;
-      if (r->mech != mech) {
+      if (r->mech != m) {
         ++it;
         continue;
       }
-      long take = std::min(r->n - r->launched, cap - total);
+      long take = std::min(r->n - r->launched, std::max(1L, (cap - work) / r->n_steps));
       L.batch.push_back({r, r->launched, take});
       r->launched += take;
       total += take;
+      work += take * r->n_steps;
       if (r->launched == r->n) it = ready.erase(it);
       else ++it;
     }
-    if (total > L.list_cap) {
-      if (L.d_list) (void)hipFree(L.d_list);
-      if (L.h_list) (void)hipHostFree(L.h_list);
-      L.list_cap = std::max(total, 2 * L.list_cap);
-      HIP_OK(hipMalloc(&L.d_list, L.list_cap * sizeof(int)));
-      HIP_OK(hipHostMalloc((void**)&L.h_list, L.list_cap * sizeof(int), hipHostMallocDefault));
-    }
+    int* h_list = A.h_list[li];
+    int* d_list = A.d_list[li];
     long q = 0;
     for (const Piece& pc : L.batch) {
       Request* r = pc.r;
-      for (long c = 0; c < pc.count; ++c) L.h_list[q++] = (int)(r->slot0 + pc.first + c);
+      for (long c = 0; c < pc.count; ++c) h_list[q++] = (int)(r->slot0 + pc.first + c);
       if (!r->up_ev) upload(A, r, L.st);
       else HIP_OK(hipStreamWaitEvent(L.st, r->up_ev, 0));
       r->state = kRunning;
     }
-    HIP_OK(hipMemcpyAsync(L.d_list, L.h_list, total * sizeof(int), hipMemcpyHostToDevice, L.st));
+    HIP_OK(hipMemcpyAsync(d_list, h_list, total * sizeof(int), hipMemcpyHostToDevice, L.st));
     HIP_OK(hipMemsetAsync(L.d_counter, 0, sizeof(int), L.st));
     CellBdfArgs a = A.S->cell_args();
     a.nc = (int)total;
@@ -253,32 +283,20 @@ bool Server::launch() {
     a.t1 = 0.0;
     a.counter = L.d_counter;
     a.stats = A.stats;
-    a.cell_list = L.d_list;
+    a.cell_list = d_list;
     a.cell_t1 = A.t1;
     a.cell_nsteps = A.nsteps;
     a.fail_count = A.fail;
+    a.wave_scratch = A.scratch[li];
     int grid = 0, block = 0;
     size_t lds = 0;
     A.S->cell_launch_shape(total, &grid, &block, &lds);
-    if (A.S->P.mp) {
-      size_t waves = (size_t)grid * (block / kWave);
-      if (L.scratch.size() < mechs.size()) {
-        L.scratch.resize(mechs.size(), nullptr);
-        L.scratch_waves.resize(mechs.size(), 0);
-      }
-      if (L.scratch_waves[mech] < waves) {
-        if (L.scratch[mech]) HIP_OK(hipFree(L.scratch[mech]));
-        HIP_OK(hipMalloc(&L.scratch[mech], waves * cell_bdf_wave_scratch(A.S->P) * sizeof(double)));
-        L.scratch_waves[mech] = waves;
-      }
-      a.wave_scratch = L.scratch[mech];
-    }
     launch_cell_bdf(a, grid, block, lds, L.st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(L.ev, L.st));
     L.busy = true;
-    L.cells = 0;
-    for (const Piece& pc : L.batch) L.cells += pc.count * pc.r->n_steps;
+    L.mech = m;
+    L.cells = work;
     any = true;
     std::lock_guard<std::mutex> g(mu);
     batches += 1;
@@ -302,7 +320,7 @@ bool Server::poll_lanes() {
       Request* r = pc.r;
       r->finished += pc.count;
       if (r->finished < r->n) continue;
-      Arena& A = *mechs[r->mech];
+      Arena& A = *mech(r->mech);
       size_t d = sizeof(double);
       HIP_OK(hipMemcpyAsync(r->out_state, A.state + r->slot0 * A.ns, r->n * A.ns * d,
                             hipMemcpyDeviceToHost, L.st));
@@ -341,7 +359,7 @@ bool Server::poll_copies() {
     copying.pop_back();
     any = true;
     std::lock_guard<std::mutex> g(mu);
-    Arena& A = *mechs[r->mech];
+    Arena& A = *mech(r->mech);
     for (long c = 0; c < r->n; ++c) r->failed |= A.h_fail[r->slot0 + c] != 0;
     A.release(r->slot0, r->n);
     r->t_done = now;
@@ -460,22 +478,33 @@ extern "C" int camp_server_add_mechanism(void* server, void* solver_data, long m
   A->upd_off = dev_upload(off, A->owned);
   HIP_OK(hipHostMalloc((void**)&A->h_fail, (size_t)n * sizeof(int), hipHostMallocDefault));
   A->free_ranges[0] = n;
-  // a launch takes at most twice the cells the GPU holds at once
+  // a launch takes at most twice the cells the GPU holds at once (counted
+  // in cell-timesteps); its cell list and multi-phase scratch are
+  // allocated here, per stream, at that size
   int grid = 0, block = 0;
   size_t lds = 0;
   S->cell_launch_shape(1L << 30, &grid, &block, &lds);
-  long per_launch = 2L * grid * (block / kWave);
+  A->chunk = 2L * grid * (block / kWave);
+  size_t waves = (size_t)grid * (block / kWave);
+  for (size_t li = 0; li < s->lanes.size(); ++li) {
+    A->d_list.push_back(dev_alloc<int>((size_t)A->chunk, A->owned));
+    int* h = nullptr;
+    HIP_OK(hipHostMalloc((void**)&h, (size_t)A->chunk * sizeof(int), hipHostMallocDefault));
+    A->h_list.push_back(h);
+    A->scratch.push_back(S->P.mp ? dev_alloc<double>(waves * cell_bdf_wave_scratch(S->P), A->owned)
+                                 : nullptr);
+  }
   std::lock_guard<std::mutex> g(s->mu);
-  s->mechs.push_back(std::move(A));
-  s->chunk.push_back(per_launch);
-  return (int)s->mechs.size() - 1;
+  if (s->n_mechs == kMaxMechs) die("camp_server_add_mechanism: too many mechanisms (16)");
+  s->mechs_[s->n_mechs] = std::move(A);
+  return s->n_mechs++;
 }
 
 extern "C" long camp_server_capacity(void* server, int mech) {
   Server* s = SV(server);
   std::lock_guard<std::mutex> g(s->mu);
-  if (mech < 0 || mech >= (int)s->mechs.size()) return -1;
-  return s->mechs[mech]->cap;
+  if (mech < 0 || mech >= s->n_mechs) return -1;
+  return s->mech(mech)->cap;
 }
 
 extern "C" long camp_server_submit(void* server, int mech, int n_cells, const double* state,
@@ -483,8 +512,8 @@ extern "C" long camp_server_submit(void* server, int mech, int n_cells, const do
                                    int n_steps, double* out_state, double* out_stats) {
   Server* s = SV(server);
   std::lock_guard<std::mutex> g(s->mu);
-  if (mech < 0 || mech >= (int)s->mechs.size()) return -1;
-  Arena& A = *s->mechs[mech];
+  if (mech < 0 || mech >= s->n_mechs) return -1;
+  Arena& A = *s->mech(mech);
   if (n_cells < 1 || n_cells > A.cap || n_steps < 1 || !(dt > 0.0)) return -1;
   if (A.n_upd > 0 && !base_rates) return -1;
   std::unique_ptr<Request> r(new Request());
@@ -527,8 +556,8 @@ extern "C" int camp_server_set_pool(void* server, int mech, long n_cells, const 
   Arena* A;
   {
     std::lock_guard<std::mutex> g(s->mu);
-    if (mech < 0 || mech >= (int)s->mechs.size() || n_cells < 1) return 1;
-    A = s->mechs[mech].get();
+    if (mech < 0 || mech >= s->n_mechs || n_cells < 1) return 1;
+    A = s->mech(mech);
   }
   if (A->pool_n) return 1;
   size_t d = sizeof(double);
@@ -555,8 +584,8 @@ extern "C" int camp_server_replay(void* server, int n_requests, const double* t_
     Arena* A;
     {
       std::lock_guard<std::mutex> g(s->mu);
-      if (mech[i] < 0 || mech[i] >= (int)s->mechs.size()) return -1;
-      A = s->mechs[mech[i]].get();
+      if (mech[i] < 0 || mech[i] >= s->n_mechs) return -1;
+      A = s->mech(mech[i]);
     }
     if (!A->pool_n || offset[i] < 0 || offset[i] + n_cells[i] > A->pool_n) return -1;
     std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
@@ -606,13 +635,11 @@ extern "C" void camp_server_free(void* server) {
     (void)hipStreamDestroy(L.st);
     (void)hipEventDestroy(L.ev);
     (void)hipFree(L.d_counter);
-    if (L.d_list) (void)hipFree(L.d_list);
-    if (L.h_list) (void)hipHostFree(L.h_list);
-    for (double* p : L.scratch)
-      if (p) (void)hipFree(p);
   }
-  for (auto& A : s->mechs) {
+  for (int m = 0; m < s->n_mechs; ++m) {
+    Arena* A = s->mech(m);
     for (void* p : A->owned) (void)hipFree(p);
+    for (int* h : A->h_list) (void)hipHostFree(h);
     (void)hipHostFree(A->h_fail);
     if (A->pool_n) {
       (void)hipHostFree(A->pool_state);

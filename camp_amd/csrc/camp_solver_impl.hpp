@@ -79,6 +79,7 @@ struct Solver {
   int poison = 0;  // CellBdfArgs::poison for per-cell runs
   // solver_set_eval_jac: finite-difference check of every evaluated Jacobian
   bool eval_jac = false;
+  long jac_eval_fails = 0;
   // block-path buffers
   double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_y = nullptr,
          *d_tempv = nullptr, *d_ftemp = nullptr, *d_last = nullptr, *d_acinit = nullptr,

@@ -33,10 +33,13 @@ CONV = AVOGADRO / UNIV_GAS_CONST * 1e-12
 # Reaction type codes, src/rxn_solver.c:19-35
 RXN_ARRHENIUS = 1
 RXN_TROE = 2
+RXN_CMAQ_H2O2 = 3
+RXN_CMAQ_OH_HNO3 = 4
 RXN_PHOTOLYSIS = 5
 RXN_FIRST_ORDER_LOSS = 12
 RXN_EMISSION = 13
-TYPE_CODES = {"ARRHENIUS": RXN_ARRHENIUS, "TROE": RXN_TROE, "PHOTOLYSIS": RXN_PHOTOLYSIS,
+TYPE_CODES = {"ARRHENIUS": RXN_ARRHENIUS, "TROE": RXN_TROE, "CMAQ_H2O2": RXN_CMAQ_H2O2,
+              "CMAQ_OH_HNO3": RXN_CMAQ_OH_HNO3, "PHOTOLYSIS": RXN_PHOTOLYSIS,
               "FIRST_ORDER_LOSS": RXN_FIRST_ORDER_LOSS, "EMISSION": RXN_EMISSION}
 UPDATABLE = ("PHOTOLYSIS", "EMISSION", "FIRST_ORDER_LOSS")
 
@@ -67,6 +70,7 @@ class Mechanism:
     reactions: list
     rel_tol: float = DEFAULT_REL_TOL
     name: str = "mechanism"
+    init_conc: list = None     # per species "init conc" of the input files (ppm)
 
     @property
     def n_state(self):
@@ -81,7 +85,7 @@ def pack_reaction(r):
 
     Returns (type_code, int_params, float_params, n_env)."""
     p = r.params
-    if r.type in ("ARRHENIUS", "TROE", "PHOTOLYSIS"):
+    if r.type in ("ARRHENIUS", "TROE", "PHOTOLYSIS", "CMAQ_H2O2", "CMAQ_OH_HNO3"):
         nr, npd = len(r.reactants), len(r.products)
         head = [nr, npd] + ([r.rxn_id] if r.type == "PHOTOLYSIS" else [])
         ints = head + [s + 1 for s in r.reactants] + [s + 1 for s, _ in r.products]
@@ -108,6 +112,24 @@ def pack_reaction(r):
                       p.get("kinf_A", 1.0), p.get("kinf_B", 0.0), p.get("kinf_C", 0.0),
                       p.get("Fc", 0.6), p.get("N", 1.0), scaling, CONV] + yields
             return RXN_TROE, ints, floats, 1
+        if r.type == "CMAQ_H2O2":
+            # rxn_CMAQ_H2O2.F90 initialize: k = k1 + [M] k2, [M] (ppm -> #/cc
+            # through conv) folded into k2_A with a factor 1e6
+            k1A, k2A = p.get("k1_A", 1.0), p.get("k2_A", 1.0)
+            if p.get("time unit") == "MIN":
+                k1A, k2A = k1A / 60.0, k2A / 60.0
+            floats = [k1A, p.get("k1_B", 0.0), p.get("k1_C", 0.0), k2A * 1.0e6,
+                      p.get("k2_B", 0.0), p.get("k2_C", 0.0), CONV] + yields
+            return RXN_CMAQ_H2O2, ints, floats, 1
+        if r.type == "CMAQ_OH_HNO3":
+            # rxn_CMAQ_OH_HNO3.F90 initialize: k = k0 + k3 [M] / (1 + k3 [M] /
+            # k2), k3_A times 1e6, time unit MIN as SCALING = 1/60
+            scaling = 1.0 / 60.0 if p.get("time unit") == "MIN" else 1.0
+            floats = [p.get("k0_A", 1.0), p.get("k0_B", 0.0), p.get("k0_C", 0.0),
+                      p.get("k2_A", 1.0), p.get("k2_B", 0.0), p.get("k2_C", 0.0),
+                      p.get("k3_A", 1.0) * 1.0e6, p.get("k3_B", 0.0), p.get("k3_C", 0.0),
+                      scaling, CONV] + yields
+            return RXN_CMAQ_OH_HNO3, ints, floats, 1
         floats = [p.get("scaling factor", 1.0)] + yields
         return RXN_PHOTOLYSIS, ints, floats, 2
     if r.type == "EMISSION":
@@ -130,54 +152,88 @@ def pack(mech):
 def load_camp_json(objects, name="json"):
     """Build a Mechanism from a list of ``camp-data`` objects.
 
-    Follows camp_core.F90:486-637 (object dispatch) and chem_spec_data.F90
-    (species order = order of appearance, 'tracer type', 'absolute
-    integration tolerance').  Updatable reactions get ids 1, 2, ... in
-    mechanism order, standing in for generate_int_id() (util.F90)."""
+    Follows camp_core.F90:486-637 (object dispatch: CHEM_SPEC,
+    RELATIVE_TOLERANCE, MECHANISM) and chem_spec_data.F90:212-320 (a
+    species named again -- in another file -- is merged into the first
+    object of that name, which fixes its place in the state; later
+    properties update earlier ones; 'tracer type', 'absolute integration
+    tolerance', 'init conc').  Mechanisms of the same name are merged too.
+    Updatable reactions get ids 1, 2, ... in mechanism order, standing in
+    for generate_int_id() (util.F90)."""
     if isinstance(objects, dict):
         objects = objects["camp-data"]
-    species, var_type, abs_tol = [], [], []
+    spec = {}          # name -> merged properties, in order of first appearance
     rel_tol = DEFAULT_REL_TOL
-    rxns_json = []
+    mechs = {}         # mechanism name -> reactions (first-appearance order)
     for obj in objects:
         t = obj.get("type")
         if t == "CHEM_SPEC":
             if obj.get("phase", "GAS") != "GAS":
                 raise ValueError("only gas-phase species are supported")
-            species.append(obj["name"])
-            tt = obj.get("tracer type", "VARIABLE")
-            var_type.append(CHEM_SPEC_CONSTANT if tt == "CONSTANT" else CHEM_SPEC_VARIABLE)
-            abs_tol.append(float(obj.get("absolute integration tolerance", DEFAULT_ABS_TOL)))
+            props = spec.setdefault(obj["name"], {})
+            old = props.get("tracer type")
+            new = obj.get("tracer type")
+            if old is not None and new is not None and old != new:
+                raise ValueError("Type mismatch for species " + obj["name"])
+            props.update({k: v for k, v in obj.items() if k not in ("name", "type")})
         elif t == "RELATIVE_TOLERANCE":
             rel_tol = float(obj["value"])
         elif t == "MECHANISM":
-            rxns_json.extend(obj["reactions"])
+            mechs.setdefault(obj.get("name", ""), []).extend(obj["reactions"])
+    species = list(spec)
+    var_type, abs_tol, init = [], [], []
+    for s in species:
+        props = spec[s]
+        tt = props.get("tracer type", "VARIABLE")
+        if tt not in ("VARIABLE", "CONSTANT"):
+            raise ValueError("unsupported tracer type %s for %s" % (tt, s))
+        var_type.append(CHEM_SPEC_CONSTANT if tt == "CONSTANT" else CHEM_SPEC_VARIABLE)
+        abs_tol.append(float(props.get("absolute integration tolerance", DEFAULT_ABS_TOL)))
+        init.append(float(props.get("init conc", 0.0)))
     idx = {s: i for i, s in enumerate(species)}
     reactions = []
     next_id = 1
-    for rj in rxns_json:
+    for rj in (r for rx in mechs.values() for r in rx):
         rtype = rj["type"]
         params = {k: v for k, v in rj.items() if k not in ("type", "reactants", "products")}
         reactants = []
-        for s, props in rj.get("reactants", {}).items():
-            reactants += [idx[s]] * int(props.get("qty", 1))
-        products = [(idx[s], float(props.get("yield", 1.0)))
-                    for s, props in rj.get("products", {}).items()]
-        if rtype == "EMISSION":
-            products = [(idx[rj["species"]], 1.0)]
-        if rtype == "FIRST_ORDER_LOSS":
-            reactants = [idx[rj["species"]]]
+        try:
+            for s, props in rj.get("reactants", {}).items():
+                reactants += [idx[s]] * int(props.get("qty", 1))
+            products = [(idx[s], float(props.get("yield", 1.0)))
+                        for s, props in rj.get("products", {}).items()]
+            if rtype == "EMISSION":
+                products = [(idx[rj["species"]], 1.0)]
+            if rtype == "FIRST_ORDER_LOSS":
+                reactants = [idx[rj["species"]]]
+        except KeyError as e:
+            raise ValueError("reaction %s names species %s, which no CHEM_SPEC object "
+                             "declares" % (rj.get("rxn id", rtype), e.args[0])) from None
         r = Reaction(rtype, reactants, products, params)
         if rtype in UPDATABLE:
             r.rxn_id = next_id
             next_id += 1
         reactions.append(r)
-    return Mechanism(species, var_type, abs_tol, reactions, rel_tol, name)
+    return Mechanism(species, var_type, abs_tol, reactions, rel_tol, name, init)
 
 
 def load_camp_json_file(path):
+    """One camp-data file, or a configuration file whose "camp-files" list
+    names the files to load, as camp_core%load_files and load read them
+    (camp_core.F90:310-360, 424-486).  Relative paths resolve against the
+    configuration file's directory (the reference opens them relative to the
+    working directory, the same place when run from there)."""
+    import os
     with open(path) as f:
-        return load_camp_json(json.load(f), name=path)
+        top = json.load(f)
+    if isinstance(top, dict) and "camp-files" in top:
+        base = os.path.dirname(os.path.abspath(path))
+        objects = []
+        for rel in top["camp-files"]:
+            with open(os.path.join(base, rel)) as f:
+                objects.extend(json.load(f)["camp-data"])
+        return load_camp_json(objects, name=path)
+    return load_camp_json(top, name=path)
 
 
 # --------------------------------------------------------------------------

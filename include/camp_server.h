@@ -9,7 +9,9 @@
  * cells.  Whenever a stream is free the scheduler launches the persistent
  * kernel over the next cells of the oldest request's mechanism (from that
  * request and later ones of the same mechanism, up to twice the cells the
- * GPU holds at once; a large request spans several launches), each cell
+ * GPU holds at once counted in cell-timesteps; a large request spans
+ * several launches; one mechanism never holds every stream while another
+ * has cells waiting), each cell
  * running its request's timesteps back to back (continuous batching:
  * requests that arrive while others run join the queue and start as soon
  * as a stream frees up; a request completes when its last cells do).
@@ -35,7 +37,8 @@ void *camp_server_new(int n_streams, long max_batch_cells);
 /* Registers an initialised per-cell solver (solver_new + solver_initialize)
  * as a mechanism; its program and controls (rtol, atol, max_steps) are
  * used, its own cell buffers are not.  max_cells = 0 sizes the arena from
- * half the free HBM at the time of the call.  Returns the mechanism id. */
+ * half the free HBM at the time of the call.  Up to 16 mechanisms per
+ * server.  Returns the mechanism id. */
 int camp_server_add_mechanism(void *server, void *solver_data, long max_cells);
 
 /* Arena capacity (cells) of a mechanism. */

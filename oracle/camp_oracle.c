@@ -24,6 +24,8 @@
 /* rxn_solver.c:19-35 */
 #define RXN_ARRHENIUS 1
 #define RXN_TROE 2
+#define RXN_CMAQ_H2O2 3
+#define RXN_CMAQ_OH_HNO3 4
 #define RXN_PHOTOLYSIS 5
 #define RXN_FIRST_ORDER_LOSS 12
 #define RXN_EMISSION 13
@@ -195,10 +197,17 @@ void oracle_add_rxn(void *om, int rxn_type, int n_int, int n_float, int n_env,
 
 /* Per-type accessors for the packed layouts of the gas-phase types with
  * reactant/product lists (rxn_arrhenius.c:20-36, rxn_troe.c:20-39,
+ * rxn_CMAQ_H2O2.c:19-35, rxn_CMAQ_OH_HNO3.c:19-41,
  * rxn_photolysis.c:20-33). */
 static int nprop_int(int type) { return type == RXN_PHOTOLYSIS ? 3 : 2; }
 static int nprop_flt(int type) {
-  return type == RXN_ARRHENIUS ? 6 : type == RXN_TROE ? 10 : 1;
+  switch (type) {
+    case RXN_ARRHENIUS: return 6;
+    case RXN_TROE: return 10;
+    case RXN_CMAQ_H2O2: return 7;
+    case RXN_CMAQ_OH_HNO3: return 11;
+    default: return 1;
+  }
 }
 #define NR(d) ((d)[0])
 #define NP(d) ((d)[1])
@@ -209,7 +218,8 @@ static int nprop_flt(int type) {
 #define YIELD(f, t, x) ((f)[nprop_flt(t) + (x)])
 
 static int is_mass_action(int type) {
-  return type == RXN_ARRHENIUS || type == RXN_TROE || type == RXN_PHOTOLYSIS;
+  return type == RXN_ARRHENIUS || type == RXN_TROE || type == RXN_PHOTOLYSIS ||
+         type == RXN_CMAQ_H2O2 || type == RXN_CMAQ_OH_HNO3;
 }
 
 static void td_add(Oracle *o, int i, long double c);
@@ -1412,6 +1422,30 @@ static void update_env_cell(Oracle *o, int cell, const double *env_data) {
         e[0] = (k0 / (1.0 + kinf)) *
                pow(f[6], (1.0 / (1.0 + pow(log10(kinf) / f[7], 2)))) *
                pow(conv, NR(id) - 1) * f[8];
+        break;
+      }
+      case RXN_CMAQ_H2O2: {
+        /* rxn_CMAQ_H2O2.c:106-124: k = (k1 + k2 [M]) conv^(nr-1), [M] in k2_A */
+        double conv = f[6] * P / T;
+        e[0] = (f[0] * (f[2] == 0.0 ? 1.0 : exp(f[2] / T)) *
+                    (f[1] == 0.0 ? 1.0 : pow(T / ((double)300.0), f[1])) +
+                f[3] * (f[5] == 0.0 ? 1.0 : exp(f[5] / T)) *
+                    (f[4] == 0.0 ? 1.0 : pow(T / ((double)300.0), f[4])) * conv) *
+               pow(conv, NR(id) - 1);
+        break;
+      }
+      case RXN_CMAQ_OH_HNO3: {
+        /* rxn_CMAQ_OH_HNO3.c:110-133: k = (k0 + k3 / (1 + k3 / k2))
+         * conv^(nr-1) SCALING, [M] in k3_A */
+        double conv = f[10] * P / T;
+        double k2 = f[3] * (f[5] == 0.0 ? 1.0 : exp(f[5] / T)) *
+                    (f[4] == 0.0 ? 1.0 : pow(T / ((double)300.0), f[4]));
+        double k3 = f[6] * (f[8] == 0.0 ? 1.0 : exp(f[8] / T)) *
+                    (f[7] == 0.0 ? 1.0 : pow(T / ((double)300.0), f[7])) * conv;
+        e[0] = (f[0] * (f[2] == 0.0 ? 1.0 : exp(f[2] / T)) *
+                    (f[1] == 0.0 ? 1.0 : pow(T / ((double)300.0), f[1])) +
+                k3 / (((double)1.0) + k3 / k2)) *
+               pow(conv, NR(id) - 1) * f[9];
         break;
       }
       case RXN_PHOTOLYSIS:

@@ -121,3 +121,45 @@ def test_replay_poisson_requests():
     c = srv.counters()
     assert c["done"] == k and c["cell_steps"] > 0
     srv.close()
+
+
+def _mp(n):
+    from camp_amd.multiphase import multiphase_cells, multiphase_mechanism
+    mpm = multiphase_mechanism(1)
+    return mpm, multiphase_cells(mpm, n, seed=2)
+
+
+def test_multiphase_request_equals_solver_run_and_is_the_same_in_a_mix():
+    """Multi-phase requests through the server: a request of more cells
+    than one launch takes (1 100 cells x 2 timesteps > 2 048 cell-timesteps)
+    equals solver_run bit for bit; a small request gives the same bits
+    alone and amid gas and multi-phase traffic."""
+    n = 1100
+    mpm, mcells = _mp(n)
+    ref = CampSolver(mpm, n)
+    ref.set_rates(mcells.rates)
+    a = mcells.state.copy()
+    for _ in range(2):
+        assert ref.solve(a, mcells.env, 0.0, 120.0) == 0
+    gmech, gcells = _cb05(64)
+    srv = CampServer(n_streams=4)
+    msol = CampSolver(mpm, 1)
+    mm = srv.add_mechanism(msol, max_cells=4096)
+    gsol = CampSolver(gmech, 1)
+    mg = srv.add_mechanism(gsol, max_cells=4096)
+    mbr = base_rate_matrix(msol, mcells.rates, n)
+    gbr = base_rate_matrix(gsol, gcells.rates, 64)
+    alone = srv.submit(mm, mcells.state[:6], mcells.env[:6], mbr[:6], 300.0, n_steps=1)
+    assert alone.wait()
+    big = srv.submit(mm, mcells.state, mcells.env, mbr, 120.0, n_steps=2, want_stats=True)
+    others = [srv.submit(mg, gcells.state, gcells.env, gbr, 180.0, n_steps=3)]
+    mixed = srv.submit(mm, mcells.state[:6], mcells.env[:6], mbr[:6], 300.0, n_steps=1)
+    others.append(srv.submit(mm, mcells.state[40:90], mcells.env[40:90], mbr[40:90], 60.0,
+                             n_steps=2))
+    assert big.wait() and mixed.wait() and all(o.wait() for o in others)
+    assert np.array_equal(big.state, a)
+    assert np.array_equal(big.stats, ref.cell_stats())
+    assert np.array_equal(alone.state, mixed.state)
+    c = srv.counters()
+    assert c["done"] == 5 and c["failed"] == 0 and c["batches"] >= 4
+    srv.close()

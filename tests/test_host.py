@@ -11,8 +11,8 @@ import sys
 import numpy as np
 import pytest
 
-from camp_amd.mechanisms import (CONV, box_mechanism, cb05_scale_mechanism, load_camp_json, pack,
-                                 synthetic_cells, RXN_TROE)
+from camp_amd.mechanisms import (CONV, box_mechanism, cb05_scale_mechanism, load_camp_json,
+                                 load_camp_json_file, pack, synthetic_cells, RXN_TROE)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -37,10 +37,14 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
 
 
+CB05CL_AE5 = os.path.join(HERE, "golden", "cb05cl_ae5", "config.json")
+
+
 @pytest.mark.parametrize("mech", [box_mechanism(1), cb05_scale_mechanism(1),
                                   load_camp_json(json.load(open(os.path.join(
-                                      HERE, "golden", "rxn_arrhenius.json"))))],
-                         ids=["box", "cb05", "arrhenius_json"])
+                                      HERE, "golden", "rxn_arrhenius.json")))),
+                                  load_camp_json_file(CB05CL_AE5)],
+                         ids=["box", "cb05", "arrhenius_json", "cb05cl_ae5"])
 def test_product_pattern_and_ids_match_oracle(oracle_lib, mech):
     from camp_amd.core import CampSolver
     s = CampSolver(mech, 3)          # solver_initialize runs on the host
@@ -91,7 +95,8 @@ def test_seeded_inputs_are_reproducible_and_batch_independent():
 
 
 def test_setup_error_exits_like_the_reference():
-    """camp_solver.c:93-96: setup errors print and exit(EXIT_FAILURE)."""
+    """camp_solver.c:93-96: setup errors print and exit(EXIT_FAILURE); here a
+    reaction type outside the supported set (14, wet deposition)."""
     code = ("import sys; sys.path.insert(0, %r)\n"
             "from camp_amd._lib import lib\n"
             "import numpy as np, ctypes as C\n"
@@ -100,7 +105,7 @@ def test_setup_error_exits_like_the_reference():
             "h = lib().solver_new(3, 1, vt.ctypes.data_as(ip), 1, 4, 1, 1,"
             " 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)\n"
             "ints = np.array([1, 1, 0, 0], np.int32); fl = np.zeros(1)\n"
-            "lib().rxn_add_condensed_data(3, 4, 1, 1, ints.ctypes.data_as(ip),"
+            "lib().rxn_add_condensed_data(14, 4, 1, 1, ints.ctypes.data_as(ip),"
             " fl.ctypes.data_as(dp), h)\n"
             "at = np.full(3, 1e-8)\n"
             "lib().solver_initialize(h, at.ctypes.data_as(dp), 1e-4, 100, 10)\n" % ROOT)
@@ -181,3 +186,42 @@ def test_elementary_functions_are_correctly_rounded():
             ref = [float(exact(mpmath.mpf(float(a)), mpmath.mpf(float(b)))) for a, b in zip(x, yy)]
         bad = np.flatnonzero(out != np.array(ref))
         assert bad.size == 0, (fn, x[bad[:3]], yy[bad[:3]], out[bad[:3]])
+
+
+def test_camp_files_config_loads_the_real_cb05():
+    """camp_core's camp-files list (camp_core.F90:310-360): the real
+    CB05cl_ae5 mechanism, species, tolerance and initial-condition files,
+    with species objects of the same name merged across files.  The
+    reference test counts 186 reactions (test_cb05cl_ae5.F90:367-370)."""
+    from camp_amd.core import CampSolver
+    m = load_camp_json_file(CB05CL_AE5)
+    assert len(m.reactions) == 186
+    types = {}
+    for r in m.reactions:
+        types[r.type] = types.get(r.type, 0) + 1
+    assert types == {"ARRHENIUS": 142, "PHOTOLYSIS": 25, "TROE": 15, "CMAQ_H2O2": 3,
+                     "CMAQ_OH_HNO3": 1}
+    assert m.rel_tol == 1.0e-4
+    const = sorted(s for s, v in zip(m.species, m.var_type) if v == 2)
+    assert const == ["CH4", "H2", "H2O", "M", "N2", "O2"]
+    assert m.init_conc[m.species.index("M")] == 1.0e6
+    assert m.abs_tol[m.species.index("NO2")] == 1.0e-3
+    s = CampSolver(m, 2)
+    assert (s.nv, s.nnz) == (74, 671)
+
+
+def test_cmaq_packing_follows_fortran_packers():
+    """rxn_CMAQ_H2O2.F90 / rxn_CMAQ_OH_HNO3.F90 initialize: float layouts
+    [k1_A, k1_B, k1_C, k2_A*1e6, k2_B, k2_C, CONV, yields] and [k0_A, k0_B,
+    k0_C, k2_A, k2_B, k2_C, k3_A*1e6, k3_B, k3_C, SCALING, CONV, yields],
+    defaults A = 1, B = C = 0, time unit MIN dividing the A's by 60 (H2O2)
+    or giving SCALING = 1/60 (OH_HNO3)."""
+    from camp_amd.mechanisms import Reaction, pack_reaction
+    t, ints, fl, ne = pack_reaction(Reaction("CMAQ_H2O2", [0, 0], [(1, 0.5)],
+                                             {"k1_A": 2.0, "k2_C": 5.0, "time unit": "MIN"}))
+    assert (t, ne) == (3, 1) and ints[:6] == [2, 1, 1, 1, 2, 0]
+    assert fl == [2.0 / 60.0, 0.0, 0.0, 1.0 / 60.0 * 1.0e6, 0.0, 5.0, CONV, 0.5]
+    t, ints, fl, ne = pack_reaction(Reaction("CMAQ_OH_HNO3", [0, 1], [(2, 1.0)],
+                                             {"k3_A": 3.0, "time unit": "MIN"}))
+    assert (t, ne) == (4, 1) and ints[:5] == [2, 1, 1, 2, 3]
+    assert fl == [1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 3.0e6, 0.0, 0.0, 1.0 / 60.0, CONV, 1.0]

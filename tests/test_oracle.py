@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from camp_amd.mechanisms import (box_mechanism, cb05_scale_mechanism, load_camp_json,
-                                 synthetic_cells, pack, TYPE_CODES)
+                                 load_camp_json_file, synthetic_cells, pack, TYPE_CODES)
 from tests.analytic_cases import KATS, all_cases, reference_bar
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -110,7 +110,9 @@ def test_oracle_analytic_solutions(O, name, case):
 # ------------------------------------------- the reference's own code ----
 MECHS = [("box", box_mechanism(1)), ("cb05", cb05_scale_mechanism(1)),
          ("arrhenius_json", load_camp_json(json.load(open(os.path.join(HERE, "golden",
-                                                                       "rxn_arrhenius.json")))))]
+                                                                       "rxn_arrhenius.json"))))),
+         ("cb05cl_ae5", load_camp_json_file(os.path.join(HERE, "golden", "cb05cl_ae5",
+                                                         "config.json")))]
 
 
 @pytest.mark.parametrize("name,mech", MECHS)
@@ -125,9 +127,14 @@ def test_pattern_and_ids_match_reference_code(O, name, mech):
     assert np.array_equal(ids, o.int_data())
 
 
-def test_rate_constants_match_reference_code(O):
+@pytest.mark.parametrize("name", ["cb05", "cb05cl_ae5"])
+def test_rate_constants_match_reference_code(O, name):
+    """Rate constants and rate updates bit-equal to the reference's own
+    compiled C, on the CB05-scale mechanism and on the real CB05cl_ae5
+    (Arrhenius, Troe, CMAQ_H2O2, CMAQ_OH_HNO3, photolysis) over seeded
+    temperatures and pressures."""
     ref_or_skip(O)
-    mech = cb05_scale_mechanism(1)
+    mech = dict(MECHS)[name]
     n = 64
     cells = synthetic_cells(mech, n, seed=5)
     o = O.Oracle(mech, n)
@@ -313,3 +320,34 @@ def test_deriv_blend_returns_the_estimate_where_production_and_loss_cancel(O):
     others = np.ones(mech.n_state, bool)
     others[[-4, -3, -1]] = False
     assert np.allclose(blend[others], pure[others], rtol=1e-12, atol=0)
+
+
+def test_cmaq_rate_constants_closed_form(O):
+    """The real CB05cl_ae5's CMAQ_H2O2 (R34: k = k1 + k2 [M]) and CMAQ_OH_HNO3
+    (R29: k = k0 + k3 [M] / (1 + k3 [M] / k2)) rate constants against the
+    closed forms of the CMAQ rate expressions CMAQ_9 and CMAQ_8 the
+    mechanism's "orig params" give (rate constants in cm3 s-1 times [M] in
+    molecules cm-3), converted to ppm-1 s-1, at the reference test's
+    conditions (test_cb05cl_ae5.F90:198-206: T 272.5 K, P 0.8 atm)."""
+    import math
+    from camp_amd.mechanisms import AVOGADRO, UNIV_GAS_CONST
+    mech = dict(MECHS)["cb05cl_ae5"]
+    T = 272.5
+    P = float(np.float32(0.8)) * 101325.0
+    o = O.Oracle(mech, 1)
+    o.update_env(np.array([[T, P]]))
+    k = o.rxn_env()[0]
+    M = AVOGADRO / UNIV_GAS_CONST * P / T * 1e-6    # air molecules per cm3
+    ppm = M * 1e-6                                  # molecules per cm3 per ppm
+    # each reaction's first env slot (photolysis reactions hold two)
+    offs = np.concatenate([[0], np.cumsum([p[3] for p in pack(mech)])])
+    idx = {r.params.get("rxn id"): offs[i] for i, r in enumerate(mech.reactions)}
+    # R34: HO2 + HO2 -> H2O2, CMAQ_9(2.3E-13, -600, 1.7E-33, -1000)
+    k34 = (2.3e-13 * math.exp(600.0 / T) + 1.7e-33 * math.exp(1000.0 / T) * M) * ppm
+    # R29: OH + HNO3 -> NO3, CMAQ_8(2.4E-14, -460, 2.7E-17, -2199, 6.5E-34, -1335)
+    k0, k2, k3 = (2.4e-14 * math.exp(460.0 / T), 2.7e-17 * math.exp(2199.0 / T),
+                  6.5e-34 * math.exp(1335.0 / T) * M)
+    k29 = (k0 + k3 / (1.0 + k3 / k2)) * ppm
+    for rid, want in (("R34", k34), ("R29", k29)):
+        got = k[idx[rid]]
+        assert abs(got - want) <= 1e-12 * want, (rid, got, want)
