@@ -287,6 +287,9 @@ def main():
     ap.add_argument("--cells", type=int, default=100000)
     ap.add_argument("--dt", type=float, default=180.0)
     ap.add_argument("--cells-per-block", type=int, default=0)
+    ap.add_argument("--placement", type=int, default=-1,
+                    help="per-cell kernel cell placement (0: history in LDS, 1: in global "
+                         "memory); with --cells-per-block skips the autotuner")
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-cells-per-core", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
@@ -335,6 +338,8 @@ def main():
     def fresh():
         s = CampSolver(mech, args.cells, mode=MODE_PER_CELL)
         s.set_rates(cells.rates)
+        if args.placement >= 0:
+            s.set_placement(args.placement)
         if args.cells_per_block:
             s.set_cells_per_block(args.cells_per_block)
         return s
@@ -355,10 +360,12 @@ def main():
         # the spun-up cells (copies; the workload's cells are untouched)
         chosen, cand_ms = solver.autotune(st.data_ptr(), env.data_ptr(), 0.0, args.dt,
                                           n_sample=min(args.cells, 16384))
-        tune = {"cells_per_workgroup": chosen,
-                "candidate_ms": {str(k): v for k, v in cand_ms.items()},
+        tune = {"placement": ("history in LDS", "history in global memory")[chosen[0]],
+                "cells_per_workgroup": chosen[1],
+                "candidate_ms": {"%s/%d" % (("lds", "global")[p], c): v
+                                 for (p, c), v in cand_ms.items() if v is not None},
                 "sample_cells": min(args.cells, 16384)}
-        progress("autotuned cells per workgroup: %d %s" % (chosen, cand_ms))
+        progress("autotuned: %s %s" % (chosen, tune["candidate_ms"]))
     torch.cuda.synchronize()
     step_ms, kern_ms, stats = [], [], []
     failed_cell_steps = 0

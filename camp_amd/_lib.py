@@ -60,6 +60,11 @@ SIGNATURES = {
     "camp_eval_deriv": (_i, [_p, _dp, _dp, _d, _i, _dp]),
     "camp_eval_jac": (_i, [_p, _dp, _dp, _d, _dp]),
     "camp_eval_lu_solve": (_i, [_p, _dp, _d, _dp]),
+    "camp_solver_check_jac": (_i, [_p, _dp, _dp, _d, _dp, _ip]),
+    "camp_solver_set_placement": (_i, [_p, _i]),
+    "camp_solver_set_eval_kernels": (_i, [_p, _i]),
+    "camp_solver_tune_fj": (_i, [_p, _p, _p, _i, _dp]),
+    "camp_solver_set_fj_lanes": (_i, [_p, _i, _i]),
     "camp_solver_get_cell_stats": (None, [_p, _dp]),
     "camp_solver_last_kernel_ms": (_d, [_p]),
     "camp_solver_set_profiling": (None, [_p, _i]),

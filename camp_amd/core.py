@@ -116,22 +116,29 @@ class CampSolver:
     def set_cells_per_block(self, n):
         return lib().camp_solver_set_cells_per_block(self.h, n)
 
+    def set_placement(self, place):
+        """0: cell history in LDS; 1: in global memory (more cells per CU)."""
+        return lib().camp_solver_set_placement(self.h, int(place))
+
     def autotune(self, state_ptr, env_ptr, t0, t1, n_sample=0):
-        """Pick cells per workgroup for the per-cell kernel by timing each
-        candidate on a sample of the device-resident cells (copies).
-        Returns (chosen, {candidate: ms or None when it does not fit})."""
-        ms = np.zeros(6)
+        """Pick the per-cell kernel's cell placement and cells per workgroup
+        by timing each candidate on a sample of the device-resident cells
+        (copies).  Returns ((placement, cells per workgroup),
+        {(placement, cells per workgroup): ms or None when it does not fit})."""
+        ms = np.zeros(12)
         best = lib().camp_solver_autotune(self.h, C.c_void_p(state_ptr), C.c_void_p(env_ptr),
                                           float(t0), float(t1), int(n_sample), _dp(ms))
-        return best, {c: (float(v) if v >= 0 else None)
-                      for c, v in zip((1, 2, 4, 8, 16, 32), ms)}
+        cands = [(p, c) for p in (0, 1) for c in (1, 2, 4, 8, 16, 32)]
+        return (best // 100, best % 100), {k: (float(v) if v >= 0 else None)
+                                           for k, v in zip(cands, ms)}
 
     def set_guess_helper(self, on):
         lib().camp_solver_set_guess_helper(self.h, int(bool(on)))
 
     def set_cell_order(self, order=None):
         """Queue order of the per-cell kernel (a permutation of the cells;
-        None: index order).  Results do not depend on it."""
+        None: the default, longest first by the previous call's step
+        counts).  Results do not depend on it."""
         if order is None:
             rc = lib().camp_solver_set_cell_order(self.h, None, 0)
         else:
@@ -239,6 +246,21 @@ class CampSolver:
                                   C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def set_fj_lanes(self, f_lanes, j_lanes):
+        """Lanes per cell (8, 16, 32) of the block path's f and J kernels."""
+        return lib().camp_solver_set_fj_lanes(self.h, int(f_lanes), int(j_lanes))
+
+    def tune_fj(self, state_ptr, env_ptr, reps=10):
+        """Time the block path's f/J kernels at 8, 16 and 32 lanes per cell
+        and keep the fastest.  Returns ((f lanes, J lanes),
+        {"f": {lanes: ms}, "jac": {lanes: ms}})."""
+        ms = np.zeros(6)
+        best = lib().camp_solver_tune_fj(self.h, C.c_void_p(state_ptr), C.c_void_p(env_ptr),
+                                         int(reps), _dp(ms))
+        w = (8, 16, 32)
+        return (best % 100, best // 100), {"f": dict(zip(w, ms[:3].tolist())),
+                                           "jac": dict(zip(w, ms[3:].tolist()))}
+
     def pattern(self):
         L = lib()
         rcp = np.zeros(self.ns + 1, np.int32)
@@ -265,6 +287,30 @@ class CampSolver:
         out = np.zeros((self.n_cells, self.nnz))
         rc = lib().camp_eval_jac(self.h, _dp(state), _dp(env), float(time_step), _dp(out))
         return rc, out
+
+    def check_jac(self, state, env, time_step=1.0, jac=None):
+        """Finite-difference check of J on the GPU (camp_solver_check_jac):
+        (total failing elements or -1, failing elements per cell)."""
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        env = np.ascontiguousarray(env, dtype=np.float64)
+        fails = np.zeros(self.n_cells, np.int32)
+        jp = None
+        if jac is not None:
+            jac = np.ascontiguousarray(jac, dtype=np.float64)
+            jp = _dp(jac)
+        rc = lib().camp_solver_check_jac(self.h, _dp(state), _dp(env), float(time_step), jp,
+                                         _ip(fails))
+        return rc, fails
+
+    def set_eval_kernels(self, group):
+        """camp_eval_deriv / camp_eval_jac on the lane-group kernels (1) or
+        the per-wavefront code of the per-cell kernel (0)."""
+        return lib().camp_solver_set_eval_kernels(self.h, int(group))
+
+    def set_eval_jac(self, on=True):
+        """solver_set_eval_jac: check every cell's last Jacobian after each
+        solve; failures are counted as Jac_eval_fails."""
+        lib().solver_set_eval_jac(self.h, bool(on))
 
     def eval_lu_solve(self, jac, gamma, b):
         jac = np.ascontiguousarray(jac, dtype=np.float64)

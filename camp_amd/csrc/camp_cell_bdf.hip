@@ -665,33 +665,37 @@ struct CellCv {
 
 }  // namespace
 
-template <bool MP>
-__global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
+// MP: multi-phase program; GH: the Nordsieck history, 7 of the 10 work
+// vectors and the J history in global memory (kPlaceGlobal; always for
+// multi-phase programs).  The gas-phase variant with its history in LDS is
+// compiled for one wavefront per SIMD, the others for two, so that the
+// cells their smaller LDS footprint lets onto a CU can all be resident.
+template <bool MP, bool GH>
+__global__ void __launch_bounds__(256, (GH && !MP) ? 2 : 1) k_cell_bdf(CellBdfArgs a) {
   extern __shared__ double smem[];
   const DevProgram& P = a.P;
+  constexpr int kPlace = GH ? kPlaceGlobal : kPlaceLds;
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
   int nv = P.nv;
-  size_t per_cell = cell_bdf_lds_doubles(P);
+  size_t per_cell = cell_bdf_lds_doubles(P, kPlace);
   double* base = smem + (size_t)wv * per_cell;
-  double* scratch = a.wave_scratch
-                        ? a.wave_scratch + ((size_t)blockIdx.x * (blockDim.x / kWave) + wv) *
-                                               cell_bdf_wave_scratch(P)
-                        : nullptr;
+  double* scratch = GH ? a.wave_scratch + ((size_t)blockIdx.x * (blockDim.x / kWave) + wv) *
+                                              cell_bdf_wave_scratch(P, kPlace)
+                       : nullptr;
   CellCv<MP> cv(P);
   cv.lane = lane;
   cv.nv = nv;
   cv.s_state = base;
   cv.s_k = cv.s_state + P.ns;
   cv.mpl = mp_lds(P, cv.s_k + P.n_env,
-                  scratch ? scratch + P.lu_nnz + kWave + 1 + P.rnnz : nullptr);
+                  MP ? scratch + P.lu_nnz + kWave + 1 + P.rnnz : nullptr);
   cv.s_rate = cv.s_k + P.n_env + mp_lds_doubles(P);
   double* v = cv.s_rate + 2 * (size_t)P.n_terms;
-  // Multi-phase cells keep the Nordsieck history and the work vectors
-  // other than y, acor and w in their global scratch (after the LU, J_rxn
-  // and Jacobian scalars), so that four cells fit a CU's LDS; gas-phase
-  // cells keep them all in LDS.
-  double* g = MP ? scratch + P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) : nullptr;
-  double*& vh = MP ? g : v;
+  // With GH the Nordsieck history and the work vectors other than y, acor
+  // and w live in the wavefront's global scratch (for multi-phase programs
+  // after the LU, J_rxn and Jacobian scalars); else all in LDS.
+  double* g = MP ? scratch + P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) : scratch;
+  double*& vh = GH ? g : v;
   cv.znb = vh;
   vh += 6 * (size_t)nv;
   cv.last = vh; vh += nv;
@@ -704,8 +708,8 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   cv.acor = v; v += nv;
   cv.y = v; v += nv;
   cv.w = v; v += nv;
-  // Gas-phase cells keep their J history (J_solver, J_state, J_deriv) in
-  // LDS for the whole solve; multi-phase cells use it in place in HBM.
+  // Without GH the J history (J_solver, J_state, J_deriv) is staged in LDS
+  // for the whole solve; with GH it is used in place in HBM.
   double *h_js = nullptr, *h_jy = nullptr, *h_jf = nullptr;
   // the f gather's scratch follows the LU values (gas: also the LU's sinks)
   if (MP) {
@@ -717,9 +721,11 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
     cv.Jr = nullptr;
     v += P.lu_nnz;
     cv.s_fb = v; v += fb_sink_doubles(P);
-    h_js = v; v += P.nnz;
-    h_jy = v; v += nv;
-    h_jf = v; v += nv;
+    if (!GH) {
+      h_js = v; v += P.nnz;
+      h_jy = v; v += nv;
+      h_jf = v; v += nv;
+    }
   }
   cv.l = v;
   cv.tq = v + 6;
@@ -742,8 +748,8 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
       if (a.poison) {
         const double nan = __builtin_nan("");
         for (size_t i = lane; i < per_cell; i += kWave) base[i] = nan;
-        if (scratch)
-          for (size_t i = lane; i < cell_bdf_wave_scratch(P); i += kWave) scratch[i] = nan;
+        if (GH)
+          for (size_t i = lane; i < cell_bdf_wave_scratch(P, kPlace); i += kWave) scratch[i] = nan;
         __threadfence_block();
         wsync();
       }
@@ -752,7 +758,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
       double* g_js = a.Js + (size_t)cell * P.nnz;
       double* g_jy = a.Jy + (size_t)cell * nv;
       double* g_jf = a.Jf + (size_t)cell * nv;
-      if (MP) {
+      if (GH) {
         cv.Js = g_js;
         cv.Jy = g_jy;
         cv.Jf = g_jf;
@@ -808,7 +814,7 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
           }
         }
       }
-      if (!MP) {
+      if (!GH) {
         wsync();
         for (int e = lane; e < P.nnz; e += kWave) g_js[e] = h_js[e];
         for (int i = lane; i < nv; i += kWave) {
@@ -834,31 +840,34 @@ __global__ void __launch_bounds__(256, 1) k_cell_bdf(CellBdfArgs a) {
   }
 }
 
-// Cells of multi-phase programs need more than 64 KiB of LDS per block;
-// gfx950 provides 160 KiB per CU, opted into per kernel.
-static const void* cell_bdf_fn(bool mp) {
-  return mp ? (const void*)k_cell_bdf<true> : (const void*)k_cell_bdf<false>;
+// Per-cell LDS beyond 64 KiB per block (multi-phase programs, or several
+// cells per workgroup) is opted into per kernel; gfx950 has 160 KiB per CU.
+static const void* cell_bdf_fn(const DevProgram& P, int place) {
+  if (P.mp) return (const void*)k_cell_bdf<true, true>;
+  return cell_placement(P, place) == kPlaceGlobal ? (const void*)k_cell_bdf<false, true>
+                                                  : (const void*)k_cell_bdf<false, false>;
 }
-static void allow_lds(bool mp, size_t lds) {
+static void allow_lds(const void* fn, size_t lds) {
   if (lds > 65536)
-    (void)hipFuncSetAttribute(cell_bdf_fn(mp), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s) {
-  allow_lds(a.P.mp != 0, lds);
+  allow_lds(cell_bdf_fn(a.P, a.place), lds);
   if (a.P.mp)
-    hipLaunchKernelGGL(k_cell_bdf<true>, dim3(grid), dim3(block), lds, s, a);
+    hipLaunchKernelGGL((k_cell_bdf<true, true>), dim3(grid), dim3(block), lds, s, a);
+  else if (cell_placement(a.P, a.place) == kPlaceGlobal)
+    hipLaunchKernelGGL((k_cell_bdf<false, true>), dim3(grid), dim3(block), lds, s, a);
   else
-    hipLaunchKernelGGL(k_cell_bdf<false>, dim3(grid), dim3(block), lds, s, a);
+    hipLaunchKernelGGL((k_cell_bdf<false, false>), dim3(grid), dim3(block), lds, s, a);
 }
 
-int cell_bdf_blocks_per_cu(bool mp, int block, size_t lds) {
+int cell_bdf_blocks_per_cu(const DevProgram& P, int place, int block, size_t lds) {
   if (block > 256 || lds > 160 * 1024) return 0;
-  allow_lds(mp, lds);
+  const void* fn = cell_bdf_fn(P, place);
+  allow_lds(fn, lds);
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, cell_bdf_fn(mp), block, lds) != hipSuccess)
-    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, lds) != hipSuccess) return 0;
   return n;
 }
 

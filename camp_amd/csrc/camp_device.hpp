@@ -24,6 +24,15 @@ struct DevProgram {
   int ns, nv, nnz, n_env, n_terms, lu_nnz, n_rxn;
   const int* dep_state;
   const int* j_ptr;
+  const int *colptr, *rowval;  // solver pattern (dep ids, CSC)
+  // lane-group f/J programs (gas-phase programs; Program::gf_rec), one per
+  // group width (8, 16, 32 lanes per cell)
+  int g_f_rows[3], g_j_rows[3], n_k, n_fzero;
+  const int4 *gf_rec[3], *gj_rec[3];   // 2 int4 per record
+  const int *k_slot, *f_count, *f_zero;
+  const int *r_ptr, *r_elem, *r_col;  // row view of the solver pattern
+  int n_cs;                            // state variables that are not solver variables
+  const int* cs_ids;
   const int *perm, *m_to_lu, *diag_pos, *e_diag;
   const int *l_pos, *l_row, *u_pos, *u_row;
   const uint64_t* up_rec;  // LU update records: dst | l << 16 | u << 32, rows of 64

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "camp_device.hpp"
+#include "camp_model.hpp"
 #include "camp_mp_device.hpp"
 
 namespace camp {
@@ -14,23 +15,37 @@ constexpr int kCellStats = 10;  // nst nfe nsetups netf nni ncfn nje flag hu nex
 // two parts of f (state update + rates, species gather + estimate blend)
 constexpr int kCellProf = 7;
 
-// LDS doubles one wavefront needs for a cell in the persistent kernel:
-// state, rate data, the multi-phase extras, Nordsieck history (6 vectors),
-// 10 work vectors, the LU factors (for multi-phase programs the LU, the
-// history and 7 of the work vectors live in global scratch instead) and 24
-// slots for the BDF coefficient arrays l, tq and tau.
-__host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P) {
-  return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms +
-         (P.mp ? 3 : 16) * (size_t)P.nv + fb_sink_doubles(P) +
-         (P.mp ? 0 : (size_t)P.lu_nnz + P.nnz + 2 * (size_t)P.nv) + 24;
+// Where a cell's working set lives in the persistent kernel.
+//  kPlaceLds     everything in LDS: state, rate data, per-term rates,
+//                Nordsieck history (6 vectors), 10 work vectors, the f
+//                gather's scratch, the LU factors, the J history and 24
+//                slots for the BDF coefficient arrays l, tq and tau (gas
+//                phase only: 4 cells per CU for the CB05-scale mechanism);
+//  kPlaceGlobal  the Nordsieck history and 7 of the 10 work vectors in
+//                per-wavefront global scratch and the J history used in
+//                place in HBM, so that more cells fit a CU's LDS (6 for the
+//                CB05-scale mechanism; compiled for two wavefronts per SIMD).
+//                Multi-phase programs always run so, with their LU factors,
+//                reaction Jacobian and per-instance Jacobian scalars in the
+//                scratch as well.
+constexpr int kPlaceLds = 0, kPlaceGlobal = 1;
+__host__ __device__ inline int cell_placement(const DevProgram& P, int place) {
+  return P.mp ? kPlaceGlobal : place;
 }
-// Global scratch doubles per wavefront of multi-phase programs: LU factors
-// (with the factorisation's 64 sink slots and the solve's zero slot), the
-// reaction Jacobian, the per-instance Jacobian scalars, and the Nordsieck
-// history and 7 of the 10 work vectors (13 vectors), so that four cells fit
-// a CU's LDS.
-__host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P) {
-  return P.mp ? (size_t)P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) + 13 * (size_t)P.nv : 0;
+__host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P, int place) {
+  const bool g = cell_placement(P, place) == kPlaceGlobal;
+  return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms +
+         (g ? 3 : 16) * (size_t)P.nv + fb_sink_doubles(P) +
+         (P.mp ? 0 : (size_t)P.lu_nnz) + (g ? 0 : (size_t)P.nnz + 2 * (size_t)P.nv) + 24;
+}
+// Global scratch doubles per wavefront: for multi-phase programs the LU
+// factors (with the factorisation's 64 sink slots and the solve's zero
+// slot), the reaction Jacobian and the per-instance Jacobian scalars, then
+// (both programs, kPlaceGlobal) the Nordsieck history and 7 of the 10 work
+// vectors (13 vectors).
+__host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P, int place) {
+  if (P.mp) return (size_t)P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) + 13 * (size_t)P.nv;
+  return place == kPlaceGlobal ? 13 * (size_t)P.nv : 0;
 }
 
 struct CellBdfArgs {
@@ -52,11 +67,12 @@ struct CellBdfArgs {
   const double* cell_t1 = nullptr;
   double* prof = nullptr;  // [cells][kCellProf] when phase timing is on
   int* fail_count = nullptr;  // per cell: incremented on each failed solve
-  double* wave_scratch = nullptr;  // [grid waves][cell_bdf_wave_scratch] (multi-phase)
+  double* wave_scratch = nullptr;  // [grid waves][cell_bdf_wave_scratch] (kPlaceGlobal)
   // Optional: cell solves cell_nsteps[cell] consecutive timesteps over
   // [t0, t1] in this launch, each exactly as a separate launch would
   // (state written back and restaged between them); nullptr: one.
   const int* cell_nsteps = nullptr;
+  int place = kPlaceLds;  // cell_placement(P, place) selects the kernel variant
   // Check of per-cell independence: when non-zero, every wavefront fills its
   // whole working memory (the cell's LDS region and, for multi-phase
   // programs, its global scratch) with quiet NaNs before it stages each
@@ -65,7 +81,9 @@ struct CellBdfArgs {
 };
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);
-int cell_bdf_blocks_per_cu(bool mp, int block, size_t lds);
+// workgroups per CU of the variant (program kind and placement) at this
+// block size and LDS, 0 if it does not fit
+int cell_bdf_blocks_per_cu(const DevProgram& P, int place, int block, size_t lds);
 
 __global__ void k_rates(DevProgram P, int nc, const double* env, double* rxn_env);
 __global__ void k_deriv(DevProgram P, int nc, const double* y, const double* state,
@@ -82,6 +100,26 @@ __host__ __device__ inline size_t fj_lds_doubles(const DevProgram& P) {
          fb_lds_doubles(P);
 }
 inline int fj_waves_per_block(const DevProgram& P) { return P.mp ? 1 : 4; }
+
+// Lane-group f(y) and J(y) of gas-phase programs (lanes per cell = 8, 16
+// or 32; camp_kernels.hip) and the compact rate-constant array they read.
+__global__ void k_kval(DevProgram P, int nc, const double* rxn_env, double* kval);
+void launch_group_deriv(int lanes, const DevProgram& P, int nc, const double* y,
+                        const double* state, const double* kval, const double* Js,
+                        const double* Jy, const double* Jf, double* fout, double h, int use_est,
+                        int* fail, hipStream_t st);
+void launch_group_jac(int lanes, const DevProgram& P, int nc, const double* y,
+                      const double* state, const double* kval, double* J, double h,
+                      hipStream_t st);
+// Per-cell LDS stride of the lane-group kernels: the state, (the blended f)
+// the gather's running sums 2 nv + 2, the compact rate constants; odd, so
+// that the cells of a wavefront start in different LDS banks.
+__host__ __device__ inline int group_cell_stride(const DevProgram& P, bool fsum) {
+  return (P.ns + (fsum ? 2 * P.nv + 2 : 0) + P.n_k) | 1;
+}
+inline size_t group_lds_bytes(const DevProgram& P, int lanes, bool fsum) {
+  return (size_t)(kWave / lanes) * group_cell_stride(P, fsum) * sizeof(double);
+}
 __global__ void k_lu_factor(DevProgram P, int nc, const double* J, double gamma, double* LU,
                             int* fail);
 __global__ void k_lu_solve(DevProgram P, int nc, const double* LU, double* b, double s);
@@ -110,6 +148,12 @@ __global__ void k_vec_project(long n, const double* zn0, double* acor, double* y
 __global__ void k_vec_tstar(long n, const double* tmp1, const double* corr, double h_j,
                             double* out);
 __global__ void k_fill(long n, double v, double* x);
+__global__ void k_check_jac(DevProgram P, int nc, const double* y, const double* state,
+                            const double* env, const double* rxn_env, const double* J, double h_ts,
+                            int* cell_fails);
+inline size_t check_jac_lds_doubles(const DevProgram& P) {
+  return fj_lds_doubles(P) + 4 * (size_t)P.nv + 2 * (size_t)P.nnz;
+}
 // renv[c][upd_off[u]] = rates[c][u] for c < n_cells, u < n_upd
 __global__ void k_scatter_rates(int n_cells, int n_upd, const int* upd_off, const double* rates,
                                 int n_env, double* renv);

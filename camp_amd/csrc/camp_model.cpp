@@ -234,6 +234,136 @@ std::vector<int> markowitz_order(int n, const std::vector<int>& colptr,
   return order;
 }
 
+// The lane-group f and J program (see Program::gf_rec): whole species
+// packed longest first into G bins for f, contiguous element runs
+// of near-equal record counts for J, and the compact k index map.
+static void build_group_program(Program& P, int w) {
+  const int G = kGroupWidths[w], nv = P.nv;
+  // compact k: the rxn_env slot of each term, in term order
+  std::vector<int> kc_of_slot(std::max(P.n_env, 1), -1);
+  P.k_slot.clear();
+  for (int t = 0; t < P.n_terms; ++t) {
+    int sl = P.t_kidx[t];
+    if (kc_of_slot[sl] < 0) {
+      kc_of_slot[sl] = (int)P.k_slot.size();
+      P.k_slot.push_back(sl);
+    }
+  }
+  P.n_k = (int)P.k_slot.size();
+  if (P.k_slot.empty()) P.k_slot.push_back(0);
+  auto put_coef = [](int* r, double c) {
+    uint64_t bits;
+    std::memcpy(&bits, &c, sizeof(bits));
+    r[0] = (int)(uint32_t)(bits & 0xFFFFFFFFu);
+    r[1] = (int)(uint32_t)(bits >> 32);
+  };
+  // f: species into bins, longest first, each to the least-loaded bin
+  P.f_count.assign(nv, 0);
+  P.f_zero.clear();
+  std::vector<int> sp(nv);
+  for (int i = 0; i < nv; ++i) {
+    sp[i] = i;
+    P.f_count[i] = P.f_ptr[i + 1] - P.f_ptr[i];
+    if (P.f_count[i] == 0) P.f_zero.push_back(i);
+  }
+  std::stable_sort(sp.begin(), sp.end(),
+                   [&](int a, int b) { return P.f_count[a] > P.f_count[b]; });
+  std::vector<std::vector<int>> fbin(G);
+  std::vector<int> load(G, 0);
+  for (int i : sp) {
+    if (P.f_count[i] == 0) continue;
+    int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    fbin[b].push_back(i);
+    load[b] += P.f_count[i];
+  }
+  // each bin walks its species in index order
+  for (auto& b : fbin) std::sort(b.begin(), b.end());
+  // rows: the longest bin, rounded up to a multiple of 4, plus 4 rows that
+  // the kernels' look-ahead loads but never processes
+  P.g_f_rows[w] = (std::max(1, *std::max_element(load.begin(), load.end())) + 3) / 4 * 4;
+  const int f_alloc = P.g_f_rows[w] + 4;
+  std::vector<int>& gf = P.gf_rec[w];
+  gf.assign((size_t)8 * G * f_alloc, 0);
+  for (int j = 0; j < G; ++j) {
+    int r = 0;
+    for (int i : fbin[j])
+      for (int q = P.f_ptr[i]; q < P.f_ptr[i + 1]; ++q, ++r) {
+        int* d = &gf[(size_t)8 * ((size_t)r * G + j)];
+        int t = P.f_term[q];
+        d[0] = i | ((q + 1 == P.f_ptr[i + 1]) ? (1 << 30) : 0);
+        d[1] = P.f_guard[q];
+        put_coef(&d[2], P.f_coef[q]);
+        d[4] = kc_of_slot[P.t_kidx[t]] | (P.t_flags[t] << 28);
+        for (int m = 0; m < 3; ++m) d[5 + m] = m < P.t_nreact[t] ? P.t_react[3 * t + m] : -1;
+      }
+    for (; r < f_alloc; ++r) {
+      int* d = &gf[(size_t)8 * ((size_t)r * G + j)];
+      d[0] = nv | (1 << 30);
+      d[1] = -1;
+      put_coef(&d[2], 0.0);
+      d[4] = 0;
+      d[5] = d[6] = d[7] = -1;
+    }
+  }
+  // J: contiguous element runs with near-equal record counts; an element
+  // without records gets one record of coefficient 0, so that every element
+  // is written
+  const int nnz = P.nnz;
+  auto nrec = [&](int e) { return std::max(1, P.j_ptr[e + 1] - P.j_ptr[e]); };
+  std::vector<long> acc(nnz + 1, 0);
+  for (int e = 0; e < nnz; ++e) acc[e + 1] = acc[e] + nrec(e);
+  const long total = acc[nnz];
+  std::vector<int> first(G + 1, nnz);
+  first[0] = 0;
+  for (int j = 1; j < G; ++j) {
+    long target = total * j / G;
+    int e = first[j - 1];
+    while (e < nnz && acc[e] < target) ++e;
+    first[j] = e;
+  }
+  int rows = 1;
+  for (int j = 0; j < G; ++j) rows = std::max(rows, (int)(acc[first[j + 1]] - acc[first[j]]));
+  P.g_j_rows[w] = (rows + 3) / 4 * 4;
+  const int j_alloc = P.g_j_rows[w] + 4;
+  std::vector<int>& gj = P.gj_rec[w];
+  gj.assign((size_t)8 * G * j_alloc, 0);
+  for (int j = 0; j < G; ++j) {
+    int r = 0;
+    for (int e = first[j]; e < first[j + 1]; ++e) {
+      if (P.j_ptr[e] == P.j_ptr[e + 1]) {
+        int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
+        d[0] = e | (1 << 30);
+        d[1] = -1;
+        put_coef(&d[2], 0.0);
+        d[4] = 0;
+        d[5] = d[6] = -1;
+        d[7] = 0;
+        ++r;
+        continue;
+      }
+      for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q, ++r) {
+        int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
+        d[0] = e | ((q + 1 == P.j_ptr[e + 1]) ? (1 << 30) : 0);
+        d[1] = P.je_b[q];
+        put_coef(&d[2], P.je_c[q]);
+        d[4] = kc_of_slot[P.je_a[4 * q]];
+        d[5] = P.je_a[4 * q + 1];
+        d[6] = P.je_a[4 * q + 2];
+        d[7] = P.je_a[4 * q + 3];
+      }
+    }
+    for (; r < j_alloc; ++r) {
+      int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
+      d[0] = nnz | (1 << 30);
+      d[1] = -1;
+      put_coef(&d[2], 0.0);
+      d[4] = 0;
+      d[5] = d[6] = -1;
+      d[7] = 0;
+    }
+  }
+}
+
 std::string Model::initialize() {
   if (n_added != n_rxn) return "not all reactions were added before solver_initialize";
   // --- reaction Jacobian pattern: diagonal + per-type used elements -------
@@ -645,6 +775,9 @@ std::string Model::initialize() {
     P.fe_rec[4 * q + 2] = (int)(uint32_t)(bits & 0xFFFFFFFFu);
     P.fe_rec[4 * q + 3] = (int)(uint32_t)(bits >> 32);
   }
+
+  if (!P.mp)
+    for (int w = 0; w < 3; ++w) build_group_program(P, w);
 
   // row view of the solver pattern (entries of each row in column order)
   P.r_ptr.assign(nv + 1, 0);

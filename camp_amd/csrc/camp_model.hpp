@@ -34,6 +34,8 @@ constexpr int kRxnFirstOrderLoss = 12;
 constexpr int kRxnEmission = 13;
 
 // Flags of a rate term.
+// lanes per cell of the lane-group f/J kernels (the tunable widths)
+constexpr int kGroupWidths[3] = {8, 16, 32};
 constexpr int kTermMassAction = 1;   // skip contributions when the rate is 0
 constexpr int kTermRoundRate = 2;    // rate product rounded to double (loss)
 constexpr int kTermSlot = 4;         // contribution read from a per-cell slot
@@ -121,6 +123,26 @@ struct Program {
   int ts_fwd_blocks = 0, ts_bwd_blocks = 0;
   std::vector<uint64_t> up_rec;
   long n_updates() const { return (long)up_dst.size(); }
+
+  // ---- lane-group f and J (gas-phase programs) ------------------------------
+  // G lanes integrate one cell's f(y) or J(y), 64 / G cells per wavefront,
+  // G one of kGroupWidths.  The rate constants the terms use are read from a
+  // compact per-cell array, k_slot[i] being the rxn_env slot of compact
+  // index i.  gf_rec: the f records of whole species, the species packed
+  // longest first into G bins (the least-loaded bin takes the next
+  // species); row r of lane j at (r * G + j), 8 ints each:
+  // {target species | last record of its species << 30, guard state id (-1
+  // none), coefficient (low, high word), compact k index | term flags << 28,
+  // reactant state ids x3 (-1 none)}; padding rows target species nv with
+  // coefficient 0.  gj_rec: the J records of contiguous element runs of
+  // near-equal record counts per lane, 8 ints each: {target element | last
+  // record of its element << 30, guard state id, coefficient (low, high),
+  // compact k index, other reactants x2 (-1 none), the differentiated
+  // reactant's state id}; padding rows target element nnz.  f_zero: the
+  // species without records (their f is 0).
+  // One program per group width kGroupWidths[w] (lanes per cell).
+  int g_f_rows[3] = {0, 0, 0}, g_j_rows[3] = {0, 0, 0}, n_k = 0;
+  std::vector<int> gf_rec[3], gj_rec[3], k_slot, f_count, f_zero;
 
   // ---- multi-phase tables (empty for gas-phase models) -------------------
   bool mp = false;

@@ -79,6 +79,7 @@ struct Arena {
   std::vector<int*> d_list, h_list;
   std::vector<double*> scratch;
   long chunk = 0;
+  int place = 0;  // the solver's cell placement when registered
   // replay pool (pinned host memory)
   long pool_n = 0;
   double *pool_state = nullptr, *pool_env = nullptr, *pool_rates = nullptr, *pool_out = nullptr;
@@ -288,9 +289,10 @@ bool Server::launch() {
     a.cell_nsteps = A.nsteps;
     a.fail_count = A.fail;
     a.wave_scratch = A.scratch[li];
+    a.place = A.place;
     int grid = 0, block = 0;
     size_t lds = 0;
-    A.S->cell_launch_shape(total, &grid, &block, &lds);
+    A.S->cell_launch_shape(total, &grid, &block, &lds, A.place);
     launch_cell_bdf(a, grid, block, lds, L.st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(L.ev, L.st));
@@ -483,7 +485,8 @@ extern "C" int camp_server_add_mechanism(void* server, void* solver_data, long m
   // allocated here, per stream, at that size
   int grid = 0, block = 0;
   size_t lds = 0;
-  S->cell_launch_shape(1L << 30, &grid, &block, &lds);
+  A->place = cell_placement(S->P, S->place);
+  S->cell_launch_shape(1L << 30, &grid, &block, &lds, A->place);
   A->chunk = 2L * grid * (block / kWave);
   size_t waves = (size_t)grid * (block / kWave);
   for (size_t li = 0; li < s->lanes.size(); ++li) {
@@ -491,8 +494,8 @@ extern "C" int camp_server_add_mechanism(void* server, void* solver_data, long m
     int* h = nullptr;
     HIP_OK(hipHostMalloc((void**)&h, (size_t)A->chunk * sizeof(int), hipHostMallocDefault));
     A->h_list.push_back(h);
-    A->scratch.push_back(S->P.mp ? dev_alloc<double>(waves * cell_bdf_wave_scratch(S->P), A->owned)
-                                 : nullptr);
+    size_t per_wave = cell_bdf_wave_scratch(S->P, A->place);
+    A->scratch.push_back(per_wave ? dev_alloc<double>(waves * per_wave, A->owned) : nullptr);
   }
   std::lock_guard<std::mutex> g(s->mu);
   if (s->n_mechs == kMaxMechs) die("camp_server_add_mechanism: too many mechanisms (16)");

@@ -44,6 +44,35 @@ void Solver::setup_device() {
   P.n_rxn = m.n_rxn;
   P.dep_state = dev_upload(G.dep_state, owned);
   P.j_ptr = dev_upload(G.j_ptr, owned);
+  P.colptr = dev_upload(G.colptr, owned);
+  P.rowval = dev_upload(G.rowval, owned);
+  P.n_k = G.n_k;
+  for (int w = 0; w < 3; ++w) {
+    P.g_f_rows[w] = G.g_f_rows[w];
+    P.g_j_rows[w] = G.g_j_rows[w];
+    P.gf_rec[w] = G.gf_rec[w].empty()
+                      ? nullptr
+                      : reinterpret_cast<const int4*>(dev_upload(G.gf_rec[w], owned));
+    P.gj_rec[w] = G.gj_rec[w].empty()
+                      ? nullptr
+                      : reinterpret_cast<const int4*>(dev_upload(G.gj_rec[w], owned));
+  }
+  P.k_slot = dev_upload(G.k_slot, owned);
+  P.f_count = dev_upload(G.f_count, owned);
+  P.n_fzero = (int)G.f_zero.size();
+  P.f_zero = dev_upload(G.f_zero, owned);
+  P.r_ptr = dev_upload(G.r_ptr, owned);
+  P.r_elem = dev_upload(G.r_elem, owned);
+  P.r_col = dev_upload(G.r_col, owned);
+  {
+    std::vector<char> dep(G.ns, 0);
+    for (int d : G.dep_state) dep[d] = 1;
+    std::vector<int> cs;
+    for (int i = 0; i < G.ns; ++i)
+      if (!dep[i]) cs.push_back(i);
+    P.n_cs = (int)cs.size();
+    P.cs_ids = dev_upload(cs, owned);
+  }
   P.perm = dev_upload(G.perm, owned);
   P.m_to_lu = dev_upload(G.m_to_lu, owned);
   P.diag_pos = dev_upload(G.diag_pos, owned);
@@ -172,12 +201,61 @@ void Solver::upload_renv() {
   renv_dirty = false;
 }
 
+void check_jac_device(Solver* s, const double* d_y, const double* d_state, const double* d_env,
+                      const double* d_J, double time_step, int* d_fails);
+
+void Solver::refresh_kval() {
+  if (P.mp || P.n_k == 0) return;
+  hipLaunchKernelGGL(k_kval, dim3(nblk((long)m.nc * P.n_k)), dim3(256), 0, stream, P, m.nc, d_renv,
+                     kval());
+}
+
+void Solver::launch_deriv(const double* y, const double* st, const double* env, double* out,
+                          double h, int use_est, int* fail) {
+  int nc = m.nc;
+  if (!P.mp) {
+    launch_group_deriv(f_lanes, P, nc, y, st, kval(), d_Js, d_Jy, d_Jf, out, h, use_est, fail,
+                       stream);
+    return;
+  }
+  int wpb = fj_waves_per_block(P);
+  size_t lds = wpb * fj_lds_doubles(P) * sizeof(double);
+  hipLaunchKernelGGL(k_deriv, dim3(std::min(nblk(nc, wpb), 4096)), dim3(64 * wpb), lds, stream, P,
+                     nc, y, st, env, d_renv, d_Js, d_Jy, d_Jf, out, h, use_est, fail);
+}
+
+void Solver::launch_jac(const double* y, const double* st, const double* env, double* J, double h,
+                        double* jr) {
+  int nc = m.nc;
+  if (!P.mp) {
+    launch_group_jac(j_lanes, P, nc, y, st, kval(), J, h, stream);
+    return;
+  }
+  int wpb = fj_waves_per_block(P);
+  size_t lds = wpb * fj_lds_doubles(P) * sizeof(double);
+  hipLaunchKernelGGL(k_jac, dim3(std::min(nblk(nc, wpb), 4096)), dim3(64 * wpb), lds, stream, P,
+                     nc, y, st, env, d_renv, J, h, jr);
+}
+
 int Solver::run(double* st, double* env, double t0, double t1) {
   upload_renv();
   if (m.n_rxn == 0) return CAMP_SOLVER_SUCCESS;  // no_solve, camp_solver.c:240
   if (mode == CAMP_MODE_BLOCK && P.mp)
     die("multi-phase mechanisms run in the per-cell mode (CAMP_MODE_PER_CELL)");
-  return mode == CAMP_MODE_BLOCK ? run_block(st, env, t0, t1) : run_cells(st, env, t0, t1);
+  int rc = mode == CAMP_MODE_BLOCK ? run_block(st, env, t0, t1) : run_cells(st, env, t0, t1);
+  if (eval_jac) {
+    // solver_set_eval_jac: each cell's last Jacobian (its J history) against
+    // difference quotients of f at the state it was evaluated at; a cell
+    // that fails counts as one Jacobian evaluation failure
+    if (!d_jac_fails) d_jac_fails = dev_alloc<int>((size_t)m.nc, owned);
+    check_jac_device(this, d_Jy, st, env, d_Js, t1 - t0, d_jac_fails);
+    std::vector<int> f(m.nc);
+    HIP_OK(hipMemcpyAsync(f.data(), d_jac_fails, (size_t)m.nc * sizeof(int),
+                          hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    for (int c = 0; c < m.nc; ++c) jac_eval_fails += f[c] != 0;
+  }
+  return rc;
 }
 
 CellBdfArgs Solver::cell_args() const {
@@ -188,17 +266,19 @@ CellBdfArgs Solver::cell_args() const {
   a.max_steps = m.max_steps;
   a.max_conv_fails = m.max_conv_fails;
   a.guess_on = guess_on;
+  a.place = cell_placement(P, place);
   return a;
 }
 
-void Solver::cell_launch_shape(long nq, int* grid, int* block, size_t* lds) const {
+void Solver::cell_launch_shape(long nq, int* grid, int* block, size_t* lds, int at_place) const {
   int cpb = cells_per_block;
+  const int place = at_place < 0 ? this->place : at_place;
   *block = kWave * cpb;
-  *lds = (size_t)cpb * cell_bdf_lds_doubles(P) * sizeof(double);
+  *lds = (size_t)cpb * cell_bdf_lds_doubles(P, place) * sizeof(double);
   int dev = 0, ncu = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  int per_cu = cell_bdf_blocks_per_cu(P.mp != 0, *block, *lds);
+  int per_cu = cell_bdf_blocks_per_cu(P, place, *block, *lds);
   if (per_cu <= 0) die("per-cell kernel does not fit on a compute unit (LDS)");
   *grid = (int)std::max<long>(1, std::min<long>((long)per_cu * ncu, (nq + cpb - 1) / cpb));
 }
@@ -225,7 +305,7 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   int grid = 0, block = 0;
   size_t lds = 0;
   cell_launch_shape(m.nc, &grid, &block, &lds);
-  a.wave_scratch = wave_scratch((size_t)grid * (block / kWave));
+  a.wave_scratch = wave_scratch((size_t)grid * (block / kWave), place);
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
@@ -258,6 +338,26 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   }
   bstats = s;
   stats_from_cells = true;
+  if (!explicit_order) {
+    // Next call's queue order: the cells by this call's BDF step count,
+    // most first (a counting sort on quarter octaves), so that a launch's
+    // tail holds the cheap cells.  A cell's result does not depend on the
+    // order or on which wavefront takes it.
+    constexpr int kKeys = 128;
+    auto key = [&](int c) {
+      double n = stats[(size_t)c * kCellStats];
+      int k = n > 0.0 ? (int)(4.0 * std::log2(1.0 + n)) : 0;
+      return kKeys - 1 - std::min(kKeys - 1, std::max(0, k));
+    };
+    std::vector<int> start(kKeys + 1, 0);
+    for (int c = 0; c < m.nc; ++c) start[key(c) + 1]++;
+    for (int k = 0; k < kKeys; ++k) start[k + 1] += start[k];
+    h_order.resize(m.nc);
+    for (int c = 0; c < m.nc; ++c) h_order[start[key(c)]++] = c;
+    if (!d_order) d_order = dev_alloc<int>((size_t)m.nc, owned);
+    HIP_OK(hipMemcpy(d_order, h_order.data(), (size_t)m.nc * sizeof(int),
+                     hipMemcpyHostToDevice));
+  }
   if (profiling) {
     std::vector<double> pr((size_t)m.nc * kCellProf);
     HIP_OK(hipMemcpy(pr.data(), d_prof, pr.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -376,10 +476,7 @@ struct BlockCv {
   int deriv(const double* yv, double* out, int use_est) {
     double ts = next_h > 0.0 ? next_h : init_h;
     HIP_OK(hipMemsetAsync(S.d_fail, 0, sizeof(int), s));
-    size_t lds = 4 * fj_lds_doubles(S.P) * sizeof(double);
-    hipLaunchKernelGGL(k_deriv, dim3(std::min(nblk(nc, 4), 4096)), dim3(256), lds, s, S.P, nc,
-                       yv, st, (const double*)nullptr, S.d_renv, S.d_Js, S.d_Jy, S.d_Jf, out, ts,
-                       use_est, S.d_fail);
+    S.launch_deriv(yv, st, nullptr, out, ts, use_est, S.d_fail);
     return check_fail();
   }
   int f(const double* yv, double* out) {
@@ -389,9 +486,7 @@ struct BlockCv {
   int jac(const double* yv, double* fy) {
     if (deriv(yv, fy, 0)) return 1;
     double ts = next_h > 0.0 ? next_h : init_h;
-    size_t lds = 4 * fj_lds_doubles(S.P) * sizeof(double);
-    hipLaunchKernelGGL(k_jac, dim3(std::min(nblk(nc, 4), 4096)), dim3(256), lds, s, S.P, nc, yv,
-                       st, (const double*)nullptr, S.d_renv, S.d_Js, ts, (double*)nullptr);
+    S.launch_jac(yv, st, nullptr, S.d_Js, ts, nullptr);
     copy(S.d_Jy, yv);
     copy(S.d_Jf, fy);
     return 0;
@@ -835,6 +930,7 @@ int Solver::run_block(double* st, double* env, double t0, double t1) {
                      d_var_type, st);
   hipLaunchKernelGGL(k_rates, dim3(nblk((long)nc * m.n_rxn)), dim3(256), 0, stream, P, nc, env,
                      d_renv);
+  refresh_kval();
   hipLaunchKernelGGL(k_state_gather, dim3(nblk((long)nc * P.nv)), dim3(256), 0, stream, P, nc, st,
                      d_zn);
   BlockCv cv(*this, st);
@@ -1174,18 +1270,35 @@ extern "C" int camp_solver_set_cells_per_block(void* solver_data, int cpb) {
   Solver* s = SD(solver_data);
   if (cpb == 0) cpb = 1;
   if (cpb == 1 || cpb == 2 || cpb == 4 || cpb == 8 || cpb == 16 || cpb == 32) {
-    size_t lds = (size_t)cpb * cell_bdf_lds_doubles(s->P) * sizeof(double);
-    if (!s->m.initialized || !s->stream || cell_bdf_blocks_per_cu(s->P.mp != 0, kWave * cpb, lds) > 0)
+    size_t lds = (size_t)cpb * cell_bdf_lds_doubles(s->P, s->place) * sizeof(double);
+    if (!s->m.initialized || !s->stream ||
+        cell_bdf_blocks_per_cu(s->P, s->place, kWave * cpb, lds) > 0)
       s->cells_per_block = cpb;
   }
   return s->cells_per_block;
 }
 
-// Autotuner for the persistent per-cell kernel: every cells-per-workgroup
-// candidate in {1, 2, 4, 8, 16, 32} that fits a compute unit solves the same
-// sample of cells (every (nc / n_sample)-th cell, copied with its rate data
-// and J history, so the solver's own cells are untouched) over [t0, t1];
-// the fastest by HIP events on the solver's stream is kept.
+extern "C" int camp_solver_set_eval_kernels(void* solver_data, int group) {
+  camp::Solver* s = SD(solver_data);
+  int old = s->eval_group ? 1 : 0;
+  s->eval_group = group != 0;
+  return old;
+}
+
+extern "C" int camp_solver_set_placement(void* solver_data, int place) {
+  using namespace camp;
+  Solver* s = SD(solver_data);
+  if (place == kPlaceLds || place == kPlaceGlobal) s->place = place;
+  return cell_placement(s->P, s->place);
+}
+
+// Autotuner for the persistent per-cell kernel.  Candidates: the cell
+// placement (history in LDS: fewer cells per CU; history in global memory:
+// more cells per CU, two wavefronts per SIMD) times cells per workgroup
+// {1, 2, 4, 8, 16, 32}; every candidate that fits a compute unit solves the
+// same sample of cells (every (nc / n_sample)-th cell, copied with its rate
+// data and J history, so the solver's own cells are untouched) over
+// [t0, t1], and the fastest by HIP events on the solver's stream is kept.
 extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
                                     const double* d_env, double t0, double t1, int n_sample,
                                     double* ms_out) {
@@ -1219,54 +1332,64 @@ extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
                        src, dst);
   };
   const int cand[6] = {1, 2, 4, 8, 16, 32};
-  int best = s->cells_per_block;
+  const int keep_cpb = s->cells_per_block, keep_place = s->place;
+  int best = keep_cpb, best_place = cell_placement(P, keep_place);
   double best_ms = -1.0;
-  for (int k = 0; k < 6; ++k) {
-    int cpb = cand[k];
-    size_t lds = (size_t)cpb * cell_bdf_lds_doubles(P) * sizeof(double);
-    if (ms_out) ms_out[k] = -1.0;
-    if (cell_bdf_blocks_per_cu(P.mp != 0, kWave * cpb, lds) <= 0) continue;
-    gather(d_state, P.ns, st);
-    gather(d_env, 2, env);
-    gather(s->d_renv, P.n_env, renv);
-    gather(s->d_Js, P.nnz, js);
-    gather(s->d_Jy, P.nv, jy);
-    gather(s->d_Jf, P.nv, jf);
-    CellBdfArgs a = s->cell_args();
-    a.nc = (int)ns;
-    a.state = st;
-    a.env = env;
-    a.rxn_env = renv;
-    a.Js = js;
-    a.Jy = jy;
-    a.Jf = jf;
-    a.t0 = t0;
-    a.t1 = t1;
-    a.counter = s->d_counter;
-    a.stats = stats;
-    int keep = s->cells_per_block;
-    s->cells_per_block = cpb;
-    int grid = 0, block = 0;
-    s->cell_launch_shape(ns, &grid, &block, &lds);
-    s->cells_per_block = keep;
-    a.wave_scratch = s->wave_scratch((size_t)grid * (block / kWave));
-    HIP_OK(hipMemsetAsync(s->d_counter, 0, sizeof(int), s->stream));
-    HIP_OK(hipEventRecord(s->ev0, s->stream));
-    launch_cell_bdf(a, grid, block, lds, s->stream);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipEventRecord(s->ev1, s->stream));
-    HIP_OK(hipStreamSynchronize(s->stream));
-    float ms = 0.f;
-    HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-    if (ms_out) ms_out[k] = ms;
-    if (best_ms < 0.0 || ms < best_ms) {
-      best_ms = ms;
-      best = cpb;
+  for (int pl = 0; pl < 2; ++pl) {
+    const int place = pl == 0 ? kPlaceLds : kPlaceGlobal;
+    for (int k = 0; k < 6; ++k) {
+      double* slot = ms_out ? &ms_out[pl * 6 + k] : nullptr;
+      if (slot) *slot = -1.0;
+      // multi-phase programs have one placement
+      if (cell_placement(P, place) != place) continue;
+      int cpb = cand[k];
+      size_t lds = (size_t)cpb * cell_bdf_lds_doubles(P, place) * sizeof(double);
+      if (cell_bdf_blocks_per_cu(P, place, kWave * cpb, lds) <= 0) continue;
+      gather(d_state, P.ns, st);
+      gather(d_env, 2, env);
+      gather(s->d_renv, P.n_env, renv);
+      gather(s->d_Js, P.nnz, js);
+      gather(s->d_Jy, P.nv, jy);
+      gather(s->d_Jf, P.nv, jf);
+      s->cells_per_block = cpb;
+      s->place = place;
+      CellBdfArgs a = s->cell_args();
+      a.nc = (int)ns;
+      a.state = st;
+      a.env = env;
+      a.rxn_env = renv;
+      a.Js = js;
+      a.Jy = jy;
+      a.Jf = jf;
+      a.t0 = t0;
+      a.t1 = t1;
+      a.counter = s->d_counter;
+      a.stats = stats;
+      int grid = 0, block = 0;
+      s->cell_launch_shape(ns, &grid, &block, &lds);
+      a.wave_scratch = s->wave_scratch((size_t)grid * (block / kWave), place);
+      s->cells_per_block = keep_cpb;
+      s->place = keep_place;
+      HIP_OK(hipMemsetAsync(s->d_counter, 0, sizeof(int), s->stream));
+      HIP_OK(hipEventRecord(s->ev0, s->stream));
+      launch_cell_bdf(a, grid, block, lds, s->stream);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipEventRecord(s->ev1, s->stream));
+      HIP_OK(hipStreamSynchronize(s->stream));
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+      if (slot) *slot = ms;
+      if (best_ms < 0.0 || ms < best_ms) {
+        best_ms = ms;
+        best = cpb;
+        best_place = place;
+      }
     }
   }
   for (void* p : tmp) HIP_OK(hipFree(p));
   s->cells_per_block = best;
-  return best;
+  s->place = best_place;
+  return best + 100 * best_place;
 }
 
 extern "C" void camp_solver_set_guess_helper(void* solver_data, int on) { SD(solver_data)->guess_on = on != 0; }
@@ -1276,6 +1399,7 @@ extern "C" int camp_solver_set_cell_order(void* solver_data, const int* order, i
   Solver* s = SD(solver_data);
   if (!order || n == 0) {
     s->h_order.clear();
+    s->explicit_order = false;
     return 0;
   }
   if (n != s->m.nc) return 1;
@@ -1287,6 +1411,7 @@ extern "C" int camp_solver_set_cell_order(void* solver_data, const int* order, i
   s->ensure_device();
   if (!s->d_order) s->d_order = dev_alloc<int>((size_t)n, s->owned);
   s->h_order.assign(order, order + n);
+  s->explicit_order = true;
   HIP_OK(hipMemcpy(s->d_order, s->h_order.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
   return 0;
 }
@@ -1367,11 +1492,16 @@ extern "C" int camp_eval_deriv(void* solver_data, const double* state, const dou
   double* d_f = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
   eval_prepare(s, state, env, d_y);
   HIP_OK(hipMemsetAsync(s->d_fail, 0, sizeof(int), s->stream));
-  int wpb = camp::fj_waves_per_block(P);
-  size_t lds = wpb * camp::fj_lds_doubles(P) * sizeof(double);
-  hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, wpb), 4096)), dim3(64 * wpb), lds,
-                     s->stream, P, nc, d_y, s->d_state, s->d_env, s->d_renv, s->d_Js, s->d_Jy,
-                     s->d_Jf, d_f, time_step, use_deriv_est, s->d_fail);
+  if (s->eval_group && !P.mp) {
+    s->refresh_kval();
+    s->launch_deriv(d_y, s->d_state, s->d_env, d_f, time_step, use_deriv_est, s->d_fail);
+  } else {
+    int wpb = camp::fj_waves_per_block(P);
+    size_t lds = wpb * camp::fj_lds_doubles(P) * sizeof(double);
+    hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, wpb), 4096)), dim3(64 * wpb),
+                       lds, s->stream, P, nc, d_y, s->d_state, s->d_env, s->d_renv, s->d_Js,
+                       s->d_Jy, s->d_Jf, d_f, time_step, use_deriv_est, s->d_fail);
+  }
   int fail = 0;
   HIP_OK(hipMemcpyAsync(&fail, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipMemcpyAsync(deriv, d_f, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToHost,
@@ -1397,9 +1527,14 @@ extern "C" int camp_eval_jac(void* solver_data, const double* state, const doubl
   hipLaunchKernelGGL(camp::k_deriv, dim3(std::min(camp::nblk(nc, wpb), 4096)), dim3(64 * wpb), lds,
                      s->stream, P, nc, d_y, s->d_state, s->d_env, s->d_renv, s->d_Js, s->d_Jy,
                      s->d_Jf, s->d_Jf, time_step, 0, s->d_fail);
-  hipLaunchKernelGGL(camp::k_jac, dim3(std::min(camp::nblk(nc, wpb), 4096)), dim3(64 * wpb), lds,
-                     s->stream, P, nc, d_y, s->d_state, s->d_env, s->d_renv, s->d_Js, time_step,
-                     d_jr);
+  if (s->eval_group && !P.mp) {
+    s->refresh_kval();
+    s->launch_jac(d_y, s->d_state, s->d_env, s->d_Js, time_step, nullptr);
+  } else {
+    hipLaunchKernelGGL(camp::k_jac, dim3(std::min(camp::nblk(nc, wpb), 4096)), dim3(64 * wpb),
+                       lds, s->stream, P, nc, d_y, s->d_state, s->d_env, s->d_renv, s->d_Js,
+                       time_step, d_jr);
+  }
   HIP_OK(hipMemcpyAsync(s->d_Jy, d_y, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToDevice,
                         s->stream));
   int fail = 0;
@@ -1409,6 +1544,64 @@ extern "C" int camp_eval_jac(void* solver_data, const double* state, const doubl
   HIP_OK(hipStreamSynchronize(s->stream));
   for (void* p : tmp) (void)hipFree(p);
   return fail;
+}
+
+namespace camp {
+// Finite-difference check of per-cell Jacobians on the device (k_check_jac):
+// fails[c] = failing elements of cell c (-1: f() failed at its state).
+void check_jac_device(Solver* s, const double* d_y, const double* d_state, const double* d_env,
+                      const double* d_J, double time_step, int* d_fails) {
+  const DevProgram& P = s->P;
+  size_t lds = check_jac_lds_doubles(P) * sizeof(double);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)k_check_jac,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_check_jac, dim3(std::min(s->m.nc, 16384)), dim3(64), lds, s->stream, P,
+                     s->m.nc, d_y, d_state, d_env, s->d_renv, d_J, time_step, d_fails);
+  HIP_OK(hipGetLastError());
+}
+}  // namespace camp
+
+extern "C" int camp_solver_check_jac(void* solver_data, const double* state, const double* env,
+                                     double time_step, const double* jac, int* fails) {
+  Solver* s = SD(solver_data);
+  if (!s->m.initialized) return -1;
+  s->ensure_device();
+  int nc = s->m.nc;
+  const camp::DevProgram& P = s->P;
+  std::vector<void*> tmp;
+  double* d_y = camp::dev_alloc<double>((size_t)nc * P.nv, tmp);
+  int* d_fails = camp::dev_alloc<int>((size_t)nc, tmp);
+  double* d_J = s->d_Js;
+  if (jac) {
+    // the caller's Jacobian: rate constants and state staged as for J(y)
+    eval_prepare(s, state, env, d_y);
+    d_J = camp::dev_alloc<double>((size_t)nc * P.nnz, tmp);
+    HIP_OK(hipMemcpyAsync(d_J, jac, (size_t)nc * P.nnz * sizeof(double), hipMemcpyHostToDevice,
+                          s->stream));
+  } else {
+    std::vector<double> J((size_t)nc * P.nnz);
+    if (camp_eval_jac(solver_data, state, env, time_step, J.data()) != 0) {
+      for (void* p : tmp) (void)hipFree(p);
+      return -1;
+    }
+    HIP_OK(hipMemcpyAsync(d_y, s->d_Jy, (size_t)nc * P.nv * sizeof(double),
+                          hipMemcpyDeviceToDevice, s->stream));
+  }
+  camp::check_jac_device(s, d_y, s->d_state, s->d_env, d_J, time_step, d_fails);
+  std::vector<int> f(nc);
+  HIP_OK(hipMemcpyAsync(f.data(), d_fails, (size_t)nc * sizeof(int), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (void* p : tmp) (void)hipFree(p);
+  long total = 0;
+  bool fbad = false;
+  for (int c = 0; c < nc; ++c) {
+    if (fails) fails[c] = f[c];
+    if (f[c] < 0) fbad = true;
+    else total += f[c];
+  }
+  return fbad ? -1 : (int)std::min<long>(total, std::numeric_limits<int>::max());
 }
 
 extern "C" int camp_eval_lu_solve(void* solver_data, const double* jac, double gamma, double* b) {
@@ -1490,9 +1683,13 @@ extern "C" void camp_solver_work_model(void* solver_data, double* out8) {
   out8[7] = 8.0 * (G.lu_nnz + 2.0 * G.nv);    // solve: read LU, b; write x
 }
 
-extern "C" int camp_solver_time_fj(void* solver_data, const double* d_state, const double* d_env,
-                                   int reps, double* ms_deriv, double* ms_jac) {
-  Solver* s = SD(solver_data);
+// f(y) and J(y) of the block path timed on device-resident data (HIP
+// events on the solver's stream, `reps` launches after one untimed).  With
+// tune, every lanes-per-cell width (8, 16, 32) of the lane-group kernels is
+// timed and the fastest kept for f and for J; ms_out then holds
+// [f at 8, 16, 32, J at 8, 16, 32].
+static int time_fj_impl(Solver* s, const double* d_state, const double* d_env, int reps,
+                        double* ms_deriv, double* ms_jac, bool tune, double* ms_out) {
   s->ensure_device();
   int nc = s->m.nc;
   const camp::DevProgram& P = s->P;
@@ -1503,32 +1700,65 @@ extern "C" int camp_solver_time_fj(void* solver_data, const double* d_state, con
   s->upload_renv();
   hipLaunchKernelGGL(camp::k_rates, dim3(camp::nblk((long)nc * s->m.n_rxn)), dim3(256), 0,
                      s->stream, P, nc, d_env, s->d_renv);
+  s->refresh_kval();
   hipLaunchKernelGGL(camp::k_state_gather, dim3(camp::nblk((long)nc * P.nv)), dim3(256), 0,
                      s->stream, P, nc, d_state, d_y);
-  int wpb = camp::fj_waves_per_block(P);
-  size_t lds = wpb * camp::fj_lds_doubles(P) * sizeof(double);
-  int grid = std::min(camp::nblk(nc, wpb), 1 << 20);
   double* d_jr = P.mp ? camp::dev_alloc<double>((size_t)nc * P.rnnz, tmp) : nullptr;
-  float ms = 0.f;
-  HIP_OK(hipEventRecord(s->ev0, s->stream));
-  for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(camp::k_deriv, dim3(grid), dim3(64 * wpb), lds, s->stream, P, nc, d_y, d_state,
-                       d_env, s->d_renv, s->d_Js, s->d_Jy, s->d_Jf, d_f, 1.0, 0, s->d_fail);
-  HIP_OK(hipEventRecord(s->ev1, s->stream));
-  HIP_OK(hipEventSynchronize(s->ev1));
-  HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-  *ms_deriv = ms / reps;
-  HIP_OK(hipEventRecord(s->ev0, s->stream));
-  for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(camp::k_jac, dim3(grid), dim3(64 * wpb), lds, s->stream, P, nc, d_y, d_state,
-                       d_env, s->d_renv, d_J, 1.0, d_jr);
-  HIP_OK(hipEventRecord(s->ev1, s->stream));
-  HIP_OK(hipEventSynchronize(s->ev1));
-  HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-  *ms_jac = ms / reps;
+  auto time = [&](auto&& launch) {
+    float ms = 0.f;
+    launch();  // one untimed launch
+    HIP_OK(hipEventRecord(s->ev0, s->stream));
+    for (int r = 0; r < reps; ++r) launch();
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
+    HIP_OK(hipEventSynchronize(s->ev1));
+    HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    return (double)ms / reps;
+  };
+  auto run_f = [&] { s->launch_deriv(d_y, d_state, d_env, d_f, 1.0, 0, s->d_fail); };
+  auto run_j = [&] { s->launch_jac(d_y, d_state, d_env, d_J, 1.0, d_jr); };
+  if (tune && !P.mp) {
+    const int widths[3] = {8, 16, 32};
+    double bf = -1.0, bj = -1.0;
+    int wf = s->f_lanes, wj = s->j_lanes;
+    for (int w = 0; w < 3; ++w) {
+      s->f_lanes = s->j_lanes = widths[w];
+      double mf = time(run_f), mj = time(run_j);
+      if (ms_out) {
+        ms_out[w] = mf;
+        ms_out[3 + w] = mj;
+      }
+      if (bf < 0.0 || mf < bf) { bf = mf; wf = widths[w]; }
+      if (bj < 0.0 || mj < bj) { bj = mj; wj = widths[w]; }
+    }
+    s->f_lanes = wf;
+    s->j_lanes = wj;
+  }
+  ms_deriv[0] = time(run_f);
+  ms_jac[0] = time(run_j);
   HIP_OK(hipStreamSynchronize(s->stream));
   for (void* p : tmp) (void)hipFree(p);
   return 0;
+}
+
+extern "C" int camp_solver_time_fj(void* solver_data, const double* d_state, const double* d_env,
+                                   int reps, double* ms_deriv, double* ms_jac) {
+  return time_fj_impl(SD(solver_data), d_state, d_env, reps, ms_deriv, ms_jac, false, nullptr);
+}
+
+extern "C" int camp_solver_set_fj_lanes(void* solver_data, int f_lanes, int j_lanes) {
+  camp::Solver* s = SD(solver_data);
+  auto ok = [](int w) { return w == 8 || w == 16 || w == 32; };
+  if (ok(f_lanes)) s->f_lanes = f_lanes;
+  if (ok(j_lanes)) s->j_lanes = j_lanes;
+  return s->f_lanes + 100 * s->j_lanes;
+}
+
+extern "C" int camp_solver_tune_fj(void* solver_data, const double* d_state, const double* d_env,
+                                   int reps, double* ms_out) {
+  camp::Solver* s = SD(solver_data);
+  double a = 0.0, b = 0.0;
+  time_fj_impl(s, d_state, d_env, reps, &a, &b, true, ms_out);
+  return s->f_lanes + 100 * s->j_lanes;
 }
 
 extern "C" void camp_solver_get_rxn_env(void* solver_data, double* rxn_env) {

@@ -60,6 +60,7 @@ struct Solver {
   Model m;
   int mode = CAMP_MODE_PER_CELL;
   int cells_per_block = 1;
+  int place = kPlaceLds;  // per-cell kernel variant (cell_placement)
   int guess_on = 1;
   bool renv_dirty = true;
   hipStream_t stream = nullptr;
@@ -72,19 +73,42 @@ struct Solver {
   double *d_Js = nullptr, *d_Jy = nullptr, *d_Jf = nullptr, *d_stats = nullptr;
   int *d_counter = nullptr, *d_fail = nullptr, *d_var_type = nullptr;
   // Queue order of the per-cell kernel (a permutation of the cells; empty:
-  // index order).  Results do not depend on it; it only decides which cells
-  // the launch starts first.
+  // index order): the caller's (explicit_order), else longest first by the
+  // previous call's step counts.  Results do not depend on it; it only
+  // decides which cells the launch starts first.
   std::vector<int> h_order;
   int* d_order = nullptr;
+  bool explicit_order = false;
   int poison = 0;  // CellBdfArgs::poison for per-cell runs
   // solver_set_eval_jac: finite-difference check of every evaluated Jacobian
   bool eval_jac = false;
   long jac_eval_fails = 0;
+  int* d_jac_fails = nullptr;
+  // camp_eval_deriv / camp_eval_jac with the lane-group kernels (the plain
+  // path's) instead of the per-wavefront ones (the per-cell kernel's code)
+  bool eval_group = false;
+  // lanes per cell of the block path's f and J kernels (8, 16 or 32)
+  int f_lanes = 16, j_lanes = 16;
   // block-path buffers
   double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_y = nullptr,
          *d_tempv = nullptr, *d_ftemp = nullptr, *d_last = nullptr, *d_acinit = nullptr,
          *d_gh1 = nullptr, *d_gh2 = nullptr, *d_LU = nullptr, *d_red = nullptr,
          *d_jtmp = nullptr;
+  // compact rate constants of the lane-group f/J kernels (gas-phase
+  // programs): [nc][n_k], refreshed with the rate constants
+  double* d_kval = nullptr;
+  double* kval() {
+    if (!d_kval) d_kval = dev_alloc<double>((size_t)m.nc * std::max(P.n_k, 1), owned);
+    return d_kval;
+  }
+  // f(y) and J(y) of all cells on the solver's stream (lane-group kernels for
+  // gas-phase programs, one wavefront per cell for multi-phase programs);
+  // refresh_kval after the rate constants change
+  void refresh_kval();
+  void launch_deriv(const double* y, const double* st, const double* env, double* out, double h,
+                    int use_est, int* fail);
+  void launch_jac(const double* y, const double* st, const double* env, double* J, double h,
+                  double* jr);
   std::vector<double> h_red;
   BlockStats bstats;
   bool stats_from_cells = false;
@@ -92,15 +116,16 @@ struct Solver {
   // seconds reported as RHS_time__s / Jac_time__s
   bool profiling = false;
   double* d_prof = nullptr;
-  // per-wavefront global scratch of multi-phase programs (grow-only)
+  // per-wavefront global scratch of the kPlaceGlobal variants (grow-only)
   double* d_wave_scratch = nullptr;
-  size_t wave_scratch_waves = 0;
-  double* wave_scratch(size_t waves) {
-    if (!P.mp) return nullptr;
-    if (waves > wave_scratch_waves) {
+  size_t wave_scratch_doubles = 0;
+  double* wave_scratch(size_t waves, int at_place) {
+    size_t n = waves * cell_bdf_wave_scratch(P, at_place);
+    if (n == 0) return nullptr;
+    if (n > wave_scratch_doubles) {
       if (d_wave_scratch) HIP_OK(hipFree(d_wave_scratch));
-      HIP_OK(hipMalloc(&d_wave_scratch, waves * cell_bdf_wave_scratch(P) * sizeof(double)));
-      wave_scratch_waves = waves;
+      HIP_OK(hipMalloc(&d_wave_scratch, n * sizeof(double)));
+      wave_scratch_doubles = n;
     }
     return d_wave_scratch;
   }
@@ -124,7 +149,8 @@ struct Solver {
   // Per-cell kernel arguments with the solver's program and controls filled
   // in (buffers left to the caller), and the launch shape for nq queued cells.
   CellBdfArgs cell_args() const;
-  void cell_launch_shape(long nq, int* grid, int* block, size_t* lds) const;
+  // (at_place < 0: the solver's own placement)
+  void cell_launch_shape(long nq, int* grid, int* block, size_t* lds, int at_place = -1) const;
   int run(double* d_state_in, double* d_env_in, double t0, double t1);
   int run_cells(double* st, double* env, double t0, double t1);
   int run_block(double* st, double* env, double t0, double t1);

@@ -155,9 +155,11 @@ void aero_rep_free_update_data(void *update_data);
 
 /* replace solver_set_debug_out and solver_set_eval_jac,
  * src/camp_solver.c:502-533 (CAMP_DEBUG builds; camp_solver_data.F90:
- * 100-118): debug output is accepted and ignored; eval_jac turns on the
- * finite-difference check of every Jacobian evaluated by the solver, whose
- * failures solver_get_statistics reports as Jac_eval_fails */
+ * 100-118): debug output is accepted and ignored; eval_jac turns on a
+ * finite-difference check (camp_solver_check_jac's rule) of every cell's
+ * last Jacobian after each solver_run, and solver_get_statistics reports the
+ * cells that fail it as Jac_eval_fails (the reference checks inside every
+ * Jac() call, camp_solver.c:1116-1122) */
 int solver_set_debug_out(void *solver_data, bool do_output);
 int solver_set_eval_jac(void *solver_data, bool eval_jac);
 
@@ -175,13 +177,22 @@ int camp_solver_set_mode(void *solver_data, int mode);
  * effect. */
 int camp_solver_set_cells_per_block(void *solver_data, int cells_per_block);
 
+/* Cell placement of the per-cell kernel (gas-phase programs; multi-phase
+ * programs always use 1): 0 = each cell's Nordsieck history, work vectors
+ * and J history in LDS (fewer cells per compute unit), 1 = those in global
+ * memory (more cells per compute unit, two wavefronts per SIMD).  Returns
+ * the placement in effect.  Results do not depend on it. */
+int camp_solver_set_placement(void *solver_data, int place);
+
 /* Autotuner for the per-cell kernel: solves a sample of n_sample cells
  * (every (n_cells / n_sample)-th; 0 = up to 8192) over [t_initial, t_final]
- * from the device-resident state/env once per cells-per-workgroup candidate
- * {1, 2, 4, 8, 16, 32} that fits a compute unit, on copies (the solver's own
- * cells, rate data and J history are untouched), and keeps the fastest.
- * ms_out[6] receives each candidate's kernel time (-1: does not fit).
- * Returns the chosen cells per workgroup. */
+ * from the device-resident state/env once per candidate -- placement {0, 1}
+ * times cells per workgroup {1, 2, 4, 8, 16, 32} -- that fits a compute
+ * unit, on copies (the solver's own cells, rate data and J history are
+ * untouched), and keeps the fastest.  ms_out[12] receives each candidate's
+ * kernel time, placement-major (-1: does not fit, or not available for the
+ * program).  Returns the chosen cells per workgroup plus 100 times the chosen
+ * placement. */
 int camp_solver_autotune(void *solver_data, const double *d_state,
                          const double *d_env, double t_initial, double t_final,
                          int n_sample, double *ms_out);
@@ -191,10 +202,11 @@ int camp_solver_autotune(void *solver_data, const double *d_state,
 void camp_solver_set_guess_helper(void *solver_data, int on);
 
 /* Queue order of the per-cell kernel: order[n] is a permutation of the
- * cells, which the launch's wavefronts then take first to last (n = 0 or
- * order = NULL restores index order).  Every cell's result is independent
- * of the order.  Returns 0, or 1 if order is not a permutation of n_cells
- * cells. */
+ * cells, which the launch's wavefronts then take first to last.  n = 0 or
+ * order = NULL restores the default: index order on the first call, then
+ * the cells by the previous call's BDF step counts, most first (so that a
+ * launch ends on cheap cells).  Every cell's result is independent of the
+ * order.  Returns 0, or 1 if order is not a permutation of n_cells cells. */
 int camp_solver_set_cell_order(void *solver_data, const int *order, int n);
 
 /* Independence check of the per-cell kernel (on = 1): before it stages each
@@ -239,6 +251,26 @@ int camp_eval_deriv(void *solver_data, const double *state, const double *env,
                     double time_step, int use_deriv_est, double *deriv);
 int camp_eval_jac(void *solver_data, const double *state, const double *env,
                   double time_step, double *jac);
+/* Finite-difference Jacobian check on the GPU, check_Jac's role
+ * (camp_solver.c:1146-1257) with its pass rule: of the difference
+ * quotients of f_i at the steps h = y_j * s, s = 1e-12 .. 1 by decades
+ * (forward differences at h and h/2, Richardson-extrapolated, with a
+ * truncation plus rounding error estimate err), the one with the smallest
+ * err must agree with J_ij to within max(1.2 err, 1e-9) absolute or 1e-4
+ * relative; columns of species below 1e-30 are skipped.  jac (n_cells x nnz, CSC element order) is checked as
+ * given, or, when NULL, J(y) is evaluated first as camp_eval_jac does.
+ * fails[n_cells] (may be NULL) receives each cell's failing elements.
+ * Returns the total, or -1 when f() fails at a cell's state. */
+int camp_solver_check_jac(void *solver_data, const double *state,
+                          const double *env, double time_step,
+                          const double *jac, int *fails);
+
+/* Kernel family of camp_eval_deriv / camp_eval_jac for gas-phase
+ * mechanisms: 0 (default) the one-wavefront-per-cell f/J code the per-cell
+ * kernel runs, 1 the lane-group kernels of the block path (8 lanes per
+ * cell).  Returns the previous setting. */
+int camp_solver_set_eval_kernels(void *solver_data, int group);
+
 /* Batched LU solve test hook: factor I - gamma*J per cell (jac n_cells x
  * nnz) and solve for b (n_cells x n_dep) in place. Returns 0 or 1. */
 int camp_eval_lu_solve(void *solver_data, const double *jac, double gamma,
@@ -277,6 +309,18 @@ void camp_solver_work_model(void *solver_data, double *out8);
 int camp_solver_time_fj(void *solver_data, const double *d_state,
                         const double *d_env, int reps, double *ms_deriv,
                         double *ms_jac);
+
+/* Autotuner of the block path's f(y) and J(y) kernels (gas-phase
+ * mechanisms): times the lane-group kernels at 8, 16 and 32 lanes per cell
+ * on the device-resident state/env (reps launches each) and keeps the
+ * fastest width for f and for J.  ms_out[6] receives [f at 8, 16, 32, J at
+ * 8, 16, 32] (ms per launch).  Returns the chosen f width plus 100 times
+ * the chosen J width. */
+int camp_solver_tune_fj(void *solver_data, const double *d_state,
+                        const double *d_env, int reps, double *ms_out);
+/* Set the lanes per cell (8, 16 or 32; other values leave it) of the block
+ * path's f and J kernels; returns f lanes plus 100 times J lanes. */
+int camp_solver_set_fj_lanes(void *solver_data, int f_lanes, int j_lanes);
 
 /* Copy of the device-side per-cell reaction environment data (n_cells x
  * n_rxn_env_param: rate constants and base rates) as last computed. */

@@ -45,11 +45,24 @@ def test_rate_constants_match_oracle(oracle_lib):
     assert np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz])) <= F_RTOL
 
 
+# "wave": one wavefront per cell, the f/J code the per-cell kernel runs;
+# 8, 16, 32: the block path's lane-group kernels at that many lanes per cell
+KERNELS = ["wave", 8, 16, 32]
+
+
+def _use(sol, kernels):
+    sol.set_eval_kernels(kernels != "wave")
+    if kernels != "wave":
+        sol.set_fj_lanes(kernels, kernels)
+
+
+@pytest.mark.parametrize("kernels", KERNELS)
 @pytest.mark.parametrize("use_est", [False, True])
-def test_deriv_matches_oracle(oracle_lib, use_est):
-    n = 128
+def test_deriv_matches_oracle(oracle_lib, use_est, kernels):
+    n = 131  # not a multiple of the lane-group kernels' 8 cells per wavefront
     mech, cells = cb05_inputs(n)
     sol = CampSolver(mech, n)
+    _use(sol, kernels)
     orc = oracle_lib.Oracle(mech, n)
     _set_rates_both(sol, orc, cells.rates)
     st = _random_states(mech, n)
@@ -70,10 +83,12 @@ def test_deriv_matches_oracle(oracle_lib, use_est):
     assert not bad.any(), "max rel err %g" % np.max(err / np.maximum(scale, 1e-300))
 
 
-def test_jac_matches_oracle(oracle_lib):
-    n = 128
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_jac_matches_oracle(oracle_lib, kernels):
+    n = 131
     mech, cells = cb05_inputs(n)
     sol = CampSolver(mech, n)
+    _use(sol, kernels)
     orc = oracle_lib.Oracle(mech, n)
     _set_rates_both(sol, orc, cells.rates)
     st = _random_states(mech, n)
@@ -85,9 +100,11 @@ def test_jac_matches_oracle(oracle_lib):
     assert not (err > F_RTOL * np.abs(j_ref)).any(), "max abs err %g" % err.max()
 
 
-def test_negative_state_fails_like_reference(oracle_lib):
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_negative_state_fails_like_reference(oracle_lib, kernels):
     mech, cells = cb05_inputs(4)
     sol = CampSolver(mech, 4)
+    _use(sol, kernels)
     st = cells.state.copy()
     st[2, 3] = -1e-20
     rc, _ = sol.eval_deriv(st, cells.env)
@@ -223,9 +240,10 @@ def test_cell_alone_equals_cell_in_mix():
 
 
 def test_autotune_picks_a_fitting_candidate_and_leaves_cells_untouched():
-    """The autotuner times every cells-per-workgroup candidate that fits on
-    copies of a sample; afterwards the solver's cells solve exactly as on a
-    solver set to the chosen value, and as with one cell per workgroup."""
+    """The autotuner times every candidate (cell placement x cells per
+    workgroup) that fits on copies of a sample; afterwards the solver's
+    cells solve exactly as on a solver set to the chosen candidate, and as
+    with each placement at one cell per workgroup."""
     import torch
     mech, cells = cb05_inputs(2048, night_fraction=0.5, stressed_fraction=0.1)
     dev = torch.device("cuda:0")
@@ -235,17 +253,20 @@ def test_autotune_picks_a_fitting_candidate_and_leaves_cells_untouched():
     st = torch.from_numpy(cells.state).to(dev)
     chosen, ms = tuned.autotune(st.data_ptr(), env.data_ptr(), 0.0, 180.0, n_sample=1024)
     print("autotune", chosen, ms)
-    assert chosen in (1, 2, 4, 8, 16, 32) and ms[chosen] is not None
-    assert ms[1] is not None and ms[32] is None  # 32 wavefronts exceed the kernel's block bound
+    assert ms[chosen] is not None
+    # both placements run at one cell per workgroup; LDS rules out 8 and more
+    assert ms[(0, 1)] is not None and ms[(1, 1)] is not None
+    assert ms[(0, 8)] is None and ms[(1, 32)] is None
     assert torch.equal(st, torch.from_numpy(cells.state).to(dev))
     assert tuned.solve_device(st.data_ptr(), env.data_ptr(), 0.0, 180.0) == 0
-    for cpb in (chosen, 1):
+    for place, cpb in (chosen, (0, 1), (1, 1), (1, 2)):
         ref = CampSolver(mech, 2048)
         ref.set_rates(cells.rates)
+        assert ref.set_placement(place) == place
         assert ref.set_cells_per_block(cpb) == cpb
         s2 = torch.from_numpy(cells.state).to(dev)
         assert ref.solve_device(s2.data_ptr(), env.data_ptr(), 0.0, 180.0) == 0
-        assert torch.equal(st, s2)
+        assert torch.equal(st, s2), (place, cpb)
 
 
 def test_deriv_estimate_and_blend_match_oracle_on_affine_mechanism(oracle_lib):
