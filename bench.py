@@ -396,28 +396,34 @@ def main():
     hbm_bytes_per_launch = 8.0 * args.cells * (2 * mech.n_state + 2 + 2 * n_env
                                                + 2 * (solver.nnz + 2 * solver.nv))
     traffic = recorded_traffic(args.cells)
-    roofline = {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
+    traffic = traffic or {}
+    roofline = {"bound": "fp64_valu", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS,
-                "traffic": traffic["k_cell_bdf"] if traffic else None,
+                "traffic": traffic.get("k_cell_bdf"),
                 "kernel": "k_cell_bdf", "avg_launch_ms": kernel_s * 1e3 / args.steps,
                 "flops_per_launch": total_flops / args.steps,
                 "algorithmic_hbm_bytes_per_launch": hbm_bytes_per_launch,
                 "note": "FP64 VALU work (peak 78.6 TFLOP/s is the FP64 figure); the cell "
                         "stays in LDS, HBM moves only state, rates and J history"}
 
-    # plain-path f/J kernels against HBM
+    # plain-path f/J kernels against HBM (lanes per cell tuned first)
+    fj_lanes, fj_cand = solver.tune_fj(st.data_ptr(), env.data_ptr(), reps=10)
     ms_f, ms_j = solver.time_fj(st.data_ptr(), env.data_ptr(), reps=10)
     gbs_f = args.cells * bytes_["f"] / (ms_f / 1e3) / 1e9
     gbs_j = args.cells * bytes_["jac"] / (ms_j / 1e3) / 1e9
     roofline_fj = {
         "deriv": {"bound": "hbm", "achieved": gbs_f, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": gbs_f / HBM_PEAK_GBS, "avg_launch_ms": ms_f,
-                  "bytes_per_cell": bytes_["f"],
-                  "traffic": traffic["k_deriv"] if traffic else None},
+                  "bytes_per_cell": bytes_["f"], "lanes_per_cell": fj_lanes[0],
+                  "candidate_ms": fj_cand["f"],
+                  "traffic": traffic.get("k_deriv_g") if traffic else None},
         "jac": {"bound": "hbm", "achieved": gbs_j, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": gbs_j / HBM_PEAK_GBS, "avg_launch_ms": ms_j,
-                "bytes_per_cell": bytes_["jac"],
-                "traffic": traffic["k_jac"] if traffic else None}}
+                "bytes_per_cell": bytes_["jac"], "lanes_per_cell": fj_lanes[1],
+                "candidate_ms": fj_cand["jac"],
+                "traffic": traffic.get("k_jac_g") if traffic else None},
+        "what": "the block path's lane-group f(y) and J(y) kernels (no blend), rate "
+                "constants already computed, on the 100k-cell workload's spun-up state"}
 
     plain = None
     if args.plain_cells > 0:
