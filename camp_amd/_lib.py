@@ -65,6 +65,7 @@ SIGNATURES = {
     "camp_solver_set_eval_kernels": (_i, [_p, _i]),
     "camp_solver_tune_fj": (_i, [_p, _p, _p, _i, _dp]),
     "camp_solver_set_fj_lanes": (_i, [_p, _i, _i]),
+    "camp_solver_run_device_noalloc": (_i, [_p, _p, _p, _d, _d]),
     "camp_solver_get_cell_stats": (None, [_p, _dp]),
     "camp_solver_last_kernel_ms": (_d, [_p]),
     "camp_solver_set_profiling": (None, [_p, _i]),

@@ -10,8 +10,12 @@
 // in LDS (Nordsieck history, work vectors, rate constants, LU factors);
 // only the cell's saved Jacobian history (J_solver, J_state, J_deriv,
 // camp_solver.c:1111-1114) is in HBM, because CAMP keeps it across calls.
+#include <cstdio>
+#include <mutex>
+
 #include "camp_device.hpp"
 #include "camp_kernels.hpp"
+#include "camp_solver_impl.hpp"
 
 namespace camp {
 
@@ -871,4 +875,72 @@ int cell_bdf_blocks_per_cu(const DevProgram& P, int place, int block, size_t lds
   return n;
 }
 
+// ---------------------------------------------------------------------
+// The no-allocation entry point's scratch: the queue counter and the
+// wavefronts' global scratch of camp_solver_run_device_noalloc.  Host code
+// never uses the array's name as an address: each device's copy is looked
+// up once, with that device current, with hipGetSymbolAddress.
+// ---------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+}  // namespace
+
+// This device's copy of the scratch (nullptr, with a message, on failure).
+double* noalloc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    std::fprintf(stderr, "camp_solver_run_device_noalloc: no current HIP device\n");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      std::fprintf(stderr, "camp_solver_run_device_noalloc: hipGetSymbolAddress failed: %s\n",
+                   hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace camp
+
+// camp_solver_run_device without any allocation (see include/camp_solver.h).
+extern "C" int camp_solver_run_device_noalloc(void* solver_data, double* d_state, double* d_env,
+                                              double t_initial, double t_final) {
+  using namespace camp;
+  Solver* s = static_cast<Solver*>(solver_data);
+  if (!s || !s->m.initialized) return CAMP_NOALLOC_NOT_READY;
+  // the solver's own device buffers are made by its first device call
+  if (!s->stream) {
+    std::fprintf(stderr, "camp_solver_run_device_noalloc: the solver's device buffers do not "
+                         "exist yet (any earlier call that uses the device makes them)\n");
+    return CAMP_NOALLOC_NOT_READY;
+  }
+  if (s->mode != CAMP_MODE_PER_CELL || s->profiling || s->eval_jac) {
+    std::fprintf(stderr, "camp_solver_run_device_noalloc: per-cell mode only, without phase "
+                         "timing or the Jacobian check\n");
+    return CAMP_NOALLOC_UNSUPPORTED;
+  }
+  const size_t per_wave = cell_bdf_wave_scratch(s->P, s->place);
+  if (per_wave > kNoAllocScratchDoubles - kNoAllocHeaderDoubles) {
+    std::fprintf(stderr, "camp_solver_run_device_noalloc: one wavefront needs %zu doubles of "
+                         "scratch, more than the %zu available\n",
+                 per_wave, kNoAllocScratchDoubles - kNoAllocHeaderDoubles);
+    return CAMP_NOALLOC_TOO_LARGE;
+  }
+  double* sc = noalloc_scratch();
+  if (!sc) return CAMP_NOALLOC_NOT_READY;
+  s->upload_renv();
+  if (s->m.n_rxn == 0) return CAMP_SOLVER_SUCCESS;
+  int rc = s->run_cells(d_state, d_env, t_initial, t_final, sc, kNoAllocScratchDoubles);
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return rc;
+}

@@ -48,6 +48,12 @@ __host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P, int
   return place == kPlaceGlobal ? 13 * (size_t)P.nv : 0;
 }
 
+// The no-allocation entry point's scratch (a file-scope device array of
+// kNoAllocScratchDoubles doubles, 64 MiB): the first kNoAllocHeaderDoubles
+// (256 bytes) hold the queue counter, the rest the wavefronts' scratch.
+constexpr size_t kNoAllocScratchDoubles = (size_t)8 << 20;
+constexpr size_t kNoAllocHeaderDoubles = 32;
+
 struct CellBdfArgs {
   DevProgram P;
   int nc;

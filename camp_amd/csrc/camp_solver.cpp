@@ -192,6 +192,8 @@ void Solver::setup_device() {
   d_counter = dev_alloc<int>(1, owned);
   d_fail = dev_alloc<int>(1, owned);
   d_var_type = dev_upload(m.var_type, owned);
+  h_stats.assign(nc * kCellStats, 0.0);
+  h_order.reserve(nc);
 }
 
 void Solver::upload_renv() {
@@ -283,7 +285,8 @@ void Solver::cell_launch_shape(long nq, int* grid, int* block, size_t* lds, int 
   *grid = (int)std::max<long>(1, std::min<long>((long)per_cu * ncu, (nq + cpb - 1) / cpb));
 }
 
-int Solver::run_cells(double* st, double* env, double t0, double t1) {
+int Solver::run_cells(double* st, double* env, double t0, double t1, double* noalloc,
+                      size_t noalloc_doubles) {
   CellBdfArgs a = cell_args();
   a.nc = m.nc;
   a.state = st;
@@ -296,19 +299,32 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   a.t1 = t1;
   a.counter = d_counter;
   a.stats = d_stats;
-  if (!h_order.empty()) a.cell_list = d_order;
+  if (order_valid) a.cell_list = d_order;
   a.poison = poison;
-  if (profiling) {
+  if (profiling && !noalloc) {
     if (!d_prof) d_prof = dev_alloc<double>((size_t)m.nc * kCellProf, owned);
     a.prof = d_prof;
   }
   int grid = 0, block = 0;
   size_t lds = 0;
   cell_launch_shape(m.nc, &grid, &block, &lds);
-  a.wave_scratch = wave_scratch((size_t)grid * (block / kWave), place);
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
-  HIP_OK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
+  if (noalloc) {
+    // the queue counter in the scratch's first 256 bytes, the wavefronts'
+    // scratch after it; as many workgroups as the scratch holds
+    const size_t per_wave = cell_bdf_wave_scratch(P, place);
+    if (per_wave > 0) {
+      long waves_fit = (long)((noalloc_doubles - kNoAllocHeaderDoubles) / per_wave);
+      grid = (int)std::max<long>(1, std::min<long>(grid, waves_fit / (block / kWave)));
+    }
+    a.counter = reinterpret_cast<int*>(noalloc);
+    a.wave_scratch = per_wave > 0 ? noalloc + kNoAllocHeaderDoubles : nullptr;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, 1L, 0.0, noalloc);
+  } else {
+    a.wave_scratch = wave_scratch((size_t)grid * (block / kWave), place);
+    HIP_OK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
+  }
   HIP_OK(hipEventRecord(ev0, stream));
   launch_cell_bdf(a, grid, block, lds, stream);
   HIP_OK(hipGetLastError());
@@ -317,8 +333,9 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
   last_kernel_ms = ms;
-  std::vector<double> stats((size_t)m.nc * kCellStats);
-  HIP_OK(hipMemcpy(stats.data(), d_stats, stats.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double>& stats = h_stats;
+  HIP_OK(hipMemcpy(stats.data(), d_stats, (size_t)m.nc * kCellStats * sizeof(double),
+                   hipMemcpyDeviceToHost));
   // One integrator per cell: the counters are sums over the cells, the
   // flag is the lowest-numbered failing cell's, and the step sizes are the
   // smallest over the cells (the stiffest cell's, the step a shared
@@ -338,7 +355,7 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
   }
   bstats = s;
   stats_from_cells = true;
-  if (!explicit_order) {
+  if (!explicit_order && (d_order || !noalloc)) {
     // Next call's queue order: the cells by this call's BDF step count,
     // most first (a counting sort on quarter octaves), so that a launch's
     // tail holds the cheap cells.  A cell's result does not depend on the
@@ -349,7 +366,7 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
       int k = n > 0.0 ? (int)(4.0 * std::log2(1.0 + n)) : 0;
       return kKeys - 1 - std::min(kKeys - 1, std::max(0, k));
     };
-    std::vector<int> start(kKeys + 1, 0);
+    int start[kKeys + 1] = {0};
     for (int c = 0; c < m.nc; ++c) start[key(c) + 1]++;
     for (int k = 0; k < kKeys; ++k) start[k + 1] += start[k];
     h_order.resize(m.nc);
@@ -357,8 +374,9 @@ int Solver::run_cells(double* st, double* env, double t0, double t1) {
     if (!d_order) d_order = dev_alloc<int>((size_t)m.nc, owned);
     HIP_OK(hipMemcpy(d_order, h_order.data(), (size_t)m.nc * sizeof(int),
                      hipMemcpyHostToDevice));
+    order_valid = true;
   }
-  if (profiling) {
+  if (profiling && !noalloc) {
     std::vector<double> pr((size_t)m.nc * kCellProf);
     HIP_OK(hipMemcpy(pr.data(), d_prof, pr.size() * sizeof(double), hipMemcpyDeviceToHost));
     int khz = 0;
@@ -1400,6 +1418,7 @@ extern "C" int camp_solver_set_cell_order(void* solver_data, const int* order, i
   if (!order || n == 0) {
     s->h_order.clear();
     s->explicit_order = false;
+    s->order_valid = false;
     return 0;
   }
   if (n != s->m.nc) return 1;
@@ -1412,6 +1431,7 @@ extern "C" int camp_solver_set_cell_order(void* solver_data, const int* order, i
   if (!s->d_order) s->d_order = dev_alloc<int>((size_t)n, s->owned);
   s->h_order.assign(order, order + n);
   s->explicit_order = true;
+  s->order_valid = true;
   HIP_OK(hipMemcpy(s->d_order, s->h_order.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
   return 0;
 }

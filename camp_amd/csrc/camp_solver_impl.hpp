@@ -79,6 +79,10 @@ struct Solver {
   std::vector<int> h_order;
   int* d_order = nullptr;
   bool explicit_order = false;
+  bool order_valid = false;  // d_order holds an order for the next launch
+  // per-cell statistics copied back after each per-cell launch (sized at
+  // set-up, so that a launch allocates no host memory either)
+  std::vector<double> h_stats;
   int poison = 0;  // CellBdfArgs::poison for per-cell runs
   // solver_set_eval_jac: finite-difference check of every evaluated Jacobian
   bool eval_jac = false;
@@ -152,7 +156,11 @@ struct Solver {
   // (at_place < 0: the solver's own placement)
   void cell_launch_shape(long nq, int* grid, int* block, size_t* lds, int at_place = -1) const;
   int run(double* d_state_in, double* d_env_in, double t0, double t1);
-  int run_cells(double* st, double* env, double t0, double t1);
+  // noalloc: the no-allocation entry point's scratch (device pointer and
+  // capacity in doubles): queue counter and wavefront scratch come from it,
+  // the wavefront count is capped to fit it, and nothing is allocated
+  int run_cells(double* st, double* env, double t0, double t1, double* noalloc = nullptr,
+                size_t noalloc_doubles = 0);
   int run_block(double* st, double* env, double t0, double t1);
 };
 

@@ -222,6 +222,28 @@ void camp_solver_set_poison(void *solver_data, int on);
 int camp_solver_run_device(void *solver_data, double *d_state, double *d_env,
                            double t_initial, double t_final);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it).  Same as camp_solver_run_device, but the per-cell kernel's
+ * queue counter and wavefront scratch come from a fixed 64 MiB device array
+ * in the library (one per GPU, kept until the process ends) and nothing is
+ * allocated, on the device or on the host.  Per-cell mode only, without
+ * phase timing or the Jacobian check; the solver's own buffers must exist
+ * (any earlier call that uses the device makes them, e.g. solver_run,
+ * camp_solver_run_device or camp_solver_get_rxn_env).  Launches as many
+ * wavefronts as the array holds (fewer than camp_solver_run_device for
+ * multi-phase mechanisms, so slower; results are the same).  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU race.
+ * Returns 0 / 1 like solver_run, or CAMP_NOALLOC_NOT_READY (solver not
+ * initialised or its device buffers missing), CAMP_NOALLOC_UNSUPPORTED
+ * (block mode, timing or Jacobian check on), CAMP_NOALLOC_TOO_LARGE (one
+ * wavefront's scratch exceeds the array), with a message on stderr. */
+#define CAMP_NOALLOC_NOT_READY -1
+#define CAMP_NOALLOC_UNSUPPORTED -2
+#define CAMP_NOALLOC_TOO_LARGE -3
+int camp_solver_run_device_noalloc(void *solver_data, double *d_state, double *d_env,
+                                   double t_initial, double t_final);
+
 /* Set the base rate of updatable reaction `unique_id` (photolysis,
  * emission, first-order loss) for every cell from rates[n_cells]; the bulk
  * equivalent of one rxn_update_data call per cell. */
