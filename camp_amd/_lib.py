@@ -66,6 +66,12 @@ SIGNATURES = {
     "camp_solver_tune_fj": (_i, [_p, _p, _p, _i, _dp]),
     "camp_solver_set_fj_lanes": (_i, [_p, _i, _i]),
     "camp_solver_run_device_noalloc": (_i, [_p, _p, _p, _d, _d]),
+    "rxn_wet_deposition_create_rate_update_data": (_p, []),
+    "rxn_wet_deposition_set_rate_update_data": (None, [_p, _i, _d]),
+    "rxn_condensed_phase_photolysis_create_rate_update_data": (_p, []),
+    "rxn_condensed_phase_photolysis_set_rate_update_data": (None, [_p, _i, _d]),
+    "aero_rep_single_particle_create_number_update_data": (_p, []),
+    "aero_rep_single_particle_set_number_update_data__n_m3": (None, [_p, _i, _i, _d]),
     "camp_solver_get_cell_stats": (None, [_p, _dp]),
     "camp_solver_last_kernel_ms": (_d, [_p]),
     "camp_solver_set_profiling": (None, [_p, _i]),

@@ -7,8 +7,8 @@
 // free, the scheduler takes the next cells of the oldest request's
 // mechanism -- from that request and the ones after it, a request being
 // split across launches when it is larger than what is left -- up to a
-// chunk of twice the cells the GPU holds at once, counted in
-// cell-timesteps, and launches the persistent per-cell kernel over them
+// chunk of twice the cells the GPU holds at once, and launches the
+// persistent per-cell kernel over them
 // (each cell runs all of its request's timesteps).  One mechanism never
 // holds every stream while another has cells waiting.  Launches allocate
 // nothing: cell lists and multi-phase scratch are per stream and sized when
@@ -239,20 +239,22 @@ bool Server::launch() {
         }
     }
     Arena& A = *mech(m);
-    // Cost-aware chunk: each cell runs all of its request's timesteps in the
-    // launch, so a chunk is capped in cell-timesteps (twice the cells the GPU
-    // holds at once), not in cells: launches of multi-step requests are no
-    // longer than others, and the streams free up at an even pace.
+    // A chunk: up to twice the cells the GPU holds at once, so that the
+    // launch fills the GPU and its tail overlaps the next launch on another
+    // stream.  (Capping chunks in cell-timesteps instead, so that multi-step
+    // requests make launches no longer than others, was measured: launches
+    // of multi-step requests then held fewer cells than the GPU holds,
+    // capacity fell by a third and the latency spread grew.)
     long cap = std::min(max_batch, A.chunk);
     L.batch.clear();
     long total = 0, work = 0;
-    for (auto it = ready.begin(); it != ready.end() && work < cap;) {
+    for (auto it = ready.begin(); it != ready.end() && total < cap;) {
       Request* r = *it;
       if (r->mech != m) {
         ++it;
         continue;
       }
-      long take = std::min(r->n - r->launched, std::max(1L, (cap - work) / r->n_steps));
+      long take = std::min(r->n - r->launched, cap - total);
       L.batch.push_back({r, r->launched, take});
       r->launched += take;
       total += take;
@@ -480,9 +482,8 @@ extern "C" int camp_server_add_mechanism(void* server, void* solver_data, long m
   A->upd_off = dev_upload(off, A->owned);
   HIP_OK(hipHostMalloc((void**)&A->h_fail, (size_t)n * sizeof(int), hipHostMallocDefault));
   A->free_ranges[0] = n;
-  // a launch takes at most twice the cells the GPU holds at once (counted
-  // in cell-timesteps); its cell list and multi-phase scratch are
-  // allocated here, per stream, at that size
+  // a launch takes at most twice the cells the GPU holds at once; its cell
+  // list and wavefront scratch are allocated here, per stream, at that size
   int grid = 0, block = 0;
   size_t lds = 0;
   A->place = cell_placement(S->P, S->place);

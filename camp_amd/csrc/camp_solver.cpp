@@ -1231,6 +1231,39 @@ extern "C" void aero_rep_print_data(void* solver_data) {
 // aero_rep_solver.c:589
 extern "C" void aero_rep_free_update_data(void* update_data) { std::free(update_data); }
 
+// Payload helpers of reaction and representation types this solver does not
+// integrate (a mechanism that uses them stops at solver_initialize with the
+// setup error), exported so that every Fortran module of the reference
+// links: rxn_wet_deposition.c:218-239 and rxn_condensed_phase_photolysis.c:
+// 464-485 ({int rxn_id; double rate}, as photolysis), aero_rep_single_particle
+// .c:405-433 ({int aero_rep_id; int update_type (0: number); int
+// particle_id; double number_conc}).  rxn_update_data and
+// aero_rep_update_data find no matching reaction or representation for them.
+extern "C" void* rxn_wet_deposition_create_rate_update_data(void) { return new_rate_payload(); }
+extern "C" void rxn_wet_deposition_set_rate_update_data(void* u, int rxn_id, double base_rate) {
+  set_rate_payload(u, rxn_id, base_rate);
+}
+extern "C" void* rxn_condensed_phase_photolysis_create_rate_update_data(void) {
+  return new_rate_payload();
+}
+extern "C" void rxn_condensed_phase_photolysis_set_rate_update_data(void* u, int photo_id,
+                                                                    double base_rate) {
+  set_rate_payload(u, photo_id, base_rate);
+}
+extern "C" void* aero_rep_single_particle_create_number_update_data(void) {
+  void* p = std::malloc(3 * sizeof(int) + sizeof(double));
+  if (!p) camp::die("allocating space for number update data");
+  return p;
+}
+extern "C" void aero_rep_single_particle_set_number_update_data__n_m3(void* update_data,
+                                                                     int aero_rep_id,
+                                                                     int particle_id,
+                                                                     double number_conc) {
+  int head[3] = {aero_rep_id, 0, particle_id};
+  std::memcpy(update_data, head, sizeof(head));
+  std::memcpy(static_cast<char*>(update_data) + sizeof(head), &number_conc, sizeof(double));
+}
+
 // aero_rep_modal_binned_mass.c:677-738: payload {int aero_rep_id, int
 // update_type, int section_id, double value} (3 ints then a double, packed)
 static void* new_section_payload() {

@@ -9,9 +9,8 @@
  * cells.  Whenever a stream is free the scheduler launches the persistent
  * kernel over the next cells of the oldest request's mechanism (from that
  * request and later ones of the same mechanism, up to twice the cells the
- * GPU holds at once counted in cell-timesteps; a large request spans
- * several launches; one mechanism never holds every stream while another
- * has cells waiting), each cell
+ * GPU holds at once; a large request spans several launches; one mechanism
+ * never holds every stream while another has cells waiting), each cell
  * running its request's timesteps back to back (continuous batching:
  * requests that arrive while others run join the queue and start as soon
  * as a stream frees up; a request completes when its last cells do).

@@ -153,6 +153,25 @@ void aero_rep_modal_binned_mass_set_gsd_update_data(void *update_data,
                                                     int section_id, double gsd);
 void aero_rep_free_update_data(void *update_data);
 
+/* replace the payload helpers of the reaction and representation types this
+ * solver does not integrate, so that every Fortran module links
+ * (rxn_wet_deposition.F90:124-144 / rxn_wet_deposition.c:218-239,
+ * rxn_condensed_phase_photolysis.F90:140-160 / .c:464-485,
+ * aero_rep_single_particle.F90:169-189 / .c:405-433); a
+ * mechanism with those types stops at solver_initialize (setup error) */
+void *rxn_wet_deposition_create_rate_update_data(void);
+void rxn_wet_deposition_set_rate_update_data(void *update_data, int rxn_id,
+                                             double base_rate);
+void *rxn_condensed_phase_photolysis_create_rate_update_data(void);
+void rxn_condensed_phase_photolysis_set_rate_update_data(void *update_data,
+                                                         int photo_id,
+                                                         double base_rate);
+void *aero_rep_single_particle_create_number_update_data(void);
+void aero_rep_single_particle_set_number_update_data__n_m3(void *update_data,
+                                                           int aero_rep_id,
+                                                           int particle_id,
+                                                           double number_conc);
+
 /* replace solver_set_debug_out and solver_set_eval_jac,
  * src/camp_solver.c:502-533 (CAMP_DEBUG builds; camp_solver_data.F90:
  * 100-118): debug output is accepted and ignored; eval_jac turns on a

@@ -32,14 +32,36 @@ def _exported():
 
 def test_every_fortran_bound_function_is_exported():
     """A Fortran build of the reference links against this library: every
-    bind(c) name of camp_solver_data.F90 and of the supported types'
-    modules (rxn_photolysis/emission/first_order_loss.F90,
-    aero_rep_modal_binned_mass.F90) is a defined dynamic symbol."""
+    bind(c) name of camp_solver_data.F90, of the supported types' modules
+    (rxn_photolysis/emission/first_order_loss.F90,
+    aero_rep_modal_binned_mass.F90) and of the modules of types it does not
+    integrate (rxn_wet_deposition.F90, rxn_condensed_phase_photolysis.F90,
+    aero_rep_single_particle.F90) is a defined dynamic symbol."""
     exported = _exported()
-    want = sorted({n for names in _symbols()["supported"].values() for n in names})
-    assert len(want) >= 27
+    sym = _symbols()
+    want = sorted({n for part in ("supported", "unsupported_types")
+                   for names in sym[part].values() for n in names})
+    assert len(want) >= 33
     missing = [n for n in want if n not in exported]
     assert not missing, missing
+
+
+def test_unsupported_types_payloads_follow_the_reference_layouts():
+    """rxn_wet_deposition.c:218-239, rxn_condensed_phase_photolysis.c:
+    464-485: {int id; double rate}; aero_rep_single_particle.c:405-433:
+    {int aero_rep_id; int update_type = 0 (number); int particle_id; double
+    number_conc}."""
+    from camp_amd._lib import lib
+    L = lib()
+    for kind in ("wet_deposition", "condensed_phase_photolysis"):
+        p = getattr(L, "rxn_%s_create_rate_update_data" % kind)()
+        getattr(L, "rxn_%s_set_rate_update_data" % kind)(p, 5, 1.25)
+        assert struct.unpack("<id", C.string_at(p, 12)) == (5, 1.25)
+        L.rxn_free_update_data(p)
+    p = L.aero_rep_single_particle_create_number_update_data()
+    L.aero_rep_single_particle_set_number_update_data__n_m3(p, 3, 7, 2.0e9)
+    assert struct.unpack("<iiid", C.string_at(p, 20)) == (3, 0, 7, 2.0e9)
+    L.aero_rep_free_update_data(p)
 
 
 @pytest.mark.skipif(not os.path.isdir("/root/reference/src"), reason="reference tree absent")
