@@ -113,6 +113,10 @@ class CampSolver:
     def set_mode(self, mode):
         return lib().camp_solver_set_mode(self.h, mode)
 
+    def set_graphs(self, on):
+        """Block mode: Newton iterations replayed as HIP graphs (default)."""
+        return lib().camp_solver_set_graphs(self.h, int(bool(on)))
+
     def set_cells_per_block(self, n):
         return lib().camp_solver_set_cells_per_block(self.h, n)
 

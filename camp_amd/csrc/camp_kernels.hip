@@ -238,9 +238,11 @@ __global__ void __launch_bounds__(64) k_deriv_g(DevProgram P, int nc, const doub
                                                 const double* __restrict__ Jy,
                                                 const double* __restrict__ Jf,
                                                 double* __restrict__ fout, double h,
+                                                const double* __restrict__ hp,
                                                 int* __restrict__ fail) {
   extern __shared__ double smem[];
   constexpr int C = kWave / G;
+  if (hp) h = *hp;  // time step from device memory (graph replays)
   const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave, wpb = blockDim.x / kWave;
   const int cs = lane / G, j = lane % G;
   const int SC = group_cell_stride(P, EST), nv = P.nv;
@@ -393,7 +395,7 @@ template <int G>
 static void launch_group_fj(bool jac, const DevProgram& P, int nc, const double* y,
                             const double* state, const double* kval, const double* Js,
                             const double* Jy, const double* Jf, double* out, double h,
-                            int use_est, int* fail, hipStream_t st) {
+                            const double* hp, int use_est, int* fail, hipStream_t st) {
   constexpr int C = kWave / G;
   int blocks = (nc + C - 1) / C;
   blocks = blocks < 1 ? 1 : blocks > (1 << 22) ? (1 << 22) : blocks;
@@ -403,38 +405,38 @@ static void launch_group_fj(bool jac, const DevProgram& P, int nc, const double*
   } else if (use_est) {
     hipLaunchKernelGGL((k_deriv_g<G, true>), dim3(blocks), dim3(kWave),
                        group_lds_bytes(P, G, true), st, P, nc, y, state, kval, Js, Jy, Jf, out,
-                       h, fail);
+                       h, hp, fail);
   } else {
     hipLaunchKernelGGL((k_deriv_g<G, false>), dim3(blocks), dim3(kWave),
                        group_lds_bytes(P, G, false), st, P, nc, y, state, kval, Js, Jy, Jf, out,
-                       h, fail);
+                       h, hp, fail);
   }
 }
 
 void launch_group_deriv(int lanes, const DevProgram& P, int nc, const double* y,
                         const double* state, const double* kval, const double* Js,
-                        const double* Jy, const double* Jf, double* fout, double h, int use_est,
-                        int* fail, hipStream_t st) {
+                        const double* Jy, const double* Jf, double* fout, double h,
+                        const double* hp, int use_est, int* fail, hipStream_t st) {
   if (lanes == 8)
-    launch_group_fj<8>(false, P, nc, y, state, kval, Js, Jy, Jf, fout, h, use_est, fail, st);
+    launch_group_fj<8>(false, P, nc, y, state, kval, Js, Jy, Jf, fout, h, hp, use_est, fail, st);
   else if (lanes == 16)
-    launch_group_fj<16>(false, P, nc, y, state, kval, Js, Jy, Jf, fout, h, use_est, fail, st);
+    launch_group_fj<16>(false, P, nc, y, state, kval, Js, Jy, Jf, fout, h, hp, use_est, fail, st);
   else
-    launch_group_fj<32>(false, P, nc, y, state, kval, Js, Jy, Jf, fout, h, use_est, fail, st);
+    launch_group_fj<32>(false, P, nc, y, state, kval, Js, Jy, Jf, fout, h, hp, use_est, fail, st);
 }
 
 void launch_group_jac(int lanes, const DevProgram& P, int nc, const double* y,
                       const double* state, const double* kval, double* J, double h,
                       hipStream_t st) {
   if (lanes == 8)
-    launch_group_fj<8>(true, P, nc, y, state, kval, nullptr, nullptr, nullptr, J, h, 0, nullptr,
-                       st);
+    launch_group_fj<8>(true, P, nc, y, state, kval, nullptr, nullptr, nullptr, J, h, nullptr, 0,
+                       nullptr, st);
   else if (lanes == 16)
-    launch_group_fj<16>(true, P, nc, y, state, kval, nullptr, nullptr, nullptr, J, h, 0, nullptr,
-                        st);
+    launch_group_fj<16>(true, P, nc, y, state, kval, nullptr, nullptr, nullptr, J, h, nullptr, 0,
+                        nullptr, st);
   else
-    launch_group_fj<32>(true, P, nc, y, state, kval, nullptr, nullptr, nullptr, J, h, 0, nullptr,
-                        st);
+    launch_group_fj<32>(true, P, nc, y, state, kval, nullptr, nullptr, nullptr, J, h, nullptr, 0,
+                        nullptr, st);
 }
 
 // Plain path: LU of I - gamma*J for all cells (LU staged in LDS).
@@ -458,9 +460,11 @@ __global__ void __launch_bounds__(64) k_lu_factor(DevProgram P, int nc, const do
 
 // Plain path: solve in place, then scale by s (CVDls gamma-ratio scaling).
 __global__ void __launch_bounds__(64) k_lu_solve(DevProgram P, int nc, const double* __restrict__ LU,
-                                                 double* __restrict__ b, double s) {
+                                                 double* __restrict__ b, double s,
+                                                 const double* __restrict__ sp) {
   extern __shared__ double smem[];
   int lane = threadIdx.x;
+  if (sp) s = *sp;  // scale from device memory (graph replays)
   double* s_lu = smem;  // LU values, then the factorisation's sinks and the solve's zero slot
   double* s_b = s_lu + P.lu_nnz + kWave + 1;
   double* s_w = s_b + P.nv;
@@ -534,9 +538,14 @@ __global__ void k_vec_complete(long n, int q, Coef6 l6, const double* __restrict
 // Newton residual: tempv = gamma*ftemp - (rl1*zn1 + acor)
 __global__ void k_vec_resid(long n, double rl1, const double* __restrict__ zn1,
                             const double* __restrict__ acor, double gamma,
-                            const double* __restrict__ ftemp, double* __restrict__ tempv) {
+                            const double* __restrict__ ftemp, double* __restrict__ tempv,
+                            const double* __restrict__ ctl) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (ctl) {  // rl1 and gamma from device memory (graph replays)
+    rl1 = ctl[0];
+    gamma = ctl[1];
+  }
   double t = rl1 * zn1[i] + acor[i];
   tempv[i] = gamma * ftemp[i] - t;
 }
@@ -816,6 +825,11 @@ __global__ void __launch_bounds__(64) k_check_jac(DevProgram P, int nc, const do
     if (lane == 0) cell_fails[c] = nfail;
     wsync();
   }
+}
+
+__global__ void k_fill_int(long n, int v, int* x) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
 }
 
 __global__ void k_fill(long n, double v, double* x) {

@@ -110,10 +110,11 @@ inline int fj_waves_per_block(const DevProgram& P) { return P.mp ? 1 : 4; }
 // Lane-group f(y) and J(y) of gas-phase programs (lanes per cell = 8, 16
 // or 32; camp_kernels.hip) and the compact rate-constant array they read.
 __global__ void k_kval(DevProgram P, int nc, const double* rxn_env, double* kval);
+// hp (may be null): the time step read from device memory instead of h
 void launch_group_deriv(int lanes, const DevProgram& P, int nc, const double* y,
                         const double* state, const double* kval, const double* Js,
-                        const double* Jy, const double* Jf, double* fout, double h, int use_est,
-                        int* fail, hipStream_t st);
+                        const double* Jy, const double* Jf, double* fout, double h,
+                        const double* hp, int use_est, int* fail, hipStream_t st);
 void launch_group_jac(int lanes, const DevProgram& P, int nc, const double* y,
                       const double* state, const double* kval, double* J, double h,
                       hipStream_t st);
@@ -128,7 +129,9 @@ inline size_t group_lds_bytes(const DevProgram& P, int lanes, bool fsum) {
 }
 __global__ void k_lu_factor(DevProgram P, int nc, const double* J, double gamma, double* LU,
                             int* fail);
-__global__ void k_lu_solve(DevProgram P, int nc, const double* LU, double* b, double s);
+// sp (may be null): the scale read from device memory instead of s
+__global__ void k_lu_solve(DevProgram P, int nc, const double* LU, double* b, double s,
+                           const double* sp);
 __global__ void k_vec_lin(long n, double a, const double* x, double b, const double* y,
                           double* z);
 __global__ void k_vec_scale(long n, double a, double* x);
@@ -139,8 +142,9 @@ struct Coef6 {
 };
 __global__ void k_vec_complete(long n, int q, Coef6 l6, const double* acor, double* zn, int save,
                                double* last, int projected, const double* y);
+// ctl (may be null): rl1 = ctl[0], gamma = ctl[1] from device memory
 __global__ void k_vec_resid(long n, double rl1, const double* zn1, const double* acor,
-                            double gamma, const double* ftemp, double* tempv);
+                            double gamma, const double* ftemp, double* tempv, const double* ctl);
 __global__ void k_vec_newton_update(long n, const double* b, double* acor, const double* zn0,
                                     double* y);
 __global__ void k_vec_ewt(long n, int nv, double rtol, const double* atol, const double* y,
@@ -154,6 +158,7 @@ __global__ void k_vec_project(long n, const double* zn0, double* acor, double* y
 __global__ void k_vec_tstar(long n, const double* tmp1, const double* corr, double h_j,
                             double* out);
 __global__ void k_fill(long n, double v, double* x);
+__global__ void k_fill_int(long n, int v, int* x);
 __global__ void k_check_jac(DevProgram P, int nc, const double* y, const double* state,
                             const double* env, const double* rxn_env, const double* J, double h_ts,
                             int* cell_fails);

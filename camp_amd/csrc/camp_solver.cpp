@@ -216,8 +216,8 @@ void Solver::launch_deriv(const double* y, const double* st, const double* env, 
                           double h, int use_est, int* fail) {
   int nc = m.nc;
   if (!P.mp) {
-    launch_group_deriv(f_lanes, P, nc, y, st, kval(), d_Js, d_Jy, d_Jf, out, h, use_est, fail,
-                       stream);
+    launch_group_deriv(f_lanes, P, nc, y, st, kval(), d_Js, d_Jy, d_Jf, out, h, nullptr, use_est,
+                       fail, stream);
     return;
   }
   int wpb = fj_waves_per_block(P);
@@ -668,12 +668,11 @@ struct BlockCv {
                        S.d_fail);
     return check_fail();
   }
-  void lsolve(double* b) {
-    double sc = gamrat != 1.0 ? 2.0 / (1.0 + gamrat) : 1.0;
+  double lsolve_scale() const { return gamrat != 1.0 ? 2.0 / (1.0 + gamrat) : 1.0; }
+  void lsolve(double* b, const double* sp = nullptr) {
     hipLaunchKernelGGL(k_lu_solve, dim3(std::min(nc, 16384)), dim3(64),
                        ((size_t)S.P.lu_nnz + kWave + 1 + 2 * (size_t)S.P.nv) * sizeof(double), s,
-                       S.P, nc,
-                       S.d_LU, b, sc);
+                       S.P, nc, S.d_LU, b, lsolve_scale(), sp);
   }
   // Non-negativity safeguard, the per-cell kernel's rule: components of y
   // below -SMALL by at most their tolerance go to TINY through corr.
@@ -685,12 +684,111 @@ struct BlockCv {
     hipLaunchKernelGGL(k_vec_project, dim3(nblk(n)), dim3(256), 0, s, n, zn[0], corr, y);
     return 0;
   }
+  // ---- HIP graphs of the Newton iteration ---------------------------------
+  // One Newton iteration's device work up to the host's convergence decision
+  // is captured once per solver as graph A: the per-iteration scalars (rl1,
+  // gamma, the solve's scale, the time step) copied from pinned host memory,
+  // the residual, the batched LU solve, the WRMS partial sums of the
+  // correction, the update of acor and y, and the partial maxima of the
+  // negative-concentration test of y -- with both partial arrays copied to
+  // pinned host memory.  Graph B is f(y) into ftemp with its failure flag
+  // (captured per state pointer and f kernel width).  A Newton iteration then costs one graph
+  // launch and one synchronisation, plus graph B when it continues, instead
+  // of seven launches and three synchronisations; the kernels and the host's
+  // reductions are the same, so the results are identical.  No memset is
+  // recorded in a graph (flags are cleared with a fill kernel).
+  void graph_capture(hipGraphExec_t* out, bool b) {
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    if (!b) {
+      HIP_OK(hipMemcpyAsync(S.d_ctl, S.h_ctl, 4 * sizeof(double), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_vec_resid, dim3(nblk(n)), dim3(256), 0, s, n, 0.0, zn[1], acor, 0.0,
+                         ftemp, tempv, (const double*)S.d_ctl);
+      lsolve(tempv, S.d_ctl + 2);
+      hipLaunchKernelGGL(k_vec_wsq, dim3(kRedBlocks), dim3(256), 0, s, n, tempv, ewt, S.d_red);
+      hipLaunchKernelGGL(k_vec_newton_update, dim3(nblk(n)), dim3(256), 0, s, n, tempv, acor,
+                         zn[0], y);
+      hipLaunchKernelGGL(k_vec_negmax, dim3(kRedBlocks), dim3(256), 0, s, n, y, ewt,
+                         S.d_red + kRedBlocks);
+      HIP_OK(hipMemcpyAsync(S.h_red_pin, S.d_red, 2 * kRedBlocks * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    } else {
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, 1L, 0, S.d_fail);
+      launch_group_deriv(S.f_lanes, S.P, nc, y, st, S.kval(), S.d_Js, S.d_Jy, S.d_Jf, ftemp,
+                         0.0, S.d_ctl + 3, 1, S.d_fail, s);
+      HIP_OK(hipMemcpyAsync(S.h_fail_pin, S.d_fail, sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    HIP_OK(hipStreamEndCapture(s, &g));
+    HIP_OK(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+    HIP_OK(hipGraphDestroy(g));
+  }
+  void graphs_ready() {
+    if (!S.h_ctl) {
+      HIP_OK(hipHostMalloc((void**)&S.h_ctl, 4 * sizeof(double), hipHostMallocDefault));
+      HIP_OK(hipHostMalloc((void**)&S.h_red_pin, 2 * kRedBlocks * sizeof(double),
+                           hipHostMallocDefault));
+      HIP_OK(hipHostMalloc((void**)&S.h_fail_pin, sizeof(int), hipHostMallocDefault));
+      S.d_ctl = dev_alloc<double>(4, S.owned);
+    }
+    if (!S.graph_a) graph_capture(&S.graph_a, false);
+    if (!S.graph_b || S.graph_b_state != st || S.graph_b_lanes != S.f_lanes) {
+      if (S.graph_b) HIP_OK(hipGraphExecDestroy(S.graph_b));
+      graph_capture(&S.graph_b, true);
+      S.graph_b_state = st;
+      S.graph_b_lanes = S.f_lanes;
+    }
+  }
+  int newton_iteration_graph() {
+    graphs_ready();
+    int m = 0;
+    double del = 0.0, delp = 0.0;
+    for (;;) {
+      S.h_ctl[0] = rl1;
+      S.h_ctl[1] = gamma;
+      S.h_ctl[2] = lsolve_scale();
+      S.h_ctl[3] = next_h > 0.0 ? next_h : init_h;
+      HIP_OK(hipGraphLaunch(S.graph_a, s));
+      HIP_OK(hipStreamSynchronize(s));
+      nni++;
+      double acc = 0.0, worst = S.h_red_pin[kRedBlocks];
+      for (int i = 0; i < kRedBlocks; ++i) acc += S.h_red_pin[i];
+      for (int i = 1; i < kRedBlocks; ++i) worst = std::max(worst, S.h_red_pin[kRedBlocks + i]);
+      del = std::sqrt(acc / (double)n);
+      if (m > 0) crate = std::fmax(CRDOWN * crate, del / delp);
+      double dcon = del * std::fmin(1.0, crate) / tq[4];
+      if (dcon <= 1.0) {
+        int pr = project_with(acor, worst);
+        if (pr > 0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+        projected = pr == 0;
+        acnrm = (m == 0 && pr < 0) ? del : wrms(acor);
+        jcur = 0;
+        return CV_SUCCESS;
+      }
+      m++;
+      if ((m == MAXCOR) || ((m >= 2) && (del > RDIV * delp))) return jcur ? CONV_FAIL : TRY_AGAIN;
+      delp = del;
+      if (project_with(acor, worst) > 0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+      nfe++;
+      HIP_OK(hipGraphLaunch(S.graph_b, s));
+      HIP_OK(hipStreamSynchronize(s));
+      if (*S.h_fail_pin) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
+    }
+  }
+  // project() with the worst excursion already known
+  int project_with(double* corr, double worst) {
+    if (worst > 1.0) return 1;
+    if (worst < 0.0) return -1;
+    hipLaunchKernelGGL(k_vec_project, dim3(nblk(n)), dim3(256), 0, s, n, zn[0], corr, y);
+    return 0;
+  }
+
   int newton_iteration() {
+    if (S.use_graphs && !S.P.mp) return newton_iteration_graph();
     int m = 0;
     double del = 0.0, delp = 0.0;
     for (;;) {
       hipLaunchKernelGGL(k_vec_resid, dim3(nblk(n)), dim3(256), 0, s, n, rl1, zn[1], acor, gamma,
-                         ftemp, tempv);
+                         ftemp, tempv, (const double*)nullptr);
       lsolve(tempv);
       nni++;
       del = wrms(tempv);
@@ -1674,8 +1772,7 @@ extern "C" int camp_eval_lu_solve(void* solver_data, const double* jac, double g
                      gamma, d_LU, s->d_fail);
   hipLaunchKernelGGL(camp::k_lu_solve, dim3(std::min(nc, 16384)), dim3(64),
                      ((size_t)P.lu_nnz + camp::kWave + 1 + 2 * (size_t)P.nv) * sizeof(double),
-                     s->stream, P, nc, d_LU,
-                     d_b, 1.0);
+                     s->stream, P, nc, d_LU, d_b, 1.0, (const double*)nullptr);
   int fail = 0;
   HIP_OK(hipMemcpyAsync(&fail, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipMemcpyAsync(b, d_b, (size_t)nc * P.nv * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -1796,6 +1893,13 @@ static int time_fj_impl(Solver* s, const double* d_state, const double* d_env, i
 extern "C" int camp_solver_time_fj(void* solver_data, const double* d_state, const double* d_env,
                                    int reps, double* ms_deriv, double* ms_jac) {
   return time_fj_impl(SD(solver_data), d_state, d_env, reps, ms_deriv, ms_jac, false, nullptr);
+}
+
+extern "C" int camp_solver_set_graphs(void* solver_data, int on) {
+  camp::Solver* s = SD(solver_data);
+  int old = s->use_graphs ? 1 : 0;
+  s->use_graphs = on != 0;
+  return old;
 }
 
 extern "C" int camp_solver_set_fj_lanes(void* solver_data, int f_lanes, int j_lanes) {

@@ -93,6 +93,13 @@ struct Solver {
   bool eval_group = false;
   // lanes per cell of the block path's f and J kernels (8, 16 or 32)
   int f_lanes = 16, j_lanes = 16;
+  // block path: Newton iterations replayed as HIP graphs (BlockCv)
+  bool use_graphs = true;
+  hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
+  const double* graph_b_state = nullptr;
+  int graph_b_lanes = 0;
+  double *h_ctl = nullptr, *h_red_pin = nullptr, *d_ctl = nullptr;  // pinned / device
+  int* h_fail_pin = nullptr;
   // block-path buffers
   double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_y = nullptr,
          *d_tempv = nullptr, *d_ftemp = nullptr, *d_last = nullptr, *d_acinit = nullptr,
@@ -136,6 +143,11 @@ struct Solver {
   double rhs_time_s = 0.0, jac_time_s = 0.0;
 
   ~Solver() {
+    if (graph_a) (void)hipGraphExecDestroy(graph_a);
+    if (graph_b) (void)hipGraphExecDestroy(graph_b);
+    if (h_ctl) (void)hipHostFree(h_ctl);
+    if (h_red_pin) (void)hipHostFree(h_red_pin);
+    if (h_fail_pin) (void)hipHostFree(h_fail_pin);
     for (void* p : owned) (void)hipFree(p);
     if (d_wave_scratch) (void)hipFree(d_wave_scratch);
     if (ev0) (void)hipEventDestroy(ev0);

@@ -190,6 +190,11 @@ int solver_set_eval_jac(void *solver_data, bool eval_jac);
 /* Select the integration mode; returns the previous mode. */
 int camp_solver_set_mode(void *solver_data, int mode);
 
+/* Block mode: replay each Newton iteration's device work as HIP graphs (1,
+ * default) or launch it kernel by kernel (0).  Results are identical.
+ * Returns the previous setting. */
+int camp_solver_set_graphs(void *solver_data, int on);
+
 /* Cells per workgroup of the persistent kernel (1, 2, 4, 8, 16, 32; one
  * wavefront per cell; a value whose workgroup does not fit a compute unit
  * is ignored); 0 selects the built-in default.  Returns the value in
