@@ -360,9 +360,10 @@ def main():
         # the spun-up cells (copies; the workload's cells are untouched)
         chosen, cand_ms = solver.autotune(st.data_ptr(), env.data_ptr(), 0.0, args.dt,
                                           n_sample=min(args.cells, 16384))
-        tune = {"placement": ("history in LDS", "history in global memory")[chosen[0]],
+        tune = {"placement": ("history in LDS", "history in global memory",
+                              "history and LU in global memory")[chosen[0]],
                 "cells_per_workgroup": chosen[1],
-                "candidate_ms": {"%s/%d" % (("lds", "global")[p], c): v
+                "candidate_ms": {"%s/%d" % (("lds", "global", "global_lu")[p], c): v
                                  for (p, c), v in cand_ms.items() if v is not None},
                 "sample_cells": min(args.cells, 16384)}
         progress("autotuned: %s %s" % (chosen, tune["candidate_ms"]))

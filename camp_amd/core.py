@@ -121,7 +121,8 @@ class CampSolver:
         return lib().camp_solver_set_cells_per_block(self.h, n)
 
     def set_placement(self, place):
-        """0: cell history in LDS; 1: in global memory (more cells per CU)."""
+        """0: cell history in LDS; 1: in global memory (more cells per CU);
+        2: history and LU factors in global memory (more still)."""
         return lib().camp_solver_set_placement(self.h, int(place))
 
     def autotune(self, state_ptr, env_ptr, t0, t1, n_sample=0):
@@ -129,10 +130,10 @@ class CampSolver:
         by timing each candidate on a sample of the device-resident cells
         (copies).  Returns ((placement, cells per workgroup),
         {(placement, cells per workgroup): ms or None when it does not fit})."""
-        ms = np.zeros(12)
+        ms = np.zeros(18)
         best = lib().camp_solver_autotune(self.h, C.c_void_p(state_ptr), C.c_void_p(env_ptr),
                                           float(t0), float(t1), int(n_sample), _dp(ms))
-        cands = [(p, c) for p in (0, 1) for c in (1, 2, 4, 8, 16, 32)]
+        cands = [(p, c) for p in (0, 1, 2) for c in (1, 2, 4, 8, 16, 32)]
         return (best // 100, best % 100), {k: (float(v) if v >= 0 else None)
                                            for k, v in zip(cands, ms)}
 

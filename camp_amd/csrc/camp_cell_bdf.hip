@@ -669,16 +669,21 @@ struct CellCv {
 
 }  // namespace
 
-// MP: multi-phase program; GH: the Nordsieck history, 7 of the 10 work
-// vectors and the J history in global memory (kPlaceGlobal; always for
-// multi-phase programs).  The gas-phase variant with its history in LDS is
-// compiled for one wavefront per SIMD, the others for two, so that the
-// cells their smaller LDS footprint lets onto a CU can all be resident.
-template <bool MP, bool GH>
-__global__ void __launch_bounds__(256, (GH && !MP) ? 2 : 1) k_cell_bdf(CellBdfArgs a) {
+// MP: multi-phase program; PL: the placement (kPlaceLds, kPlaceGlobal:
+// the Nordsieck history, 7 of the 10 work vectors and the J history in
+// global memory, kPlaceGlobalLu: the LU factors as well; multi-phase
+// programs run kPlaceGlobal with global LU factors).  Each variant is
+// compiled for the wavefronts per SIMD its LDS footprint lets onto a CU
+// (1, 2 and 3 for the CB05-scale mechanism), which bounds its registers.
+template <bool MP, int PL>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(MP || PL == kPlaceLds ? 1 : PL == kPlaceGlobal ? 2 : 3)))
+    k_cell_bdf(CellBdfArgs a) {
   extern __shared__ double smem[];
   const DevProgram& P = a.P;
-  constexpr int kPlace = GH ? kPlaceGlobal : kPlaceLds;
+  constexpr bool GH = MP || PL != kPlaceLds;
+  constexpr bool GL = MP || PL == kPlaceGlobalLu;
+  constexpr int kPlace = MP ? kPlaceGlobal : PL;
   int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
   int nv = P.nv;
   size_t per_cell = cell_bdf_lds_doubles(P, kPlace);
@@ -696,9 +701,12 @@ __global__ void __launch_bounds__(256, (GH && !MP) ? 2 : 1) k_cell_bdf(CellBdfAr
   cv.s_rate = cv.s_k + P.n_env + mp_lds_doubles(P);
   double* v = cv.s_rate + 2 * (size_t)P.n_terms;
   // With GH the Nordsieck history and the work vectors other than y, acor
-  // and w live in the wavefront's global scratch (for multi-phase programs
-  // after the LU, J_rxn and Jacobian scalars); else all in LDS.
-  double* g = MP ? scratch + P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) : scratch;
+  // and w live in the wavefront's global scratch (after the LU where it is
+  // global, and for multi-phase programs the J_rxn and Jacobian scalars);
+  // else all in LDS.
+  double* g = MP   ? scratch + P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P)
+              : GL ? scratch + P.lu_nnz + kWave + 1
+                   : scratch;
   double*& vh = GH ? g : v;
   cv.znb = vh;
   vh += 6 * (size_t)nv;
@@ -721,9 +729,13 @@ __global__ void __launch_bounds__(256, (GH && !MP) ? 2 : 1) k_cell_bdf(CellBdfAr
     cv.Jr = scratch + P.lu_nnz + kWave + 1;
     cv.s_fb = v; v += fb_sink_doubles(P);
   } else {
-    cv.lu = v;
     cv.Jr = nullptr;
-    v += P.lu_nnz;
+    if (GL) {
+      cv.lu = scratch;
+    } else {
+      cv.lu = v;
+      v += P.lu_nnz;
+    }
     cv.s_fb = v; v += fb_sink_doubles(P);
     if (!GH) {
       h_js = v; v += P.nnz;
@@ -847,9 +859,12 @@ __global__ void __launch_bounds__(256, (GH && !MP) ? 2 : 1) k_cell_bdf(CellBdfAr
 // Per-cell LDS beyond 64 KiB per block (multi-phase programs, or several
 // cells per workgroup) is opted into per kernel; gfx950 has 160 KiB per CU.
 static const void* cell_bdf_fn(const DevProgram& P, int place) {
-  if (P.mp) return (const void*)k_cell_bdf<true, true>;
-  return cell_placement(P, place) == kPlaceGlobal ? (const void*)k_cell_bdf<false, true>
-                                                  : (const void*)k_cell_bdf<false, false>;
+  if (P.mp) return (const void*)k_cell_bdf<true, kPlaceGlobal>;
+  switch (cell_placement(P, place)) {
+    case kPlaceGlobal: return (const void*)k_cell_bdf<false, kPlaceGlobal>;
+    case kPlaceGlobalLu: return (const void*)k_cell_bdf<false, kPlaceGlobalLu>;
+    default: return (const void*)k_cell_bdf<false, kPlaceLds>;
+  }
 }
 static void allow_lds(const void* fn, size_t lds) {
   if (lds > 65536)
@@ -859,11 +874,13 @@ static void allow_lds(const void* fn, size_t lds) {
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s) {
   allow_lds(cell_bdf_fn(a.P, a.place), lds);
   if (a.P.mp)
-    hipLaunchKernelGGL((k_cell_bdf<true, true>), dim3(grid), dim3(block), lds, s, a);
+    hipLaunchKernelGGL((k_cell_bdf<true, kPlaceGlobal>), dim3(grid), dim3(block), lds, s, a);
   else if (cell_placement(a.P, a.place) == kPlaceGlobal)
-    hipLaunchKernelGGL((k_cell_bdf<false, true>), dim3(grid), dim3(block), lds, s, a);
+    hipLaunchKernelGGL((k_cell_bdf<false, kPlaceGlobal>), dim3(grid), dim3(block), lds, s, a);
+  else if (cell_placement(a.P, a.place) == kPlaceGlobalLu)
+    hipLaunchKernelGGL((k_cell_bdf<false, kPlaceGlobalLu>), dim3(grid), dim3(block), lds, s, a);
   else
-    hipLaunchKernelGGL((k_cell_bdf<false, false>), dim3(grid), dim3(block), lds, s, a);
+    hipLaunchKernelGGL((k_cell_bdf<false, kPlaceLds>), dim3(grid), dim3(block), lds, s, a);
 }
 
 int cell_bdf_blocks_per_cu(const DevProgram& P, int place, int block, size_t lds) {

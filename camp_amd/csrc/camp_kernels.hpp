@@ -28,23 +28,34 @@ constexpr int kCellProf = 7;
 //                Multi-phase programs always run so, with their LU factors,
 //                reaction Jacobian and per-instance Jacobian scalars in the
 //                scratch as well.
-constexpr int kPlaceLds = 0, kPlaceGlobal = 1;
+//  kPlaceGlobalLu  as kPlaceGlobal, and the LU factors (with the
+//                factorisation's sink slots and the solve's zero slot) in
+//                the scratch too (gas phase: 12 cells per CU for the
+//                CB05-scale mechanism, compiled for three wavefronts per
+//                SIMD).
+constexpr int kPlaceLds = 0, kPlaceGlobal = 1, kPlaceGlobalLu = 2, kNumPlacements = 3;
 __host__ __device__ inline int cell_placement(const DevProgram& P, int place) {
   return P.mp ? kPlaceGlobal : place;
 }
+// LU factors in the wavefront's global scratch
+__host__ __device__ inline bool cell_lu_global(const DevProgram& P, int place) {
+  return P.mp || place == kPlaceGlobalLu;
+}
 __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P, int place) {
-  const bool g = cell_placement(P, place) == kPlaceGlobal;
+  const bool g = cell_placement(P, place) != kPlaceLds;
   return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms +
          (g ? 3 : 16) * (size_t)P.nv + fb_sink_doubles(P) +
-         (P.mp ? 0 : (size_t)P.lu_nnz) + (g ? 0 : (size_t)P.nnz + 2 * (size_t)P.nv) + 24;
+         (cell_lu_global(P, place) ? 0 : (size_t)P.lu_nnz) +
+         (g ? 0 : (size_t)P.nnz + 2 * (size_t)P.nv) + 24;
 }
-// Global scratch doubles per wavefront: for multi-phase programs the LU
-// factors (with the factorisation's 64 sink slots and the solve's zero
-// slot), the reaction Jacobian and the per-instance Jacobian scalars, then
-// (both programs, kPlaceGlobal) the Nordsieck history and 7 of the 10 work
-// vectors (13 vectors).
+// Global scratch doubles per wavefront: the LU factors where they are global
+// (with the factorisation's 64 sink slots and the solve's zero slot), for
+// multi-phase programs then the reaction Jacobian and the per-instance
+// Jacobian scalars, then (kPlaceGlobal, kPlaceGlobalLu) the Nordsieck
+// history and 7 of the 10 work vectors (13 vectors).
 __host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P, int place) {
   if (P.mp) return (size_t)P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) + 13 * (size_t)P.nv;
+  if (place == kPlaceGlobalLu) return (size_t)P.lu_nnz + kWave + 1 + 13 * (size_t)P.nv;
   return place == kPlaceGlobal ? 13 * (size_t)P.nv : 0;
 }
 

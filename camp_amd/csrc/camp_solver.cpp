@@ -1437,13 +1437,14 @@ extern "C" int camp_solver_set_eval_kernels(void* solver_data, int group) {
 extern "C" int camp_solver_set_placement(void* solver_data, int place) {
   using namespace camp;
   Solver* s = SD(solver_data);
-  if (place == kPlaceLds || place == kPlaceGlobal) s->place = place;
+  if (place >= 0 && place < kNumPlacements) s->place = place;
   return cell_placement(s->P, s->place);
 }
 
 // Autotuner for the persistent per-cell kernel.  Candidates: the cell
 // placement (history in LDS: fewer cells per CU; history in global memory:
-// more cells per CU, two wavefronts per SIMD) times cells per workgroup
+// more cells per CU, two wavefronts per SIMD; history and LU factors in
+// global memory: three wavefronts per SIMD) times cells per workgroup
 // {1, 2, 4, 8, 16, 32}; every candidate that fits a compute unit solves the
 // same sample of cells (every (nc / n_sample)-th cell, copied with its rate
 // data and J history, so the solver's own cells are untouched) over
@@ -1484,8 +1485,8 @@ extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
   const int keep_cpb = s->cells_per_block, keep_place = s->place;
   int best = keep_cpb, best_place = cell_placement(P, keep_place);
   double best_ms = -1.0;
-  for (int pl = 0; pl < 2; ++pl) {
-    const int place = pl == 0 ? kPlaceLds : kPlaceGlobal;
+  for (int pl = 0; pl < kNumPlacements; ++pl) {
+    const int place = pl;
     for (int k = 0; k < 6; ++k) {
       double* slot = ms_out ? &ms_out[pl * 6 + k] : nullptr;
       if (slot) *slot = -1.0;

@@ -204,16 +204,17 @@ int camp_solver_set_cells_per_block(void *solver_data, int cells_per_block);
 /* Cell placement of the per-cell kernel (gas-phase programs; multi-phase
  * programs always use 1): 0 = each cell's Nordsieck history, work vectors
  * and J history in LDS (fewer cells per compute unit), 1 = those in global
- * memory (more cells per compute unit, two wavefronts per SIMD).  Returns
+ * memory (more cells per compute unit, two wavefronts per SIMD), 2 = those
+ * and the LU factors in global memory (three wavefronts per SIMD).  Returns
  * the placement in effect.  Results do not depend on it. */
 int camp_solver_set_placement(void *solver_data, int place);
 
 /* Autotuner for the per-cell kernel: solves a sample of n_sample cells
  * (every (n_cells / n_sample)-th; 0 = up to 8192) over [t_initial, t_final]
- * from the device-resident state/env once per candidate -- placement {0, 1}
+ * from the device-resident state/env once per candidate -- placement {0, 1, 2}
  * times cells per workgroup {1, 2, 4, 8, 16, 32} -- that fits a compute
  * unit, on copies (the solver's own cells, rate data and J history are
- * untouched), and keeps the fastest.  ms_out[12] receives each candidate's
+ * untouched), and keeps the fastest.  ms_out[18] receives each candidate's
  * kernel time, placement-major (-1: does not fit, or not available for the
  * program).  Returns the chosen cells per workgroup plus 100 times the chosen
  * placement. */

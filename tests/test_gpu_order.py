@@ -6,7 +6,8 @@ workload with night and x1e3 cells, 3 072 multi-phase cells), two
 solver_run calls each (the second uses the J history the first left), are
 solved in index order, in a random permutation with every wavefront's
 working memory filled with NaNs before each cell, in reverse order with the
-cell history in global memory, and longest-first by the first call's step
+cell history in global memory, permuted and poisoned with the history and
+LU factors in global memory, and longest-first by the first call's step
 counts; then the same cells go through the request server as interleaved
 requests of both mechanisms.  Every result is compared bit for bit.  This is
 the guarantee of the reference's multi- versus single-cell test
@@ -76,6 +77,7 @@ def test_results_do_not_depend_on_order_placement_or_leftover_memory(reference_r
     perm = np.random.default_rng(7).permutation(n)
     _same(ref, _two_calls(mech, cells, order=perm, poison=True))
     _same(ref, _two_calls(mech, cells, order=np.arange(n)[::-1], place=1))
+    _same(ref, _two_calls(mech, cells, order=perm, poison=True, place=2))
     _same(ref, _two_calls(mech, cells, longest=True, poison=True))
 
 
