@@ -53,6 +53,30 @@ def make_inputs(n_cells):
     return mech, cells
 
 
+def host_info():
+    """The host the CPU baseline ran on: usable cores, the CPU model, and how
+    the oracle was compiled."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    from oracle import oracle
+    return {"nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(),
+            "cpu_model": model, "oracle_build": "gcc " + oracle.NATIVE_FLAGS}
+
+
+def _pool_map(fn, jobs):
+    """Oracle work in spawned processes (one per job), which load the
+    -O3 -march=native build of the oracle."""
+    os.environ["CAMP_ORACLE_NATIVE"] = "1"
+    with mp.get_context("spawn").Pool(len(jobs)) as pool:
+        return pool.map(fn, jobs)
+
+
 def _cpu_worker(args):
     n_cells, c0, c1, warm, steps, dt = args
     from oracle.oracle import Oracle
@@ -77,23 +101,109 @@ def cpu_baseline(n_cells, warm, steps, dt, cores, per_core):
     """Oracle per-cell solve of cells spread over the workload: the same W
     untimed and K timed steps as the GPU; rate = cells x K / slowest worker."""
     from oracle import oracle
-    oracle.build(ref=False)
+    oracle.build_native()
     jobs = []
     for i in range(cores):
         c0 = i * (n_cells // cores)
         jobs.append((n_cells, c0, c0 + per_core, warm, steps, dt))
-    if cores == 1:
-        times = [_cpu_worker(jobs[0])]
-    else:
-        with mp.get_context("spawn").Pool(cores) as pool:
-            times = pool.map(_cpu_worker, jobs)
+    times = _pool_map(_cpu_worker, jobs)
     wall = max(times)
     return {"value": cores * per_core * steps / wall, "unit": "cell-timesteps/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "host": host_info(),
             "sample": "%d cells (%d blocks of %d consecutive cells spread over the %d-cell "
                       "workload) x %d timed steps of %g s after %d untimed, per-cell "
                       "CVODE-algorithm BDF, one process per core" % (
                           cores * per_core, cores, per_core, n_cells, steps, dt, warm)}
+
+
+def _cpu_block_worker(args):
+    c0, c1, warm, steps, dt = args
+    from oracle.oracle import Oracle
+    from camp_amd.mechanisms import cb05_scale_mechanism, synthetic_cells
+    mech = cb05_scale_mechanism(1)
+    cells = synthetic_cells(mech, c1 - c0, seed=2, cell_offset=c0)
+    o = Oracle(mech, c1 - c0)
+    o.set_rates(cells.rates)
+    st = np.ascontiguousarray(cells.state)
+    env = np.ascontiguousarray(cells.env)
+    for _ in range(warm):
+        o.run(st, env, 0.0, dt, mode=0)
+    t = time.perf_counter()
+    for _ in range(steps):
+        o.run(st, env, 0.0, dt, mode=0)
+    return time.perf_counter() - t
+
+
+def cpu_baseline_block(n_cells, warm, steps, dt, cores, per_core):
+    """The reference's own semantics on the CPU -- one CVODE over all cells
+    of a solver_run call -- on a bounded sample of the plain-path workload:
+    one block of `per_core` consecutive cells per core, the blocks spread
+    over the workload (each block its own BDF with shared step control, as
+    one MPI rank per core would run it); the same W untimed and K timed
+    steps as the GPU's plain path.  Larger blocks cost more per cell (the
+    stiffest cell sets every cell's step), so this favours the CPU over one
+    block of all cells."""
+    from oracle import oracle
+    oracle.build_native()
+    jobs = []
+    for i in range(cores):
+        c0 = i * (n_cells // cores)
+        jobs.append((c0, c0 + per_core, warm, steps, dt))
+    wall = max(_pool_map(_cpu_block_worker, jobs))
+    return {"value": cores * per_core * steps / wall, "unit": "cell-timesteps/s",
+            "cores": cores, "kind": "port",
+            "sample": "%d cells of the %d-cell plain-path workload: %d blocks of %d consecutive "
+                      "cells spread over it, one BDF with shared step control per block, one "
+                      "process per core, %d timed steps of %g s after %d untimed" % (
+                          cores * per_core, n_cells, cores, per_core, steps, dt, warm)}
+
+
+def _box_worker(args):
+    reps, = args
+    from oracle.oracle import Oracle
+    from camp_amd.mechanisms import box_mechanism, synthetic_cells
+    mech = box_mechanism(1)
+    cells = synthetic_cells(mech, 1, seed=2, T=298.0, P=101325.0)
+    best = None
+    for _ in range(reps):
+        o = Oracle(mech, 1)
+        o.set_rates(cells.rates)
+        st = np.ascontiguousarray(cells.state).copy()
+        env = np.ascontiguousarray(cells.env)
+        t = time.perf_counter()
+        for _ in range(60):
+            assert o.run(st, env, 0.0, 60.0, mode=1) == 0
+        el = time.perf_counter() - t
+        best = el if best is None else min(best, el)
+    return best
+
+
+def box_model_leg(reps=20):
+    """The reference's CPU-runnable configuration: the 10-species,
+    15-reaction box model, one cell at 298 K, one hour in 60 s steps at
+    rtol 1e-4 -- the oracle on one core, and the per-cell GPU kernel (one
+    launch per step, host arrays, as solver_run is called)."""
+    from oracle import oracle
+    oracle.build_native()
+    cpu_s = _pool_map(_box_worker, [(reps,)])[0]
+    from camp_amd.core import CampSolver
+    from camp_amd.mechanisms import box_mechanism, synthetic_cells
+    mech = box_mechanism(1)
+    cells = synthetic_cells(mech, 1, seed=2, T=298.0, P=101325.0)
+    s = CampSolver(mech, 1)
+    s.set_rates(cells.rates)
+    st = cells.state.copy()
+    assert s.solve(st, cells.env, 0.0, 60.0) == 0
+    st = cells.state.copy()
+    t = time.perf_counter()
+    for _ in range(60):
+        assert s.solve(st, cells.env, 0.0, 60.0) == 0
+    gpu_s = time.perf_counter() - t
+    s.close()
+    return {"what": "box model (10 species, 15 Arrhenius), 1 cell, 60 x 60 s steps, rtol 1e-4",
+            "unit": "cell-timesteps/s", "cpu_one_core": 60.0 / cpu_s,
+            "gpu_solver_run": 60.0 / gpu_s,
+            "note": "one cell cannot use a GPU; the per-call launch and copies dominate"}
 
 
 def serving(mech, solver, pool_state, env, rates, n_requests, mp_max_cells, mp_spun=None,
@@ -290,7 +400,7 @@ def main():
     ap.add_argument("--placement", type=int, default=-1,
                     help="per-cell kernel cell placement (0: history in LDS, 1: in global "
                          "memory); with --cells-per-block skips the autotuner")
-    ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-cores", type=int, default=min(16, len(os.sched_getaffinity(0))))
     ap.add_argument("--cpu-cells-per-core", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-cells", type=int, default=10000,
@@ -429,6 +539,11 @@ def main():
     plain = None
     if args.plain_cells > 0:
         plain = plain_path(mech, args.plain_cells, args.plain_steps, args.dt)
+        if not args.no_cpu:
+            progress("cpu block-mode baseline on %d cores" % args.cpu_cores)
+            plain["cpu_baseline_block"] = cpu_baseline_block(args.plain_cells, 1,
+                                                             args.plain_steps, args.dt,
+                                                             args.cpu_cores, 100)
 
     mpleg = mp_spun = None
     if args.mp_cells > 0:
@@ -442,11 +557,14 @@ def main():
                         args.serve_batch_cells)
         progress("serving leg done")
 
-    cpu = None
+    cpu = box = None
     if not args.no_cpu:
         progress("cpu baseline on %d cores" % args.cpu_cores)
         cpu = cpu_baseline(args.cells, args.warmup, args.steps, args.dt, args.cpu_cores,
                            args.cpu_cells_per_core)
+        progress("box model leg")
+        box = box_model_leg()
+    half = max(1, args.steps // 2)
 
     out = {
         "metric": "cell-timesteps/s at 100k cells, CB05-scale FP64; p50/p99 request latency",
@@ -464,7 +582,12 @@ def main():
                         "what": "serving request latency at 70% load"} if serve else None),
         "step_ms": {"p50": float(np.percentile(step_ms, 50)),
                     "p99": float(np.percentile(step_ms, 99)),
+                    "each": [round(x, 2) for x in step_ms],
                     "what": "one 180 s step over all cells"},
+        "steady_state": {"value": args.cells * half / (sum(step_ms[-half:]) / 1e3),
+                         "unit": "cell-timesteps/s",
+                         "what": "the last %d timed steps only (the cells are still relaxing "
+                                 "from the seeded state: earlier steps take longer)" % half},
         "failed_cell_steps": {"timed": failed_cell_steps, "warmup": warm_failed},
         "serving": serve,
         "solver_stats_per_cell_step": {"steps": float(tot[0] / args.cells / args.steps),
@@ -478,6 +601,7 @@ def main():
         "roofline": roofline,
         "roofline_fj": roofline_fj,
         "cpu_baseline": cpu,
+        "box_model": box,
     }
     print(json.dumps(out))
 

@@ -25,6 +25,25 @@ _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
 
 
+# CAMP_ORACLE_NATIVE=1 selects liboracle_native.so (the same source built
+# -O3 -march=native for this host: the benchmark's CPU baseline)
+if os.environ.get("CAMP_ORACLE_NATIVE") == "1":
+    LIB_PATH = os.path.join(HERE, "liboracle_native.so")
+
+NATIVE_FLAGS = "-std=c99 -O3 -march=native -fPIC -ffp-contract=off"
+
+
+def build_native(quiet=True):
+    """Compile liboracle_native.so for this host (always rebuilt, since a
+    copy built on another machine may not run here)."""
+    path = os.path.join(HERE, "liboracle_native.so")
+    if os.path.exists(path):
+        os.remove(path)
+    subprocess.run(["make", "-s", "-C", HERE, "native"], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+    return path
+
+
 def build(ref=True, quiet=True):
     """Compile liboracle.so, and oracle/_ref when the reference tree exists."""
     targets = ["oracle"]

@@ -77,3 +77,33 @@ def dense_from_csc(colptr, rowval, vals, n):
         for k in range(colptr[j], colptr[j + 1]):
             J[rowval[k], j] = vals[k]
     return J
+
+
+def _oracle_chunk(args):
+    mech, st, env, rates, rtol, atol, max_steps, steps, dt = args
+    from oracle.oracle import Oracle
+    o = Oracle(mech, st.shape[0], rel_tol=rtol, abs_tol=np.full(mech.n_state, atol),
+               max_steps=max_steps)
+    o.set_rates(rates)
+    x = np.ascontiguousarray(st).copy()
+    for _ in range(steps):
+        assert o.run(x, np.ascontiguousarray(env), 0.0, dt, mode=1) == 0
+    return x
+
+
+def oracle_per_cell(mech, state, env, rates, rtol, atol, steps, dt, max_steps=200000,
+                    workers=8):
+    """The oracle's per-cell BDF over `steps` steps of dt for these cells,
+    split over a process pool (CPU only); returns the final states."""
+    import multiprocessing as mp
+    parts = [p for p in np.array_split(np.arange(state.shape[0]), workers) if len(p)]
+    jobs = [(mech, state[p], env[p], {k: v[p] for k, v in rates.items()}, rtol, atol, max_steps,
+             steps, dt) for p in parts]
+    with mp.get_context("spawn").Pool(len(jobs)) as pool:
+        out = pool.map(_oracle_chunk, jobs)
+    return np.concatenate(out, axis=0)
+
+
+def bar_units(x, ref, rtol=1e-4, atol=1e-8):
+    """Per cell, the largest |x - ref| / (rtol |ref| + atol) over species."""
+    return np.max(np.abs(x - ref) / (rtol * np.abs(ref) + atol), axis=1)

@@ -189,8 +189,8 @@ def test_cb05_rtol_1e4_error_no_worse_than_oracle(oracle_lib, mode):
     spread: the oracle is also run with T perturbed by 1e-15 and 1e-14
     (relative), and the HIP path's median and 90th percentile of the cells'
     worst errors, in units of the 1e-4 / 1e-8 bar, may not exceed twice the
-    largest the oracle shows."""
-    n = 64 if mode == MODE_PER_CELL else 8
+    largest the oracle shows (no floor)."""
+    n = 64 if mode == MODE_PER_CELL else 32
     mech, cells, a, b = _cb05_pair(oracle_lib, n, mode, 1.0e-4, 1.0e-8)
     atol = np.full(mech.n_state, 1.0e-14)
     tight = oracle_lib.Oracle(mech, n, rel_tol=1.0e-10, abs_tol=atol, max_steps=200000)
@@ -221,8 +221,8 @@ def test_cb05_rtol_1e4_error_no_worse_than_oracle(oracle_lib, mode):
     g50, g90 = np.percentile(e_gpu, 50), np.percentile(e_gpu, 90)
     print("mode", mode, "error in bar units (median, p90): hip %.3g %.3g, oracle under "
           "perturbation %s %s" % (g50, g90, np.round(o50, 3), np.round(o90, 3)))
-    assert g50 <= 2.0 * max(max(o50), 1.0)
-    assert g90 <= 2.0 * max(max(o90), 1.0)
+    assert g50 <= 2.0 * max(o50)
+    assert g90 <= 2.0 * max(o90)
 
 
 def test_cell_alone_equals_cell_in_mix():
