@@ -236,9 +236,11 @@ def serving(mech, solver, pool_state, env, rates, n_requests, mp_max_cells, mp_s
             dev = torch.device("cuda:0")
             st_d = torch.from_numpy(mcells.state).to(dev)
             env_d = torch.from_numpy(mcells.env).to(dev)
+            progress("serving: spinning up %d multi-phase pool cells" % mp_pool)
             sp.solve_device(st_d.data_ptr(), env_d.data_ptr(), 0.0, dt)
             torch.cuda.synchronize()
             mstate = st_d.cpu().numpy()
+            progress("serving: pool ready")
         msol = CampSolver(mpm, 1)
         mpid = srv.add_mechanism(msol)
         srv.set_pool(mpid, np.ascontiguousarray(mstate[:mp_pool]), mcells.env[:mp_pool],
@@ -257,7 +259,10 @@ def serving(mech, solver, pool_state, env, rates, n_requests, mp_max_cells, mp_s
 
     k_cap = max(16, n_requests // 2)
     m, sizes, off, dts, steps = draw(k_cap)
+    progress("serving: capacity pass, %d requests at once" % k_cap)
     lat, span, failed_cap = srv.replay(np.zeros(k_cap), m, sizes, off, dts, steps)
+    progress("serving: capacity %.2f requests/s; %d Poisson arrivals at 70%%"
+             % (k_cap / span, n_requests))
     cap_rps = k_cap / span
     capacity = float(np.sum(sizes * steps)) / span
     m, sizes, off, dts, steps = draw(n_requests)

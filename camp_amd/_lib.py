@@ -84,6 +84,11 @@ SIGNATURES = {
     "camp_solver_work_model": (None, [_p, _dp]),
     # include/camp_server.h
     "camp_server_new": (_p, [_i, _l]),
+    "camp_hostmodel_new": (_p, [_p, _i, _i, _i, _i, _i, _i, _ip, _ip, _i]),
+    "camp_hostmodel_integrate": (_i, [_p, _d, _i, _i, _i, _i, _p, _p, _p, _i, _p, _p]),
+    "camp_hostmodel_set_state": (None, [_p, _dp]),
+    "camp_hostmodel_gather": (None, [_p, _i, _i, _i, _i, _p, _p, _p, _i, _p, _p, _dp, _dp]),
+    "camp_hostmodel_free": (None, [_p]),
     "camp_server_add_mechanism": (_i, [_p, _p, _l]),
     "camp_server_capacity": (_l, [_p, _i]),
     "camp_solver_rate_layout": (_i, [_p, _ip, _ip]),

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcamp_mi355x.so")
 SOURCES = ["camp_model.cpp", "camp_solver.cpp", "camp_server.cpp", "camp_kernels.hip",
-           "camp_cell_bdf.hip"]
+           "camp_cell_bdf.hip", "camp_host_model.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # bit-for-bit control of every multiply-add (the error-free transforms
          # of the double-double accumulators depend on it)

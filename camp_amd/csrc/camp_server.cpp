@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdio>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -584,6 +585,27 @@ extern "C" int camp_server_replay(void* server, int n_requests, const double* t_
   Server* s = SV(server);
   std::vector<long> ids(n_requests, -1);
   auto t0 = Clock::now();
+  // a progress line on stderr every 20 s of a long replay
+  Clock::time_point said = t0;
+  double done0 = 0.0;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    done0 = s->n_done;
+  }
+  auto progress = [&](int submitted) {
+    auto now = Clock::now();
+    if (now - said < std::chrono::seconds(20)) return;
+    said = now;
+    double done = 0.0;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      done = s->n_done - done0;
+    }
+    std::fprintf(stderr, "[server] replay: %d of %d requests submitted, %.0f done, %.0f s\n",
+                 submitted, n_requests, done,
+                 std::chrono::duration<double>(now - t0).count());
+    std::fflush(stderr);
+  };
   for (int i = 0; i < n_requests; ++i) {
     Arena* A;
     {
@@ -592,8 +614,12 @@ extern "C" int camp_server_replay(void* server, int n_requests, const double* t_
       A = s->mech(mech[i]);
     }
     if (!A->pool_n || offset[i] < 0 || offset[i] + n_cells[i] > A->pool_n) return -1;
-    std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
-                                           std::chrono::duration<double>(t_arrive[i])));
+    const auto due = t0 + std::chrono::duration_cast<Clock::duration>(
+                              std::chrono::duration<double>(t_arrive[i]));
+    while (Clock::now() < due) {
+      std::this_thread::sleep_until(std::min(due, Clock::now() + std::chrono::seconds(5)));
+      progress(i);
+    }
     long o = offset[i];
     ids[i] = camp_server_submit(server, mech[i], n_cells[i], A->pool_state + o * A->ns,
                                 A->pool_env + o * 2, A->pool_rates + o * std::max(A->n_upd, 1),
@@ -605,7 +631,11 @@ extern "C" int camp_server_replay(void* server, int n_requests, const double* t_
   for (int i = 0; i < n_requests; ++i) {
     std::unique_lock<std::mutex> g(s->mu);
     Request* r = s->reqs[ids[i]].get();
-    s->cv_done.wait(g, [r] { return r->state == kDone; });
+    while (!s->cv_done.wait_for(g, std::chrono::seconds(5), [r] { return r->state == kDone; })) {
+      g.unlock();
+      progress(n_requests);
+      g.lock();
+    }
     latency_s[i] = std::chrono::duration<double>(r->t_done - r->t_submit).count();
     last = std::max(last, r->t_done);
     failed += r->failed ? 1 : 0;
