@@ -499,6 +499,22 @@ def main():
     elapsed = time.perf_counter() - t0
     value = args.cells * args.steps / elapsed
 
+    # the same solver through solver_run on host arrays (state and env
+    # copied over PCIe inside each call), two steps after the timed ones
+    host_st = st.cpu().numpy().copy()
+    host_env = np.ascontiguousarray(cells.env).copy()
+    t_h = time.perf_counter()
+    k_h = 0.0
+    for _ in range(2):
+        solver.solve(host_st, host_env, 0.0, args.dt)
+        k_h += solver.last_kernel_ms()
+    el_h = time.perf_counter() - t_h
+    host_leg = {"value": args.cells * 2 / el_h, "unit": "cell-timesteps/s",
+                "ms_per_step": 1e3 * el_h / 2, "kernel_ms_per_step": k_h / 2,
+                "what": "solver_run on host arrays (state and env copied to and from HBM in "
+                        "each call), two steps after the timed ones"}
+    progress("host-array solver_run: %.0f ms/step (kernel %.0f)" % (1e3 * el_h / 2, k_h / 2))
+
     flops, bytes_ = solver.work_model()
     tot = np.sum(stats, axis=0)  # nst nfe nsetups netf nni ncfn nje
     nfe, nsetups, nni, nje = tot[1], tot[2], tot[4], tot[6]
@@ -607,6 +623,7 @@ def main():
         "roofline_fj": roofline_fj,
         "cpu_baseline": cpu,
         "box_model": box,
+        "host_arrays": host_leg,
     }
     print(json.dumps(out))
 
