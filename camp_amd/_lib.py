@@ -80,6 +80,7 @@ SIGNATURES = {
     "camp_solver_get_rxn_env": (None, [_p, _dp]),
     "camp_cr_math": (None, [_i, _i, _dp, _dp, _dp]),
     "camp_solver_autotune": (_i, [_p, _p, _p, _d, _d, _i, _dp]),
+    "camp_solver_phase_cycles": (_i, [_p, _p, _p, _i, _i, _dp]),
     "camp_solver_time_fj": (_i, [_p, _p, _p, _i, _dp, _dp]),
     "camp_solver_work_model": (None, [_p, _dp]),
     # include/camp_server.h

@@ -137,6 +137,14 @@ class CampSolver:
         return (best // 100, best % 100), {k: (float(v) if v >= 0 else None)
                                            for k, v in zip(cands, ms)}
 
+    def phase_cycles(self, state_ptr, env_ptr, n_sample=0, reps=4):
+        """Cycles of f, J, LU and one solve in the per-cell kernel (dict), or
+        None where not measurable (multi-phase, placement 2)."""
+        out = np.zeros(4)
+        rc = lib().camp_solver_phase_cycles(self.h, C.c_void_p(state_ptr), C.c_void_p(env_ptr),
+                                            int(n_sample), int(reps), _dp(out))
+        return None if rc else dict(zip(("f", "jac", "lu", "solve"), out.tolist()))
+
     def set_guess_helper(self, on):
         lib().camp_solver_set_guess_helper(self.h, int(bool(on)))
 

@@ -611,6 +611,42 @@ struct CellCv {
     return CV_SUCCESS;
   }
 
+  // Measurement: the cycles of f(y), J(y), the LU factorisation (gamma =
+  // 10) and one triangular solve at the cell's staged state, averaged over
+  // reps repetitions, into pr[0..3].  Integrates nothing.
+  __device__ int bench_phases(int reps, double* pr) {
+    for (int i = lane; i < nv; i += kWave) y[i] = zn(0)[i];
+    wsync();
+    next_h = init_h = 1.0;
+    if (jac(y, ftemp)) return -101;
+    long long cf = 0, cj = 0, clu = 0, cs = 0;
+    for (int r = 0; r < reps; ++r) {
+      long long t0 = clock64();
+      f(y, ftemp);
+      long long t1 = clock64();
+      jac(y, ftemp);
+      long long t2 = clock64();
+      if (cell_lu_factor(P, lane, Js, 10.0, lu)) return -102;
+      long long t3 = clock64();
+      for (int i = lane; i < nv; i += kWave) tempv[i] = ftemp[i];
+      wsync();
+      long long t4 = clock64();
+      lsolve(tempv);
+      long long t5 = clock64();
+      cf += t1 - t0;
+      cj += t2 - t1;
+      clu += t3 - t2;
+      cs += t5 - t4;
+    }
+    if (lane == 0) {
+      pr[0] = (double)cf / reps;
+      pr[1] = (double)cj / reps;
+      pr[2] = (double)clu / reps;
+      pr[3] = (double)cs / reps;
+    }
+    return -100;
+  }
+
   // CVodeReInit + CVodeSetInitStep + CVode(tout, CV_NORMAL); zn(0) holds y0
   // on entry, y holds y(tout) on successful exit.
   __device__ __forceinline__ int solve(double t0, double tout) {
@@ -675,16 +711,19 @@ struct CellCv {
 // programs run kPlaceGlobal with global LU factors).  Each variant is
 // compiled for the wavefronts per SIMD its LDS footprint lets onto a CU
 // (1, 2 and 3 for the CB05-scale mechanism), which bounds its registers.
-template <bool MP, int PL>
+// BENCH: the measurement variant (bench_phases instead of the solve).
+template <bool MP, int PL, bool BENCH = false>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(MP || PL == kPlaceLds ? 1 : PL == kPlaceGlobal ? 2 : 3)))
+    __attribute__((amdgpu_waves_per_eu(MP || PL == kPlaceLds ? 1 : 2)))
     k_cell_bdf(CellBdfArgs a) {
   extern __shared__ double smem[];
   const DevProgram& P = a.P;
   constexpr bool GH = MP || PL != kPlaceLds;
   constexpr bool GL = MP || PL == kPlaceGlobalLu;
   constexpr int kPlace = MP ? kPlaceGlobal : PL;
-  int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  // the wavefront index is uniform: a scalar, so the cell's LDS and scratch
+  // pointers derived from it live in scalar registers
+  int lane = threadIdx.x % kWave, wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   int nv = P.nv;
   size_t per_cell = cell_bdf_lds_doubles(P, kPlace);
   double* base = smem + (size_t)wv * per_cell;
@@ -750,12 +789,12 @@ __global__ void __launch_bounds__(256)
   cv.guess_on = a.guess_on;
   cv.maxcf = a.max_conv_fails;
   cv.max_steps = a.max_steps;
-  cv.prof = a.prof != nullptr;
+  cv.prof = !BENCH && a.prof != nullptr;
 
   for (;;) {
     int q = 0;
     if (lane == 0) q = atomicAdd(a.counter, 1);
-    q = __shfl(q, 0, kWave);
+    q = __builtin_amdgcn_readfirstlane(q);  // lane 0's ticket, as a scalar
     if (q >= a.nc) break;
     int cell = a.cell_list ? a.cell_list[q] : q;
     double t1 = a.cell_t1 ? a.cell_t1[cell] : a.t1;
@@ -813,7 +852,11 @@ __global__ void __launch_bounds__(256)
       wsync();
       cv.c_f = cv.c_j = cv.c_lu = cv.c_sol = cv.c_fr = cv.c_fg = 0;
       long long c_start = cv.tick();
-      int flag = cv.solve(a.t0, t1);
+      int flag;
+      if constexpr (BENCH)
+        flag = cv.bench_phases(a.bench_reps, a.prof + (size_t)cell * kCellProf);
+      else
+        flag = cv.solve(a.t0, t1);
       if (flag >= 0) {
         for (int i = lane; i < nv; i += kWave) {
           double yv = cv.y[i];
@@ -881,6 +924,22 @@ void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipS
     hipLaunchKernelGGL((k_cell_bdf<false, kPlaceGlobalLu>), dim3(grid), dim3(block), lds, s, a);
   else
     hipLaunchKernelGGL((k_cell_bdf<false, kPlaceLds>), dim3(grid), dim3(block), lds, s, a);
+}
+
+void launch_cell_bdf_bench(const CellBdfArgs& a, int grid, int block, size_t lds,
+                           hipStream_t s) {
+  if (a.P.mp) die("phase measurement: gas-phase programs only");
+  allow_lds(cell_bdf_fn(a.P, a.place), lds);
+  if (cell_placement(a.P, a.place) == kPlaceGlobal) {
+    allow_lds((const void*)k_cell_bdf<false, kPlaceGlobal, true>, lds);
+    hipLaunchKernelGGL((k_cell_bdf<false, kPlaceGlobal, true>), dim3(grid), dim3(block), lds, s,
+                       a);
+  } else if (cell_placement(a.P, a.place) == kPlaceLds) {
+    allow_lds((const void*)k_cell_bdf<false, kPlaceLds, true>, lds);
+    hipLaunchKernelGGL((k_cell_bdf<false, kPlaceLds, true>), dim3(grid), dim3(block), lds, s, a);
+  } else {
+    die("phase measurement: placements 0 and 1 only");
+  }
 }
 
 int cell_bdf_blocks_per_cu(const DevProgram& P, int place, int block, size_t lds) {

@@ -88,6 +88,18 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+// Read-only program data at a wave-uniform index through the scalar cache
+// (the program is never written while a kernel runs): the value lands in
+// scalar registers without a vector load and a wait on it.
+// (Host builds of the shared routines read it directly.)
+template <typename T>
+__host__ __device__ __forceinline__ T ld_uniform(const T* p, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+  return p[i];
+#endif
+}
 // Wavefront reductions with DPP lane moves (no LDS round trip): quad
 // butterflies, half-row and row mirrors give every lane its row's value,
 // then row broadcasts of lanes 15 and 31 carry the rows into lane 63, whose
@@ -385,7 +397,7 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
   for (int pass = 0; pass * kWave < nv; ++pass) {
     int slot = pass * kWave + lane;
     int i = slot < nv ? P.fs_species[slot] : -1;
-    int4 hd = P.fl_hdr[pass];
+    int4 hd = ld_uniform(P.fl_hdr, pass);
     const int2* er = P.re_ell + hd.z + lane;
     double acc = 0.0;
     if (use_est) {
@@ -536,7 +548,7 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
   // value -- and the pivot needs a single wavefront sync.
   constexpr int R = 5;
   const uint64_t sink = (uint64_t)(P.lu_nnz + lane);  // {dst = sink, l = 0, u = 0}
-  int4 h0 = P.lf_hdr[0], h1 = P.lf_hdr[1];
+  int4 h0 = ld_uniform(P.lf_hdr, 0), h1 = ld_uniform(P.lf_hdr, 1);
   int l_p = P.lf_pos[h0.x + lane];
   int nb0 = h0.z >> 16;
   uint64_t cur[R];
@@ -549,7 +561,7 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     cur[r] = r < nb0 ? v : sink;
   }
   for (int k = 0; k < P.nv; ++k) {
-    int4 h2 = P.lf_hdr[k + 2];
+    int4 h2 = ld_uniform(P.lf_hdr, k + 2);
     int n_lp = P.lf_pos[h1.x + lane];
     int nb1 = h1.z >> 16;
     uint64_t nxt[R];
@@ -664,33 +676,35 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
   const int4* tb = P.ts_tbl + 2 * lane;
   int blk = 0;
   int4 t0 = tb[0], t1 = tb[1];
+  int4 u0 = tb[2 * kWave], u1 = tb[2 * kWave + 1];
   const int kf = nv - 1;  // forward pivots 0 .. nv-2
   const int split = kf < kWave ? kf : kWave;
-  // the table has one block of padding past the last, so the look-ahead
-  // load of the next block never leaves it
+  // blocks are fetched two ahead (a block's work is shorter than the
+  // memory latency); the table has two blocks of padding past the last, so
+  // the look-ahead loads never leave it
   for (int kb = 0; kb < split; kb += B) {
-    int4 n0 = tb[2 * kWave * (blk + 1)], n1 = tb[2 * kWave * (blk + 1) + 1];
+    int4 n0 = tb[2 * kWave * (blk + 2)], n1 = tb[2 * kWave * (blk + 2) + 1];
     tri_block<1, false>(kb, 0, t0, t1, lu, 1.0, 1.0, x0, x1);
-    t0 = n0; t1 = n1; ++blk;
+    t0 = u0; t1 = u1; u0 = n0; u1 = n1; ++blk;
   }
   for (int kb = split; kb < kf; kb += B) {
-    int4 n0 = tb[2 * kWave * (blk + 1)], n1 = tb[2 * kWave * (blk + 1) + 1];
+    int4 n0 = tb[2 * kWave * (blk + 2)], n1 = tb[2 * kWave * (blk + 2) + 1];
     tri_block<1, true>(kb, kWave, t0, t1, lu, 1.0, 1.0, x0, x1);
-    t0 = n0; t1 = n1; ++blk;
+    t0 = u0; t1 = u1; u0 = n0; u1 = n1; ++blk;
   }
   double d0 = has0 ? 1.0 / lu[P.diag_pos[r0]] : 0.0;
   double d1 = has1 ? 1.0 / lu[P.diag_pos[r1]] : 0.0;
   x0 *= d0;
   x1 *= d1;
   for (int kb = nv - 1; kb >= kWave; kb -= B) {
-    int4 n0 = tb[2 * kWave * (blk + 1)], n1 = tb[2 * kWave * (blk + 1) + 1];
+    int4 n0 = tb[2 * kWave * (blk + 2)], n1 = tb[2 * kWave * (blk + 2) + 1];
     tri_block<-1, true>(kb, kWave, t0, t1, lu, d0, d1, x0, x1);
-    t0 = n0; t1 = n1; ++blk;
+    t0 = u0; t1 = u1; u0 = n0; u1 = n1; ++blk;
   }
   for (int kb = (nv - 1 < kWave - 1 ? nv - 1 : kWave - 1); kb >= 0; kb -= B) {
-    int4 n0 = tb[2 * kWave * (blk + 1)], n1 = tb[2 * kWave * (blk + 1) + 1];
+    int4 n0 = tb[2 * kWave * (blk + 2)], n1 = tb[2 * kWave * (blk + 2) + 1];
     tri_block<-1, false>(kb, 0, t0, t1, lu, d0, d1, x0, x1);
-    t0 = n0; t1 = n1; ++blk;
+    t0 = u0; t1 = u1; u0 = n0; u1 = n1; ++blk;
   }
   wsync();
   if (has0) b[P.perm[r0]] = x0;
@@ -719,8 +733,8 @@ __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, dou
     // {column range, diagonal position} of pivot k (empty outside 0..nv-1)
     auto range = [&](int k) -> int4 {
       if (k < 0 || k >= P.nv) return int4{0, 0, 0, 0};
-      int4 h = P.lu_hdr[2 * k];
-      int4 h2 = P.lu_hdr[2 * k + 1];
+      int4 h = ld_uniform(P.lu_hdr, 2 * k);
+      int4 h2 = ld_uniform(P.lu_hdr, 2 * k + 1);
       return fwd ? int4{h.x, h.y, h2.x, 0} : int4{h2.y, h2.z, h2.x, 0};
     };
     int k0 = fwd ? 0 : P.nv - 1, step = fwd ? 1 : -1;

@@ -95,9 +95,13 @@ struct CellBdfArgs {
   // programs, its global scratch) with quiet NaNs before it stages each
   // cell, so that any value a cell reads before writing shows in its result.
   int poison = 0;
+  int bench_reps = 0;  // launch_cell_bdf_bench: repetitions per phase
 };
 
 void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);
+// the measurement variant: per cell, the cycles of f, J, LU and one solve
+// into prof[cell][0..3] (gas-phase programs, placements 0 and 1)
+void launch_cell_bdf_bench(const CellBdfArgs& a, int grid, int block, size_t lds, hipStream_t s);
 // workgroups per CU of the variant (program kind and placement) at this
 // block size and LDS, 0 if it does not fit
 int cell_bdf_blocks_per_cu(const DevProgram& P, int place, int block, size_t lds);

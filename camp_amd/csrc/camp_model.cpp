@@ -1066,7 +1066,7 @@ std::string Model::initialize() {
     for (int kb = split; kb < kf; kb += 8, ++P.ts_fwd_blocks) block(kb, 1, split, kf);
     for (int kb = nv - 1; kb >= W; kb -= 8, ++P.ts_bwd_blocks) block(kb, -1, W, nv);
     for (int kb = std::min(nv - 1, W - 1); kb >= 0; kb -= 8, ++P.ts_bwd_blocks) block(kb, -1, 0, W);
-    P.ts_tbl.insert(P.ts_tbl.end(), (size_t)8 * W, zero | (zero << 16));  // look-ahead padding
+    P.ts_tbl.insert(P.ts_tbl.end(), (size_t)2 * 8 * W, zero | (zero << 16));  // look-ahead padding
   }
   // The factorisation's program: per pivot, the L positions to scale and the
   // update records {dst, l, u}, each padded to whole rows of 64 (at least

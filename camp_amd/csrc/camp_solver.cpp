@@ -1481,6 +1481,19 @@ extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
     hipLaunchKernelGGL(k_gather_rows, dim3(nblk(n)), dim3(256), 0, s->stream, ns, len, stride,
                        src, dst);
   };
+  // The sample's queue order: longest first by the last call's BDF step
+  // counts, as a solve orders its cells (else index order).
+  int* d_sample_order = nullptr;
+  if (s->stats_from_cells && s->h_stats.size() >= (size_t)nc * kCellStats) {
+    std::vector<std::pair<double, int>> by(ns);
+    for (long i = 0; i < ns; ++i) by[i] = {-s->h_stats[(size_t)(i * stride) * kCellStats], (int)i};
+    std::stable_sort(by.begin(), by.end());
+    std::vector<int> ord(ns);
+    for (long i = 0; i < ns; ++i) ord[i] = by[i].second;
+    std::vector<void*> o;
+    d_sample_order = dev_upload(ord, o);
+    tmp.push_back(d_sample_order);
+  }
   const int cand[6] = {1, 2, 4, 8, 16, 32};
   const int keep_cpb = s->cells_per_block, keep_place = s->place;
   int best = keep_cpb, best_place = cell_placement(P, keep_place);
@@ -1515,6 +1528,7 @@ extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
       a.t1 = t1;
       a.counter = s->d_counter;
       a.stats = stats;
+      a.cell_list = d_sample_order;
       int grid = 0, block = 0;
       s->cell_launch_shape(ns, &grid, &block, &lds);
       a.wave_scratch = s->wave_scratch((size_t)grid * (block / kWave), place);
@@ -1540,6 +1554,79 @@ extern "C" int camp_solver_autotune(void* solver_data, const double* d_state,
   s->cells_per_block = best;
   s->place = best_place;
   return best + 100 * best_place;
+}
+
+// Phase measurement of the per-cell kernel (gas-phase programs, placements
+// 0 and 1): every wavefront stages a cell of a sample (copies, as the
+// autotuner takes them) and times f(y), J(y), the LU factorisation and one
+// triangular solve at its state, reps times each; out4 receives the mean
+// cycles over the sample.  Returns 0, or 1 when the program or placement is
+// not measurable this way.
+extern "C" int camp_solver_phase_cycles(void* solver_data, const double* d_state,
+                                        const double* d_env, int n_sample, int reps,
+                                        double* out4) {
+  using namespace camp;
+  Solver* s = SD(solver_data);
+  if (!s->m.initialized || s->P.mp || cell_placement(s->P, s->place) == kPlaceGlobalLu ||
+      reps < 1)
+    return 1;
+  s->ensure_device();
+  s->upload_renv();
+  const DevProgram& P = s->P;
+  long nc = s->m.nc;
+  long ns = n_sample > 0 ? std::min<long>(n_sample, nc) : std::min<long>(nc, 4096);
+  long stride = std::max<long>(1, nc / ns);
+  std::vector<void*> tmp;
+  auto take = [&](long n) { return dev_alloc<double>((size_t)std::max<long>(n, 1), tmp); };
+  double* st = take(ns * P.ns);
+  double* env = take(ns * 2);
+  double* renv = take(ns * P.n_env);
+  double* js = take(ns * P.nnz);
+  double* jy = take(ns * P.nv);
+  double* jf = take(ns * P.nv);
+  double* stats = take(ns * kCellStats);
+  double* prof = take(ns * kCellProf);
+  auto gather = [&](const double* src, int len, double* dst) {
+    long n = ns * len;
+    hipLaunchKernelGGL(k_gather_rows, dim3(nblk(n)), dim3(256), 0, s->stream, ns, len, stride,
+                       src, dst);
+  };
+  gather(d_state, P.ns, st);
+  gather(d_env, 2, env);
+  gather(s->d_renv, P.n_env, renv);
+  gather(s->d_Js, P.nnz, js);
+  gather(s->d_Jy, P.nv, jy);
+  gather(s->d_Jf, P.nv, jf);
+  CellBdfArgs a = s->cell_args();
+  a.nc = (int)ns;
+  a.state = st;
+  a.env = env;
+  a.rxn_env = renv;
+  a.Js = js;
+  a.Jy = jy;
+  a.Jf = jf;
+  a.counter = s->d_counter;
+  a.stats = stats;
+  a.prof = prof;
+  a.bench_reps = reps;
+  int grid = 0, block = 0;
+  size_t lds = 0;
+  s->cell_launch_shape(ns, &grid, &block, &lds);
+  a.wave_scratch = s->wave_scratch((size_t)grid * (block / kWave), s->place);
+  HIP_OK(hipMemsetAsync(s->d_counter, 0, sizeof(int), s->stream));
+  launch_cell_bdf_bench(a, grid, block, lds, s->stream);
+  HIP_OK(hipGetLastError());
+  std::vector<double> h((size_t)ns * kCellProf);
+  HIP_OK(hipMemcpyAsync(h.data(), prof, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < 4; ++k) {
+    double acc = 0.0;
+    for (long c = 0; c < ns; ++c) acc += h[(size_t)c * kCellProf + k];
+    out4[k] = acc / (double)ns;
+  }
+  for (void* p : tmp) HIP_OK(hipFree(p));
+  return 0;
 }
 
 extern "C" void camp_solver_set_guess_helper(void* solver_data, int on) { SD(solver_data)->guess_on = on != 0; }

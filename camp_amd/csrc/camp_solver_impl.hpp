@@ -60,7 +60,9 @@ struct Solver {
   Model m;
   int mode = CAMP_MODE_PER_CELL;
   int cells_per_block = 1;
-  int place = kPlaceLds;  // per-cell kernel variant (cell_placement)
+  // per-cell kernel variant (cell_placement): history in global memory, the
+  // fastest for the CB05-scale mechanism (six cells per CU instead of four)
+  int place = kPlaceGlobal;
   int guess_on = 1;
   bool renv_dirty = true;
   hipStream_t stream = nullptr;

@@ -204,9 +204,9 @@ int camp_solver_set_cells_per_block(void *solver_data, int cells_per_block);
 /* Cell placement of the per-cell kernel (gas-phase programs; multi-phase
  * programs always use 1): 0 = each cell's Nordsieck history, work vectors
  * and J history in LDS (fewer cells per compute unit), 1 = those in global
- * memory (more cells per compute unit, two wavefronts per SIMD), 2 = those
- * and the LU factors in global memory (three wavefronts per SIMD).  Returns
- * the placement in effect.  Results do not depend on it. */
+ * memory (more cells per compute unit, two wavefronts per SIMD; the
+ * default), 2 = those and the LU factors in global memory.  Returns the
+ * placement in effect.  Results do not depend on it. */
 int camp_solver_set_placement(void *solver_data, int place);
 
 /* Autotuner for the per-cell kernel: solves a sample of n_sample cells
@@ -221,6 +221,17 @@ int camp_solver_set_placement(void *solver_data, int place);
 int camp_solver_autotune(void *solver_data, const double *d_state,
                          const double *d_env, double t_initial, double t_final,
                          int n_sample, double *ms_out);
+
+/* Measurement hook: the per-cell kernel's cycles for f(y), J(y), the LU
+ * factorisation and one triangular solve, each timed reps times at the
+ * state of every cell of a sample of n_sample cells (every
+ * (n_cells/n_sample)-th, copies; 0 = up to 4096) with the solver's placement
+ * and cells per workgroup, as many cells resident as in a solve.  out4
+ * receives the mean cycles over the sample.  Returns 0, or 1 for
+ * multi-phase programs and placement 2. */
+int camp_solver_phase_cycles(void *solver_data, const double *d_state,
+                             const double *d_env, int n_sample, int reps,
+                             double *out4);
 
 /* Enable (1, default) or disable (0) the negative-prediction guess helper
  * (camp_solver.c:1321-1408). */
