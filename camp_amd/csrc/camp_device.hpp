@@ -547,6 +547,7 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
   // pivot's reciprocal -- the same product that is stored as the scaled L
   // value -- and the pivot needs a single wavefront sync.
   constexpr int R = 5;
+  bool bad = false;
   const uint64_t sink = (uint64_t)(P.lu_nnz + lane);  // {dst = sink, l = 0, u = 0}
   int4 h0 = ld_uniform(P.lf_hdr, 0), h1 = ld_uniform(P.lf_hdr, 1);
   int l_p = P.lf_pos[h0.x + lane];
@@ -579,7 +580,11 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
       oa[r] = lu[(int)((cur[r] >> 16) & 0xFFFF)];
       ob[r] = lu[(int)((cur[r] >> 32) & 0xFFFF)];
     }
-    if (piv == 0.0 || !isfinite(piv)) return 1;
+    // a zero or non-finite pivot fails the factorisation; the check is
+    // folded into a flag tested after the loop, so it does not make the
+    // pivot's load a wait ahead of the others (the values computed after a
+    // bad pivot are discarded with the failure)
+    bad |= !(piv != 0.0 && isfinite(piv));
     double rinv = 1.0 / piv;
     lu[l_p] = lval * rinv;
     for (int r = 1; r < (h0.z & 0xFFFF); ++r) {
@@ -615,7 +620,7 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
 #pragma unroll
     for (int r = 0; r < R; ++r) cur[r] = nxt[r];
   }
-  return 0;
+  return bad ? 1 : 0;
 }
 
 __device__ __forceinline__ double readlane_d(double v, int l) {

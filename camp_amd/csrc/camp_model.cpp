@@ -1098,9 +1098,10 @@ std::string Model::initialize() {
     h[2] = rl | (ru << 16);
     h[3] = P.diag_pos[k];
   }
-  // 5 rows of sink records past the end: the device loads a pivot's first
-  // 5 rows unconditionally and keeps those past its own
-  for (int q = 0; q < 5 * 64; ++q) P.up_rec.push_back((uint64_t)(P.lu_nnz + q % 64));
+  // 16 rows of sink records past the end: the device loads a pivot's first
+  // rows (a fixed number, at most 16) unconditionally and keeps those past
+  // its own
+  for (int q = 0; q < 16 * 64; ++q) P.up_rec.push_back((uint64_t)(P.lu_nnz + q % 64));
   initialized = true;
   return std::string();
 }
