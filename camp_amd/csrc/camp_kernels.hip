@@ -330,8 +330,8 @@ __device__ __forceinline__ GjOps gj_ops(int4 x, int4 w, const double* sy, const 
 }
 // cell_jac's record: k * the other reactants; loss records (coefficient -1)
 // unguarded, others guarded; the element's sum stored after its last record
-__device__ __forceinline__ void gj_acc(const GjOps& o, double h, int nnz, bool valid, double* Jc,
-                                       dd& D) {
+// (into the cell's LDS copy of J)
+__device__ __forceinline__ void gj_acc(const GjOps& o, double h, int nnz, double* Jc, dd& D) {
   double v = o.k;
   if (o.h1) v *= o.y1;
   if (o.h2) v *= o.y2;
@@ -344,7 +344,7 @@ __device__ __forceinline__ void gj_acc(const GjOps& o, double h, int nnz, bool v
   }
   D = dd_add(D, dd{add, 0.0});
   if (o.last) {
-    if (valid && o.e < nnz) Jc[o.e] = D.hi;
+    if (o.e < nnz) Jc[o.e] = D.hi;
     D = dd{0.0, 0.0};
   }
 }
@@ -359,17 +359,18 @@ __global__ void __launch_bounds__(64) k_jac_g(DevProgram P, int nc, const double
   const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave, wpb = blockDim.x / kWave;
   const int cs = lane / G, j = lane % G;
   const int SC = group_cell_stride(P, false);
-  double* s = smem + (size_t)wv * C * SC;
+  const int nnz = P.nnz;
+  double* s = smem + (size_t)wv * C * (SC + nnz);
   double* sy = s + cs * SC;
   const double* sk = sy + P.ns;
+  // the wavefront's cells' Jacobians, in LDS until they are written out
+  double* sJ = s + (size_t)C * SC;
+  double* Jc = sJ + (size_t)cs * nnz;
   const int4* rec = P.gj_rec[g_index<G>()] + 2 * j;
   const int rows = P.g_j_rows[g_index<G>()];
   constexpr int RS = 2 * G;
   for (int base = (blockIdx.x * wpb + wv) * C; base < nc; base += gridDim.x * wpb * C) {
     g_stage<G, false>(P, nc, base, lane, state, y, kval, s);
-    const int c = base + cs;
-    const bool valid = c < nc;
-    double* Jc = J + (size_t)(valid ? c : base) * P.nnz;
     dd D = {0.0, 0.0};
     int4 a0 = rec[0], b0 = rec[1], a1 = rec[RS], b1 = rec[RS + 1];
     int4 a2 = rec[2 * RS], b2 = rec[2 * RS + 1], a3 = rec[3 * RS], b3 = rec[3 * RS + 1];
@@ -380,11 +381,18 @@ __global__ void __launch_bounds__(64) k_jac_g(DevProgram P, int nc, const double
       a2 = nx[2 * RS]; b2 = nx[2 * RS + 1]; a3 = nx[3 * RS]; b3 = nx[3 * RS + 1];
       GjOps o0 = gj_ops(x0, w0, sy, sk), o1 = gj_ops(x1, w1, sy, sk),
             o2 = gj_ops(x2, w2, sy, sk), o3 = gj_ops(x3, w3, sy, sk);
-      gj_acc(o0, h, P.nnz, valid, Jc, D);
-      gj_acc(o1, h, P.nnz, valid, Jc, D);
-      gj_acc(o2, h, P.nnz, valid, Jc, D);
-      gj_acc(o3, h, P.nnz, valid, Jc, D);
+      gj_acc(o0, h, nnz, Jc, D);
+      gj_acc(o1, h, nnz, Jc, D);
+      gj_acc(o2, h, nnz, Jc, D);
+      gj_acc(o3, h, nnz, Jc, D);
     }
+    wsync();
+    // the cells base .. base+C-1 are consecutive in J: one contiguous,
+    // coalesced write of their Jacobians (every element has a record)
+    const int n_cells = nc - base < C ? nc - base : C;
+    const size_t n = (size_t)n_cells * nnz;
+    double* out = J + (size_t)base * nnz;
+    for (size_t i = lane; i < n; i += kWave) out[i] = sJ[i];
     wsync();
   }
 }
@@ -400,8 +408,12 @@ static void launch_group_fj(bool jac, const DevProgram& P, int nc, const double*
   int blocks = (nc + C - 1) / C;
   blocks = blocks < 1 ? 1 : blocks > (1 << 22) ? (1 << 22) : blocks;
   if (jac) {
-    hipLaunchKernelGGL((k_jac_g<G>), dim3(blocks), dim3(kWave), group_lds_bytes(P, G, false), st,
-                       P, nc, y, state, kval, out, h);
+    const size_t lds = group_jac_lds_bytes(P, G);
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void*)k_jac_g<G>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_jac_g<G>), dim3(blocks), dim3(kWave), lds, st, P, nc, y, state, kval,
+                       out, h);
   } else if (use_est) {
     hipLaunchKernelGGL((k_deriv_g<G, true>), dim3(blocks), dim3(kWave),
                        group_lds_bytes(P, G, true), st, P, nc, y, state, kval, Js, Jy, Jf, out,

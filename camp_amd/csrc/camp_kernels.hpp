@@ -142,6 +142,11 @@ __host__ __device__ inline int group_cell_stride(const DevProgram& P, bool fsum)
 inline size_t group_lds_bytes(const DevProgram& P, int lanes, bool fsum) {
   return (size_t)(kWave / lanes) * group_cell_stride(P, fsum) * sizeof(double);
 }
+// The J kernel also stages its cells' Jacobians (nnz each) and writes them
+// out contiguously.
+inline size_t group_jac_lds_bytes(const DevProgram& P, int lanes) {
+  return group_lds_bytes(P, lanes, false) + (size_t)(kWave / lanes) * P.nnz * sizeof(double);
+}
 __global__ void k_lu_factor(DevProgram P, int nc, const double* J, double gamma, double* LU,
                             int* fail);
 // sp (may be null): the scale read from device memory instead of s
