@@ -535,8 +535,12 @@ def main():
                 "kernel": "k_cell_bdf", "avg_launch_ms": kernel_s * 1e3 / args.steps,
                 "flops_per_launch": total_flops / args.steps,
                 "algorithmic_hbm_bytes_per_launch": hbm_bytes_per_launch,
-                "note": "FP64 VALU work (peak 78.6 TFLOP/s is the FP64 figure); the cell "
-                        "stays in LDS, HBM moves only state, rates and J history"}
+                "placement": ("history in LDS", "history in global memory",
+                              "history and LU in global memory")[solver.set_placement(-1)],
+                "note": "FP64 VALU work (peak 78.6 TFLOP/s is the FP64 figure); algorithmic "
+                        "HBM bytes are state, env, rates and J history in and out; with the "
+                        "history in global memory the cells' Nordsieck history, work vectors "
+                        "and J history also stream through L2 ('traffic')"}
 
     # plain-path f/J kernels against HBM (lanes per cell tuned first)
     fj_lanes, fj_cand = solver.tune_fj(st.data_ptr(), env.data_ptr(), reps=10)
