@@ -46,6 +46,30 @@ __device__ __forceinline__ double rpoweri(double b, int e) {
   return e < 0 ? 1.0 / p : p;
 }
 
+// A wave-uniform double kept in scalar registers: every assignment reads the
+// first active lane's value (all lanes hold the same one), so the value
+// lives in an SGPR pair between uses instead of occupying two VGPRs for the
+// whole solve (the integrator's step-control scalars).
+struct UDouble {
+  double v;
+  UDouble() = default;
+  __device__ UDouble(double x) : v(uni(x)) {}
+  __device__ UDouble& operator=(double x) {
+    v = uni(x);
+    return *this;
+  }
+  __device__ UDouble& operator+=(double x) { return *this = v + x; }
+  __device__ UDouble& operator-=(double x) { return *this = v - x; }
+  __device__ UDouble& operator*=(double x) { return *this = v * x; }
+  __device__ UDouble& operator/=(double x) { return *this = v / x; }
+  __device__ operator double() const { return v; }
+  __device__ static double uni(double x) {
+    int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+  }
+};
+
 // MP selects the multi-phase instantiation: gas-phase programs run a kernel
 // without the aerosol code (smaller code and register footprint).
 template <bool MP>
@@ -60,9 +84,9 @@ struct CellCv {
   double *Js, *Jy, *Jf;
   MpLds mpl;   // multi-phase extras (LDS); unused for gas-phase programs
   double* Jr;  // reaction Jacobian scratch (global) of multi-phase programs
-  double tn, h, hscale, hprime, next_h, eta, etamax, etaq, etaqm1, etaqp1, hu, init_h, rtol;
+  UDouble tn, h, hscale, hprime, next_h, eta, etamax, etaq, etaqm1, etaqp1, hu, init_h, rtol;
   double *l, *tq, *tau;  // small coefficient arrays, in LDS (uniform values)
-  double rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
+  UDouble rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
   int q, qprime, L, qwait, jcur, use_acinit, guess_on, maxcf, max_steps, projected;
   int nst, nfe, nsetups, nni, ncfn, netf, nje, nstlp, nstlj;
   // optional shader-clock counters (cycles in f, Jac, LU factor, LU solve)

@@ -88,6 +88,17 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+// The lane index made opaque to the compiler at the start of a phase (f, J,
+// LU, solve): per-lane program addresses derived from it are then computed
+// inside the phase instead of being hoisted to the kernel's start and held
+// in registers through the whole solve (they raised the per-cell kernel's
+// register use past the two-wavefront bound).
+__host__ __device__ __forceinline__ int phase_lane(int lane) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(lane));
+#endif
+  return lane;
+}
 // Read-only program data at a wave-uniform index through the scalar cache
 // (the program is never written while a kernel runs): the value lands in
 // scalar registers without a vector load and a wait on it.
@@ -254,6 +265,7 @@ __device__ __forceinline__ void cell_rate_constants(const DevProgram& P, int lan
 // first-order loss rounds k * y to double (rxn_first_order_loss.c:150).
 __host__ __device__ __forceinline__ void cell_rates(const DevProgram& P, int lane, const double* s_state,
                                            const double* s_k, double* s_rate) {
+  lane = phase_lane(lane);
   for (int t = lane; t < P.n_terms; t += kWave) {
     int4 tr = P.t_rec[t];
     int fl = tr.x >> 28;
@@ -369,6 +381,7 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
                                            const double* Jf, bool use_est, double h,
                                            double* out, double* s_fb,
                                            const double* s_slot = nullptr) {
+  lane = phase_lane(lane);
   const int nv = P.nv, L = P.fb_rows;
   const double* slot_or_rate = s_slot ? s_slot : s_rate;
   {
@@ -458,6 +471,7 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
 // returns 1 (wave-uniform) if any y < -SMALL, else copies y into s_state.
 __device__ __forceinline__ int cell_set_state(const DevProgram& P, int lane, const double* y,
                                               double* s_state) {
+  lane = phase_lane(lane);
   int bad = 0;
   for (int i = lane; i < P.nv; i += kWave) bad |= (y[i] < -1.0e-30);
   bad = wave_or(bad);
@@ -499,6 +513,7 @@ __host__ __device__ __forceinline__ dd jac_elem_mass_action(const DevProgram& P,
 
 __device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const double* s_state,
                                          const double* s_k, double h, double* J) {
+  lane = phase_lane(lane);
   for (int e = lane; e < P.nnz; e += kWave) {
     dd D = {0.0, 0.0};
     for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q) {
@@ -527,6 +542,7 @@ __device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const do
 // ---------------------------------------------------------------------
 __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, const double* J,
                                               double gamma, double* lu) {
+  lane = phase_lane(lane);
   for (int p = lane; p < P.lu_nnz; p += kWave) lu[p] = 0.0;
   wsync();
   for (int e = lane; e < P.nnz; e += kWave) {
@@ -669,6 +685,7 @@ __device__ __forceinline__ void tri_block(int kb, int k_base, int4 t0, int4 t1, 
 // sits on the chain.
 __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane, double* lu,
                                                   double* b) {
+  lane = phase_lane(lane);
   const int nv = P.nv;
   const int r0 = lane, r1 = lane + kWave;
   const bool has0 = r0 < nv, has1 = r1 < nv;
@@ -720,6 +737,7 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
 // Solve (LU) x = b in place; b in dep order, w a scratch vector (nv).
 __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, double* lu,
                                               double* b, double* w) {
+  lane = phase_lane(lane);
   if (P.ts_tbl) {
     cell_lu_solve_reg(P, lane, lu, b);
     return;

@@ -424,6 +424,7 @@ __device__ __forceinline__ void mp_cell_env(const DevProgram& P, int lane, doubl
 // state, then ZSR writes the new water.  Jac() runs f() first, so its own
 // update sees the water that f() left.
 __device__ __forceinline__ void mp_refresh(const DevProgram& P, int lane, double* s_state, MpLds L) {
+  lane = phase_lane(lane);
   for (int i = lane; i < P.n_pi; i += kWave) mp_phase_update(P, i, s_state, L.aero + kAeroSlots * i);
   wsync();
   for (int z = lane; z < P.n_zi; z += kWave) mp_zsr_water(P, z, L.sub_env, s_state);
@@ -434,6 +435,7 @@ __device__ __forceinline__ void mp_refresh(const DevProgram& P, int lane, double
 // multi-phase reactions into their f slots.
 __device__ __forceinline__ void mp_f_prologue(const DevProgram& P, int lane, double* s_state,
                                               const double* s_k, MpLds L) {
+  lane = phase_lane(lane);
   mp_refresh(P, lane, s_state, L);
   for (int i = lane; i < P.n_sp; i += kWave) mp_simpol_f(P, i, s_state, s_k, L.aero, L.slot);
   for (int i = lane; i < P.n_aq; i += kWave) mp_aqeq_f(P, i, s_state, s_k, L.slot);
@@ -445,6 +447,7 @@ __device__ __forceinline__ void mp_f_prologue(const DevProgram& P, int lane, dou
 // element 0 is then set to 1 (the map's dummy, camp_solver.c:1099).
 __device__ __forceinline__ void mp_j_prologue(const DevProgram& P, int lane, const double* s_state,
                                               const double* s_k, MpLds L) {
+  lane = phase_lane(lane);
   for (int i = lane; i < P.n_pi; i += kWave)
     mp_phase_partial_sums(P, i, s_state, L.aero + kAeroSlots * i);
   for (int q = lane; q < P.pnnz; q += kWave) L.jp[q] = 0.0;
@@ -465,6 +468,7 @@ __device__ __forceinline__ void mp_j_prologue(const DevProgram& P, int lane, con
 __device__ __forceinline__ void cell_jac_mp(const DevProgram& P, int lane, const double* s_state,
                                             const double* s_k, double h, MpLds L, double* Jr,
                                             double* J) {
+  lane = phase_lane(lane);
   for (int e = lane; e < P.rnnz; e += kWave) {
     dd D = jac_elem_mass_action(P, e, s_state, s_k, h);
     D = mp_jac_records(P, e, s_state, L.aero, L.jsc, D);
