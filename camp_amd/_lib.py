@@ -66,6 +66,7 @@ SIGNATURES = {
     "camp_solver_tune_fj": (_i, [_p, _p, _p, _i, _dp]),
     "camp_solver_set_fj_lanes": (_i, [_p, _i, _i]),
     "camp_solver_set_graphs": (_i, [_p, _i]),
+    "camp_solver_cell_lds_bytes": (_l, [_p, _i]),
     "camp_solver_run_device_noalloc": (_i, [_p, _p, _p, _d, _d]),
     "rxn_wet_deposition_create_rate_update_data": (_p, []),
     "rxn_wet_deposition_set_rate_update_data": (None, [_p, _i, _d]),

@@ -771,18 +771,33 @@ __global__ void __launch_bounds__(256)
               : GL ? scratch + P.lu_nnz + kWave + 1
                    : scratch;
   double*& vh = GH ? g : v;
+  constexpr bool kNewtonLds = !MP && PL == kPlaceGlobal;
   cv.znb = vh;
   vh += 6 * (size_t)nv;
   cv.last = vh; vh += nv;
   cv.acinit = vh; vh += nv;
   cv.ewt = vh; vh += nv;
-  cv.tempv = vh; vh += nv;
-  cv.ftemp = vh; vh += nv;
+  // gas-phase cells with the history in global memory keep the Newton
+  // iteration's two work vectors in LDS as well (six cells still fit a CU)
+  if (kNewtonLds) {
+    cv.tempv = v; v += nv;
+    cv.ftemp = v; v += nv;
+  } else {
+    cv.tempv = vh; vh += nv;
+    cv.ftemp = vh; vh += nv;
+  }
   cv.gh1 = vh; vh += nv;
   cv.gh2 = vh; vh += nv;
   cv.acor = v; v += nv;
   cv.y = v; v += nv;
-  cv.w = v; v += nv;
+  // the column solve's scratch w (nv > 128 only: smaller programs solve in
+  // registers) stays in LDS except for gas-phase cells with the history in
+  // global memory, whose LDS it would push past six cells per CU
+  if (kNewtonLds) {
+    cv.w = vh; vh += nv;
+  } else {
+    cv.w = v; v += nv;
+  }
   // Without GH the J history (J_solver, J_state, J_deriv) is staged in LDS
   // for the whole solve; with GH it is used in place in HBM.
   double *h_js = nullptr, *h_jy = nullptr, *h_jf = nullptr;

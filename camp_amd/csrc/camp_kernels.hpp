@@ -21,18 +21,19 @@ constexpr int kCellProf = 7;
 //                gather's scratch, the LU factors, the J history and 24
 //                slots for the BDF coefficient arrays l, tq and tau (gas
 //                phase only: 4 cells per CU for the CB05-scale mechanism);
-//  kPlaceGlobal  the Nordsieck history and 7 of the 10 work vectors in
-//                per-wavefront global scratch and the J history used in
-//                place in HBM, so that more cells fit a CU's LDS (6 for the
-//                CB05-scale mechanism; compiled for two wavefronts per SIMD).
-//                Multi-phase programs always run so, with their LU factors,
-//                reaction Jacobian and per-instance Jacobian scalars in the
-//                scratch as well.
-//  kPlaceGlobalLu  as kPlaceGlobal, and the LU factors (with the
-//                factorisation's sink slots and the solve's zero slot) in
-//                the scratch too (gas phase: 12 cells per CU for the
-//                CB05-scale mechanism, compiled for three wavefronts per
-//                SIMD).
+//  kPlaceGlobal  the Nordsieck history and most work vectors in
+//                per-wavefront global scratch (gas phase: all but y, acor,
+//                tempv and ftemp; multi-phase: all but y, acor and w) and
+//                the J history used in place in HBM, so that more cells fit
+//                a CU's LDS (6 for the CB05-scale mechanism; compiled for
+//                two wavefronts per SIMD).  Multi-phase programs always run
+//                so, with their LU factors, reaction Jacobian and
+//                per-instance Jacobian scalars in the scratch as well.
+//  kPlaceGlobalLu  the Nordsieck history, 7 of the 10 work vectors and the
+//                LU factors (with the factorisation's sink slots and the
+//                solve's zero slot) in the scratch (gas phase: 12 cells per
+//                CU for the CB05-scale mechanism, compiled for two
+//                wavefronts per SIMD).
 constexpr int kPlaceLds = 0, kPlaceGlobal = 1, kPlaceGlobalLu = 2, kNumPlacements = 3;
 __host__ __device__ inline int cell_placement(const DevProgram& P, int place) {
   return P.mp ? kPlaceGlobal : place;
@@ -41,10 +42,17 @@ __host__ __device__ inline int cell_placement(const DevProgram& P, int place) {
 __host__ __device__ inline bool cell_lu_global(const DevProgram& P, int place) {
   return P.mp || place == kPlaceGlobalLu;
 }
+// work vectors of a cell in LDS: all 16 with the history in LDS; for
+// gas-phase cells with the history in global memory y, acor and the Newton
+// iteration's tempv and ftemp; otherwise y, acor and the solve's w
+__host__ __device__ inline int cell_lds_vectors(const DevProgram& P, int place) {
+  const int pl = cell_placement(P, place);
+  return pl == kPlaceLds ? 16 : (!P.mp && pl == kPlaceGlobal) ? 4 : 3;
+}
 __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P, int place) {
   const bool g = cell_placement(P, place) != kPlaceLds;
   return (size_t)P.ns + P.n_env + mp_lds_doubles(P) + 2 * (size_t)P.n_terms +
-         (g ? 3 : 16) * (size_t)P.nv + fb_sink_doubles(P) +
+         (size_t)cell_lds_vectors(P, place) * P.nv + fb_sink_doubles(P) +
          (cell_lu_global(P, place) ? 0 : (size_t)P.lu_nnz) +
          (g ? 0 : (size_t)P.nnz + 2 * (size_t)P.nv) + 24;
 }
@@ -52,11 +60,12 @@ __host__ __device__ inline size_t cell_bdf_lds_doubles(const DevProgram& P, int 
 // (with the factorisation's 64 sink slots and the solve's zero slot), for
 // multi-phase programs then the reaction Jacobian and the per-instance
 // Jacobian scalars, then (kPlaceGlobal, kPlaceGlobalLu) the Nordsieck
-// history and 7 of the 10 work vectors (13 vectors).
+// history and the work vectors not in LDS (13 vectors; 12 for gas-phase
+// kPlaceGlobal).
 __host__ __device__ inline size_t cell_bdf_wave_scratch(const DevProgram& P, int place) {
   if (P.mp) return (size_t)P.lu_nnz + kWave + 1 + P.rnnz + mp_jsc_doubles(P) + 13 * (size_t)P.nv;
   if (place == kPlaceGlobalLu) return (size_t)P.lu_nnz + kWave + 1 + 13 * (size_t)P.nv;
-  return place == kPlaceGlobal ? 13 * (size_t)P.nv : 0;
+  return place == kPlaceGlobal ? 12 * (size_t)P.nv : 0;
 }
 
 // The no-allocation entry point's scratch (a file-scope device array of

@@ -1434,6 +1434,14 @@ extern "C" int camp_solver_set_eval_kernels(void* solver_data, int group) {
   return old;
 }
 
+extern "C" long camp_solver_cell_lds_bytes(void* solver_data, int place) {
+  using namespace camp;
+  Solver* s = SD(solver_data);
+  if (!s->m.initialized || place < 0 || place >= kNumPlacements) return -1;
+  s->ensure_device();
+  return (long)(cell_bdf_lds_doubles(s->P, cell_placement(s->P, place)) * sizeof(double));
+}
+
 extern "C" int camp_solver_set_placement(void* solver_data, int place) {
   using namespace camp;
   Solver* s = SD(solver_data);

@@ -209,6 +209,11 @@ int camp_solver_set_cells_per_block(void *solver_data, int cells_per_block);
  * placement in effect.  Results do not depend on it. */
 int camp_solver_set_placement(void *solver_data, int place);
 
+/* LDS bytes one cell of the per-cell kernel uses with the given placement
+ * (-1 for an unknown placement or an uninitialised solver; makes the
+ * solver's device buffers). */
+long camp_solver_cell_lds_bytes(void *solver_data, int place);
+
 /* Autotuner for the per-cell kernel: solves a sample of n_sample cells
  * (every (n_cells / n_sample)-th; 0 = up to 8192) over [t_initial, t_final]
  * from the device-resident state/env once per candidate -- placement {0, 1, 2}
