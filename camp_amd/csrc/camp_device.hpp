@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include "camp_dd.hpp"
+#include "camp_model.hpp"
 
 namespace camp {
 
@@ -29,6 +30,8 @@ struct DevProgram {
   // group width (8, 16, 32 lanes per cell)
   int g_f_rows[3], g_j_rows[3], n_k, n_fzero;
   const int4 *gf_rec[3], *gj_rec[3];   // 2 int4 per record
+  const int4* cj_rec;                  // per-cell J (Program::cj_rec), 2 int4 per record
+  int cj_rows;
   const int *k_slot, *f_count, *f_zero;
   const int *r_ptr, *r_elem, *r_col;  // row view of the solver pattern
   int n_cs;                            // state variables that are not solver variables
@@ -88,6 +91,14 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+// Wait for every outstanding global load and store of the wavefront
+// (s_waitcnt vmcnt(0); expcnt and lgkmcnt left open).  Placed in uniform
+// control flow before a loop that keeps loads in flight across iterations:
+// the compiler counts those loads exactly only when no store is pending on
+// some path into the loop; with one pending (stores and loads complete in no
+// fixed order) it waits for all loads at the top of every iteration, and
+// the look-ahead hides nothing.
+__device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 // The lane index made opaque to the compiler at the start of a phase (f, J,
 // LU, solve): per-lane program addresses derived from it are then computed
 // inside the phase instead of being hoisted to the kernel's start and held
@@ -487,7 +498,7 @@ __device__ __forceinline__ int cell_set_state(const DevProgram& P, int lane, con
 // ---------------------------------------------------------------------
 // J(y) of one cell into the solver Jacobian (CSC element order): per-type
 // calc_jac_contrib + jacobian_output + jac_map (camp_solver.c:1052-1106,
-// Jacobian.c:223-241).  Lane e gathers the contributions of element e.
+// Jacobian.c:223-241).
 // ---------------------------------------------------------------------
 // Mass-action contributions to Jacobian element e of the gather space
 // (solver elements, or reaction-pattern elements for multi-phase programs).
@@ -511,26 +522,75 @@ __host__ __device__ __forceinline__ dd jac_elem_mass_action(const DevProgram& P,
   return D;
 }
 
+// Operands of one J record.
+struct GjOps {
+  int e, last, g, h1, h2;
+  double coef, k, y1, y2, yw, sg;
+};
+__device__ __forceinline__ GjOps gj_ops(int4 x, int4 w, const double* sy, const double* sk) {
+  GjOps o;
+  o.e = x.x & 0x3FFFFFFF;
+  o.last = (x.x >> 30) & 1;
+  o.g = x.y;
+  o.coef = hilo_to_double(x.w, x.z);
+  o.h1 = w.y >= 0;
+  o.h2 = w.z >= 0;
+  o.k = sk[w.x];
+  o.y1 = sy[o.h1 ? w.y : 0];
+  o.y2 = sy[o.h2 ? w.z : 0];
+  o.yw = sy[w.w >= 0 ? w.w : 0];
+  o.sg = sy[o.g >= 0 ? o.g : 0];
+  return o;
+}
+// One J record: k * the other reactants; loss records (coefficient -1)
+// unguarded, others guarded (rxn_arrhenius.c:186-207 and alike); the
+// element's double-double sum stored after its last record.  A guarded
+// record that fails adds an exact zero, which leaves the sum as skipping it
+// would.
+__device__ __forceinline__ void gj_acc(const GjOps& o, double h, int nnz, double* Jc, dd& D) {
+  double v = o.k;
+  if (o.h1) v *= o.y1;
+  if (o.h2) v *= o.y2;
+  double add;
+  if (o.coef == -1.0) {
+    add = -v;
+  } else {
+    bool keep = o.g < 0 || -v * o.yw * o.coef * h <= o.sg;
+    add = keep ? o.coef * v : 0.0;
+  }
+  D = dd_add(D, dd{add, 0.0});
+  if (o.last) {
+    if (o.e < nnz) Jc[o.e] = D.hi;
+    D = dd{0.0, 0.0};
+  }
+}
+
+// J(y) of one gas-phase cell: lane j walks its run of whole elements in the
+// per-cell J program (runs of near-equal record counts), four records in
+// flight while the current four are accumulated, and writes each element
+// after its last record.
 __device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const double* s_state,
                                          const double* s_k, double h, double* J) {
   lane = phase_lane(lane);
-  for (int e = lane; e < P.nnz; e += kWave) {
-    dd D = {0.0, 0.0};
-    for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q) {
-      int4 a = P.je_a[q];
-      double c = P.je_c[q];
-      double v = s_k[a.x];
-      if (a.y >= 0) v *= s_state[a.y];
-      if (a.z >= 0) v *= s_state[a.z];
-      if (c == -1.0) {
-        D = dd_add(D, dd{-v, 0.0});
-      } else {
-        int g = P.je_b[q];
-        if (g >= 0 && !(-v * s_state[a.w] * c * h <= s_state[g])) continue;
-        D = dd_add(D, dd{c * v, 0.0});
-      }
-    }
-    J[e] = D.hi;
+  vm_drain();
+  const int nnz = P.nnz;
+  constexpr int RS = 2 * kWave;
+  const int4* rec = P.cj_rec + 2 * lane;
+  dd D = {0.0, 0.0};
+  // the program has 4 rows of padding past cj_rows for the look-ahead
+  int4 a0 = rec[0], b0 = rec[1], a1 = rec[RS], b1 = rec[RS + 1];
+  int4 a2 = rec[2 * RS], b2 = rec[2 * RS + 1], a3 = rec[3 * RS], b3 = rec[3 * RS + 1];
+  for (int r = 0; r < P.cj_rows; r += 4) {
+    int4 x0 = a0, w0 = b0, x1 = a1, w1 = b1, x2 = a2, w2 = b2, x3 = a3, w3 = b3;
+    const int4* nx = rec + (size_t)(r + 4) * RS;
+    a0 = nx[0]; b0 = nx[1]; a1 = nx[RS]; b1 = nx[RS + 1];
+    a2 = nx[2 * RS]; b2 = nx[2 * RS + 1]; a3 = nx[3 * RS]; b3 = nx[3 * RS + 1];
+    GjOps o0 = gj_ops(x0, w0, s_state, s_k), o1 = gj_ops(x1, w1, s_state, s_k),
+          o2 = gj_ops(x2, w2, s_state, s_k), o3 = gj_ops(x3, w3, s_state, s_k);
+    gj_acc(o0, h, nnz, J, D);
+    gj_acc(o1, h, nnz, J, D);
+    gj_acc(o2, h, nnz, J, D);
+    gj_acc(o3, h, nnz, J, D);
   }
 }
 
@@ -543,6 +603,7 @@ __device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const do
 __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, const double* J,
                                               double gamma, double* lu) {
   lane = phase_lane(lane);
+  vm_drain();
   for (int p = lane; p < P.lu_nnz; p += kWave) lu[p] = 0.0;
   wsync();
   for (int e = lane; e < P.nnz; e += kWave) {
@@ -562,7 +623,7 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
   // of them is written: the updates use the unscaled L value times the
   // pivot's reciprocal -- the same product that is stored as the scaled L
   // value -- and the pivot needs a single wavefront sync.
-  constexpr int R = 5;
+  constexpr int R = kLuRowBatch;
   bool bad = false;
   const uint64_t sink = (uint64_t)(P.lu_nnz + lane);  // {dst = sink, l = 0, u = 0}
   int4 h0 = ld_uniform(P.lf_hdr, 0), h1 = ld_uniform(P.lf_hdr, 1);
@@ -612,15 +673,14 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     if (nb0 > R) {
       // the remaining rows (pivots of the dense trailing block) read the
       // scaled L values; they go in batches of R rows whose loads are all
-      // issued before the first is used
+      // issued before the first is used (the program pads them to whole
+      // batches: a per-row selection on a loaded record made the compiler
+      // wait for each load before issuing the next)
       wsync();
       for (int b = R; b < nb0; b += R) {
         uint64_t x[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          uint64_t v = P.up_rec[h0.y + (b + r) * kWave + lane];
-          x[r] = b + r < nb0 ? v : sink;
-        }
+        for (int r = 0; r < R; ++r) x[r] = P.up_rec[h0.y + (b + r) * kWave + lane];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           od[r] = lu[(int)(x[r] & 0xFFFF)];

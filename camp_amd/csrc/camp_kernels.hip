@@ -308,47 +308,6 @@ __global__ void __launch_bounds__(64) k_deriv_g(DevProgram P, int nc, const doub
   }
 }
 
-// Operands of one J record.
-struct GjOps {
-  int e, last, g, h1, h2;
-  double coef, k, y1, y2, yw, sg;
-};
-__device__ __forceinline__ GjOps gj_ops(int4 x, int4 w, const double* sy, const double* sk) {
-  GjOps o;
-  o.e = x.x & 0x3FFFFFFF;
-  o.last = (x.x >> 30) & 1;
-  o.g = x.y;
-  o.coef = hilo_to_double(x.w, x.z);
-  o.h1 = w.y >= 0;
-  o.h2 = w.z >= 0;
-  o.k = sk[w.x];
-  o.y1 = sy[o.h1 ? w.y : 0];
-  o.y2 = sy[o.h2 ? w.z : 0];
-  o.yw = sy[w.w >= 0 ? w.w : 0];
-  o.sg = sy[o.g >= 0 ? o.g : 0];
-  return o;
-}
-// cell_jac's record: k * the other reactants; loss records (coefficient -1)
-// unguarded, others guarded; the element's sum stored after its last record
-// (into the cell's LDS copy of J)
-__device__ __forceinline__ void gj_acc(const GjOps& o, double h, int nnz, double* Jc, dd& D) {
-  double v = o.k;
-  if (o.h1) v *= o.y1;
-  if (o.h2) v *= o.y2;
-  double add;
-  if (o.coef == -1.0) {
-    add = -v;
-  } else {
-    bool keep = o.g < 0 || -v * o.yw * o.coef * h <= o.sg;
-    add = keep ? o.coef * v : 0.0;
-  }
-  D = dd_add(D, dd{add, 0.0});
-  if (o.last) {
-    if (o.e < nnz) Jc[o.e] = D.hi;
-    D = dd{0.0, 0.0};
-  }
-}
-
 template <int G>
 __global__ void __launch_bounds__(64) k_jac_g(DevProgram P, int nc, const double* __restrict__ y,
                                               const double* __restrict__ state,

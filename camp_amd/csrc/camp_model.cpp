@@ -234,6 +234,75 @@ std::vector<int> markowitz_order(int n, const std::vector<int>& colptr,
   return order;
 }
 
+// J records of contiguous element runs with near-equal record counts, one
+// run per lane of G (the layout of Program::gj_rec); an element without
+// records gets one record of coefficient 0, so that every element is
+// written.  kc maps the rxn_env slot of a record's rate constant to its
+// compact index; nullptr keeps the slot itself.
+static void build_j_runs(const Program& P, int G, const std::vector<int>* kc,
+                         std::vector<int>& gj, int& rows_out) {
+  auto put_coef = [](int* r, double c) {
+    uint64_t bits;
+    std::memcpy(&bits, &c, sizeof(bits));
+    r[0] = (int)(uint32_t)(bits & 0xFFFFFFFFu);
+    r[1] = (int)(uint32_t)(bits >> 32);
+  };
+  const int nnz = P.nnz;
+  auto nrec = [&](int e) { return std::max(1, P.j_ptr[e + 1] - P.j_ptr[e]); };
+  std::vector<long> acc(nnz + 1, 0);
+  for (int e = 0; e < nnz; ++e) acc[e + 1] = acc[e] + nrec(e);
+  const long total = acc[nnz];
+  std::vector<int> first(G + 1, nnz);
+  first[0] = 0;
+  for (int j = 1; j < G; ++j) {
+    long target = total * j / G;
+    int e = first[j - 1];
+    while (e < nnz && acc[e] < target) ++e;
+    first[j] = e;
+  }
+  int rows = 1;
+  for (int j = 0; j < G; ++j) rows = std::max(rows, (int)(acc[first[j + 1]] - acc[first[j]]));
+  rows_out = (rows + 3) / 4 * 4;
+  // plus 4 rows that the kernels' look-ahead loads but never processes
+  const int j_alloc = rows_out + 4;
+  gj.assign((size_t)8 * G * j_alloc, 0);
+  for (int j = 0; j < G; ++j) {
+    int r = 0;
+    for (int e = first[j]; e < first[j + 1]; ++e) {
+      if (P.j_ptr[e] == P.j_ptr[e + 1]) {
+        int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
+        d[0] = e | (1 << 30);
+        d[1] = -1;
+        put_coef(&d[2], 0.0);
+        d[4] = 0;
+        d[5] = d[6] = -1;
+        d[7] = 0;
+        ++r;
+        continue;
+      }
+      for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q, ++r) {
+        int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
+        d[0] = e | ((q + 1 == P.j_ptr[e + 1]) ? (1 << 30) : 0);
+        d[1] = P.je_b[q];
+        put_coef(&d[2], P.je_c[q]);
+        d[4] = kc ? (*kc)[P.je_a[4 * q]] : P.je_a[4 * q];
+        d[5] = P.je_a[4 * q + 1];
+        d[6] = P.je_a[4 * q + 2];
+        d[7] = P.je_a[4 * q + 3];
+      }
+    }
+    for (; r < j_alloc; ++r) {
+      int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
+      d[0] = nnz | (1 << 30);
+      d[1] = -1;
+      put_coef(&d[2], 0.0);
+      d[4] = 0;
+      d[5] = d[6] = -1;
+      d[7] = 0;
+    }
+  }
+}
+
 // The lane-group f and J program (see Program::gf_rec): whole species
 // packed longest first into G bins for f, contiguous element runs
 // of near-equal record counts for J, and the compact k index map.
@@ -305,63 +374,7 @@ static void build_group_program(Program& P, int w) {
       d[5] = d[6] = d[7] = -1;
     }
   }
-  // J: contiguous element runs with near-equal record counts; an element
-  // without records gets one record of coefficient 0, so that every element
-  // is written
-  const int nnz = P.nnz;
-  auto nrec = [&](int e) { return std::max(1, P.j_ptr[e + 1] - P.j_ptr[e]); };
-  std::vector<long> acc(nnz + 1, 0);
-  for (int e = 0; e < nnz; ++e) acc[e + 1] = acc[e] + nrec(e);
-  const long total = acc[nnz];
-  std::vector<int> first(G + 1, nnz);
-  first[0] = 0;
-  for (int j = 1; j < G; ++j) {
-    long target = total * j / G;
-    int e = first[j - 1];
-    while (e < nnz && acc[e] < target) ++e;
-    first[j] = e;
-  }
-  int rows = 1;
-  for (int j = 0; j < G; ++j) rows = std::max(rows, (int)(acc[first[j + 1]] - acc[first[j]]));
-  P.g_j_rows[w] = (rows + 3) / 4 * 4;
-  const int j_alloc = P.g_j_rows[w] + 4;
-  std::vector<int>& gj = P.gj_rec[w];
-  gj.assign((size_t)8 * G * j_alloc, 0);
-  for (int j = 0; j < G; ++j) {
-    int r = 0;
-    for (int e = first[j]; e < first[j + 1]; ++e) {
-      if (P.j_ptr[e] == P.j_ptr[e + 1]) {
-        int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
-        d[0] = e | (1 << 30);
-        d[1] = -1;
-        put_coef(&d[2], 0.0);
-        d[4] = 0;
-        d[5] = d[6] = -1;
-        d[7] = 0;
-        ++r;
-        continue;
-      }
-      for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q, ++r) {
-        int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
-        d[0] = e | ((q + 1 == P.j_ptr[e + 1]) ? (1 << 30) : 0);
-        d[1] = P.je_b[q];
-        put_coef(&d[2], P.je_c[q]);
-        d[4] = kc_of_slot[P.je_a[4 * q]];
-        d[5] = P.je_a[4 * q + 1];
-        d[6] = P.je_a[4 * q + 2];
-        d[7] = P.je_a[4 * q + 3];
-      }
-    }
-    for (; r < j_alloc; ++r) {
-      int* d = &gj[(size_t)8 * ((size_t)r * G + j)];
-      d[0] = nnz | (1 << 30);
-      d[1] = -1;
-      put_coef(&d[2], 0.0);
-      d[4] = 0;
-      d[5] = d[6] = -1;
-      d[7] = 0;
-    }
-  }
+  build_j_runs(P, G, &kc_of_slot, P.gj_rec[w], P.g_j_rows[w]);
 }
 
 std::string Model::initialize() {
@@ -776,8 +789,10 @@ std::string Model::initialize() {
     P.fe_rec[4 * q + 3] = (int)(uint32_t)(bits >> 32);
   }
 
-  if (!P.mp)
+  if (!P.mp) {
     for (int w = 0; w < 3; ++w) build_group_program(P, w);
+    build_j_runs(P, 64, nullptr, P.cj_rec, P.cj_rows);
+  }
 
   // row view of the solver pattern (entries of each row in column order)
   P.r_ptr.assign(nv + 1, 0);
@@ -1071,9 +1086,12 @@ std::string Model::initialize() {
   // The factorisation's program: per pivot, the L positions to scale and the
   // update records {dst, l, u}, each padded to whole rows of 64 (at least
   // one row of L positions) with entries that write lane l's sink slot
-  // lu_nnz + l, so the device loops need no per-lane bounds checks;
-  // lf_hdr[k] = {first L position, first update record, L rows | update
-  // rows << 16, diagonal position} (two empty pivots past the end).
+  // lu_nnz + l, so the device loops need no per-lane bounds checks; a
+  // pivot with more than kLuRowBatch rows of updates is padded with rows of
+  // sinks to a whole number of batches, so that the device loads a batch
+  // with no per-row selection; lf_hdr[k] = {first L position, first update
+  // record, L rows | update rows << 16, diagonal position} (two empty pivots
+  // past the end).
   if (P.lu_nnz + 64 > 65535) return "LU fill exceeds 65471 entries per cell";
   P.lf_pos.clear();
   P.up_rec.clear();
@@ -1084,6 +1102,7 @@ std::string Model::initialize() {
     h[1] = (int)P.up_rec.size();
     int nl = P.l_ptr[k + 1] - P.l_ptr[k], nu = P.up_ptr[k + 1] - P.up_ptr[k];
     int rl = std::max(1, (nl + 63) / 64), ru = (nu + 63) / 64;
+    if (ru > kLuRowBatch) ru = (ru + kLuRowBatch - 1) / kLuRowBatch * kLuRowBatch;
     for (int q = 0; q < 64 * rl; ++q)
       P.lf_pos.push_back(q < nl ? P.l_pos[P.l_ptr[k] + q] : P.lu_nnz + q % 64);
     for (int q = 0; q < 64 * ru; ++q) {

@@ -18,6 +18,11 @@
 
 namespace camp {
 
+// Rows of 64 LU update records the per-cell factorisation processes
+// together: a pivot's first kLuRowBatch rows, then its further rows in
+// batches of kLuRowBatch (the program pads those to whole batches).
+constexpr int kLuRowBatch = 5;
+
 // src/camp_common.h:34-45, src/time_derivative.h:18
 constexpr int kVariable = 1;
 constexpr double kSmall = 1.0e-30;
@@ -143,6 +148,12 @@ struct Program {
   // One program per group width kGroupWidths[w] (lanes per cell).
   int g_f_rows[3] = {0, 0, 0}, g_j_rows[3] = {0, 0, 0}, n_k = 0;
   std::vector<int> gf_rec[3], gj_rec[3], k_slot, f_count, f_zero;
+  // The per-cell kernels' J program (gas-phase programs): gj_rec's layout
+  // for the 64 lanes of a wavefront, with the rate constant's rxn_env slot
+  // in place of the compact k index; cj_rows rows (a multiple of 4) plus 4
+  // rows of padding.
+  int cj_rows = 0;
+  std::vector<int> cj_rec;
 
   // ---- multi-phase tables (empty for gas-phase models) -------------------
   bool mp = false;

@@ -57,6 +57,9 @@ void Solver::setup_device() {
                       ? nullptr
                       : reinterpret_cast<const int4*>(dev_upload(G.gj_rec[w], owned));
   }
+  P.cj_rows = G.cj_rows;
+  P.cj_rec = G.cj_rec.empty() ? nullptr
+                              : reinterpret_cast<const int4*>(dev_upload(G.cj_rec, owned));
   P.k_slot = dev_upload(G.k_slot, owned);
   P.f_count = dev_upload(G.f_count, owned);
   P.n_fzero = (int)G.f_zero.size();
