@@ -139,7 +139,7 @@ class CampSolver:
 
     def phase_cycles(self, state_ptr, env_ptr, n_sample=0, reps=4):
         """Cycles of f, J, LU and one solve in the per-cell kernel (dict), or
-        None where not measurable (multi-phase, placement 2)."""
+        None where not measurable (gas-phase programs at placement 2)."""
         out = np.zeros(4)
         rc = lib().camp_solver_phase_cycles(self.h, C.c_void_p(state_ptr), C.c_void_p(env_ptr),
                                             int(n_sample), int(reps), _dp(out))

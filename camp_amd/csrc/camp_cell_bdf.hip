@@ -163,11 +163,32 @@ struct CellCv {
     return 0;
   }
 
+  // The history updates below work on one row of the Nordsieck array at a
+  // time, staged in registers: all of its loads are issued together and each
+  // value is stored once, instead of a chain of dependent load-store steps
+  // through the array (which may live in global memory).  Every element
+  // sees the same operations in the same order as the column-wise loops of
+  // CVODE (cvRescale, cvPredict, cvRestore, cvCompleteStep).
+  __device__ __forceinline__ void hist_load(int i, double* z) const {
+#pragma unroll
+    for (int j = 0; j <= QMAX; ++j)
+      if (j <= q) z[j] = zn(j)[i];
+  }
+
   __device__ __forceinline__ void rescale() {
+    double fac[QMAX + 1];
     double factor = eta;
-    for (int j = 1; j <= q; ++j) {
-      for (int i = lane; i < nv; i += kWave) zn(j)[i] *= factor;
+#pragma unroll
+    for (int j = 1; j <= QMAX; ++j) {
+      fac[j] = factor;
       factor *= eta;
+    }
+    for (int i = lane; i < nv; i += kWave) {
+      double z[QMAX + 1];
+      hist_load(i, z);
+#pragma unroll
+      for (int j = 1; j <= QMAX; ++j)
+        if (j <= q) zn(j)[i] = z[j] * fac[j];
     }
     h = hscale * eta;
     next_h = h;
@@ -175,20 +196,32 @@ struct CellCv {
     wsync();
   }
 
+  // zn[j-1] += zn[j] (SIGN 1) or -= (SIGN -1), k = 1..q, j = q..k
+  template <int SIGN>
+  __device__ __forceinline__ void pascal() {
+    for (int i = lane; i < nv; i += kWave) {
+      double z[QMAX + 1];
+      hist_load(i, z);
+#pragma unroll
+      for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+        for (int j = QMAX; j >= k; --j)
+          if (j <= q) z[j - 1] = SIGN > 0 ? z[j - 1] + z[j] : z[j - 1] - z[j];
+#pragma unroll
+      for (int j = 0; j < QMAX; ++j)
+        if (j < q) zn(j)[i] = z[j];
+    }
+    wsync();
+  }
+
   __device__ __forceinline__ void predict() {
     tn += h;
-    for (int i = lane; i < nv; i += kWave)
-      for (int k = 1; k <= q; ++k)
-        for (int j = q; j >= k; --j) zn(j - 1)[i] += zn(j)[i];
-    wsync();
+    pascal<1>();
   }
 
   __device__ __forceinline__ void restore(double saved_t) {
     tn = saved_t;
-    for (int i = lane; i < nv; i += kWave)
-      for (int k = 1; k <= q; ++k)
-        for (int j = q; j >= k; --j) zn(j - 1)[i] -= zn(j)[i];
-    wsync();
+    pascal<-1>();
   }
 
   __device__ __forceinline__ void set_tq(double hsum, double alpha0, double alpha0_hat, double xi_inv,
@@ -538,16 +571,27 @@ struct CellCv {
     tau[1] = h;
     qwait--;
     bool save = (qwait == 1) && (q != QMAX);
+    double lj[QMAX + 1];
+#pragma unroll
+    for (int j = 0; j <= QMAX; ++j) lj[j] = j <= q ? l[j] : 0.0;
     for (int i = lane; i < nv; i += kWave) {
       double a = acor[i];
-      for (int j = 0; j <= q; ++j) zn(j)[i] += l[j] * a;
-      if (save) zn(QMAX)[i] = a;
+      double z[QMAX + 1];
+      hist_load(i, z);
+#pragma unroll
+      for (int j = 0; j <= QMAX; ++j)
+        if (j <= q) z[j] += lj[j] * a;
       // after a projection, components at TINY restart flat
       if (projected && y[i] == 1.0e-60) {
-        zn(0)[i] = 1.0e-60;
-        for (int j = 1; j <= q; ++j) zn(j)[i] = 0.0;
+        z[0] = 1.0e-60;
+#pragma unroll
+        for (int j = 1; j <= QMAX; ++j) z[j] = 0.0;
       }
-      last[i] = zn(0)[i];
+#pragma unroll
+      for (int j = 0; j <= QMAX; ++j)
+        if (j <= q) zn(j)[i] = z[j];
+      if (save) zn(QMAX)[i] = a;
+      last[i] = z[0];
     }
     if (save) saved_tq5 = tq[5];
     wsync();
@@ -967,9 +1011,12 @@ void launch_cell_bdf(const CellBdfArgs& a, int grid, int block, size_t lds, hipS
 
 void launch_cell_bdf_bench(const CellBdfArgs& a, int grid, int block, size_t lds,
                            hipStream_t s) {
-  if (a.P.mp) die("phase measurement: gas-phase programs only");
   allow_lds(cell_bdf_fn(a.P, a.place), lds);
-  if (cell_placement(a.P, a.place) == kPlaceGlobal) {
+  if (a.P.mp) {
+    allow_lds((const void*)k_cell_bdf<true, kPlaceGlobal, true>, lds);
+    hipLaunchKernelGGL((k_cell_bdf<true, kPlaceGlobal, true>), dim3(grid), dim3(block), lds, s,
+                       a);
+  } else if (cell_placement(a.P, a.place) == kPlaceGlobal) {
     allow_lds((const void*)k_cell_bdf<false, kPlaceGlobal, true>, lds);
     hipLaunchKernelGGL((k_cell_bdf<false, kPlaceGlobal, true>), dim3(grid), dim3(block), lds, s,
                        a);

@@ -1578,10 +1578,9 @@ extern "C" int camp_solver_phase_cycles(void* solver_data, const double* d_state
                                         double* out4) {
   using namespace camp;
   Solver* s = SD(solver_data);
-  if (!s->m.initialized || s->P.mp || cell_placement(s->P, s->place) == kPlaceGlobalLu ||
-      reps < 1)
-    return 1;
+  if (!s->m.initialized || reps < 1) return 1;
   s->ensure_device();
+  if (cell_placement(s->P, s->place) == kPlaceGlobalLu) return 1;
   s->upload_renv();
   const DevProgram& P = s->P;
   long nc = s->m.nc;

@@ -232,8 +232,8 @@ int camp_solver_autotune(void *solver_data, const double *d_state,
  * state of every cell of a sample of n_sample cells (every
  * (n_cells/n_sample)-th, copies; 0 = up to 4096) with the solver's placement
  * and cells per workgroup, as many cells resident as in a solve.  out4
- * receives the mean cycles over the sample.  Returns 0, or 1 for
- * multi-phase programs and placement 2. */
+ * receives the mean cycles over the sample.  Returns 0, or 1 for gas-phase
+ * programs at placement 2. */
 int camp_solver_phase_cycles(void *solver_data, const double *d_state,
                              const double *d_env, int n_sample, int reps,
                              double *out4);
