@@ -794,6 +794,89 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
   wsync();
 }
 
+// Column sweeps of the triangular solves (nv > 128): one stage per pivot k
+// -- its column range, the rows and LU positions of its first 64 entries
+// (one per lane), their LU values and, backward, the pivot's diagonal.
+struct ColStage {
+  int4 r;
+  int row, pos;
+  double val, dg;
+};
+template <bool FWD>
+__device__ __forceinline__ void col_fetch_rows(const DevProgram& P, int lane, int k, ColStage& S) {
+  const int* pos_a = FWD ? P.l_pos : P.u_pos;
+  const int* row_a = FWD ? P.l_row : P.u_row;
+  if (k < 0 || k >= P.nv) {
+    S.r = int4{0, 0, 0, 0};
+  } else {
+    int4 h = ld_uniform(P.lu_hdr, 2 * k);
+    int4 h2 = ld_uniform(P.lu_hdr, 2 * k + 1);
+    S.r = FWD ? int4{h.x, h.y, h2.x, 0} : int4{h2.y, h2.z, h2.x, 0};
+  }
+  int q = S.r.x + lane;
+  S.row = q < S.r.y ? row_a[q] : -1;
+  S.pos = q < S.r.y ? pos_a[q] : 0;
+}
+template <bool FWD>
+__device__ __forceinline__ void col_fetch_vals(const DevProgram& P, int k, const double* lu,
+                                               ColStage& S) {
+  S.val = S.row >= 0 ? lu[S.pos] : 0.0;
+  S.dg = (!FWD && k >= 0 && k < P.nv) ? lu[S.r.z] : 1.0;
+}
+// Pivot k of a sweep: x_k (divided by the diagonal, backward) updates the
+// rows of its column.
+template <bool FWD>
+__device__ __forceinline__ void col_pivot(const DevProgram& P, int lane, int k, const ColStage& S,
+                                          const double* lu, double* w) {
+  const int* pos_a = FWD ? P.l_pos : P.u_pos;
+  const int* row_a = FWD ? P.l_row : P.u_row;
+  double xk = w[k];
+  if (!FWD) {
+    xk = xk / S.dg;
+    wsync();
+    if (lane == 0) w[k] = xk;
+  }
+  if (S.row >= 0) w[S.row] = w[S.row] - S.val * xk;
+  for (int q = S.r.x + kWave + lane; q < S.r.y; q += kWave) {
+    int i = row_a[q];
+    w[i] = w[i] - lu[pos_a[q]] * xk;
+  }
+  wsync();
+}
+// One step of the pipelined sweep: the LU values of pivot k + 2 (stage C,
+// whose rows were fetched a step earlier), pivot k (stage A), then the rows
+// of pivot k + 3 into A.
+template <bool FWD>
+__device__ __forceinline__ void col_step(const DevProgram& P, int lane, int n, const double* lu,
+                                         double* w, ColStage& A, ColStage& C) {
+  constexpr int st = FWD ? 1 : -1;
+  const int k = FWD ? n : P.nv - 1 - n;
+  col_fetch_vals<FWD>(P, k + 2 * st, lu, C);
+  if (n < P.nv) col_pivot<FWD>(P, lane, k, A, lu, w);
+  col_fetch_rows<FWD>(P, lane, k + 3 * st, A);
+}
+// The sweep, unrolled by three over three stages so that each stage keeps
+// its registers: LU values are fetched two pivots ahead and rows three
+// ahead, with no copy between stages that would wait for the newest load.
+template <bool FWD>
+__device__ __forceinline__ void col_sweep(const DevProgram& P, int lane, const double* lu,
+                                          double* w) {
+  constexpr int st = FWD ? 1 : -1;
+  const int k0 = FWD ? 0 : P.nv - 1;
+  ColStage S0, S1, S2;
+  col_fetch_rows<FWD>(P, lane, k0, S0);
+  col_fetch_rows<FWD>(P, lane, k0 + st, S1);
+  col_fetch_rows<FWD>(P, lane, k0 + 2 * st, S2);
+  col_fetch_vals<FWD>(P, k0, lu, S0);
+  col_fetch_vals<FWD>(P, k0 + st, lu, S1);
+  vm_drain();
+  for (int n = 0; n < P.nv; n += 3) {
+    col_step<FWD>(P, lane, n, lu, w, S0, S2);
+    col_step<FWD>(P, lane, n + 1, lu, w, S1, S0);
+    col_step<FWD>(P, lane, n + 2, lu, w, S2, S1);
+  }
+}
+
 // Solve (LU) x = b in place; b in dep order, w a scratch vector (nv).
 __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, double* lu,
                                               double* b, double* w) {
@@ -803,56 +886,13 @@ __device__ __forceinline__ void cell_lu_solve(const DevProgram& P, int lane, dou
     return;
   }
   // Column sweeps with the solution in LDS (nv > 128).  Pivot k's L (or U)
-  // column: the first 64 entries go one per lane, the rest inline.  The
-  // program and LU reads form a software pipeline over pivots -- header of
-  // k+2, positions and rows of k+1, LU values of k+1 -- so a pivot waits on
-  // LDS only (the LU of multi-phase cells lives in global memory).
+  // column: the first 64 entries go one per lane, the rest inline; the
+  // program and LU reads run ahead of the pivots (col_sweep), so a pivot
+  // waits on LDS only (the LU of multi-phase cells lives in global memory).
   for (int k = lane; k < P.nv; k += kWave) w[k] = b[P.perm[k]];
   wsync();
-  for (int dir = 0; dir < 2; ++dir) {
-    const bool fwd = dir == 0;
-    const int* pos_a = fwd ? P.l_pos : P.u_pos;
-    const int* row_a = fwd ? P.l_row : P.u_row;
-    // {column range, diagonal position} of pivot k (empty outside 0..nv-1)
-    auto range = [&](int k) -> int4 {
-      if (k < 0 || k >= P.nv) return int4{0, 0, 0, 0};
-      int4 h = ld_uniform(P.lu_hdr, 2 * k);
-      int4 h2 = ld_uniform(P.lu_hdr, 2 * k + 1);
-      return fwd ? int4{h.x, h.y, h2.x, 0} : int4{h2.y, h2.z, h2.x, 0};
-    };
-    int k0 = fwd ? 0 : P.nv - 1, step = fwd ? 1 : -1;
-    int4 r0 = range(k0), r1 = range(k0 + step), r2 = range(k0 + 2 * step);
-    int q0 = r0.x + lane, q1 = r1.x + lane;
-    int row0 = q0 < r0.y ? row_a[q0] : -1, pos0 = q0 < r0.y ? pos_a[q0] : 0;
-    int row1 = q1 < r1.y ? row_a[q1] : -1, pos1 = q1 < r1.y ? pos_a[q1] : 0;
-    double val0 = row0 >= 0 ? lu[pos0] : 0.0;
-    double dg0 = fwd ? 1.0 : lu[r0.z];
-    for (int n = 0; n < P.nv; ++n) {
-      int k = k0 + step * n;
-      // stages issued for later pivots, each from data loaded one pivot
-      // earlier: range of k+3, positions/rows of k+2, values of k+1
-      int4 r3 = range(k + 3 * step);
-      int q2 = r2.x + lane;
-      int row2 = q2 < r2.y ? row_a[q2] : -1, pos2 = q2 < r2.y ? pos_a[q2] : 0;
-      double val1 = row1 >= 0 ? lu[pos1] : 0.0;
-      double dg1 = (!fwd && n + 1 < P.nv) ? lu[r1.z] : 1.0;
-      double xk = w[k];
-      if (!fwd) {
-        xk = xk / dg0;
-        wsync();
-        if (lane == 0) w[k] = xk;
-      }
-      if (row0 >= 0) w[row0] = w[row0] - val0 * xk;
-      for (int q = r0.x + kWave + lane; q < r0.y; q += kWave) {
-        int i = row_a[q];
-        w[i] = w[i] - lu[pos_a[q]] * xk;
-      }
-      wsync();
-      r0 = r1; r1 = r2; r2 = r3;
-      row0 = row1; val0 = val1; dg0 = dg1;
-      row1 = row2; pos1 = pos2;
-    }
-  }
+  col_sweep<true>(P, lane, lu, w);
+  col_sweep<false>(P, lane, lu, w);
   for (int k = lane; k < P.nv; k += kWave) b[P.perm[k]] = w[k];
   wsync();
 }
