@@ -76,6 +76,8 @@ struct DevProgram {
   const int4* js_a;
   const double* js_c;
   const int *jm_ptr, *jm_rxn, *jm_param;
+  const int* mj_elist;  // J_rxn assembly order (Program::mj_elist)
+  int mj_rows;
 };
 
 

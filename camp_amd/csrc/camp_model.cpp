@@ -377,6 +377,25 @@ static void build_group_program(Program& P, int w) {
   build_j_runs(P, G, &kc_of_slot, P.gj_rec[w], P.g_j_rows[w]);
 }
 
+// The per-cell kernels' J_rxn assembly order for multi-phase programs:
+// reaction elements grouped by their record signature (the number of
+// mass-action records, then the kinds of the special records in order), so
+// that a row of 64 lanes runs one code path.
+static void build_mp_j_order(Program& P) {
+  const int rn = P.rnnz;
+  std::vector<std::vector<int>> sig(rn);
+  for (int e = 0; e < rn; ++e) {
+    sig[e].push_back(P.j_ptr[e + 1] - P.j_ptr[e]);
+    for (int q = P.js_ptr[e]; q < P.js_ptr[e + 1]; ++q) sig[e].push_back(P.js_a[4 * q]);
+  }
+  std::vector<int> order(rn);
+  for (int e = 0; e < rn; ++e) order[e] = e;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sig[x] < sig[y]; });
+  P.mj_rows = (rn + 63) / 64;
+  P.mj_elist.assign((size_t)64 * std::max(P.mj_rows, 1), -1);
+  for (int i = 0; i < rn; ++i) P.mj_elist[i] = order[i];
+}
+
 std::string Model::initialize() {
   if (n_added != n_rxn) return "not all reactions were added before solver_initialize";
   // --- reaction Jacobian pattern: diagonal + per-type used elements -------
@@ -792,6 +811,8 @@ std::string Model::initialize() {
   if (!P.mp) {
     for (int w = 0; w < 3; ++w) build_group_program(P, w);
     build_j_runs(P, 64, nullptr, P.cj_rec, P.cj_rows);
+  } else {
+    build_mp_j_order(P);
   }
 
   // row view of the solver pattern (entries of each row in column order)

@@ -183,6 +183,12 @@ struct Program {
   // solver element e = sum over [jm_ptr[e], jm_ptr[e+1]) of J_rxn[jm_rxn] * J_params[jm_param]
   std::vector<int> jm_ptr, jm_rxn, jm_param;
   int pnnz = 0;
+  // the order in which the per-cell kernels assemble J_rxn: mj_rows rows of
+  // 64 reaction elements (lane l of row r takes mj_elist[64 r + l], -1 past
+  // the end), elements sorted by the kinds of their records so that the
+  // lanes of a row run the same record code
+  int mj_rows = 0;
+  std::vector<int> mj_elist;
 };
 
 

@@ -469,7 +469,10 @@ __device__ __forceinline__ void cell_jac_mp(const DevProgram& P, int lane, const
                                             const double* s_k, double h, MpLds L, double* Jr,
                                             double* J) {
   lane = phase_lane(lane);
-  for (int e = lane; e < P.rnnz; e += kWave) {
+  // elements in the program's order, whose rows share their record kinds
+  for (int r = 0; r < P.mj_rows; ++r) {
+    int e = P.mj_elist[r * kWave + lane];
+    if (e < 0) continue;
     dd D = jac_elem_mass_action(P, e, s_state, s_k, h);
     D = mp_jac_records(P, e, s_state, L.aero, L.jsc, D);
     Jr[e] = D.hi;

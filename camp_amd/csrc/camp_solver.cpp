@@ -179,6 +179,8 @@ void Solver::setup_device() {
     P.js_ptr = dev_upload(G.js_ptr, owned);
     P.js_a = reinterpret_cast<const int4*>(dev_upload(G.js_a, owned));
     P.js_c = dev_upload(G.js_c, owned);
+    P.mj_rows = G.mj_rows;
+    P.mj_elist = dev_upload(G.mj_elist, owned);
     P.jm_ptr = dev_upload(G.jm_ptr, owned);
     P.jm_rxn = dev_upload(G.jm_rxn, owned);
     P.jm_param = dev_upload(G.jm_param, owned);
