@@ -602,6 +602,73 @@ __device__ __forceinline__ void cell_jac(const DevProgram& P, int lane, const do
 // a zero or non-finite pivot (a recoverable linear-setup failure).  lu
 // holds lu_nnz values and 64 sink slots after them (scratch).
 // ---------------------------------------------------------------------
+// One pivot of the factorisation's program: its header {lf_pos row start,
+// up_rec row start, L rows | update rows << 16, diagonal position}, the
+// lane's first L position and its first kLuRowBatch update records.
+struct LuPivot {
+  int4 h;
+  int l_p;
+  uint64_t rec[kLuRowBatch];
+};
+__device__ __forceinline__ void lu_fetch_pivot(const DevProgram& P, int lane, int k, LuPivot& S) {
+  // lf_hdr has two empty pivots past the end; later ones are never used
+  S.h = ld_uniform(P.lf_hdr, k < P.nv + 2 ? k : P.nv + 1);
+  S.l_p = P.lf_pos[S.h.x + lane];
+#pragma unroll
+  for (int r = 0; r < kLuRowBatch; ++r) S.rec[r] = P.up_rec[S.h.y + r * kWave + lane];
+}
+// Eliminate one pivot; returns true on a zero or non-finite pivot.
+template <int R>
+__device__ __forceinline__ bool lu_pivot(const DevProgram& P, int lane, const LuPivot& S,
+                                         double* lu) {
+  const int nb = S.h.z >> 16;
+  double piv = lu[S.h.w];
+  double lval = lu[S.l_p];
+  double od[R], oa[R], ob[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    od[r] = lu[(int)(S.rec[r] & 0xFFFF)];
+    oa[r] = lu[(int)((S.rec[r] >> 16) & 0xFFFF)];
+    ob[r] = lu[(int)((S.rec[r] >> 32) & 0xFFFF)];
+  }
+  // a zero or non-finite pivot fails the factorisation; the check is
+  // folded into a flag tested after the loop, so it does not make the
+  // pivot's load a wait ahead of the others (the values computed after a
+  // bad pivot are discarded with the failure)
+  bool bad = !(piv != 0.0 && isfinite(piv));
+  double rinv = 1.0 / piv;
+  lu[S.l_p] = lval * rinv;
+  for (int r = 1; r < (S.h.z & 0xFFFF); ++r) {
+    int p = P.lf_pos[S.h.x + r * kWave + lane];
+    lu[p] = lu[p] * rinv;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) lu[(int)(S.rec[r] & 0xFFFF)] = od[r] - (oa[r] * rinv) * ob[r];
+  if (nb > R) {
+    // the remaining rows (pivots of the dense trailing block) read the
+    // scaled L values; they go in batches of R rows whose loads are all
+    // issued before the first is used (the program pads them to whole
+    // batches: a per-row selection on a loaded record made the compiler
+    // wait for each load before issuing the next)
+    wsync();
+    for (int b = R; b < nb; b += R) {
+      uint64_t x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = P.up_rec[S.h.y + (b + r) * kWave + lane];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        od[r] = lu[(int)(x[r] & 0xFFFF)];
+        oa[r] = lu[(int)((x[r] >> 16) & 0xFFFF)];
+        ob[r] = lu[(int)((x[r] >> 32) & 0xFFFF)];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) lu[(int)(x[r] & 0xFFFF)] = od[r] - oa[r] * ob[r];
+    }
+  }
+  wsync();
+  return bad;
+}
+
 __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, const double* J,
                                               double gamma, double* lu) {
   lane = phase_lane(lane);
@@ -614,89 +681,30 @@ __device__ __forceinline__ int cell_lu_factor(const DevProgram& P, int lane, con
     lu[P.m_to_lu[e]] = v;
   }
   wsync();
-  // Software pipeline over pivots: pivot k+2's header, pivot k+1's first
-  // row of L positions and first 5 rows of update records are fetched from
-  // global memory while pivot k is eliminated.  The program is padded to
-  // rows of 64 with entries that write the lane's own sink slot lu[lu_nnz +
-  // lane], so every load and store is unconditional.  Within one pivot
-  // every update has a distinct destination (neither the pivot's L column
-  // nor its U row).  A wavefront's LDS accesses execute in program order,
-  // so the pivot, the L values and every update operand are read before any
-  // of them is written: the updates use the unscaled L value times the
-  // pivot's reciprocal -- the same product that is stored as the scaled L
-  // value -- and the pivot needs a single wavefront sync.
+  // Software pipeline over pivots, unrolled by two over two register sets:
+  // a pivot's header (scalar), first row of L positions and first R rows of
+  // update records are fetched two pivots before it is eliminated, so their
+  // latency hides behind the pivot in between (a rotation through copies
+  // would make every pivot wait for the newest loads).  The program pads
+  // every pivot to at least R rows of update records and its L positions to
+  // whole rows of 64, with entries that write the lane's own sink slot
+  // lu[lu_nnz + lane], so every load and store is unconditional.  Within one
+  // pivot every update has a distinct destination (neither the pivot's L
+  // column nor its U row).  A wavefront's LDS accesses execute in program
+  // order, so the pivot, the L values and every update operand are read
+  // before any of them is written: the updates use the unscaled L value
+  // times the pivot's reciprocal -- the same product that is stored as the
+  // scaled L value -- and the pivot needs a single wavefront sync.
   constexpr int R = kLuRowBatch;
   bool bad = false;
-  const uint64_t sink = (uint64_t)(P.lu_nnz + lane);  // {dst = sink, l = 0, u = 0}
-  int4 h0 = ld_uniform(P.lf_hdr, 0), h1 = ld_uniform(P.lf_hdr, 1);
-  int l_p = P.lf_pos[h0.x + lane];
-  int nb0 = h0.z >> 16;
-  uint64_t cur[R];
-  // a pivot's first R rows are loaded unconditionally (the program has R
-  // rows of sink records past its end) and those past its own replaced by
-  // sinks, so the loads need no branches
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    uint64_t v = P.up_rec[h0.y + r * kWave + lane];
-    cur[r] = r < nb0 ? v : sink;
-  }
-  for (int k = 0; k < P.nv; ++k) {
-    int4 h2 = ld_uniform(P.lf_hdr, k + 2);
-    int n_lp = P.lf_pos[h1.x + lane];
-    int nb1 = h1.z >> 16;
-    uint64_t nxt[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      uint64_t v = P.up_rec[h1.y + r * kWave + lane];
-      nxt[r] = r < nb1 ? v : sink;
-    }
-    double piv = lu[h0.w];
-    double lval = lu[l_p];
-    double od[R], oa[R], ob[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      od[r] = lu[(int)(cur[r] & 0xFFFF)];
-      oa[r] = lu[(int)((cur[r] >> 16) & 0xFFFF)];
-      ob[r] = lu[(int)((cur[r] >> 32) & 0xFFFF)];
-    }
-    // a zero or non-finite pivot fails the factorisation; the check is
-    // folded into a flag tested after the loop, so it does not make the
-    // pivot's load a wait ahead of the others (the values computed after a
-    // bad pivot are discarded with the failure)
-    bad |= !(piv != 0.0 && isfinite(piv));
-    double rinv = 1.0 / piv;
-    lu[l_p] = lval * rinv;
-    for (int r = 1; r < (h0.z & 0xFFFF); ++r) {
-      int p = P.lf_pos[h0.x + r * kWave + lane];
-      lu[p] = lu[p] * rinv;
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) lu[(int)(cur[r] & 0xFFFF)] = od[r] - (oa[r] * rinv) * ob[r];
-    if (nb0 > R) {
-      // the remaining rows (pivots of the dense trailing block) read the
-      // scaled L values; they go in batches of R rows whose loads are all
-      // issued before the first is used (the program pads them to whole
-      // batches: a per-row selection on a loaded record made the compiler
-      // wait for each load before issuing the next)
-      wsync();
-      for (int b = R; b < nb0; b += R) {
-        uint64_t x[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) x[r] = P.up_rec[h0.y + (b + r) * kWave + lane];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          od[r] = lu[(int)(x[r] & 0xFFFF)];
-          oa[r] = lu[(int)((x[r] >> 16) & 0xFFFF)];
-          ob[r] = lu[(int)((x[r] >> 32) & 0xFFFF)];
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) lu[(int)(x[r] & 0xFFFF)] = od[r] - oa[r] * ob[r];
-      }
-    }
-    wsync();
-    h0 = h1; h1 = h2; l_p = n_lp; nb0 = nb1;
-#pragma unroll
-    for (int r = 0; r < R; ++r) cur[r] = nxt[r];
+  LuPivot A, B;
+  lu_fetch_pivot(P, lane, 0, A);
+  lu_fetch_pivot(P, lane, 1, B);
+  for (int k = 0; k < P.nv; k += 2) {
+    bad |= lu_pivot<R>(P, lane, A, lu);
+    lu_fetch_pivot(P, lane, k + 2, A);
+    if (k + 1 < P.nv) bad |= lu_pivot<R>(P, lane, B, lu);
+    lu_fetch_pivot(P, lane, k + 3, B);
   }
   return bad ? 1 : 0;
 }

@@ -1107,12 +1107,12 @@ std::string Model::initialize() {
   // The factorisation's program: per pivot, the L positions to scale and the
   // update records {dst, l, u}, each padded to whole rows of 64 (at least
   // one row of L positions) with entries that write lane l's sink slot
-  // lu_nnz + l, so the device loops need no per-lane bounds checks; a
-  // pivot with more than kLuRowBatch rows of updates is padded with rows of
-  // sinks to a whole number of batches, so that the device loads a batch
-  // with no per-row selection; lf_hdr[k] = {first L position, first update
-  // record, L rows | update rows << 16, diagonal position} (two empty pivots
-  // past the end).
+  // lu_nnz + l, so the device loops need no per-lane bounds checks; every
+  // pivot's updates are padded with rows of sinks to a whole number (at
+  // least one) of batches of kLuRowBatch rows, so that the device loads a
+  // batch with no per-row selection; lf_hdr[k] = {first L position, first
+  // update record, L rows | update rows << 16, diagonal position} (two empty
+  // pivots past the end).
   if (P.lu_nnz + 64 > 65535) return "LU fill exceeds 65471 entries per cell";
   P.lf_pos.clear();
   P.up_rec.clear();
@@ -1122,8 +1122,9 @@ std::string Model::initialize() {
     h[0] = (int)P.lf_pos.size();
     h[1] = (int)P.up_rec.size();
     int nl = P.l_ptr[k + 1] - P.l_ptr[k], nu = P.up_ptr[k + 1] - P.up_ptr[k];
-    int rl = std::max(1, (nl + 63) / 64), ru = (nu + 63) / 64;
-    if (ru > kLuRowBatch) ru = (ru + kLuRowBatch - 1) / kLuRowBatch * kLuRowBatch;
+    int rl = std::max(1, (nl + 63) / 64);
+    int ru = std::max(kLuRowBatch, (nu + 63) / 64);
+    ru = (ru + kLuRowBatch - 1) / kLuRowBatch * kLuRowBatch;
     for (int q = 0; q < 64 * rl; ++q)
       P.lf_pos.push_back(q < nl ? P.l_pos[P.l_ptr[k] + q] : P.lu_nnz + q % 64);
     for (int q = 0; q < 64 * ru; ++q) {
@@ -1138,10 +1139,6 @@ std::string Model::initialize() {
     h[2] = rl | (ru << 16);
     h[3] = P.diag_pos[k];
   }
-  // 16 rows of sink records past the end: the device loads a pivot's first
-  // rows (a fixed number, at most 16) unconditionally and keeps those past
-  // its own
-  for (int q = 0; q < 16 * 64; ++q) P.up_rec.push_back((uint64_t)(P.lu_nnz + q % 64));
   initialized = true;
   return std::string();
 }
