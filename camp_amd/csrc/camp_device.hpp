@@ -427,24 +427,30 @@ __host__ __device__ __forceinline__ void cell_deriv(const DevProgram& P, int lan
     const int2* er = P.re_ell + hd.z + lane;
     double acc = 0.0;
     if (use_est) {
-      // re_ell has padding rows up to a multiple of 4 plus 4 beyond hd.w
-      int2 b0 = er[0], b1 = er[kWave], b2 = er[2 * kWave], b3 = er[3 * kWave];
-      for (int j = 0; j < hd.w; j += 4) {
-        int2 c[4] = {b0, b1, b2, b3};
-        b0 = er[(j + 4) * kWave];
-        b1 = er[(j + 5) * kWave];
-        b2 = er[(j + 6) * kWave];
-        b3 = er[(j + 7) * kWave];
-        double jv[4], yv[4], jyv[4];
+      // re_ell has padding rows up to a multiple of kEstBatch plus kEstBatch
+      // beyond hd.w; a batch's J-history loads are all issued before the
+      // first is used
+      constexpr int EB = kEstBatch;
+      int2 b[EB];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < EB; ++u) b[u] = er[u * kWave];
+      for (int j = 0; j < hd.w; j += EB) {
+        int2 c[EB];
+#pragma unroll
+        for (int u = 0; u < EB; ++u) {
+          c[u] = b[u];
+          b[u] = er[(j + EB + u) * kWave];
+        }
+        double jv[EB], yv[EB], jyv[EB];
+#pragma unroll
+        for (int u = 0; u < EB; ++u) {
           int e = c[u].x >= 0 ? c[u].x : 0, col = c[u].x >= 0 ? c[u].y : 0;
           jv[u] = Js[e];
           yv[u] = y[col];
           jyv[u] = Jy[col];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < EB; ++u)
           if (c[u].x >= 0) acc += jv[u] * (yv[u] - jyv[u]);
       }
     }

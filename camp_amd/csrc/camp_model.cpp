@@ -851,9 +851,10 @@ std::string Model::initialize() {
       P.fl_hdr.push_back(0);
       P.fl_hdr.push_back((int)(P.re_ell.size() / 2));
       P.fl_hdr.push_back(cr);
-      // rows padded (with -1 entries) to a multiple of 4 plus 4, so that the
-      // estimate's batches of 4 and its loads 4 rows ahead need no checks
-      int cr_alloc = (cr + 3) / 4 * 4 + 4;
+      // rows padded (with -1 entries) to a multiple of kEstBatch plus
+      // kEstBatch, so that the estimate's batches and its loads a batch ahead
+      // need no checks
+      int cr_alloc = (cr + kEstBatch - 1) / kEstBatch * kEstBatch + kEstBatch;
       for (int j = 0; j < cr_alloc; ++j)
         for (int l = 0; l < W; ++l) {
           int slot = pass * W + l, q = -1;

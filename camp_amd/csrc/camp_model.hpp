@@ -22,6 +22,8 @@ namespace camp {
 // together: a pivot's first kLuRowBatch rows, then its further rows in
 // batches of kLuRowBatch (the program pads those to whole batches).
 constexpr int kLuRowBatch = 5;
+// Row-view entries of the f estimate's ELL a lane reads per batch.
+constexpr int kEstBatch = 8;
 
 // src/camp_common.h:34-45, src/time_derivative.h:18
 constexpr int kVariable = 1;
