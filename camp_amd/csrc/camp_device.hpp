@@ -279,22 +279,34 @@ __device__ __forceinline__ void cell_rate_constants(const DevProgram& P, int lan
 __host__ __device__ __forceinline__ void cell_rates(const DevProgram& P, int lane, const double* s_state,
                                            const double* s_k, double* s_rate) {
   lane = phase_lane(lane);
-  for (int t = lane; t < P.n_terms; t += kWave) {
-    int4 tr = P.t_rec[t];
-    int fl = tr.x >> 28;
-    double k = s_k[tr.x & 0x0FFFFFFF];
-    double y0 = s_state[tr.y >= 0 ? tr.y : 0], y1 = s_state[tr.z >= 0 ? tr.z : 0],
-           y2 = s_state[tr.w >= 0 ? tr.w : 0];
-    dd rate = {k, 0.0};
-    if (fl & 2) {
-      rate = {k * y0, 0.0};
-    } else {
-      rate = tr.y >= 0 ? dd_mul_d(rate, y0) : rate;
-      rate = tr.z >= 0 ? dd_mul_d(rate, y1) : rate;
-      rate = tr.w >= 0 ? dd_mul_d(rate, y2) : rate;
+  // a lane's terms in groups of four whose program loads are issued together
+  constexpr int TG = 4;
+  for (int t0 = lane; t0 < P.n_terms; t0 += TG * kWave) {
+    int4 tr[TG];
+#pragma unroll
+    for (int g = 0; g < TG; ++g) {
+      int t = t0 + g * kWave;
+      tr[g] = t < P.n_terms ? P.t_rec[t] : make_int4(0, -1, -1, -1);
     }
-    s_rate[2 * t] = rate.hi;
-    s_rate[2 * t + 1] = rate.lo;
+#pragma unroll
+    for (int g = 0; g < TG; ++g) {
+      int t = t0 + g * kWave;
+      if (t >= P.n_terms) break;
+      int fl = tr[g].x >> 28;
+      double k = s_k[tr[g].x & 0x0FFFFFFF];
+      double y0 = s_state[tr[g].y >= 0 ? tr[g].y : 0], y1 = s_state[tr[g].z >= 0 ? tr[g].z : 0],
+             y2 = s_state[tr[g].w >= 0 ? tr[g].w : 0];
+      dd rate = {k, 0.0};
+      if (fl & 2) {
+        rate = {k * y0, 0.0};
+      } else {
+        rate = tr[g].y >= 0 ? dd_mul_d(rate, y0) : rate;
+        rate = tr[g].z >= 0 ? dd_mul_d(rate, y1) : rate;
+        rate = tr[g].w >= 0 ? dd_mul_d(rate, y2) : rate;
+      }
+      s_rate[2 * t] = rate.hi;
+      s_rate[2 * t + 1] = rate.lo;
+    }
   }
 }
 
