@@ -331,12 +331,10 @@ struct CellCv {
   }
 
   // guess_helper, camp_solver.c:1321-1408 (called with h_n = h > 0)
+  // y_n_min: the minimum of y_n, which the caller has computed
   __device__ __forceinline__ int guess_helper(double t_n, double h_n, const double* y_n, const double* y_n1,
-                              const double* hf, double* tmp1, double* corr) {
-    double mn = 1.0e308;
-    for (int i = lane; i < nv; i += kWave) mn = fmin(mn, y_n[i]);
-    mn = wave_min(mn);
-    if (mn > -1.0e-30) return 0;
+                              const double* hf, double* tmp1, double* corr, double y_n_min) {
+    if (y_n_min > -1.0e-30) return 0;
     double c = 1.0 / h_n;
     for (int i = lane; i < nv; i += kWave) {
       tmp1[i] = y_n1[i];
@@ -436,12 +434,21 @@ struct CellCv {
       wsync();
       lsolve(tempv);
       nni++;
-      del = wrms(tempv);
+      // the correction's WRMS norm, the corrected state and the per-lane sum
+      // of the new correction's norm in one pass over the rows (one round
+      // trip for the error weights and zn[0] instead of two)
+      double s_del = 0.0, s_acor = 0.0;
       for (int i = lane; i < nv; i += kWave) {
-        double a = acor[i] + tempv[i];
+        double t = tempv[i], wt = ewt[i];
+        double p = t * wt;
+        s_del += p * p;
+        double a = acor[i] + t;
         acor[i] = a;
         y[i] = zn(0)[i] + a;
+        double pa = a * wt;
+        s_acor += pa * pa;
       }
+      del = sqrt(wave_sum(s_del) / nv);
       wsync();
       if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
       double dcon = del * fmin(1.0, crate) / tq[4];
@@ -449,7 +456,8 @@ struct CellCv {
         int pr = project(acor);
         if (pr > 0) return jcur ? RHSFUNC_RECVR : TRY_AGAIN;
         projected = pr == 0;
-        acnrm = (m == 0 && pr < 0) ? del : wrms(acor);
+        // (without a projection acor is the one summed above)
+        acnrm = pr < 0 ? (m == 0 ? del : sqrt(wave_sum(s_acor) / nv)) : wrms(acor);
         jcur = 0;
         return CV_SUCCESS;
       }
@@ -467,9 +475,14 @@ struct CellCv {
                     (nst >= nstlp + MSBP) || (fabs(gamrat - 1.0) > DGMAX);
     use_acinit = 0;
     if (guess_on) {
-      for (int i = lane; i < nv; i += kWave) acinit[i] = zn(0)[i] - last[i];
+      double mn = 1.0e308;
+      for (int i = lane; i < nv; i += kWave) {
+        double z = zn(0)[i];
+        acinit[i] = z - last[i];
+        mn = fmin(mn, z);
+      }
       wsync();
-      int gh = guess_helper(tn, h, zn(0), last, acinit, gh1, gh2);
+      int gh = guess_helper(tn, h, zn(0), last, acinit, gh1, gh2, wave_min(mn));
       if (gh == 1) {
         for (int i = lane; i < nv; i += kWave) acinit[i] = gh2[i];
         wsync();
