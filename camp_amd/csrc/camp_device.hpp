@@ -78,6 +78,8 @@ struct DevProgram {
   const int *jm_ptr, *jm_rxn, *jm_param;
   const int* mj_elist;  // J_rxn assembly order (Program::mj_elist)
   int mj_rows;
+  const int4* mj_rec;   // the rows' record streams (Program::mj_rec), 2 int4 per record
+  const int2* mj_rhdr;  // per row: first record row, records
 };
 
 

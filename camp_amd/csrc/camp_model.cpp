@@ -394,6 +394,41 @@ static void build_mp_j_order(Program& P) {
   P.mj_rows = (rn + 63) / 64;
   P.mj_elist.assign((size_t)64 * std::max(P.mj_rows, 1), -1);
   for (int i = 0; i < rn; ++i) P.mj_elist[i] = order[i];
+  // the rows' record streams
+  auto put = [&](int* d, int k0, int k1, int k2, int k3, int k4, int k5, double c) {
+    uint64_t bits;
+    std::memcpy(&bits, &c, sizeof(bits));
+    d[0] = k0; d[1] = k1; d[2] = k2; d[3] = k3; d[4] = k4; d[5] = k5;
+    d[6] = (int)(uint32_t)(bits & 0xFFFFFFFFu);
+    d[7] = (int)(uint32_t)(bits >> 32);
+  };
+  P.mj_rec.clear();
+  P.mj_rhdr.clear();
+  int rec_rows = 0;
+  for (int r = 0; r < P.mj_rows; ++r) {
+    int len = 1;
+    for (int l = 0; l < 64; ++l) {
+      int e = P.mj_elist[(size_t)64 * r + l];
+      if (e >= 0)
+        len = std::max(len, (P.j_ptr[e + 1] - P.j_ptr[e]) + (P.js_ptr[e + 1] - P.js_ptr[e]));
+    }
+    P.mj_rhdr.push_back(rec_rows);
+    P.mj_rhdr.push_back(len);
+    P.mj_rec.resize((size_t)8 * 64 * (rec_rows + len), 0);  // zeros: padding records
+    for (int l = 0; l < 64; ++l) {
+      int e = P.mj_elist[(size_t)64 * r + l];
+      if (e < 0) continue;
+      int k = 0;
+      for (int q = P.j_ptr[e]; q < P.j_ptr[e + 1]; ++q, ++k)
+        put(&P.mj_rec[(size_t)8 * (64 * (size_t)(rec_rows + k) + l)], -1, P.je_a[4 * q],
+            P.je_a[4 * q + 1], P.je_a[4 * q + 2], P.je_a[4 * q + 3], P.je_b[q], P.je_c[q]);
+      for (int q = P.js_ptr[e]; q < P.js_ptr[e + 1]; ++q, ++k)
+        put(&P.mj_rec[(size_t)8 * (64 * (size_t)(rec_rows + k) + l)], P.js_a[4 * q],
+            P.js_a[4 * q + 1], P.js_a[4 * q + 2], P.js_a[4 * q + 3], 0, 0, P.js_c[q]);
+    }
+    rec_rows += len;
+  }
+  P.mj_rec.resize((size_t)8 * 64 * (rec_rows + 1), 0);  // the look-ahead's padding row
 }
 
 std::string Model::initialize() {

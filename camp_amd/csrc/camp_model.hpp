@@ -191,6 +191,15 @@ struct Program {
   // lanes of a row run the same record code
   int mj_rows = 0;
   std::vector<int> mj_elist;
+  // each row's records as one stream, lane-major: record row k of row r at
+  // mj_rhdr[2 r] + k holds, per lane, the lane's element's k-th record --
+  // its mass-action records, then its special records -- as 8 ints
+  // {kind (-1 mass action: k slot, reactants y1, y2 | the special record's
+  // kind, instance, p1, p2; 0 padding), then (mass action) the
+  // differentiated reactant and the guard, and the coefficient (low, high
+  // word)}; mj_rhdr[2 r + 1] records in row r (at least 1), and one
+  // padding record row past the last
+  std::vector<int> mj_rec, mj_rhdr;
 };
 
 

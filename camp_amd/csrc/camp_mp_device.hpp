@@ -295,91 +295,96 @@ __host__ __device__ __forceinline__ void mp_aqeq_jscalars(const DevProgram& P, i
   o[5] = w < 1.0e-4 * P.aq_small_water[i] ? 1.0 : 0.0;
 }
 
-// --- special Jacobian records of one reaction element, added to D ---------
+// --- one special Jacobian record {kind, instance, p1, p2}, c, added to D ----
 // PRODUCTION contributions add, LOSS contributions subtract (Jacobian.c:
 // 223-241 keeps them in separate long double sums and outputs prod - loss).
-__host__ __device__ __forceinline__ dd mp_jac_records(const DevProgram& P, int e, const double* s_state,
-                                             const double* s_aero, const double* s_jsc, dd D) {
+__host__ __device__ __forceinline__ dd mp_jac_record(const DevProgram& P, int4 a, double c,
+                                            const double* s_state, const double* s_aero,
+                                            const double* s_jsc, dd D) {
   const double* aq0 = s_jsc + kSpScalars * P.n_sp;
-  for (int q = P.js_ptr[e]; q < P.js_ptr[e + 1]; ++q) {
-    int4 a = P.js_a[q];
-    double c = P.js_c[q];
-    if (a.x <= kJsSimpolAP) {
-      const double* s = s_jsc + kSpScalars * a.y;
-      double N = s[0], kp = s[1], kc = s[2];
-      dd evap0 = {s[3], s[4]}, cond = {s[5], s[6]}, evap = {s[7], s[8]};
-      double dcdr = s[9], dedr = s[10], dedm = s[11], dedw = s[12];
-      switch (a.x) {
-        case kJsSimpolGG:  // LOSS N * cond_rate
-          D = dd_sub(D, dd_mul(dd_from(N), dd_from(kc)));
-          break;
-        case kJsSimpolAG:  // PRODUCTION N * cond_rate / kp
-          D = dd_add(D, dd_div_d(dd_mul(dd_from(N), dd_from(kc)), kp));
-          break;
-        case kJsSimpolGA:  // PRODUCTION N * evap_rate * act
-          D = dd_add(D, dd_mul(dd_from(N), evap0));
-          break;
-        case kJsSimpolAA:  // LOSS N * evap_rate * act / kp
-          D = dd_sub(D, dd_div_d(dd_mul(dd_from(N), evap0), kp));
-          break;
-        default: {  // phase element: species a.z of the instance's phase
-          int pi = P.sp_pi[a.y];
-          const double* ag = s_aero + kAeroSlots * pi;
-          double dens = P.pm_dens[a.z], mw = P.pm_mw[a.z];
-          double factor = 3.0 / (4.0 * kPi) / cr_powi(P.pi_dp[pi] / 2.0, 3);
-          double numj = 1.0 / dens * factor;
-          double massj = 1.0;
-          dd lm = {ag[3], ag[4]}, mo = {ag[5], ag[6]};
-          double mwj = dd_round(dd_div(dd_sub(mo, dd_mul(dd_from(1.0 / mw), lm)), dd_mul(mo, mo)));
-          double radj = 0.0;
-          if (a.x == kJsSimpolGP) {
-            D = dd_add(D, dd_from(N * dedr * radj));
-            D = dd_sub(D, dd_from(N * dcdr * radj));
-            D = dd_add(D, dd_mul(evap, dd_from(numj)));
-            D = dd_sub(D, dd_mul(cond, dd_from(numj)));
-            D = dd_add(D, dd_from(N * dedm * massj));
-            D = dd_add(D, dd_from(N * dedw * mwj));
-          } else {
-            D = dd_sub(D, dd_from(N * dedr / kp * radj));
-            D = dd_add(D, dd_from(N * dcdr / kp * radj));
-            D = dd_sub(D, dd_mul(dd_div_d(evap, kp), dd_from(numj)));
-            D = dd_add(D, dd_mul(dd_div_d(cond, kp), dd_from(numj)));
-            D = dd_sub(D, dd_from(N * dedm / kp * massj));
-            D = dd_sub(D, dd_from(N * dedw / kp * mwj));
-          }
+  if (a.x <= kJsSimpolAP) {
+    const double* s = s_jsc + kSpScalars * a.y;
+    double N = s[0], kp = s[1], kc = s[2];
+    dd evap0 = {s[3], s[4]}, cond = {s[5], s[6]}, evap = {s[7], s[8]};
+    double dcdr = s[9], dedr = s[10], dedm = s[11], dedw = s[12];
+    switch (a.x) {
+      case kJsSimpolGG:  // LOSS N * cond_rate
+        D = dd_sub(D, dd_mul(dd_from(N), dd_from(kc)));
+        break;
+      case kJsSimpolAG:  // PRODUCTION N * cond_rate / kp
+        D = dd_add(D, dd_div_d(dd_mul(dd_from(N), dd_from(kc)), kp));
+        break;
+      case kJsSimpolGA:  // PRODUCTION N * evap_rate * act
+        D = dd_add(D, dd_mul(dd_from(N), evap0));
+        break;
+      case kJsSimpolAA:  // LOSS N * evap_rate * act / kp
+        D = dd_sub(D, dd_div_d(dd_mul(dd_from(N), evap0), kp));
+        break;
+      default: {  // phase element: species a.z of the instance's phase
+        int pi = P.sp_pi[a.y];
+        const double* ag = s_aero + kAeroSlots * pi;
+        double dens = P.pm_dens[a.z], mw = P.pm_mw[a.z];
+        double factor = 3.0 / (4.0 * kPi) / cr_powi(P.pi_dp[pi] / 2.0, 3);
+        double numj = 1.0 / dens * factor;
+        double massj = 1.0;
+        dd lm = {ag[3], ag[4]}, mo = {ag[5], ag[6]};
+        double mwj = dd_round(dd_div(dd_sub(mo, dd_mul(dd_from(1.0 / mw), lm)), dd_mul(mo, mo)));
+        double radj = 0.0;
+        if (a.x == kJsSimpolGP) {
+          D = dd_add(D, dd_from(N * dedr * radj));
+          D = dd_sub(D, dd_from(N * dcdr * radj));
+          D = dd_add(D, dd_mul(evap, dd_from(numj)));
+          D = dd_sub(D, dd_mul(cond, dd_from(numj)));
+          D = dd_add(D, dd_from(N * dedm * massj));
+          D = dd_add(D, dd_from(N * dedw * mwj));
+        } else {
+          D = dd_sub(D, dd_from(N * dedr / kp * radj));
+          D = dd_add(D, dd_from(N * dcdr / kp * radj));
+          D = dd_sub(D, dd_mul(dd_div_d(evap, kp), dd_from(numj)));
+          D = dd_add(D, dd_mul(dd_div_d(cond, kp), dd_from(numj)));
+          D = dd_sub(D, dd_from(N * dedm / kp * massj));
+          D = dd_sub(D, dd_from(N * dedw / kp * mwj));
         }
       }
-      continue;
     }
-    const double* s = aq0 + kAqScalars * a.y;
-    if (s[5] != 0.0) continue;  // water below threshold: no contribution
-    dd fwd = {s[0], s[1]}, rev = {s[2], s[3]};
-    dd water = dd_from(s[4]);
-    const int* sp = P.aq_sp + 6 * a.y;
-    int nr = P.aq_nr[a.y], np = P.aq_np[a.y];
-    switch (a.x) {
-      case kJsAqRR:
-        if (fwd.hi != 0.0) D = dd_sub(D, dd_mul(dd_div_d(dd_div_d(fwd, s_state[sp[a.z]]), c), water));
-        break;
-      case kJsAqRP:
-        if (fwd.hi != 0.0) D = dd_add(D, dd_mul(dd_div_d(dd_div_d(fwd, s_state[sp[a.z]]), c), water));
-        break;
-      case kJsAqPR:
-        if (rev.hi != 0.0) D = dd_add(D, dd_mul(dd_div_d(dd_div_d(rev, s_state[sp[a.z]]), c), water));
-        break;
-      case kJsAqPP:
-        if (rev.hi != 0.0) D = dd_sub(D, dd_mul(dd_div_d(dd_div_d(rev, s_state[sp[a.z]]), c), water));
-        break;
-      case kJsAqRW:  // LOSS -fwd/m*(nr-1), PRODUCTION -rev/m*(np-1)
-        D = dd_sub(D, dd_mul(dd_div_d(dd_neg(fwd), c), dd_from((double)(nr - 1))));
-        D = dd_add(D, dd_mul(dd_div_d(dd_neg(rev), c), dd_from((double)(np - 1))));
-        break;
-      case kJsAqPW:  // PRODUCTION -fwd/m*(nr-1), LOSS -rev/m*(np-1)
-        D = dd_add(D, dd_mul(dd_div_d(dd_neg(fwd), c), dd_from((double)(nr - 1))));
-        D = dd_sub(D, dd_mul(dd_div_d(dd_neg(rev), c), dd_from((double)(np - 1))));
-        break;
-    }
+    return D;
   }
+  const double* s = aq0 + kAqScalars * a.y;
+  if (s[5] != 0.0) return D;  // water below threshold: no contribution
+  dd fwd = {s[0], s[1]}, rev = {s[2], s[3]};
+  dd water = dd_from(s[4]);
+  const int* sp = P.aq_sp + 6 * a.y;
+  int nr = P.aq_nr[a.y], np = P.aq_np[a.y];
+  switch (a.x) {
+    case kJsAqRR:
+      if (fwd.hi != 0.0) D = dd_sub(D, dd_mul(dd_div_d(dd_div_d(fwd, s_state[sp[a.z]]), c), water));
+      break;
+    case kJsAqRP:
+      if (fwd.hi != 0.0) D = dd_add(D, dd_mul(dd_div_d(dd_div_d(fwd, s_state[sp[a.z]]), c), water));
+      break;
+    case kJsAqPR:
+      if (rev.hi != 0.0) D = dd_add(D, dd_mul(dd_div_d(dd_div_d(rev, s_state[sp[a.z]]), c), water));
+      break;
+    case kJsAqPP:
+      if (rev.hi != 0.0) D = dd_sub(D, dd_mul(dd_div_d(dd_div_d(rev, s_state[sp[a.z]]), c), water));
+      break;
+    case kJsAqRW:  // LOSS -fwd/m*(nr-1), PRODUCTION -rev/m*(np-1)
+      D = dd_sub(D, dd_mul(dd_div_d(dd_neg(fwd), c), dd_from((double)(nr - 1))));
+      D = dd_add(D, dd_mul(dd_div_d(dd_neg(rev), c), dd_from((double)(np - 1))));
+      break;
+    case kJsAqPW:  // PRODUCTION -fwd/m*(nr-1), LOSS -rev/m*(np-1)
+      D = dd_add(D, dd_mul(dd_div_d(dd_neg(fwd), c), dd_from((double)(nr - 1))));
+      D = dd_sub(D, dd_mul(dd_div_d(dd_neg(rev), c), dd_from((double)(np - 1))));
+      break;
+  }
+  return D;
+}
+
+// All special Jacobian records of one reaction element, added to D.
+__host__ __device__ __forceinline__ dd mp_jac_records(const DevProgram& P, int e, const double* s_state,
+                                             const double* s_aero, const double* s_jsc, dd D) {
+  for (int q = P.js_ptr[e]; q < P.js_ptr[e + 1]; ++q)
+    D = mp_jac_record(P, P.js_a[q], P.js_c[q], s_state, s_aero, s_jsc, D);
   return D;
 }
 
@@ -462,6 +467,27 @@ __device__ __forceinline__ void mp_j_prologue(const DevProgram& P, int lane, con
   wsync();
 }
 
+// One record of a J_rxn row stream (Program::mj_rec) added to D: a
+// mass-action record as jac_elem_mass_action adds it, a special record as
+// mp_jac_record does, padding not at all.
+__device__ __forceinline__ dd mp_jac_stream_record(const DevProgram& P, int4 a, int4 b,
+                                                   const double* s_state, const double* s_k,
+                                                   double h, const double* s_aero,
+                                                   const double* s_jsc, dd D) {
+  const double c = hilo_to_double(b.w, b.z);
+  if (a.x == -1) {
+    double v = s_k[a.y];
+    if (a.z >= 0) v *= s_state[a.z];
+    if (a.w >= 0) v *= s_state[a.w];
+    if (c == -1.0) return dd_add(D, dd{-v, 0.0});
+    const int g = b.y;
+    if (g >= 0 && !(-v * s_state[b.x] * c * h <= s_state[g])) return D;
+    return dd_add(D, dd{c * v, 0.0});
+  }
+  if (a.x == 0) return D;
+  return mp_jac_record(P, a, c, s_state, s_aero, s_jsc, D);
+}
+
 // J of a multi-phase cell: the reaction Jacobian into Jr (global scratch,
 // rnnz values), then J = J_rxn * J_params over the map (camp_solver.c:
 // 1100-1106), one solver element per lane.
@@ -469,13 +495,24 @@ __device__ __forceinline__ void cell_jac_mp(const DevProgram& P, int lane, const
                                             const double* s_k, double h, MpLds L, double* Jr,
                                             double* J) {
   lane = phase_lane(lane);
-  // elements in the program's order, whose rows share their record kinds
+  vm_drain();
+  // elements in the program's order, whose rows share their record kinds;
+  // each lane reads its element's records as a stream, the next record in
+  // flight while the current one is added
   for (int r = 0; r < P.mj_rows; ++r) {
-    int e = P.mj_elist[r * kWave + lane];
-    if (e < 0) continue;
-    dd D = jac_elem_mass_action(P, e, s_state, s_k, h);
-    D = mp_jac_records(P, e, s_state, L.aero, L.jsc, D);
-    Jr[e] = D.hi;
+    const int e = P.mj_elist[r * kWave + lane];
+    const int2 rh = ld_uniform(P.mj_rhdr, r);
+    const int4* rec = P.mj_rec + 2 * ((size_t)rh.x * kWave + lane);
+    dd D = {0.0, 0.0};
+    int4 a = rec[0], b = rec[1];
+    for (int k = 0; k < rh.y; ++k) {
+      const int4 ca = a, cb = b;
+      const int4* nx = rec + 2 * (size_t)(k + 1) * kWave;
+      a = nx[0];
+      b = nx[1];
+      D = mp_jac_stream_record(P, ca, cb, s_state, s_k, h, L.aero, L.jsc, D);
+    }
+    if (e >= 0) Jr[e] = D.hi;
   }
   __threadfence_block();
   wsync();

@@ -181,6 +181,8 @@ void Solver::setup_device() {
     P.js_c = dev_upload(G.js_c, owned);
     P.mj_rows = G.mj_rows;
     P.mj_elist = dev_upload(G.mj_elist, owned);
+    P.mj_rec = reinterpret_cast<const int4*>(dev_upload(G.mj_rec, owned));
+    P.mj_rhdr = reinterpret_cast<const int2*>(dev_upload(G.mj_rhdr, owned));
     P.jm_ptr = dev_upload(G.jm_ptr, owned);
     P.jm_rxn = dev_upload(G.jm_rxn, owned);
     P.jm_param = dev_upload(G.jm_param, owned);
