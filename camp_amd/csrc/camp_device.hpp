@@ -37,7 +37,7 @@ struct DevProgram {
   int n_cs;                            // state variables that are not solver variables
   const int* cs_ids;
   const int *perm, *m_to_lu, *diag_pos, *e_diag;
-  const int *l_pos, *l_row, *u_pos, *u_row;
+  const int *l_pos, *l_row, *u_pos, *u_row;  // column solve: padded columns (Program::cs_*)
   const uint64_t* up_rec;  // LU update records: dst | l << 16 | u << 32, rows of 64
   const int4* lu_hdr;       // per pivot: {l range, update range}, {diag position, u range}
   const int* lf_pos;        // L positions the factorisation scales, rows of 64
@@ -843,14 +843,15 @@ __device__ __forceinline__ void col_fetch_rows(const DevProgram& P, int lane, in
     int4 h2 = ld_uniform(P.lu_hdr, 2 * k + 1);
     S.r = FWD ? int4{h.x, h.y, h2.x, 0} : int4{h2.y, h2.z, h2.x, 0};
   }
-  int q = S.r.x + lane;
-  S.row = q < S.r.y ? row_a[q] : -1;
-  S.pos = q < S.r.y ? pos_a[q] : 0;
+  // (columns are padded to whole rows of 64 with row -1, position 0)
+  const int q = S.r.x + lane;
+  S.row = row_a[q];
+  S.pos = pos_a[q];
 }
 template <bool FWD>
 __device__ __forceinline__ void col_fetch_vals(const DevProgram& P, int k, const double* lu,
                                                ColStage& S) {
-  S.val = S.row >= 0 ? lu[S.pos] : 0.0;
+  S.val = lu[S.pos];  // (position 0 for padding: read, never used)
   S.dg = (!FWD && k >= 0 && k < P.nv) ? lu[S.r.z] : 1.0;
 }
 // Pivot k of a sweep: x_k (divided by the diagonal, backward) updates the

@@ -1103,11 +1103,31 @@ std::string Model::initialize() {
   if (P.l_pos.empty()) { P.l_pos.push_back(0); P.l_row.push_back(0); }
   if (P.u_pos.empty()) { P.u_pos.push_back(0); P.u_row.push_back(0); }
   if (P.up_dst.empty()) { P.up_dst.push_back(0); P.up_l.push_back(0); P.up_u.push_back(0); }
+  // the column solve's padded columns: {start, start + entries} per pivot
+  P.cs_lpos.clear(); P.cs_lrow.clear(); P.cs_upos.clear(); P.cs_urow.clear();
+  std::vector<int> cl(2 * nv), cu(2 * nv);
+  auto pad_cols = [&](const std::vector<int>& ptr, const std::vector<int>& vpos,
+                      const std::vector<int>& vrow, std::vector<int>& opos, std::vector<int>& orow,
+                      std::vector<int>& rng) {
+    for (int k = 0; k < nv; ++k) {
+      int n = ptr[k + 1] - ptr[k], start = (int)opos.size();
+      rng[2 * k] = start;
+      rng[2 * k + 1] = start + n;
+      for (int q = 0; q < std::max(1, (n + 63) / 64) * 64; ++q) {
+        opos.push_back(q < n ? vpos[ptr[k] + q] : 0);
+        orow.push_back(q < n ? vrow[ptr[k] + q] : -1);
+      }
+    }
+    // entries read for the pivots past the end (header zero): the first row
+    if (opos.size() < 64) { opos.resize(64, 0); orow.resize(64, -1); }
+  };
+  pad_cols(P.l_ptr, P.l_pos, P.l_row, P.cs_lpos, P.cs_lrow, cl);
+  pad_cols(P.u_ptr, P.u_pos, P.u_row, P.cs_upos, P.cs_urow, cu);
   P.lu_hdr.assign((size_t)8 * (nv + 2), 0);
   for (int k = 0; k < nv; ++k) {
     int* h = &P.lu_hdr[(size_t)8 * k];
-    h[0] = P.l_ptr[k]; h[1] = P.l_ptr[k + 1]; h[2] = P.up_ptr[k]; h[3] = P.up_ptr[k + 1];
-    h[4] = P.diag_pos[k]; h[5] = P.u_ptr[k]; h[6] = P.u_ptr[k + 1];
+    h[0] = cl[2 * k]; h[1] = cl[2 * k + 1]; h[2] = P.up_ptr[k]; h[3] = P.up_ptr[k + 1];
+    h[4] = P.diag_pos[k]; h[5] = cu[2 * k]; h[6] = cu[2 * k + 1];
   }
   // The register triangular solve's program (nv <= 128): per block of 8
   // pivots, in the order the device sweeps visit them, and per lane t, the

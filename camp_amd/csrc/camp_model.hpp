@@ -115,6 +115,10 @@ struct Program {
   std::vector<int> l_ptr, l_pos, l_row;             // per pivot k: L(:,k) below k
   std::vector<int> u_ptr, u_pos, u_row;             // per pivot k: U(:,k) above k
   std::vector<int> up_ptr;                          // per pivot k: update range
+  // the column solve's copies of L(:,k) and U(:,k): each column padded to
+  // whole rows of 64 entries with (row -1, position 0), so that a column's
+  // first 64 entries load with no bounds check; lu_hdr's ranges index them
+  std::vector<int> cs_lpos, cs_lrow, cs_upos, cs_urow;
   std::vector<int> lu_hdr;  // per pivot k: {l_ptr[k], l_ptr[k+1], up_ptr[k], up_ptr[k+1],
                             //  diag_pos[k], u_ptr[k], u_ptr[k+1], 0} (two 16-byte
                             //  loads), then 2 empty pivots

@@ -80,10 +80,11 @@ void Solver::setup_device() {
   P.m_to_lu = dev_upload(G.m_to_lu, owned);
   P.diag_pos = dev_upload(G.diag_pos, owned);
   P.e_diag = dev_upload(G.e_diag, owned);
-  P.l_pos = dev_upload(G.l_pos, owned);
-  P.l_row = dev_upload(G.l_row, owned);
-  P.u_pos = dev_upload(G.u_pos, owned);
-  P.u_row = dev_upload(G.u_row, owned);
+  // (the column solve's padded copies; the host keeps the plain columns)
+  P.l_pos = dev_upload(G.cs_lpos, owned);
+  P.l_row = dev_upload(G.cs_lrow, owned);
+  P.u_pos = dev_upload(G.cs_upos, owned);
+  P.u_row = dev_upload(G.cs_urow, owned);
   P.lu_hdr = reinterpret_cast<const int4*>(dev_upload(G.lu_hdr, owned));
   P.up_rec = dev_upload(G.up_rec, owned);
   P.lf_pos = dev_upload(G.lf_pos, owned);
