@@ -826,23 +826,29 @@ __device__ __forceinline__ void cell_lu_solve_reg(const DevProgram& P, int lane,
 
 // Column sweeps of the triangular solves (nv > 128): one stage per pivot k
 // -- its column range, the rows and LU positions of its first 64 entries
-// (one per lane), their LU values and, backward, the pivot's diagonal.
+// (one per lane), their LU values and, backward, the pivot's diagonal -- and
+// the range of the pivot the stage takes next (rn).
 struct ColStage {
-  int4 r;
+  int4 r, rn;
   int row, pos;
   double val, dg;
 };
+// {column start, column end, diagonal position} of pivot k; zero outside
+// 0..nv-1
 template <bool FWD>
-__device__ __forceinline__ void col_fetch_rows(const DevProgram& P, int lane, int k, ColStage& S) {
+__device__ __forceinline__ int4 col_range(const DevProgram& P, int k) {
+  if (k < 0 || k >= P.nv) return int4{0, 0, 0, 0};
+  int4 h = ld_uniform(P.lu_hdr, 2 * k);
+  int4 h2 = ld_uniform(P.lu_hdr, 2 * k + 1);
+  return FWD ? int4{h.x, h.y, h2.x, 0} : int4{h2.y, h2.z, h2.x, 0};
+}
+// the stage's next pivot: its range (fetched a step earlier, in rn), rows
+// and positions
+template <bool FWD>
+__device__ __forceinline__ void col_fetch_rows(const DevProgram& P, int lane, ColStage& S) {
   const int* pos_a = FWD ? P.l_pos : P.u_pos;
   const int* row_a = FWD ? P.l_row : P.u_row;
-  if (k < 0 || k >= P.nv) {
-    S.r = int4{0, 0, 0, 0};
-  } else {
-    int4 h = ld_uniform(P.lu_hdr, 2 * k);
-    int4 h2 = ld_uniform(P.lu_hdr, 2 * k + 1);
-    S.r = FWD ? int4{h.x, h.y, h2.x, 0} : int4{h2.y, h2.z, h2.x, 0};
-  }
+  S.r = S.rn;
   // (columns are padded to whole rows of 64 with row -1, position 0)
   const int q = S.r.x + lane;
   S.row = row_a[q];
@@ -875,36 +881,43 @@ __device__ __forceinline__ void col_pivot(const DevProgram& P, int lane, int k, 
   wsync();
 }
 // One step of the pipelined sweep: the LU values of pivot k + 2 (stage C,
-// whose rows were fetched a step earlier), pivot k (stage A), then the rows
-// of pivot k + 3 into A.
+// whose rows were fetched a step earlier), pivot k (stage A), the rows of
+// pivot k + 3 into A (its range fetched a step earlier) and the range of
+// pivot k + 4 into B, the stage of the next step.
 template <bool FWD>
 __device__ __forceinline__ void col_step(const DevProgram& P, int lane, int n, const double* lu,
-                                         double* w, ColStage& A, ColStage& C) {
+                                         double* w, ColStage& A, ColStage& B, ColStage& C) {
   constexpr int st = FWD ? 1 : -1;
   const int k = FWD ? n : P.nv - 1 - n;
   col_fetch_vals<FWD>(P, k + 2 * st, lu, C);
   if (n < P.nv) col_pivot<FWD>(P, lane, k, A, lu, w);
-  col_fetch_rows<FWD>(P, lane, k + 3 * st, A);
+  col_fetch_rows<FWD>(P, lane, A);
+  B.rn = col_range<FWD>(P, k + 4 * st);
 }
 // The sweep, unrolled by three over three stages so that each stage keeps
-// its registers: LU values are fetched two pivots ahead and rows three
-// ahead, with no copy between stages that would wait for the newest load.
+// its registers: LU values are fetched two pivots ahead, rows three ahead
+// and ranges four ahead, with no copy between stages that would wait for
+// the newest load.
 template <bool FWD>
 __device__ __forceinline__ void col_sweep(const DevProgram& P, int lane, const double* lu,
                                           double* w) {
   constexpr int st = FWD ? 1 : -1;
   const int k0 = FWD ? 0 : P.nv - 1;
   ColStage S0, S1, S2;
-  col_fetch_rows<FWD>(P, lane, k0, S0);
-  col_fetch_rows<FWD>(P, lane, k0 + st, S1);
-  col_fetch_rows<FWD>(P, lane, k0 + 2 * st, S2);
+  S0.rn = col_range<FWD>(P, k0);
+  S1.rn = col_range<FWD>(P, k0 + st);
+  S2.rn = col_range<FWD>(P, k0 + 2 * st);
+  col_fetch_rows<FWD>(P, lane, S0);
+  col_fetch_rows<FWD>(P, lane, S1);
+  col_fetch_rows<FWD>(P, lane, S2);
+  S0.rn = col_range<FWD>(P, k0 + 3 * st);
   col_fetch_vals<FWD>(P, k0, lu, S0);
   col_fetch_vals<FWD>(P, k0 + st, lu, S1);
   vm_drain();
   for (int n = 0; n < P.nv; n += 3) {
-    col_step<FWD>(P, lane, n, lu, w, S0, S2);
-    col_step<FWD>(P, lane, n + 1, lu, w, S1, S0);
-    col_step<FWD>(P, lane, n + 2, lu, w, S2, S1);
+    col_step<FWD>(P, lane, n, lu, w, S0, S1, S2);
+    col_step<FWD>(P, lane, n + 1, lu, w, S1, S2, S0);
+    col_step<FWD>(P, lane, n + 2, lu, w, S2, S0, S1);
   }
 }
 
